@@ -1,0 +1,74 @@
+"""Build the native parts of the package in-tree.
+
+* ``libaiso.so``: the HIP engine (kernels.hip, engine.cpp, host.cpp) for gfx950,
+  exporting the C-ABI declared in ``include/aiso.h``;
+* the pybind11 modules ``pyAcoustic_iso_float_{nl,born,born_ext}`` that mirror the
+  reference's module names on top of that C-ABI.
+
+Run ``python -m acoustic_isotropic_operators_amd.build`` (or ``build()``).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import sysconfig
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INC = os.path.join(ROOT, "include")
+ARCH = os.environ.get("AISO_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+             # keep hipcc from emitting packed FP32 instructions
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-I" + INC, "-I" + CSRC]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {cmd[0]} ({r.returncode})")
+    return r
+
+
+def build_lib():
+    objs = []
+    bdir = os.path.join(PKG, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    for src in ("kernels.hip", "engine.cpp", "host.cpp"):
+        obj = os.path.join(bdir, src.replace(".", "_") + ".o")
+        lang = [] if src.endswith(".hip") else ["-x", "hip"]
+        _run([HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    out = os.path.join(PKG, "libaiso.so")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out, *objs])
+    return out
+
+
+def build_pybind():
+    import pybind11
+    ext = sysconfig.get_config_var("EXT_SUFFIX")
+    inc = ["-I" + pybind11.get_include(), "-I" + sysconfig.get_paths()["include"], "-I" + INC, "-I" + CSRC]
+    outs = []
+    for name in ("pyAcoustic_iso_float_nl", "pyAcoustic_iso_float_born", "pyAcoustic_iso_float_born_ext"):
+        src = os.path.join(CSRC, "pybind", name + ".cpp")
+        out = os.path.join(PKG, name + ext)
+        _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", *inc, src,
+              "-L" + PKG, "-laiso", "-Wl,-rpath,$ORIGIN", "-o", out])
+        outs.append(out)
+    return outs
+
+
+def build():
+    lib = build_lib()
+    mods = build_pybind()
+    return [lib, *mods]
+
+
+if __name__ == "__main__":
+    for p in build():
+        print(p)

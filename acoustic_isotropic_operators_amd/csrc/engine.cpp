@@ -1,0 +1,856 @@
+// engine.cpp -- the C-ABI of libaiso.so: contexts, geometry, and the
+// multi-shot operators (nonlinear, Born, offset-extended Born).
+//
+// The reference runs one shot per GPU at a time, re-creating its operator
+// objects and device buffers on every call (F/src/BornShotsGpu.cpp:82-143).
+// Here a context keeps the velocity tables, geometry lists and work buffers
+// resident on the device, and every fine time step advances a whole batch of
+// shots with one launch of the fused step kernel (kernels.hip).  Per-shot
+// results are reduced in shot order, so they do not depend on the batch size.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "aiso.h"
+#include "host.hpp"
+#include "kernels.hpp"
+
+using namespace aiso;
+
+namespace {
+
+thread_local std::string g_err;
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        throw Error(AISO_ERR_MEMORY, std::string("out of device memory in ") + what);
+    if (e != hipSuccess)
+        throw Error(AISO_ERR_HIP, std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+#define HIPC(x) hip_check((x), #x)
+
+// Grow-only device buffer.
+struct DBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void* need(size_t bytes)
+    {
+        if (bytes <= n && p) return p;
+        release();
+        if (bytes == 0) bytes = 4;
+        HIPC(hipMalloc(&p, bytes));
+        n = bytes;
+        return p;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+template <class T>
+void upload(DBuf& b, const std::vector<T>& v, hipStream_t s)
+{
+    b.need(std::max<size_t>(v.size(), 1) * sizeof(T));
+    if (!v.empty()) HIPC(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+std::atomic<long> g_geom_id{1};
+
+// Per-tile sparse lists of one geometry for one tile shape, plus the arrays
+// of the device-side spatial interpolation.
+struct DevGeom {
+    DBuf tilePtr, ent, ring, cornerReg, weight, adjPtr, adjEnt;
+    int nRing = 0, nReg = 0;
+};
+
+}  // namespace
+
+struct aiso_device {
+    Geometry g;
+    long id = g_geom_id++;
+};
+
+struct aiso_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    FdParams p;
+    bool haveVel = false;
+    std::vector<float> vel, v2dt2h, reflh;
+    DBuf v2dt2, refl, cd;
+    int batchReq = 0, batchUsed = 0, tz = 64, tx = 32;
+    size_t bytesPerShot = 0;
+    // work buffers
+    DBuf pairs, W, ss, img, dreg, sig, slots, mtil, acc, modelDev, dataDev;
+    std::map<std::tuple<long, int, int>, std::unique_ptr<DevGeom>> geoms;
+    // profiling of the step kernels
+    bool profile = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::vector<double> evPts;
+    long long launches = 0;
+    double ms = 0, pts = 0;
+};
+
+namespace {
+
+int tilesZ(const aiso_ctx* c) { return (c->p.nz - 2 * kFat + c->tz - 1) / c->tz; }
+int tilesX(const aiso_ctx* c) { return (c->p.nx - 2 * kFat + c->tx - 1) / c->tx; }
+
+// Bucket the unique points of a geometry by step-kernel tile.
+DevGeom* dev_geom(aiso_ctx* c, const aiso_device* d)
+{
+    auto key = std::make_tuple(d->id, c->tz, c->tx);
+    auto it = c->geoms.find(key);
+    if (it != c->geoms.end()) return it->second.get();
+    auto dg = std::make_unique<DevGeom>();
+    const Geometry& g = d->g;
+    const int nz = c->p.nz, nx = c->p.nx, tZ = tilesZ(c), nt = tZ * tilesX(c);
+    std::vector<std::vector<int2>> buckets(nt);
+    std::vector<int2> ring;
+    for (int r = 0; r < (int)g.regPos.size(); ++r) {
+        int pos = g.regPos[r], iz = pos % nz, ix = pos / nz;
+        if (iz >= kFat && iz < nz - kFat && ix >= kFat && ix < nx - kFat) {
+            int t = (iz - kFat) / c->tz + tZ * ((ix - kFat) / c->tx);
+            int local = ((ix - kFat) % c->tx) * c->tz + (iz - kFat) % c->tz;
+            buckets[t].push_back(make_int2(local, r));
+        } else {
+            ring.push_back(make_int2(pos, r));
+        }
+    }
+    std::vector<int> ptr(nt + 1, 0);
+    std::vector<int2> ent;
+    for (int t = 0; t < nt; ++t) {
+        ptr[t + 1] = ptr[t] + (int)buckets[t].size();
+        ent.insert(ent.end(), buckets[t].begin(), buckets[t].end());
+    }
+    upload(dg->tilePtr, ptr, c->stream);
+    upload(dg->ent, ent, c->stream);
+    upload(dg->ring, ring, c->stream);
+    upload(dg->cornerReg, g.cornerReg, c->stream);
+    upload(dg->weight, g.weight, c->stream);
+    upload(dg->adjPtr, g.adjPtr, c->stream);
+    upload(dg->adjEnt, g.adjEnt, c->stream);
+    dg->nRing = (int)ring.size();
+    dg->nReg = (int)g.regPos.size();
+    HIPC(hipStreamSynchronize(c->stream));
+    DevGeom* out = dg.get();
+    c->geoms.emplace(key, std::move(dg));
+    return out;
+}
+
+enum class Op { NlFwd, NlAdj, BornFwd, BornAdj, ExtFwd, ExtAdj };
+
+bool is_adj(Op op) { return op == Op::NlAdj || op == Op::BornAdj || op == Op::ExtAdj; }
+bool is_born(Op op) { return op != Op::NlFwd && op != Op::NlAdj; }
+bool is_ext(Op op) { return op == Op::ExtFwd || op == Op::ExtAdj; }
+
+struct Request {
+    Op op;
+    int add;
+    const float* modelIn;   // forward input
+    float* modelOut;        // adjoint output
+    const float* dataIn;
+    float* dataOut;
+    int nShotSig;           // NL model shots
+    aiso_device* const* src;
+    int nShot;
+    const float* wavelets;
+    int nWav;
+    aiso_device* const* rec;
+    int nRec;
+    int mem;
+};
+
+void launch_timed(aiso_ctx* c, unsigned mode, const StepArgs& a, int nshots)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->profile) {
+        HIPC(hipEventCreate(&e0));
+        HIPC(hipEventCreate(&e1));
+        HIPC(hipEventRecord(e0, c->stream));
+    }
+    HIPC(launch_step(mode, c->tz, c->tx, a, c->stream));
+    if (c->profile) {
+        HIPC(hipEventRecord(e1, c->stream));
+        c->ev.emplace_back(e0, e1);
+        c->evPts.push_back(double(c->p.interior()) * nshots);
+    }
+}
+
+void drain_profile(aiso_ctx* c)
+{
+    if (c->ev.empty()) return;
+    HIPC(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < c->ev.size(); ++i) {
+        float t = 0.f;
+        HIPC(hipEventElapsedTime(&t, c->ev[i].first, c->ev[i].second));
+        c->ms += t;
+        c->pts += c->evPts[i];
+        c->launches++;
+        (void)hipEventDestroy(c->ev[i].first);
+        (void)hipEventDestroy(c->ev[i].second);
+    }
+    c->ev.clear();
+    c->evPts.clear();
+}
+
+// Host signal chain for one Born shot: S = I_t(D_v(D2_t(R_src^T w)))
+// (F/include/seismicOperator2D.cpp:151-180).
+std::vector<float> born_source(const aiso_ctx* c, const Geometry& src, const float* wav)
+{
+    const FdParams& p = c->p;
+    const int nReg = (int)src.regPos.size();
+    std::vector<float> reg((size_t)nReg * p.nts), d2((size_t)nReg * p.nts), fine((size_t)nReg * p.ntw);
+    src.adjoint(false, reg.data(), wav, p.nts);
+    second_time_derivative(p, nReg, reg.data(), d2.data());
+    scale_source(p, c->vel.data(), src.regPos, p.nts, d2.data());
+    interp_time_fwd(p, nReg, d2.data(), fine.data());
+    return fine;
+}
+
+// Nonlinear source chain (F/src/nonlinearPropGpu.cpp:132-139).
+std::vector<float> nl_source(const aiso_ctx* c, const Geometry& src, const float* m)
+{
+    const FdParams& p = c->p;
+    const int nReg = (int)src.regPos.size();
+    std::vector<float> reg((size_t)nReg * p.nts), fine((size_t)nReg * p.ntw);
+    src.adjoint(false, reg.data(), m, p.nts);
+    scale_source(p, c->vel.data(), src.regPos, p.nts, reg.data());
+    interp_time_fwd(p, nReg, reg.data(), fine.data());
+    return fine;
+}
+
+void run(aiso_ctx* c, const Request& q)
+{
+    const FdParams& p = c->p;
+    if (!c->haveVel) throw Error(AISO_ERR_STATE, "velocity not set");
+    if (q.nShot < 1) throw Error(AISO_ERR_PARAM, "nShot must be >= 1");
+    if (q.nRec != 1 && q.nRec != q.nShot) throw Error(AISO_ERR_PARAM, "receiver geometry count must be 1 or nShot");
+    if ((!is_born(q.op) || is_ext(q.op)) && p.sub >= kSubMax) throw Error(AISO_ERR_PARAM, "**** ERROR: Subsampling parameter too high ****");
+    if ((!is_born(q.op) || is_ext(q.op)) && p.minPad >= kPadMax) throw Error(AISO_ERR_PARAM, "**** ERROR: Padding value is too high ****");
+    if (is_ext(q.op) && p.extension != AISO_EXT_OFFSET && p.nExt > 1)
+        throw Error(AISO_ERR_PARAM, "extended Born needs extension=offset");
+    if (is_born(q.op) && q.nWav != 1 && q.nWav != q.nShot) throw Error(AISO_ERR_PARAM, "wavelet count must be 1 or nShot");
+    if (!is_born(q.op) && q.nShotSig != 1 && q.nShotSig != q.nShot)
+        throw Error(AISO_ERR_PARAM, "model shot axis must be 1 or nShot");
+    const int nRecDev = q.rec[0]->g.nDev;
+    for (int i = 0; i < q.nRec; ++i)
+        if (q.rec[i]->g.nDev != nRecDev) throw Error(AISO_ERR_GEOM, "receivers per shot must be constant");
+    const int nSrcDev = q.src[0]->g.nDev;
+    for (int i = 0; i < q.nShot; ++i)
+        if (q.src[i]->g.nDev != nSrcDev) throw Error(AISO_ERR_GEOM, "sources per shot must be constant");
+    for (int i = 0; i < q.nShot; ++i)
+        for (int v : q.src[i]->g.gridIdx)
+            if (v >= p.nzx()) throw Error(AISO_ERR_GEOM, "source outside the velocity grid");
+    for (int i = 0; i < q.nRec; ++i)
+        for (int v : q.rec[i]->g.gridIdx)
+            if (v >= p.nzx()) throw Error(AISO_ERR_GEOM, "receiver outside the velocity grid");
+
+    hipStream_t s = c->stream;
+    const long nzx = p.nzx();
+    const int nExt = is_ext(q.op) ? p.nExt : 1;
+    const long modelN = is_born(q.op) ? nzx * nExt : (long)q.nShotSig * nSrcDev * p.nts;
+    const long dataN = (long)q.nShot * nRecDev * p.nts;
+    const int h = p.h();
+    const float* f = p.filt.data();
+
+    // ---- operands on the device
+    const float* dModelIn = nullptr;
+    float* dModelOut = nullptr;
+    const float* dDataIn = nullptr;
+    float* dDataOut = nullptr;
+    std::vector<float> hModel;   // NL models are assembled on the host
+    if (!is_born(q.op)) {
+        hModel.resize(modelN);
+        if (q.op == Op::NlFwd) {
+            if (q.mem == AISO_MEM_DEVICE) HIPC(hipMemcpy(hModel.data(), q.modelIn, modelN * 4, hipMemcpyDeviceToHost));
+            else std::memcpy(hModel.data(), q.modelIn, modelN * 4);
+        } else if (q.add) {
+            if (q.mem == AISO_MEM_DEVICE) HIPC(hipMemcpy(hModel.data(), q.modelOut, modelN * 4, hipMemcpyDeviceToHost));
+            else std::memcpy(hModel.data(), q.modelOut, modelN * 4);
+        } else {
+            std::fill(hModel.begin(), hModel.end(), 0.f);
+        }
+    } else if (q.mem == AISO_MEM_DEVICE) {
+        dModelIn = q.modelIn;
+        dModelOut = q.modelOut;
+    } else {
+        float* m = c->modelDev.need(modelN * 4) ? c->modelDev.as<float>() : nullptr;
+        if (!is_adj(q.op) || q.add) HIPC(hipMemcpyAsync(m, is_adj(q.op) ? q.modelOut : q.modelIn, modelN * 4, hipMemcpyHostToDevice, s));
+        dModelIn = m;
+        dModelOut = m;
+    }
+    if (q.mem == AISO_MEM_DEVICE) {
+        dDataIn = q.dataIn;
+        dDataOut = q.dataOut;
+    } else {
+        float* d = c->dataDev.need(dataN * 4) ? c->dataDev.as<float>() : nullptr;
+        if (is_adj(q.op) || q.add) HIPC(hipMemcpyAsync(d, is_adj(q.op) ? q.dataIn : q.dataOut, dataN * 4, hipMemcpyHostToDevice, s));
+        dDataIn = d;
+        dDataOut = d;
+    }
+
+    // ---- geometry lists
+    std::vector<DevGeom*> srcG(q.nShot), recG(q.nRec);
+    int nSrcRegMax = 1, nRecRegMax = 1;
+    for (int i = 0; i < q.nShot; ++i) {
+        srcG[i] = dev_geom(c, q.src[i]);
+        nSrcRegMax = std::max(nSrcRegMax, srcG[i]->nReg);
+    }
+    for (int i = 0; i < q.nRec; ++i) {
+        recG[i] = dev_geom(c, q.rec[i]);
+        nRecRegMax = std::max(nRecRegMax, recG[i]->nReg);
+    }
+
+    // ---- batch size from the HBM budget
+    const bool needW = is_born(q.op), needSS = is_ext(q.op) || q.op == Op::BornAdj;
+    const bool needImg = q.op == Op::BornAdj || q.op == Op::ExtAdj;
+    size_t perShot = 2 * nzx * 4 + (size_t)nRecRegMax * p.nts * 4 + (size_t)nSrcRegMax * p.ntw * 4;
+    if (needW) perShot += (size_t)p.nts * nzx * 4;
+    if (needSS) perShot += 2 * nzx * 4;
+    if (needImg) perShot += (size_t)nExt * nzx * 4;
+    c->bytesPerShot = perShot;
+    int B = c->batchReq;
+    if (B <= 0) {
+        size_t freeB = 0, totalB = 0;
+        HIPC(hipMemGetInfo(&freeB, &totalB));
+        size_t have = c->pairs.n + c->W.n + c->ss.n + c->img.n + c->dreg.n + c->sig.n;
+        size_t budget = (size_t)((freeB + have) * 0.80);
+        B = (int)std::max<size_t>(1, budget / perShot);
+        B = std::min(B, 32);
+    }
+    B = std::min(B, q.nShot);
+    c->batchUsed = B;
+
+    float* pairs = c->pairs.need((size_t)B * 2 * nzx * 4) ? c->pairs.as<float>() : nullptr;
+    float* W = needW ? (float*)c->W.need((size_t)B * p.nts * nzx * 4) : nullptr;
+    float* ss = needSS ? (float*)c->ss.need((size_t)B * 2 * nzx * 4) : nullptr;
+    float* img = needImg ? (float*)c->img.need((size_t)B * nExt * nzx * 4) : nullptr;
+    float* dreg = (float*)c->dreg.need((size_t)B * nRecRegMax * p.nts * 4);
+    float* sig = (float*)c->sig.need((size_t)B * nSrcRegMax * p.ntw * 4);
+    ShotSlot* dSlots = (ShotSlot*)c->slots.need(sizeof(ShotSlot) * B);
+
+    // ---- per-call model preparation
+    const float* mtil = nullptr;
+    float* acc = nullptr;
+    if (q.op == Op::BornFwd) {
+        float* m = (float*)c->mtil.need(nzx * 4);
+        HIPC(launch_scale_model(dModelIn, m, c->v2dt2.as<float>(), c->refl.as<float>(), p.nz, p.nx, s));
+        mtil = m;
+    } else if (q.op == Op::ExtFwd) {
+        float* m = (float*)c->mtil.need(nzx * nExt * 4);
+        HIPC(launch_scale_ext(dModelIn, m, c->refl.as<float>(), p.nz, p.nx, nExt, s));
+        mtil = m;
+    }
+    if (needImg) {
+        acc = (float*)c->acc.need(nzx * nExt * 4);
+        HIPC(launch_fill(acc, nzx * nExt, s));
+    }
+    std::vector<float> nlAcc;
+    if (q.op == Op::NlAdj && q.nShotSig == 1) nlAcc.assign((size_t)nSrcDev * p.nts, 0.f);
+
+    StepArgs base{};
+    base.slots = dSlots;
+    base.v2dt2 = c->v2dt2.as<float>();
+    base.cd = c->cd.as<float>();
+    base.mtil = mtil;
+    base.nz = p.nz; base.nx = p.nx; base.minPad = p.minPad; base.nts = p.nts; base.ntw = p.ntw;
+    base.tilesZ = tilesZ(c);
+    base.ntiles = base.tilesZ * tilesX(c);
+    for (int k = 0; k < 6; ++k) { base.zc[k] = p.zc[k]; base.xc[k] = p.xc[k]; }
+    base.wA = base.wB = base.wImg = 0;
+
+    std::vector<float> sigHost;
+    for (int b0 = 0; b0 < q.nShot; b0 += B) {
+        const int nb = std::min(B, q.nShot - b0);
+        // -- slot descriptors and source traces
+        std::vector<ShotSlot> slots(nb);
+        int maxSrcRing = 0, maxRecRing = 0;
+        sigHost.assign((size_t)nb * nSrcRegMax * p.ntw, 0.f);
+        for (int k = 0; k < nb; ++k) {
+            const int shot = b0 + k;
+            const aiso_device* sd = q.src[shot];
+            DevGeom* sg = srcG[shot];
+            DevGeom* rg = recG[q.nRec == 1 ? 0 : shot];
+            ShotSlot& S = slots[k];
+            std::memset(&S, 0, sizeof(S));
+            S.p[0] = pairs + (size_t)k * 2 * nzx;
+            S.p[1] = S.p[0] + nzx;
+            if (W) S.W = W + (size_t)k * p.nts * nzx;
+            if (ss) { S.ss[0] = ss + (size_t)k * 2 * nzx; S.ss[1] = S.ss[0] + nzx; }
+            if (img) S.img = img + (size_t)k * nExt * nzx;
+            S.dreg = dreg + (size_t)k * nRecRegMax * p.nts;
+            S.sig = sig + (size_t)k * nSrcRegMax * p.ntw;
+            S.srcTile = sg->tilePtr.as<int>();
+            S.srcEnt = sg->ent.as<int2>();
+            S.srcRing = sg->ring.as<int2>();
+            S.nSrcRing = sg->nRing;
+            S.recTile = rg->tilePtr.as<int>();
+            S.recEnt = rg->ent.as<int2>();
+            S.recRing = rg->ring.as<int2>();
+            S.nRecRing = rg->nRing;
+            maxSrcRing = std::max(maxSrcRing, sg->nRing);
+            maxRecRing = std::max(maxRecRing, rg->nRing);
+            std::vector<float> fine;
+            if (is_born(q.op)) {
+                const float* wav = q.wavelets + (size_t)(q.nWav == 1 ? 0 : shot) * nSrcDev * p.nts;
+                fine = born_source(c, sd->g, wav);
+            } else if (q.op == Op::NlFwd) {
+                fine = nl_source(c, sd->g, hModel.data() + (size_t)(q.nShotSig == 1 ? 0 : shot) * nSrcDev * p.nts);
+            }
+            if (!fine.empty()) std::memcpy(sigHost.data() + (size_t)k * nSrcRegMax * p.ntw, fine.data(), fine.size() * 4);
+        }
+        HIPC(hipMemcpyAsync(dSlots, slots.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
+        if (q.op != Op::NlAdj) HIPC(hipMemcpyAsync(sig, sigHost.data(), sigHost.size() * 4, hipMemcpyHostToDevice, s));
+        else HIPC(launch_fill(sig, (long)nb * nSrcRegMax * p.ntw, s));
+
+        StepArgs a = base;
+        a.nshots = nb;
+        a.gridLogical = a.ntiles * nb;
+
+        // -- adjoint: data traces to the regular receiver grid
+        if (is_adj(q.op)) {
+            for (int k = 0; k < nb; ++k) {
+                const int shot = b0 + k;
+                const aiso_device* rd = q.rec[q.nRec == 1 ? 0 : shot];
+                DevGeom* rg = recG[q.nRec == 1 ? 0 : shot];
+                HIPC(launch_dev_adjoint(dreg + (size_t)k * nRecRegMax * p.nts, dDataIn + (size_t)shot * nRecDev * p.nts,
+                                        rg->adjPtr.as<int>(), rg->adjEnt.as<int>(), rg->weight.as<float>(),
+                                        rg->nReg, rd->g.nb, p.nts, s));
+            }
+        } else {
+            HIPC(launch_fill(dreg, (long)nb * nRecRegMax * p.nts, s));
+        }
+
+        auto ring = [&](unsigned mode, const StepArgs& sa) {
+            int mr = 0;
+            if (mode & (M_INJ_SIG | M_REC_SIG)) mr = std::max(mr, maxSrcRing);
+            if (mode & (M_INJ_DATA | M_REC_DATA)) mr = std::max(mr, maxRecRing);
+            if (mr > 0) HIPC(launch_ring(mode, sa, mr, s));
+        };
+
+        // -- source wavefield pass (all Born-type operators)
+        if (is_born(q.op)) {
+            HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
+            a.cur = 1;
+            for (int its = 0; its < p.nts - 1; ++its)
+                for (int it2 = 1; it2 <= p.sub; ++it2) {
+                    a.its = its;
+                    a.itwInj = its * p.sub + it2 - 1;
+                    a.wA = (long)its * nzx;
+                    a.wB = (long)(its + 1) * nzx;
+                    a.fe = f[it2];
+                    a.ff = f[h + it2];
+                    a.skipA = a.fe == 0.f;
+                    a.skipB = a.ff == 0.f;
+                    a.assignA = its == 0 && it2 == 1;
+                    a.assignB = it2 == 1;
+                    launch_timed(c, MODE_SRC, a, nb);
+                    ring(MODE_SRC, a);
+                    a.cur ^= 1;
+                }
+        }
+
+        // -- second pass
+        HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
+        a.cur = 1;
+        a.assignA = a.assignB = a.skipA = a.skipB = 0;
+        if (q.op == Op::NlFwd || q.op == Op::BornFwd || q.op == Op::ExtFwd) {
+            const unsigned mode = q.op == Op::NlFwd ? MODE_NL_FWD : q.op == Op::BornFwd ? MODE_BORN_SCAT : MODE_EXT_SCAT;
+            int L = 0, R = 1;
+            if (q.op == Op::ExtFwd) HIPC(launch_ext_ss(dSlots, nb, L, 0, mtil, base.v2dt2, p.nz, p.nx, p.hExt, s));
+            for (int its = 0; its < p.nts - 1; ++its) {
+                if (q.op == Op::ExtFwd)
+                    HIPC(launch_ext_ss(dSlots, nb, R, (long)(its + 1) * nzx, mtil, base.v2dt2, p.nz, p.nx, p.hExt, s));
+                for (int it2 = 1; it2 <= p.sub; ++it2) {
+                    a.its = its;
+                    a.itwInj = its * p.sub + it2 - 1;
+                    a.fa = f[it2 - 1];
+                    a.fb = f[h + it2 - 1];
+                    a.wA = (long)its * nzx;
+                    a.wB = (long)(its + 1) * nzx;
+                    a.ssA = L;
+                    a.ssB = R;
+                    a.fe = f[it2];
+                    a.ff = f[h + it2];
+                    launch_timed(c, mode, a, nb);
+                    ring(mode, a);
+                    a.cur ^= 1;
+                }
+                std::swap(L, R);
+            }
+            for (int k = 0; k < nb; ++k) {
+                const int shot = b0 + k;
+                const aiso_device* rd = q.rec[q.nRec == 1 ? 0 : shot];
+                DevGeom* rg = recG[q.nRec == 1 ? 0 : shot];
+                HIPC(launch_dev_forward(dreg + (size_t)k * nRecRegMax * p.nts, dDataOut + (size_t)shot * nRecDev * p.nts,
+                                        rg->cornerReg.as<int>(), rg->weight.as<float>(), nRecDev, rd->g.nb, p.nts,
+                                        q.add, s));
+            }
+        } else {
+            const unsigned mode = q.op == Op::NlAdj ? MODE_NL_ADJ : q.op == Op::BornAdj ? MODE_BORN_REC : MODE_EXT_REC;
+            int L = 0, R = 1;
+            if (ss) HIPC(launch_fill(ss, (long)nb * 2 * nzx, s));
+            if (img) HIPC(launch_fill(img, (long)nb * nExt * nzx, s));
+            for (int its = p.nts - 2; its >= 0; --its) {
+                for (int it2 = p.sub - 1; it2 >= 0; --it2) {
+                    a.its = its;
+                    a.itwRec = its * p.sub + it2;
+                    a.fc = f[it2 + 1];
+                    a.fd = f[h + it2 + 1];
+                    a.ssA = L;
+                    a.ssB = R;
+                    a.fe = f[it2];
+                    a.ff = f[h + it2];
+                    a.skipA = a.fe == 0.f;
+                    a.skipB = a.ff == 0.f;
+                    a.assignA = it2 == p.sub - 1;
+                    a.assignB = 0;
+                    a.image = (q.op == Op::BornAdj && it2 == 0) ? 1 : 0;
+                    a.wImg = (long)(its + 1) * nzx;
+                    launch_timed(c, mode, a, nb);
+                    ring(mode, a);
+                    a.cur ^= 1;
+                }
+                if (q.op == Op::ExtAdj)
+                    HIPC(launch_ext_img(dSlots, nb, R, (long)(its + 1) * nzx, base.v2dt2, p.nz, p.nx, nExt, s));
+                std::swap(L, R);
+            }
+            if (q.op == Op::BornAdj) {
+                HIPC(launch_born_adj_finish(dSlots, nb, R, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
+            } else if (q.op == Op::ExtAdj) {
+                HIPC(launch_ext_img(dSlots, nb, R, 0, base.v2dt2, p.nz, p.nx, nExt, s));
+                HIPC(launch_ext_adj_finish(dSlots, nb, c->refl.as<float>(), p.nz, p.nx, nExt, s));
+            }
+            if (needImg) {
+                for (int k = 0; k < nb; ++k) HIPC(launch_accumulate(acc, img + (size_t)k * nExt * nzx, nzx * nExt, s));
+            }
+            if (q.op == Op::NlAdj) {
+                std::vector<float> fineAll((size_t)nb * nSrcRegMax * p.ntw);
+                HIPC(hipMemcpyAsync(fineAll.data(), sig, fineAll.size() * 4, hipMemcpyDeviceToHost, s));
+                HIPC(hipStreamSynchronize(s));
+                for (int k = 0; k < nb; ++k) {
+                    const int shot = b0 + k;
+                    const Geometry& sgm = q.src[shot]->g;
+                    const int nReg = (int)sgm.regPos.size();
+                    std::vector<float> reg((size_t)nReg * p.nts);
+                    interp_time_adj(p, nReg, reg.data(), fineAll.data() + (size_t)k * nSrcRegMax * p.ntw);
+                    scale_source(p, c->vel.data(), sgm.regPos, p.nts, reg.data());
+                    if (q.nShotSig == 1) {
+                        sgm.forward(true, reg.data(), nlAcc.data(), p.nts);
+                    } else {
+                        std::vector<float> sl((size_t)nSrcDev * p.nts);
+                        sgm.forward(false, reg.data(), sl.data(), p.nts);
+                        float* dst = hModel.data() + (size_t)shot * nSrcDev * p.nts;
+                        for (size_t i = 0; i < sl.size(); ++i) dst[i] += sl[i];
+                    }
+                }
+            }
+        }
+        drain_profile(c);
+    }
+
+    // ---- outputs
+    if (needImg) HIPC(launch_add_to(dModelOut, acc, nzx * nExt, q.add, s));
+    if (q.op == Op::NlAdj) {
+        if (q.nShotSig == 1)
+            for (size_t i = 0; i < nlAcc.size(); ++i) hModel[i] += nlAcc[i];
+        if (q.mem == AISO_MEM_DEVICE) HIPC(hipMemcpyAsync(q.modelOut, hModel.data(), modelN * 4, hipMemcpyHostToDevice, s));
+        else std::memcpy(q.modelOut, hModel.data(), modelN * 4);
+    }
+    if (q.mem == AISO_MEM_HOST) {
+        if (is_adj(q.op) && is_born(q.op)) HIPC(hipMemcpyAsync(q.modelOut, dModelOut, modelN * 4, hipMemcpyDeviceToHost, s));
+        if (!is_adj(q.op)) HIPC(hipMemcpyAsync(q.dataOut, dDataOut, dataN * 4, hipMemcpyDeviceToHost, s));
+    }
+    HIPC(hipStreamSynchronize(s));
+    HIPC(hipGetLastError());
+}
+
+template <class F>
+aiso_status guard(F&& f)
+{
+    try {
+        f();
+        return AISO_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return AISO_ERR_PARAM;
+    }
+}
+
+aiso_status op_call(aiso_ctx* c, Op op, int add, const float* mIn, float* mOut, const float* dIn, float* dOut,
+                    int nShotSig, aiso_device* const* src, int nShot, const float* wav, int nWav,
+                    aiso_device* const* rec, int nRec, int mem)
+{
+    return guard([&] {
+        if (!c) throw Error(AISO_ERR_STATE, "null context");
+        if (!src || !rec) throw Error(AISO_ERR_GEOM, "null geometry list");
+        HIPC(hipSetDevice(c->device));
+        Request q{op, add, mIn, mOut, dIn, dOut, nShotSig, src, nShot, wav, nWav, rec, nRec, mem};
+        run(c, q);
+    });
+}
+
+}  // namespace
+
+// ================================================================= C-ABI ==
+extern "C" {
+
+const char* aiso_last_error(void) { return g_err.c_str(); }
+
+aiso_status aiso_ctx_create(const aiso_fd_params* par, int device, aiso_ctx** out)
+{
+    return guard([&] {
+        if (!par || !out) throw Error(AISO_ERR_PARAM, "null argument");
+        auto c = std::make_unique<aiso_ctx>();
+        c->p = make_params(*par);
+        int n = 0;
+        HIPC(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n)
+            throw Error(AISO_ERR_PARAM, "**** ERROR [getGpuInfo]: One of the element of the GPU Id list is not a valid GPU Id number ****");
+        c->device = device;
+        HIPC(hipSetDevice(device));
+        HIPC(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        *out = c.release();
+    });
+}
+
+aiso_status aiso_ctx_destroy(aiso_ctx* c)
+{
+    return guard([&] {
+        if (!c) return;
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        for (auto& e : c->ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+        c->geoms.clear();
+        hipStream_t s = c->stream;
+        delete c;
+        (void)hipStreamDestroy(s);
+    });
+}
+
+aiso_status aiso_ctx_set_velocity(aiso_ctx* c, const float* vel, int mem)
+{
+    return guard([&] {
+        if (!c || !vel) throw Error(AISO_ERR_PARAM, "null argument");
+        HIPC(hipSetDevice(c->device));
+        const FdParams& p = c->p;
+        std::vector<float> v(p.nzx());
+        if (mem == AISO_MEM_DEVICE) HIPC(hipMemcpy(v.data(), vel, v.size() * 4, hipMemcpyDeviceToHost));
+        else std::memcpy(v.data(), vel, v.size() * 4);
+        check_velocity(p, v.data());
+        c->vel = std::move(v);
+        c->v2dt2h.assign(p.nzx(), 0.f);
+        c->reflh.assign(p.nzx(), 0.f);
+        velocity_scales(p, c->vel.data(), c->v2dt2h.data(), c->reflh.data());
+        upload(c->v2dt2, c->v2dt2h, c->stream);
+        upload(c->refl, c->reflh, c->stream);
+        upload(c->cd, p.cd, c->stream);
+        HIPC(hipStreamSynchronize(c->stream));
+        c->haveVel = true;
+    });
+}
+
+aiso_status aiso_ctx_set_batch(aiso_ctx* c, int n)
+{
+    return guard([&] {
+        if (!c || n < 0) throw Error(AISO_ERR_PARAM, "bad batch size");
+        c->batchReq = n;
+    });
+}
+
+aiso_status aiso_ctx_set_tile(aiso_ctx* c, int tz, int tx)
+{
+    return guard([&] {
+        if (!c) throw Error(AISO_ERR_PARAM, "null context");
+        if ((tz != 32 && tz != 64) || (tx != 8 && tx != 16 && tx != 32))
+            throw Error(AISO_ERR_PARAM, "tile must be tz in {32,64}, tx in {8,16,32}");
+        c->tz = tz;
+        c->tx = tx;
+    });
+}
+
+aiso_status aiso_ctx_info(const aiso_ctx* c, int* ntw, int* batchUsed, size_t* bytesPerShot)
+{
+    return guard([&] {
+        if (!c) throw Error(AISO_ERR_PARAM, "null context");
+        if (ntw) *ntw = c->p.ntw;
+        if (batchUsed) *batchUsed = c->batchUsed;
+        if (bytesPerShot) *bytesPerShot = c->bytesPerShot;
+    });
+}
+
+aiso_status aiso_ctx_stream(const aiso_ctx* c, void** stream)
+{
+    return guard([&] {
+        if (!c || !stream) throw Error(AISO_ERR_PARAM, "null argument");
+        *stream = (void*)c->stream;
+    });
+}
+
+aiso_status aiso_ctx_profile(aiso_ctx* c, int enable)
+{
+    return guard([&] {
+        if (!c) throw Error(AISO_ERR_PARAM, "null context");
+        drain_profile(c);
+        c->profile = enable != 0;
+        c->launches = 0;
+        c->ms = 0;
+        c->pts = 0;
+    });
+}
+
+aiso_status aiso_ctx_profile_read(aiso_ctx* c, long long* launches, double* ms, double* pts)
+{
+    return guard([&] {
+        if (!c) throw Error(AISO_ERR_PARAM, "null context");
+        drain_profile(c);
+        if (launches) *launches = c->launches;
+        if (ms) *ms = c->ms;
+        if (pts) *pts = c->pts;
+    });
+}
+
+static aiso_status wrap_dev(Geometry&& g, aiso_device** out)
+{
+    auto d = std::make_unique<aiso_device>();
+    d->g = std::move(g);
+    *out = d.release();
+    return AISO_OK;
+}
+
+aiso_status aiso_device_create_coords(int nDev, const float* z, const float* x, int nzVel, int nxVel, float oz,
+                                      float dz, float ox, float dx, int nt, int dipole, float zShift, float xShift,
+                                      aiso_device** out)
+{
+    return guard([&] {
+        if (!z || !x || !out) throw Error(AISO_ERR_PARAM, "null argument");
+        wrap_dev(geom_coords(nDev, z, x, nzVel, nxVel, oz, dz, ox, dx, nt, dipole, zShift, xShift), out);
+    });
+}
+
+aiso_status aiso_device_create_ints(int nDev, const int* z, const int* x, int nzVel, int nxVel, int nt, int dipole,
+                                    int zShift, int xShift, aiso_device** out)
+{
+    return guard([&] {
+        if (!z || !x || !out) throw Error(AISO_ERR_PARAM, "null argument");
+        wrap_dev(geom_ints(nDev, z, x, nzVel, nxVel, nt, dipole, zShift, xShift), out);
+    });
+}
+
+aiso_status aiso_device_create_grid(int nzDev, int ozDev, int dzDev, int nxDev, int oxDev, int dxDev, int nzVel,
+                                    int nxVel, int nt, int dipole, int zShift, int xShift, aiso_device** out)
+{
+    return guard([&] {
+        if (!out) throw Error(AISO_ERR_PARAM, "null argument");
+        wrap_dev(geom_grid(nzDev, ozDev, dzDev, nxDev, oxDev, dxDev, nzVel, nxVel, nt, dipole, zShift, xShift), out);
+    });
+}
+
+aiso_status aiso_device_destroy(aiso_device* d)
+{
+    delete d;
+    return AISO_OK;
+}
+
+aiso_status aiso_device_sizes(const aiso_device* d, int* nIrreg, int* nReg, int* nb)
+{
+    return guard([&] {
+        if (!d) throw Error(AISO_ERR_PARAM, "null device");
+        if (nIrreg) *nIrreg = d->g.nDev;
+        if (nReg) *nReg = (int)d->g.regPos.size();
+        if (nb) *nb = d->g.nb;
+    });
+}
+
+aiso_status aiso_device_reg_pos(const aiso_device* d, int* regPos)
+{
+    return guard([&] {
+        if (!d || !regPos) throw Error(AISO_ERR_PARAM, "null argument");
+        std::copy(d->g.regPos.begin(), d->g.regPos.end(), regPos);
+    });
+}
+
+aiso_status aiso_device_corners(const aiso_device* d, int* gridIdx, float* weight)
+{
+    return guard([&] {
+        if (!d) throw Error(AISO_ERR_PARAM, "null device");
+        if (gridIdx) std::copy(d->g.gridIdx.begin(), d->g.gridIdx.end(), gridIdx);
+        if (weight) std::copy(d->g.weight.begin(), d->g.weight.end(), weight);
+    });
+}
+
+aiso_status aiso_device_forward(const aiso_device* d, int add, const float* reg, float* irreg, int nt)
+{
+    return guard([&] {
+        if (!d || !reg || !irreg) throw Error(AISO_ERR_PARAM, "null argument");
+        d->g.forward(add != 0, reg, irreg, nt);
+    });
+}
+
+aiso_status aiso_device_adjoint(const aiso_device* d, int add, float* reg, const float* irreg, int nt)
+{
+    return guard([&] {
+        if (!d || !reg || !irreg) throw Error(AISO_ERR_PARAM, "null argument");
+        d->g.adjoint(add != 0, reg, irreg, nt);
+    });
+}
+
+aiso_status aiso_nl_forward(aiso_ctx* c, int add, const float* model, int nShotSig, float* data,
+                            aiso_device* const* src, int nShot, aiso_device* const* rec, int nRec, int mem)
+{
+    return op_call(c, Op::NlFwd, add, model, nullptr, nullptr, data, nShotSig, src, nShot, nullptr, 1, rec, nRec, mem);
+}
+
+aiso_status aiso_nl_adjoint(aiso_ctx* c, int add, float* model, int nShotSig, const float* data,
+                            aiso_device* const* src, int nShot, aiso_device* const* rec, int nRec, int mem)
+{
+    return op_call(c, Op::NlAdj, add, nullptr, model, data, nullptr, nShotSig, src, nShot, nullptr, 1, rec, nRec, mem);
+}
+
+aiso_status aiso_born_forward(aiso_ctx* c, int add, const float* model, float* data, aiso_device* const* src,
+                              int nShot, const float* wav, int nWav, aiso_device* const* rec, int nRec, int mem)
+{
+    return op_call(c, Op::BornFwd, add, model, nullptr, nullptr, data, 1, src, nShot, wav, nWav, rec, nRec, mem);
+}
+
+aiso_status aiso_born_adjoint(aiso_ctx* c, int add, float* model, const float* data, aiso_device* const* src,
+                              int nShot, const float* wav, int nWav, aiso_device* const* rec, int nRec, int mem)
+{
+    return op_call(c, Op::BornAdj, add, nullptr, model, data, nullptr, 1, src, nShot, wav, nWav, rec, nRec, mem);
+}
+
+aiso_status aiso_bornext_forward(aiso_ctx* c, int add, const float* model, float* data, aiso_device* const* src,
+                                 int nShot, const float* wav, int nWav, aiso_device* const* rec, int nRec, int mem)
+{
+    return op_call(c, Op::ExtFwd, add, model, nullptr, nullptr, data, 1, src, nShot, wav, nWav, rec, nRec, mem);
+}
+
+aiso_status aiso_bornext_adjoint(aiso_ctx* c, int add, float* model, const float* data, aiso_device* const* src,
+                                 int nShot, const float* wav, int nWav, aiso_device* const* rec, int nRec, int mem)
+{
+    return op_call(c, Op::ExtAdj, add, nullptr, model, data, nullptr, 1, src, nShot, wav, nWav, rec, nRec, mem);
+}
+
+int aiso_pad_plus(int n, int pad, int blockSize) { return pad_plus(n, pad, blockSize); }
+
+}  // extern "C"

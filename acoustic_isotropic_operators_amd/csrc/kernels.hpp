@@ -1,0 +1,97 @@
+// kernels.hpp -- argument blocks shared by the HIP kernels and their host
+// launchers.  Plain structs passed by value (kernel arguments) or through a
+// small device array (one ShotSlot per shot advanced by a launch).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace aiso {
+
+// One shot in flight: its wavefield pair and the per-shot buffers the fused
+// step kernel reads or updates.  Unused members are null.
+struct ShotSlot {
+    float* p[2];          // wavefield pair [nx][nz]; the "current" one is picked by parity
+    float* W;             // coarse source wavefield [nts][nx][nz]
+    float* ss[2];         // two coarse-rate accumulators / secondary sources [nx][nz]
+    float* img;           // per-shot image [nExt][nx][nz]
+    float* dreg;          // regular-grid receiver traces [nRecReg][nts]
+    float* sig;           // regular-grid source traces [nSrcReg][ntw]
+    const int* srcTile;   // per-tile CSR offsets into srcEnt (tiles+1)
+    const int2* srcEnt;   // (local index in tile, trace) for interior source points
+    const int* recTile;
+    const int2* recEnt;
+    const int2* srcRing;  // (grid index, trace) for points in the FAT ring
+    const int2* recRing;
+    int nSrcRing, nRecRing;
+};
+
+// Operating modes of the fused time-step kernel (template bits).
+enum : unsigned {
+    M_ADJ = 1u << 0,        // adjoint stepper (Laplacian of v2dt2*c)
+    M_SS_BORN = 1u << 1,    // + m~*(W[a]*fa) + m~*(W[b]*fb)   (Born scattered source)
+    M_SS_SLICE = 1u << 2,   // + ss[a]*fa + ss[b]*fb            (precomputed secondary source)
+    M_INJ_SIG = 1u << 3,    // + sig[s][itwInj] at source points
+    M_INJ_DATA = 1u << 4,   // + d[r][its]*fc + d[r][its+1]*fd at receiver points
+    M_CAPTURE = 1u << 5,    // A (+)= n*fe, B (+)= n*ff after damping
+    M_IMAGE = 1u << 6,      // img += Wimg * B (when args.image != 0)
+    M_REC_DATA = 1u << 7,   // d[r][its] += n*fe, d[r][its+1] += n*ff at receivers
+    M_REC_SIG = 1u << 8,    // sig[s][itwRec] += n at source points
+};
+
+struct StepArgs {
+    const ShotSlot* slots;
+    const float* v2dt2;
+    const float* cd;        // sponge table (minPad entries)
+    const float* mtil;      // scaled reflectivity for M_SS_BORN
+    int nz, nx, minPad, nts, ntw;
+    int tilesZ, ntiles, nshots, gridLogical;
+    float zc[6], xc[6];
+    int cur;                // index of the current slice in ShotSlot::p
+    int its, itwInj, itwRec;
+    long wA, wB;            // element offsets of the W slices used (capture / SS_BORN)
+    int ssA, ssB;           // ShotSlot::ss indices used as A and B
+    float fa, fb;           // pre-damp pointwise weights (b skipped when fb == 0)
+    float fc, fd;           // data-injection weights
+    float fe, ff;           // capture / record weights
+    int assignA, assignB;   // first touch of A / B: write instead of accumulate
+    int skipA, skipB;       // zero weight: leave A / B untouched
+    int image;              // apply M_IMAGE this step
+    long wImg;              // element offset of the W slice used by M_IMAGE
+};
+
+}  // namespace aiso
+
+namespace aiso {
+
+// The seven fused-step configurations used by the operators.
+constexpr unsigned MODE_NL_FWD = M_INJ_SIG | M_REC_DATA;
+constexpr unsigned MODE_NL_ADJ = M_ADJ | M_INJ_DATA | M_REC_SIG;
+constexpr unsigned MODE_SRC = M_INJ_SIG | M_CAPTURE;
+constexpr unsigned MODE_BORN_SCAT = M_SS_BORN | M_REC_DATA;
+constexpr unsigned MODE_BORN_REC = M_ADJ | M_INJ_DATA | M_CAPTURE | M_IMAGE;
+constexpr unsigned MODE_EXT_SCAT = M_SS_SLICE | M_REC_DATA;
+constexpr unsigned MODE_EXT_REC = M_ADJ | M_INJ_DATA | M_CAPTURE;
+
+hipError_t launch_step(unsigned mode, int tz, int tx, const StepArgs& a, hipStream_t s);
+hipError_t launch_ring(unsigned mode, const StepArgs& a, int maxRing, hipStream_t s);
+hipError_t launch_fill(float* p, long n, hipStream_t s);
+hipError_t launch_scale_model(const float* m, float* out, const float* v2dt2, const float* refl,
+                              int nz, int nx, hipStream_t s);
+hipError_t launch_scale_ext(const float* m, float* out, const float* refl, int nz, int nx, int nExt,
+                            hipStream_t s);
+hipError_t launch_born_adj_finish(const ShotSlot* slots, int nshots, int ssR, const float* v2dt2,
+                                  const float* refl, int nz, int nx, hipStream_t s);
+hipError_t launch_ext_ss(const ShotSlot* slots, int nshots, int dst, long wk, const float* mext,
+                         const float* v2dt2, int nz, int nx, int hExt, hipStream_t s);
+hipError_t launch_ext_img(const ShotSlot* slots, int nshots, int src, long wk, const float* v2dt2,
+                          int nz, int nx, int nExt, hipStream_t s);
+hipError_t launch_ext_adj_finish(const ShotSlot* slots, int nshots, const float* refl, int nz, int nx,
+                                 int nExt, hipStream_t s);
+hipError_t launch_accumulate(float* acc, const float* img, long n, hipStream_t s);
+hipError_t launch_add_to(float* out, const float* acc, long n, int add, hipStream_t s);
+hipError_t launch_dev_forward(const float* reg, float* out, const int* cornerReg, const float* weight,
+                              int nDev, int nb, int nt, int add, hipStream_t s);
+hipError_t launch_dev_adjoint(float* reg, const float* irreg, const int* adjPtr, const int* adjEnt,
+                              const float* weight, int nReg, int nb, int nt, hipStream_t s);
+
+}  // namespace aiso

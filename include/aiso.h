@@ -1,0 +1,178 @@
+/*
+ * aiso.h -- C-ABI of the MI355X-native 2D acoustic isotropic FD engine
+ * (libaiso.so).  Plain pointers and sizes, status codes, no C++ or torch types.
+ *
+ * The entry points replace the reference's C++ operator classes that its
+ * pybind11 modules bind (paths relative to the reference root;
+ * F/ = acoustic_iso_lib/acoustic_iso_float, PY/ = acoustic_iso_lib/python/python_float):
+ *
+ *   aiso_ctx_*         fdParam + init*Gpu + allocate*Gpu     F/src/fdParam.cpp:9-140,
+ *                                                            F/src/BornShotsGpuFunctions.cu:81-238
+ *   aiso_device_*      deviceGpu (3 constructors, fwd/adj)   F/include/deviceGpu.h:30-45,
+ *                                                            PY/pyAcoustic_iso_float_nl.cpp:18-25
+ *   aiso_nl_*          nonlinearPropShotsGpu fwd/adj         F/include/nonlinearPropShotsGpu.h:18-53,
+ *                                                            PY/pyAcoustic_iso_float_nl.cpp:27-40
+ *   aiso_born_*        BornShotsGpu fwd/adj                  F/include/BornShotsGpu.h:20-56,
+ *                                                            PY/pyAcoustic_iso_float_born.cpp:18-34
+ *   aiso_bornext_*     BornExtShotsGpu fwd/adj (offset)      F/include/BornExtShotsGpu.h:20-58
+ *   aiso_pad_plus      padding rule                          F/src/padFileGpuMain.cpp:37-66
+ *
+ * Conventions (same as the reference operators):
+ *   - grids are [ix][iz] (z fastest) on the padded grid incl. the FAT=5 ring;
+ *   - data are [iShot][iRec][its]; wavelets [iSrcDevice][its]; NL models
+ *     [iShot|0][iSrcDevice][its]; extended models [iExt][ix][iz];
+ *   - add == 0 overwrites the output, add != 0 accumulates into it;
+ *   - every call returns AISO_OK or an error code; aiso_last_error() gives the
+ *     message (the reference prints it and throws std::runtime_error).
+ * Each context owns all of its state (no globals), so contexts are independent;
+ * calls on one context must not overlap.
+ */
+#ifndef AISO_H
+#define AISO_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    AISO_OK = 0,
+    AISO_ERR_PARAM = 1,    /* inconsistent parameters (fdParam / init checks)  */
+    AISO_ERR_GEOM = 2,     /* device out of bounds, bad geometry               */
+    AISO_ERR_HIP = 3,      /* HIP runtime failure                              */
+    AISO_ERR_MEMORY = 4,   /* not enough device memory                         */
+    AISO_ERR_STATE = 5     /* call out of order (e.g. no velocity set)         */
+} aiso_status;
+
+enum { AISO_MEM_HOST = 0, AISO_MEM_DEVICE = 1 };
+enum { AISO_EXT_NONE = 0, AISO_EXT_OFFSET = 1 };
+
+/* Parfile keys read by fdParam (F/src/fdParam.cpp:14-96).  nz/nx are the
+ * padded sizes of the velocity model, oz/ox its origins. */
+typedef struct {
+    int nz, nx;
+    float dz, dx, oz, ox;
+    int nts;
+    float dts, ots;
+    int sub;
+    int zPadMinus, zPadPlus, xPadMinus, xPadPlus;
+    int fat;            /* must be 5                                         */
+    int blockSize;      /* must be 16 (model-size rule of fdParam.cpp:258-268) */
+    float alphaCos;     /* default 0.99                                      */
+    float fMax;         /* default 1000                                      */
+    int freeSurface;    /* only 0 is supported                               */
+    int nExt;           /* odd, default 1                                    */
+    int extension;      /* AISO_EXT_NONE or AISO_EXT_OFFSET                  */
+} aiso_fd_params;
+
+typedef struct aiso_ctx aiso_ctx;
+typedef struct aiso_device aiso_device;
+
+/* ---------------------------------------------------------------- errors */
+/* Message of the last failed call on this thread. */
+const char* aiso_last_error(void);
+
+/* --------------------------------------------------------------- context */
+/* fdParam + init*Gpu: validates the parameters that do not depend on the
+ * velocity (F/src/fdParam.cpp:28-96, :258-268; sub/minPad limits of
+ * F/src/BornShotsGpuFunctions.cu:154-186) and binds HIP device `device`. */
+aiso_status aiso_ctx_create(const aiso_fd_params* par, int device, aiso_ctx** out);
+aiso_status aiso_ctx_destroy(aiso_ctx* ctx);
+
+/* fdParam velocity part + allocate*Gpu: CFL <= 0.45 and dispersion >= 3
+ * checks (F/src/fdParam.cpp:228-256), v^2*dtw^2 and 2/v^3 scalings
+ * (:122-139), kept resident on the device.  vel: nx*nz floats.  The
+ * reference's setVel (F/include/BornShotsGpu.h:54) maps here. */
+aiso_status aiso_ctx_set_velocity(aiso_ctx* ctx, const float* vel, int mem);
+
+/* Shots advanced together by one launch (0 = automatic, by free HBM). */
+aiso_status aiso_ctx_set_batch(aiso_ctx* ctx, int shotsPerLaunch);
+/* Tile shape of the time-step kernel: tz in {32,64}, tx in {8,16,32}. */
+aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
+/* Derived sizes: ntw, sub, nShotsPerLaunch actually used by the last call. */
+aiso_status aiso_ctx_info(const aiso_ctx* ctx, int* ntw, int* batchUsed, size_t* bytesPerShot);
+/* The HIP stream (hipStream_t) every kernel of this context runs on. */
+aiso_status aiso_ctx_stream(const aiso_ctx* ctx, void** stream);
+/* Per-launch timing of the time-step kernels with HIP events on that stream. */
+aiso_status aiso_ctx_profile(aiso_ctx* ctx, int enable);
+/* Launch count, summed duration (ms) and summed interior point-updates of
+ * the time-step kernels timed since profiling was enabled. */
+aiso_status aiso_ctx_profile_read(aiso_ctx* ctx, long long* launches, double* ms,
+                                  double* pointUpdates);
+
+/* ------------------------------------------------------------ geometry --
+ * deviceGpu.  nzVel/nxVel are the padded velocity sizes; nt the trace length. */
+/* Constructor #1 (F/src/deviceGpu.cpp:8-84): physical coordinates, bilinear
+ * corner weights.  oz/dz/ox/dx are the velocity axes. */
+aiso_status aiso_device_create_coords(int nDev, const float* zCoord, const float* xCoord,
+                                      int nzVel, int nxVel, float oz, float dz, float ox,
+                                      float dx, int nt, int dipole, float zDipoleShift,
+                                      float xDipoleShift, aiso_device** out);
+/* Constructor #2 (F/src/deviceGpu.cpp:87-139): integer grid positions. */
+aiso_status aiso_device_create_ints(int nDev, const int* zGrid, const int* xGrid,
+                                    int nzVel, int nxVel, int nt, int dipole,
+                                    int zDipoleShift, int xDipoleShift, aiso_device** out);
+/* Constructor #3 (F/src/deviceGpu.cpp:142-199): regular o/d/n device grid. */
+aiso_status aiso_device_create_grid(int nzDev, int ozDev, int dzDev, int nxDev, int oxDev,
+                                    int dxDev, int nzVel, int nxVel, int nt, int dipole,
+                                    int zDipoleShift, int xDipoleShift, aiso_device** out);
+aiso_status aiso_device_destroy(aiso_device* dev);
+/* getNDeviceIrreg / getNDeviceReg / getRegPosUnique (F/include/deviceGpu.h:41-47). */
+aiso_status aiso_device_sizes(const aiso_device* dev, int* nDevIrreg, int* nDevReg, int* nbCorner);
+aiso_status aiso_device_reg_pos(const aiso_device* dev, int* regPos);
+/* Raw corner list (nDevIrreg*nbCorner indices and weights). */
+aiso_status aiso_device_corners(const aiso_device* dev, int* gridIdx, float* weight);
+/* deviceGpu::forward (regular [nReg][nt] -> irregular [nDev][nt]) and
+ * ::adjoint, on host buffers (F/src/deviceGpu.cpp:226-261). */
+aiso_status aiso_device_forward(const aiso_device* dev, int add, const float* reg,
+                                float* irreg, int nt);
+aiso_status aiso_device_adjoint(const aiso_device* dev, int add, float* reg,
+                                const float* irreg, int nt);
+
+/* ----------------------------------------------------------- operators --
+ * sources: nShot geometries (one per shot).  receivers: 1 (constant geometry)
+ * or nShot.  wavelets: nWavelet (1 or nShot) blocks of [nSrcDev][nts].
+ * mem: AISO_MEM_HOST or AISO_MEM_DEVICE for model and data. */
+
+/* nonlinearPropShotsGpu::forward/adjoint (F/src/nonlinearPropShotsGpu.cpp:62-152,
+ * 257-359).  model [nShotSig][nSrcDev][nts] with nShotSig in {1, nShot}. */
+aiso_status aiso_nl_forward(aiso_ctx* ctx, int add, const float* model, int nShotSig,
+                            float* data, aiso_device* const* sources, int nShot,
+                            aiso_device* const* receivers, int nRecGeom, int mem);
+aiso_status aiso_nl_adjoint(aiso_ctx* ctx, int add, float* model, int nShotSig,
+                            const float* data, aiso_device* const* sources, int nShot,
+                            aiso_device* const* receivers, int nRecGeom, int mem);
+
+/* BornShotsGpu::forward/adjoint (F/src/BornShotsGpu.cpp:60-143, 245-339).
+ * model [nx][nz]. */
+aiso_status aiso_born_forward(aiso_ctx* ctx, int add, const float* model, float* data,
+                              aiso_device* const* sources, int nShot,
+                              const float* wavelets, int nWavelet,
+                              aiso_device* const* receivers, int nRecGeom, int mem);
+aiso_status aiso_born_adjoint(aiso_ctx* ctx, int add, float* model, const float* data,
+                              aiso_device* const* sources, int nShot,
+                              const float* wavelets, int nWavelet,
+                              aiso_device* const* receivers, int nRecGeom, int mem);
+
+/* BornExtShotsGpu::forward/adjoint with extension=offset
+ * (F/src/BornExtGpu.cpp:33-134 -> F/src/BornExtShotsGpuFunctions.cu:582-724,
+ * 1848-1989).  model [nExt][nx][nz]. */
+aiso_status aiso_bornext_forward(aiso_ctx* ctx, int add, const float* model, float* data,
+                                 aiso_device* const* sources, int nShot,
+                                 const float* wavelets, int nWavelet,
+                                 aiso_device* const* receivers, int nRecGeom, int mem);
+aiso_status aiso_bornext_adjoint(aiso_ctx* ctx, int add, float* model, const float* data,
+                                 aiso_device* const* sources, int nShot,
+                                 const float* wavelets, int nWavelet,
+                                 aiso_device* const* receivers, int nRecGeom, int mem);
+
+/* ---------------------------------------------------------------- misc */
+/* padFileGpuMain padding rule (F/src/padFileGpuMain.cpp:49-66, no free
+ * surface): returns padPlus for a side of n samples with pad on the minus side. */
+int aiso_pad_plus(int n, int pad, int blockSize);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,659 @@
+/*
+ * aiso_oracle_body.h -- precision-generic body of the CPU oracle.
+ * Included twice by aiso_oracle.c with REAL/SFX set to (float,_f) and
+ * (double,_d).  TEST INFRASTRUCTURE ONLY (see aiso_oracle.h).
+ *
+ * Kernel order, indexing and rounding follow the reference CUDA kernels one
+ * for one; each function cites the kernel or driver it restates.  Loops over
+ * interior points are OpenMP-parallel over ix (each output element is written
+ * by exactly one iteration, as in the reference's one-thread-per-point launch),
+ * so results do not depend on the thread count.
+ */
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define FN(name) CAT(name, SFX)
+
+#define FAT ORC_FAT
+
+/* Laplacian coefficients, time-interpolation filter and cosine sponge table.
+ * F/src/BornShotsGpuFunctions.cu:135-147 (coefficients), :151-167 (filter),
+ * :169-182 (sponge); identical in nonlinearShotsGpuFunctions.cu:123-170.
+ * filt has 2*(sub+1) entries, cd has minPad entries. */
+void FN(orc_tables)(const orc_grid* g, REAL* zc, REAL* xc, REAL* filt, REAL* cd)
+{
+    static const double C[6] = {-2.927222222, 1.666666667, -0.238095238,
+                                0.039682539, -0.004960317, 0.000317460};
+    REAL dz = (REAL)g->dz, dx = (REAL)g->dx;
+    for (int k = 0; k < 6; ++k) {
+        zc[k] = (REAL)(C[k] / (double)(dz * dz));
+        xc[k] = (REAL)(C[k] / (double)(dx * dx));
+    }
+    int sub = g->sub, nts = g->nts, ntw = (nts - 1) * sub + 1, h = sub + 1;
+    double s = 1.0 / sqrt((double)((REAL)ntw / (REAL)nts));
+    for (int i = 0; i < h; ++i) {
+        REAL a = (REAL)(1.0 - 1.0 * i / sub);
+        REAL b = (REAL)(1.0 - (double)a);
+        filt[i] = (REAL)((double)a * s);
+        filt[i + h] = (REAL)((double)b * s);
+    }
+    REAL alpha = (REAL)g->alphaCos;
+    for (int j = 0; j < g->minPad; ++j) {
+        REAL arg = (REAL)(M_PI / (1.0 * g->minPad) * 1.0 * (g->minPad - j));
+        cd[j] = (REAL)((double)alpha + (1.0 - (double)alpha) * cos((double)arg));
+    }
+}
+
+/* v^2*dtw^2 and the reflectivity scale 2/v^3.  F/src/fdParam.cpp:23,122-139. */
+void FN(orc_vel_scales)(const orc_grid* g, const REAL* vel, REAL* v2dt2, REAL* refl)
+{
+    REAL dtw = (REAL)g->dts / (REAL)g->sub;
+    long n = (long)g->nz * g->nx;
+    for (long i = 0; i < n; ++i) {
+        REAL v = vel[i];
+        v2dt2[i] = v * v * dtw * dtw;
+        refl[i] = (REAL)(2.0 / (double)(v * v * v));
+    }
+}
+
+#define INTERIOR_BEGIN(g)                                              \
+    _Pragma("omp parallel for schedule(static)")                       \
+    for (int ix = FAT; ix < (g)->nx - FAT; ++ix) {                     \
+        for (int iz = FAT; iz < (g)->nz - FAT; ++iz) {                 \
+            long i = (long)ix * (g)->nz + iz;
+#define INTERIOR_END }}
+
+/* stepFwdGpu, F/src/kernelsGpu.cu:513-552.  In place when o == n. */
+void FN(orc_step_fwd)(const orc_grid* g, const REAL* zc, const REAL* xc,
+                      REAL* o, const REAL* c, REAL* n, const REAL* v2dt2)
+{
+    const long nz = g->nz;
+    INTERIOR_BEGIN(g)
+        REAL lap = zc[0] * c[i]
+                 + zc[1] * (c[i - 1] + c[i + 1])
+                 + zc[2] * (c[i - 2] + c[i + 2])
+                 + zc[3] * (c[i - 3] + c[i + 3])
+                 + zc[4] * (c[i - 4] + c[i + 4])
+                 + zc[5] * (c[i - 5] + c[i + 5])
+                 + xc[0] * c[i]
+                 + xc[1] * (c[i + nz] + c[i - nz])
+                 + xc[2] * (c[i + 2 * nz] + c[i - 2 * nz])
+                 + xc[3] * (c[i + 3 * nz] + c[i - 3 * nz])
+                 + xc[4] * (c[i + 4 * nz] + c[i - 4 * nz])
+                 + xc[5] * (c[i + 5 * nz] + c[i - 5 * nz]);
+        n[i] = v2dt2[i] * lap + c[i] + c[i] - o[i];
+    INTERIOR_END
+}
+
+/* stepAdjGpu, F/src/kernelsGpu.cu:574-610: the Laplacian acts on v2dt2*c
+ * (formed on the halo too).  In place when o == n.  tmp holds nz*nx values. */
+void FN(orc_step_adj)(const orc_grid* g, const REAL* zc, const REAL* xc,
+                      REAL* o, const REAL* c, REAL* n, const REAL* v2dt2, REAL* tmp)
+{
+    const long nz = g->nz, nzx = (long)g->nz * g->nx;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < nzx; ++i) tmp[i] = c[i] * v2dt2[i];
+    const REAL* s = tmp;
+    INTERIOR_BEGIN(g)
+        REAL lap = zc[0] * s[i]
+                 + zc[1] * (s[i - 1] + s[i + 1])
+                 + zc[2] * (s[i - 2] + s[i + 2])
+                 + zc[3] * (s[i - 3] + s[i + 3])
+                 + zc[4] * (s[i - 4] + s[i + 4])
+                 + zc[5] * (s[i - 5] + s[i + 5])
+                 + xc[0] * s[i]
+                 + xc[1] * (s[i + nz] + s[i - nz])
+                 + xc[2] * (s[i + 2 * nz] + s[i - 2 * nz])
+                 + xc[3] * (s[i + 3 * nz] + s[i - 3 * nz])
+                 + xc[4] * (s[i + 4 * nz] + s[i - 4 * nz])
+                 + xc[5] * (s[i + 5 * nz] + s[i - 5 * nz]);
+        o[i] = lap + c[i] + c[i] - n[i];
+    INTERIOR_END
+}
+
+/* dampCosineEdge, F/src/kernelsGpu.cu:202-219: both slices are damped. */
+void FN(orc_damp)(const orc_grid* g, const REAL* cd, REAL* p1, REAL* p2)
+{
+    const int nz = g->nz, nx = g->nx, minPad = g->minPad;
+    INTERIOR_BEGIN(g)
+        int a = iz - FAT, b = ix - FAT, c = nz - iz - 1 - FAT, d = nx - ix - 1 - FAT;
+        int m1 = a < b ? a : b, m2 = c < d ? c : d;
+        int dist = m1 < m2 ? m1 : m2;
+        if (dist < minPad) {
+            REAL damp = cd[dist];
+            p1[i] *= damp;
+            p2[i] *= damp;
+        }
+    INTERIOR_END
+}
+
+/* injectSource, F/src/kernelsGpu.cu:88-91. */
+static void FN(inject_source)(int ntw, const REAL* sig, REAL* p, int itw,
+                              const int* pos, int nPos)
+{
+    for (int s = 0; s < nPos; ++s) p[pos[s]] += sig[(long)s * ntw + itw];
+}
+
+/* recordInterpData, F/src/kernelsGpu.cu:119-127. */
+static void FN(record_interp)(int nts, int h, const REAL* f, const REAL* p, REAL* d,
+                              int its, int it2, const int* pos, int nPos)
+{
+    for (int r = 0; r < nPos; ++r) {
+        d[(long)nts * r + its] += p[pos[r]] * f[it2];
+        d[(long)nts * r + its + 1] += p[pos[r]] * f[h + it2];
+    }
+}
+
+/* interpInjectData, F/src/kernelsGpu.cu:94-99. */
+static void FN(interp_inject)(int nts, int h, const REAL* f, const REAL* d, REAL* p,
+                              int its, int it2, const int* pos, int nPos)
+{
+    for (int r = 0; r < nPos; ++r)
+        p[pos[r]] += d[(long)nts * r + its] * f[it2 + 1]
+                   + d[(long)nts * r + its + 1] * f[h + it2 + 1];
+}
+
+/* recordSource, F/src/kernelsGpu.cu:113-116. */
+static void FN(record_source)(int ntw, const REAL* p, REAL* sig, int itw,
+                              const int* pos, int nPos)
+{
+    for (int s = 0; s < nPos; ++s) sig[(long)ntw * s + itw] += p[pos[s]];
+}
+
+/* interpWavefield (F/src/kernelsGpu.cu:132-141) and
+ * extractInterpAdjointWavefield (:166-173): same arithmetic into two slices. */
+static void FN(capture2)(const orc_grid* g, const REAL* f, int h, const REAL* p,
+                         REAL* left, REAL* right, int it2)
+{
+    INTERIOR_BEGIN(g)
+        left[i] += p[i] * f[it2];
+        right[i] += p[i] * f[h + it2];
+    INTERIOR_END
+}
+
+/* injectSecondarySource, F/src/kernelsGpu.cu:102-107. */
+static void FN(inject_secondary)(const orc_grid* g, const REAL* f, int h,
+                                 const REAL* ssL, const REAL* ssR, REAL* p, int k)
+{
+    INTERIOR_BEGIN(g)
+        p[i] += ssL[i] * f[k] + ssR[i] * f[h + k];
+    INTERIOR_END
+}
+
+static void FN(swap)(REAL** a, REAL** b) { REAL* t = *a; *a = *b; *b = t; }
+
+static REAL* FN(zalloc)(long n) { return (REAL*)calloc((size_t)n, sizeof(REAL)); }
+
+/* propShotsFwdGpu, F/src/nonlinearShotsGpuFunctions.cu:227-309.
+ * sig: nSrc x ntw (already scaled and time-interpolated); dreg: nRec x nts. */
+void FN(orc_nl_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* sig,
+                    const int* srcPos, int nSrc, const int* recPos, int nRec, REAL* dreg)
+{
+    REAL zc[6], xc[6], f[2 * 101], cd[512];
+    FN(orc_tables)(g, zc, xc, f, cd);
+    const int nts = g->nts, sub = g->sub, ntw = (nts - 1) * sub + 1, h = sub + 1;
+    const long nzx = (long)g->nz * g->nx;
+    REAL* p0 = FN(zalloc)(nzx);
+    REAL* p1 = FN(zalloc)(nzx);
+    memset(dreg, 0, sizeof(REAL) * (size_t)nRec * nts);
+    for (int its = 0; its < nts - 1; ++its) {
+        for (int it2 = 1; it2 < sub + 1; ++it2) {
+            int itw = its * sub + it2;
+            FN(orc_step_fwd)(g, zc, xc, p0, p1, p0, v2dt2);
+            FN(inject_source)(ntw, sig, p0, itw - 1, srcPos, nSrc);
+            FN(orc_damp)(g, cd, p0, p1);
+            FN(record_interp)(nts, h, f, p0, dreg, its, it2, recPos, nRec);
+            FN(swap)(&p0, &p1);
+        }
+    }
+    free(p0); free(p1);
+}
+
+/* propShotsAdjGpu, F/src/nonlinearShotsGpuFunctions.cu:579-652.
+ * sig (output): nSrc x ntw, overwritten. */
+void FN(orc_nl_adj)(const orc_grid* g, const REAL* v2dt2, REAL* sig,
+                    const int* srcPos, int nSrc, const int* recPos, int nRec,
+                    const REAL* dreg)
+{
+    REAL zc[6], xc[6], f[2 * 101], cd[512];
+    FN(orc_tables)(g, zc, xc, f, cd);
+    const int nts = g->nts, sub = g->sub, ntw = (nts - 1) * sub + 1, h = sub + 1;
+    const long nzx = (long)g->nz * g->nx;
+    REAL* p0 = FN(zalloc)(nzx);
+    REAL* p1 = FN(zalloc)(nzx);
+    REAL* tmp = FN(zalloc)(nzx);
+    memset(sig, 0, sizeof(REAL) * (size_t)nSrc * ntw);
+    for (int its = nts - 2; its > -1; --its) {
+        for (int it2 = sub - 1; it2 > -1; --it2) {
+            int itw = its * sub + it2;
+            FN(orc_step_adj)(g, zc, xc, p0, p1, p0, v2dt2, tmp);
+            FN(interp_inject)(nts, h, f, dreg, p0, its, it2, recPos, nRec);
+            FN(orc_damp)(g, cd, p0, p1);
+            FN(record_source)(ntw, p0, sig, itw, srcPos, nSrc);
+            FN(swap)(&p0, &p1);
+        }
+    }
+    free(p0); free(p1); free(tmp);
+}
+
+/* Source-wavefield pass shared by every Born-type driver, e.g.
+ * F/src/BornShotsGpuFunctions.cu:281-321: W (nts slices) is zeroed, then
+ * step -> injectSource -> damp -> interpWavefield -> swap. */
+static void FN(src_pass)(const orc_grid* g, const REAL* zc, const REAL* xc,
+                         const REAL* f, const REAL* cd, const REAL* v2dt2,
+                         const REAL* sig, const int* srcPos, int nSrc, REAL* W)
+{
+    const int nts = g->nts, sub = g->sub, ntw = (nts - 1) * sub + 1, h = sub + 1;
+    const long nzx = (long)g->nz * g->nx;
+    REAL* p0 = FN(zalloc)(nzx);
+    REAL* p1 = FN(zalloc)(nzx);
+    memset(W, 0, sizeof(REAL) * (size_t)nzx * nts);
+    for (int its = 0; its < nts - 1; ++its) {
+        for (int it2 = 1; it2 < sub + 1; ++it2) {
+            int itw = its * sub + it2;
+            FN(orc_step_fwd)(g, zc, xc, p0, p1, p0, v2dt2);
+            FN(inject_source)(ntw, sig, p0, itw - 1, srcPos, nSrc);
+            FN(orc_damp)(g, cd, p0, p1);
+            FN(capture2)(g, f, h, p0, W + (long)its * nzx, W + (long)(its + 1) * nzx, it2);
+            FN(swap)(&p0, &p1);
+        }
+    }
+    free(p0); free(p1);
+}
+
+/* Scattered pass shared by Born and extended Born forward, e.g.
+ * F/src/BornShotsGpuFunctions.cu:344-376.  make_ss(k, ss) fills the secondary
+ * source for coarse sample k into a zeroed slice. */
+typedef void (*FN(ss_fn))(const orc_grid*, const void*, int, REAL*);
+
+static void FN(scat_pass)(const orc_grid* g, const REAL* zc, const REAL* xc,
+                          const REAL* f, const REAL* cd, const REAL* v2dt2,
+                          const int* recPos, int nRec, REAL* dreg,
+                          FN(ss_fn) make_ss, const void* ctx)
+{
+    const int nts = g->nts, sub = g->sub, h = sub + 1;
+    const long nzx = (long)g->nz * g->nx;
+    REAL* p0 = FN(zalloc)(nzx);
+    REAL* p1 = FN(zalloc)(nzx);
+    REAL* ssL = FN(zalloc)(nzx);
+    REAL* ssR = FN(zalloc)(nzx);
+    memset(dreg, 0, sizeof(REAL) * (size_t)nRec * nts);
+    make_ss(g, ctx, 0, ssL);
+    for (int its = 0; its < nts - 1; ++its) {
+        make_ss(g, ctx, its + 1, ssR);
+        for (int it2 = 1; it2 < sub + 1; ++it2) {
+            FN(orc_step_fwd)(g, zc, xc, p0, p1, p0, v2dt2);
+            FN(inject_secondary)(g, f, h, ssL, ssR, p0, it2 - 1);
+            FN(orc_damp)(g, cd, p0, p1);
+            FN(record_interp)(nts, h, f, p0, dreg, its, it2, recPos, nRec);
+            FN(swap)(&p0, &p1);
+        }
+        FN(swap)(&ssL, &ssR);
+        memset(ssR, 0, sizeof(REAL) * (size_t)nzx);
+    }
+    free(p0); free(p1); free(ssL); free(ssR);
+}
+
+/* Receiver pass shared by Born and extended Born adjoint, e.g.
+ * F/src/BornShotsGpuFunctions.cu:893-942.  image(k, ssR) applies the imaging
+ * condition for coarse sample k. */
+typedef void (*FN(img_fn))(const orc_grid*, void*, int, REAL*);
+
+static void FN(rec_pass)(const orc_grid* g, const REAL* zc, const REAL* xc,
+                         const REAL* f, const REAL* cd, const REAL* v2dt2,
+                         const int* recPos, int nRec, const REAL* dreg,
+                         FN(img_fn) image, void* ctx)
+{
+    const int nts = g->nts, sub = g->sub, h = sub + 1;
+    const long nzx = (long)g->nz * g->nx;
+    REAL* p0 = FN(zalloc)(nzx);
+    REAL* p1 = FN(zalloc)(nzx);
+    REAL* ssL = FN(zalloc)(nzx);
+    REAL* ssR = FN(zalloc)(nzx);
+    REAL* tmp = FN(zalloc)(nzx);
+    for (int its = nts - 2; its > -1; --its) {
+        for (int it2 = sub - 1; it2 > -1; --it2) {
+            FN(orc_step_adj)(g, zc, xc, p0, p1, p0, v2dt2, tmp);
+            FN(interp_inject)(nts, h, f, dreg, p0, its, it2, recPos, nRec);
+            FN(orc_damp)(g, cd, p0, p1);
+            FN(capture2)(g, f, h, p0, ssL, ssR, it2);
+            FN(swap)(&p0, &p1);
+        }
+        image(g, ctx, its + 1, ssR);
+        FN(swap)(&ssR, &ssL);
+        memset(ssL, 0, sizeof(REAL) * (size_t)nzx);
+    }
+    image(g, ctx, 0, ssR);
+    free(p0); free(p1); free(ssL); free(ssR); free(tmp);
+}
+
+typedef struct { const REAL* m; const REAL* W; const REAL* v2dt2; const REAL* refl; } FN(born_ctx);
+
+/* imagingFwdGpu, F/src/kernelsGpu.cu:309-316: ss = m * W[k]. */
+static void FN(born_ss)(const orc_grid* g, const void* vctx, int k, REAL* ss)
+{
+    const FN(born_ctx)* c = (const FN(born_ctx)*)vctx;
+    const long nzx = (long)g->nz * g->nx;
+    const REAL* Wk = c->W + (long)k * nzx;
+    INTERIOR_BEGIN(g)
+        ss[i] = c->m[i] * Wk[i];
+    INTERIOR_END
+}
+
+/* imagingAdjGpu, F/src/kernelsGpu.cu:318-325: m += W[k] * ss. */
+static void FN(born_img)(const orc_grid* g, void* vctx, int k, REAL* ss)
+{
+    FN(born_ctx)* c = (FN(born_ctx)*)vctx;
+    const long nzx = (long)g->nz * g->nx;
+    const REAL* Wk = c->W + (long)k * nzx;
+    REAL* m = (REAL*)c->m;
+    INTERIOR_BEGIN(g)
+        m[i] += Wk[i] * ss[i];
+    INTERIOR_END
+}
+
+/* scaleReflectivity, F/src/kernelsGpu.cu:246-252. */
+static void FN(scale_refl)(const orc_grid* g, REAL* m, const REAL* refl, const REAL* v2dt2)
+{
+    INTERIOR_BEGIN(g)
+        m[i] *= v2dt2[i] * refl[i];
+    INTERIOR_END
+}
+
+/* BornShotsFwdGpu, F/src/BornShotsGpuFunctions.cu:260-388.  model: nz*nx
+ * (read only, copied before scaling); W: nts*nz*nx scratch. */
+void FN(orc_born_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
+                      const REAL* sig, const int* srcPos, int nSrc,
+                      const int* recPos, int nRec, const REAL* model, REAL* dreg, REAL* W)
+{
+    REAL zc[6], xc[6], f[2 * 101], cd[512];
+    FN(orc_tables)(g, zc, xc, f, cd);
+    const long nzx = (long)g->nz * g->nx;
+    FN(src_pass)(g, zc, xc, f, cd, v2dt2, sig, srcPos, nSrc, W);
+    REAL* m = FN(zalloc)(nzx);
+    memcpy(m, model, sizeof(REAL) * (size_t)nzx);
+    FN(scale_refl)(g, m, refl, v2dt2);
+    FN(born_ctx) ctx = {m, W, v2dt2, refl};
+    FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(born_ss), &ctx);
+    free(m);
+}
+
+/* BornShotsAdjGpu, F/src/BornShotsGpuFunctions.cu:829-957.  model (output)
+ * is overwritten with this shot's image. */
+void FN(orc_born_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
+                      const REAL* sig, const int* srcPos, int nSrc,
+                      const int* recPos, int nRec, REAL* model, const REAL* dreg, REAL* W)
+{
+    REAL zc[6], xc[6], f[2 * 101], cd[512];
+    FN(orc_tables)(g, zc, xc, f, cd);
+    const long nzx = (long)g->nz * g->nx;
+    FN(src_pass)(g, zc, xc, f, cd, v2dt2, sig, srcPos, nSrc, W);
+    memset(model, 0, sizeof(REAL) * (size_t)nzx);
+    FN(born_ctx) ctx = {model, W, v2dt2, refl};
+    FN(rec_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(born_img), &ctx);
+    FN(scale_refl)(g, model, refl, v2dt2);
+}
+
+/* imagingOffsetFwdGpu (F/src/kernelsGpu.cu:381-396) followed by
+ * scaleSecondarySourceFd (:270-275). */
+static void FN(ext_ss)(const orc_grid* g, const void* vctx, int k, REAL* ss)
+{
+    const FN(born_ctx)* c = (const FN(born_ctx)*)vctx;
+    const int nz = g->nz, nx = g->nx, hExt = (g->nExt - 1) / 2;
+    const long nzx = (long)nz * nx;
+    INTERIOR_BEGIN(g)
+        int lo = -hExt, t = ix + 1 + FAT + hExt - nx;
+        if (t > lo) lo = t;
+        int hi = ix - hExt - FAT;
+        if (hExt < hi) hi = hExt;
+        for (int e = lo; e < hi + 1; ++e) {
+            long iModel = nzx * (e + hExt) + (long)nz * (ix - e) + iz;
+            long iSrc = nzx * k + (long)(ix - 2 * e) * nz + iz;
+            ss[i] += c->m[iModel] * c->W[iSrc];
+        }
+    INTERIOR_END
+    INTERIOR_BEGIN(g)
+        ss[i] *= c->v2dt2[i];
+    INTERIOR_END
+}
+
+/* scaleSecondarySourceFd (F/src/kernelsGpu.cu:270-275) followed by
+ * imagingOffsetAdjGpu (:398-411). */
+static void FN(ext_img)(const orc_grid* g, void* vctx, int k, REAL* ss)
+{
+    FN(born_ctx)* c = (FN(born_ctx)*)vctx;
+    const int nz = g->nz, nx = g->nx, nExt = g->nExt, hExt = (nExt - 1) / 2;
+    const long nzx = (long)nz * nx;
+    REAL* m = (REAL*)c->m;
+    INTERIOR_BEGIN(g)
+        ss[i] *= c->v2dt2[i];
+    INTERIOR_END
+#pragma omp parallel for schedule(static)
+    for (int ix = FAT + hExt; ix < nx - FAT - hExt; ++ix)
+        for (int e = 0; e < nExt; ++e)
+            for (int iz = FAT; iz < nz - FAT; ++iz) {
+                int sh = e - hExt;
+                long iModel = nzx * e + (long)nz * ix + iz;
+                long iSrc = nzx * k + (long)nz * (ix - sh) + iz;
+                long iRec = (long)nz * (ix + sh) + iz;
+                m[iModel] += ss[iRec] * c->W[iSrc];
+            }
+}
+
+/* scaleReflectivityLinExt, F/src/kernelsGpu.cu:256-267. */
+static void FN(scale_refl_ext)(const orc_grid* g, REAL* m, const REAL* refl)
+{
+    const long nzx = (long)g->nz * g->nx;
+    for (int e = 0; e < g->nExt; ++e) {
+        REAL* me = m + nzx * e;
+        INTERIOR_BEGIN(g)
+            me[i] *= refl[i];
+        INTERIOR_END
+    }
+}
+
+/* BornOffsetShotsFwdGpu, F/src/BornExtShotsGpuFunctions.cu:582-724.
+ * model: nExt*nz*nx (read only). */
+void FN(orc_bornext_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
+                         const REAL* sig, const int* srcPos, int nSrc,
+                         const int* recPos, int nRec, const REAL* model, REAL* dreg, REAL* W)
+{
+    REAL zc[6], xc[6], f[2 * 101], cd[512];
+    FN(orc_tables)(g, zc, xc, f, cd);
+    const long nzx = (long)g->nz * g->nx, nm = nzx * g->nExt;
+    FN(src_pass)(g, zc, xc, f, cd, v2dt2, sig, srcPos, nSrc, W);
+    REAL* m = FN(zalloc)(nm);
+    memcpy(m, model, sizeof(REAL) * (size_t)nm);
+    FN(scale_refl_ext)(g, m, refl);
+    FN(born_ctx) ctx = {m, W, v2dt2, refl};
+    FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(ext_ss), &ctx);
+    free(m);
+}
+
+/* BornOffsetShotsAdjGpu, F/src/BornExtShotsGpuFunctions.cu:1848-1989.
+ * model (output, nExt*nz*nx) is overwritten with this shot's image. */
+void FN(orc_bornext_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
+                         const REAL* sig, const int* srcPos, int nSrc,
+                         const int* recPos, int nRec, REAL* model, const REAL* dreg, REAL* W)
+{
+    REAL zc[6], xc[6], f[2 * 101], cd[512];
+    FN(orc_tables)(g, zc, xc, f, cd);
+    const long nzx = (long)g->nz * g->nx;
+    FN(src_pass)(g, zc, xc, f, cd, v2dt2, sig, srcPos, nSrc, W);
+    memset(model, 0, sizeof(REAL) * (size_t)nzx * g->nExt);
+    FN(born_ctx) ctx = {model, W, v2dt2, refl};
+    FN(rec_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(ext_img), &ctx);
+    FN(scale_refl_ext)(g, model, refl);
+}
+
+/* interpTimeLinTbb::forward, F/src/interpTimeLinTbb.cpp:9-62.
+ * m: nDev x nts (coarse), d: nDev x ntw (fine). */
+void FN(orc_interp_fwd)(int nts, double dtsIn, double otsIn, int sub, int nDev,
+                        int add, const REAL* m, REAL* d)
+{
+    const int ntw = (nts - 1) * sub + 1;
+    const REAL dts = (REAL)dtsIn, ots = (REAL)otsIn, dtw = dts / sub, otw = ots;
+    REAL scale = (REAL)ntw / (REAL)nts;
+    scale = (REAL)(1.0 / sqrt((double)scale));
+    if (!add) memset(d, 0, sizeof(REAL) * (size_t)nDev * ntw);
+    for (int dv = 0; dv < nDev; ++dv) {
+        const REAL* md = m + (long)dv * nts;
+        REAL* dd = d + (long)dv * ntw;
+        for (int itw = 0; itw < ntw - 1; ++itw) {
+            REAL tw = otw + itw * dtw;
+            REAL w = (tw - ots) / dts;
+            int k = (int)w;
+            w = w - k;
+            double v = ((double)md[k] * (1.0 - (double)w) + (double)(md[k + 1] * w)) * (double)scale;
+            dd[itw] = (REAL)((double)dd[itw] + v);
+        }
+        dd[ntw - 1] += scale * md[nts - 1];
+    }
+}
+
+/* interpTimeLinTbb::adjoint, F/src/interpTimeLinTbb.cpp:64-102. */
+void FN(orc_interp_adj)(int nts, double dtsIn, double otsIn, int sub, int nDev,
+                        int add, REAL* m, const REAL* d)
+{
+    const int ntw = (nts - 1) * sub + 1;
+    const REAL dts = (REAL)dtsIn, ots = (REAL)otsIn, dtw = dts / sub, otw = ots;
+    REAL scale = (REAL)ntw / (REAL)nts;
+    scale = (REAL)(1.0 / sqrt((double)scale));
+    if (!add) memset(m, 0, sizeof(REAL) * (size_t)nDev * nts);
+    for (int dv = 0; dv < nDev; ++dv) {
+        REAL* md = m + (long)dv * nts;
+        const REAL* dd = d + (long)dv * ntw;
+        for (int itw = 0; itw < ntw - 1; ++itw) {
+            REAL tw = otw + itw * dtw;
+            REAL w = (tw - ots) / dts;
+            int k = (int)w;
+            w = w - k;
+            md[k] = (REAL)((double)md[k] + (double)(scale * dd[itw]) * (1.0 - (double)w));
+            md[k + 1] += scale * dd[itw] * w;
+        }
+        md[nts - 1] += scale * dd[ntw - 1];
+    }
+}
+
+/* secondTimeDerivative::forward, F/src/secondTimeDerivative.cpp:9-28. */
+void FN(orc_d2t_fwd)(int nt, double dt, int nDev, int add, const REAL* m, REAL* d)
+{
+    REAL dtf = (REAL)dt;
+    REAL dt2 = (REAL)(1.0 / (double)(dtf * dtf));
+    if (!add) memset(d, 0, sizeof(REAL) * (size_t)nDev * nt);
+    for (int dv = 0; dv < nDev; ++dv) {
+        const REAL* x = m + (long)dv * nt;
+        REAL* y = d + (long)dv * nt;
+        y[0] = (REAL)((double)y[0] + (double)dt2 * ((double)x[1] - 2.0 * (double)x[0]));
+        for (int it = 1; it < nt - 1; ++it)
+            y[it] = (REAL)((double)y[it] + (double)dt2 * ((double)x[it + 1] - 2.0 * (double)x[it] + (double)x[it - 1]));
+        y[nt - 1] = (REAL)((double)y[nt - 1] + (double)dt2 * ((double)x[nt - 2] - 2.0 * (double)x[nt - 1]));
+    }
+}
+
+/* secondTimeDerivative::adjoint, F/src/secondTimeDerivative.cpp:30-38
+ * (the operator is symmetric). */
+void FN(orc_d2t_adj)(int nt, double dt, int nDev, int add, REAL* m, const REAL* d)
+{
+    FN(orc_d2t_fwd)(nt, dt, nDev, add, d, m);
+}
+
+/* seismicOperator2D::scaleSeismicSource, F/include/seismicOperator2D.cpp:73-86:
+ * sig[p][it] *= dtw*dtw*v[pos]*v[pos]. */
+void FN(orc_scale_source)(const orc_grid* g, const REAL* vel, const int* pos, int nPos,
+                          int nt, REAL* sig)
+{
+    REAL dtw = (REAL)g->dts / (REAL)g->sub;
+    for (int p = 0; p < nPos; ++p) {
+        REAL scale = dtw * dtw * vel[pos[p]] * vel[pos[p]];
+        for (int it = 0; it < nt; ++it) sig[(long)p * nt + it] = sig[(long)p * nt + it] * scale;
+    }
+}
+
+/* deviceGpu constructor #1 (irregular coordinates, bilinear corners),
+ * F/src/deviceGpu.cpp:8-84; bounds check :263-274.  Returns 0 or -1 when a
+ * device is out of bounds. */
+int FN(orc_dev_coords)(int nDev, const REAL* zCoord, const REAL* xCoord,
+                       int nzVel, int nxVel, double ozIn, double dzIn, double oxIn,
+                       double dxIn, int dipole, double zShiftIn, double xShiftIn,
+                       int* gridIdx, REAL* weight)
+{
+    const REAL oz = (REAL)ozIn, dz = (REAL)dzIn, ox = (REAL)oxIn, dx = (REAL)dxIn;
+    const REAL zShift = (REAL)zShiftIn, xShift = (REAL)xShiftIn;
+    const int nb = dipole == 1 ? 8 : 4;
+    REAL zMax = oz + (nzVel - 1) * dz, xMax = ox + (nxVel - 1) * dx;
+    for (int i = 0; i < nDev; ++i)
+        if (zCoord[i] >= zMax || xCoord[i] >= xMax) return -1;
+    for (int i = 0; i < nDev; ++i) {
+        int i1 = i * nb;
+        REAL wz = (zCoord[i] - oz) / dz, wx = (xCoord[i] - ox) / dx;
+        int zReg = (int)wz;
+        wz = wz - zReg;
+        wz = (REAL)(1.0 - (double)wz);
+        int xReg = (int)wx;
+        wx = wx - xReg;
+        wx = (REAL)(1.0 - (double)wx);
+        gridIdx[i1] = xReg * nzVel + zReg;
+        weight[i1] = wz * wx;
+        gridIdx[i1 + 1] = gridIdx[i1] + 1;
+        weight[i1 + 1] = (REAL)((1.0 - (double)wz) * (double)wx);
+        gridIdx[i1 + 2] = gridIdx[i1] + nzVel;
+        weight[i1 + 2] = (REAL)((double)wz * (1.0 - (double)wx));
+        gridIdx[i1 + 3] = gridIdx[i1] + nzVel + 1;
+        weight[i1 + 3] = (REAL)((1.0 - (double)wz) * (1.0 - (double)wx));
+        if (dipole == 1) {
+            REAL wzD = (zCoord[i] + zShift - oz) / dz, wxD = (xCoord[i] + xShift - ox) / dx;
+            int zRD = (int)wzD;
+            wzD = wzD - zRD;
+            wzD = (REAL)(1.0 - (double)wzD);
+            int xRD = (int)wxD;
+            wxD = wxD - xRD;
+            wxD = (REAL)(1.0 - (double)wxD);
+            gridIdx[i1 + 4] = xRD * nzVel + zRD;
+            weight[i1 + 4] = (REAL)((-1.0) * (double)wzD * (double)wxD);
+            gridIdx[i1 + 5] = gridIdx[i1 + 4] + 1;
+            weight[i1 + 5] = (REAL)((-1.0) * (1.0 - (double)wzD) * (double)wxD);
+            gridIdx[i1 + 6] = gridIdx[i1 + 4] + nzVel;
+            weight[i1 + 6] = (REAL)((-1.0) * (double)wzD * (1.0 - (double)wxD));
+            gridIdx[i1 + 7] = gridIdx[i1 + 4] + nzVel + 1;
+            weight[i1 + 7] = (REAL)((-1.0) * (1.0 - (double)wzD) * (1.0 - (double)wxD));
+        }
+    }
+    return 0;
+}
+
+/* deviceGpu::forward, F/src/deviceGpu.cpp:226-243 (regular -> irregular).
+ * cornerReg[i*nbCorner+c] is the unique-point index of that corner. */
+void FN(orc_dev_fwd)(int nDev, int nbCorner, const int* cornerReg, const REAL* weight,
+                     int nt, int add, const REAL* mReg, REAL* dIrreg)
+{
+    if (!add) memset(dIrreg, 0, sizeof(REAL) * (size_t)nDev * nt);
+    for (int i = 0; i < nDev; ++i)
+        for (int c = 0; c < nbCorner; ++c) {
+            int i1 = i * nbCorner + c, i2 = cornerReg[i1];
+            for (int it = 0; it < nt; ++it)
+                dIrreg[(long)i * nt + it] += weight[i1] * mReg[(long)i2 * nt + it];
+        }
+}
+
+/* deviceGpu::adjoint, F/src/deviceGpu.cpp:245-261 (irregular -> regular). */
+void FN(orc_dev_adj)(int nDev, int nbCorner, const int* cornerReg, const REAL* weight,
+                     int nt, int add, REAL* mReg, const REAL* dIrreg)
+{
+    int nReg = 0;
+    for (int k = 0; k < nDev * nbCorner; ++k)
+        if (cornerReg[k] + 1 > nReg) nReg = cornerReg[k] + 1;
+    if (!add) memset(mReg, 0, sizeof(REAL) * (size_t)nReg * nt);
+    for (int i = 0; i < nDev; ++i)
+        for (int c = 0; c < nbCorner; ++c) {
+            int i1 = i * nbCorner + c, i2 = cornerReg[i1];
+            for (int it = 0; it < nt; ++it)
+                mReg[(long)i2 * nt + it] += weight[i1] * dIrreg[(long)i * nt + it];
+        }
+}
+
+#undef FAT
+#undef INTERIOR_BEGIN
+#undef INTERIOR_END
+#undef FN
+#undef CAT
+#undef CAT2

@@ -1,0 +1,164 @@
+"""GPU parity: the HIP engine (through the reference-named pybind11 modules and
+the C-ABI) against the CPU oracle on the same seeded inputs.
+
+Bar: relative L2 <= 1e-5 against the float oracle on data and images (the
+tolerance the reference's own dot test uses, F/include/operator.h:50); the
+error against the double oracle is printed per result.  Integer geometry is
+compared bit-exactly.
+"""
+import numpy as np
+import pytest
+
+from cases import make_case, rel_l2
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def mods():
+    import pyAcoustic_iso_float_born as born
+    import pyAcoustic_iso_float_born_ext as bext
+    import pyAcoustic_iso_float_nl as nl
+    return nl, born, bext
+
+
+def _report(name, gpu, of, od):
+    e_f, e_d = rel_l2(gpu, of), rel_l2(gpu, od)
+    print(f"{name}: relL2 vs float oracle {e_f:.3e}, vs FP64 oracle {e_d:.3e}, float oracle vs FP64 {rel_l2(of, od):.3e}")
+    return e_f
+
+
+def _nl_model(c):
+    m = np.zeros((1, 1, c.nts), np.float32)
+    m[0, 0] = c.wavelet
+    return m
+
+
+@pytest.mark.parametrize("rec_kind,per_shot", [("grid", False), ("coords", True), ("ints", True)])
+def test_nl_forward_adjoint(mods, rec_kind, per_shot):
+    nl, _, _ = mods
+    c = make_case(seed=1, nShot=3, rec_kind=rec_kind, per_shot_rec=per_shot, src_kind="coords" if per_shot else "grid")
+    g = c.grid
+    os_, or_ = c.oracle_sources(), c.oracle_receivers()
+    op = nl.nonlinearPropShotsGpu(c.vel_vector(), c.par, c.product_sources(nl), c.product_receivers(nl))
+    m = _nl_model(c)
+    d = np.zeros((c.nShot, or_[0].nDev, c.nts), np.float32)
+    op.forward(False, m, d)
+    of = O.nl_forward(g, c.vel, os_, or_, m, "f")
+    od = O.nl_forward(g, c.vel, os_, or_, m, "d")
+    assert np.abs(of).max() > 0
+    assert _report("NL fwd", d, of, od) <= TOL
+    rng = np.random.default_rng(7)
+    dr = rng.standard_normal(d.shape).astype(np.float32)
+    ma = np.zeros_like(m)
+    op.adjoint(False, ma, dr)
+    af = O.nl_adjoint(g, c.vel, os_, or_, dr, 1, "f")
+    ad = O.nl_adjoint(g, c.vel, os_, or_, dr, 1, "d")
+    assert _report("NL adj", ma, af, ad) <= TOL
+
+
+def test_born_forward_adjoint(mods):
+    nl, born, _ = mods
+    c = make_case(seed=2, nShot=3)
+    g = c.grid
+    os_, or_ = c.oracle_sources(), c.oracle_receivers()
+    wav = c.wavelet[None, :].copy()
+    op = born.BornShotsGpu(c.vel_vector(), c.par, c.product_sources(nl), [wav], c.product_receivers(nl))
+    rng = np.random.default_rng(3)
+    m = rng.standard_normal((c.nx, c.nz)).astype(np.float32)
+    d = np.zeros((c.nShot, or_[0].nDev, c.nts), np.float32)
+    op.forward(False, m, d)
+    of = O.born_forward(g, c.vel, os_, [wav], or_, m, "f")
+    od = O.born_forward(g, c.vel, os_, [wav], or_, m, "d")
+    assert np.abs(of).max() > 0
+    assert _report("Born fwd", d, of, od) <= TOL
+    dr = rng.standard_normal(d.shape).astype(np.float32)
+    ma = np.zeros_like(m)
+    op.adjoint(False, ma, dr)
+    af = O.born_adjoint(g, c.vel, os_, [wav], or_, dr, "f")
+    ad = O.born_adjoint(g, c.vel, os_, [wav], or_, dr, "d")
+    assert _report("Born adj", ma, af, ad) <= TOL
+
+
+def test_bornext_forward_adjoint(mods):
+    nl, _, bext = mods
+    c = make_case(seed=4, nShot=2, nExt=5)
+    g = c.grid
+    os_, or_ = c.oracle_sources(), c.oracle_receivers()
+    wav = c.wavelet[None, :].copy()
+    op = bext.BornExtShotsGpu(c.vel_vector(), c.par, c.product_sources(nl), [wav], c.product_receivers(nl))
+    rng = np.random.default_rng(5)
+    m = rng.standard_normal((c.nExt, c.nx, c.nz)).astype(np.float32)
+    d = np.zeros((c.nShot, or_[0].nDev, c.nts), np.float32)
+    op.forward(False, m, d)
+    of = O.bornext_forward(g, c.vel, os_, [wav], or_, m, "f")
+    od = O.bornext_forward(g, c.vel, os_, [wav], or_, m, "d")
+    assert _report("BornExt fwd", d, of, od) <= TOL
+    dr = rng.standard_normal(d.shape).astype(np.float32)
+    ma = np.zeros_like(m)
+    op.adjoint(False, ma, dr)
+    af = O.bornext_adjoint(g, c.vel, os_, [wav], or_, dr, "f")
+    ad = O.bornext_adjoint(g, c.vel, os_, [wav], or_, dr, "d")
+    assert _report("BornExt adj", ma, af, ad) <= TOL
+
+
+def test_ring_source_and_receivers(mods):
+    """A source inside the FAT ring and receivers on the last interior row:
+    the reference injects/records there too (F/src/kernelsGpu.cu:88-127)."""
+    nl, _, _ = mods
+    c = make_case(seed=6, nShot=1, src_kind="ints", src_iz=3)
+    c.rec_args = [("grid", (1, c.nz - 6, 1, 20, 10, 2))]
+    g = c.grid
+    os_, or_ = c.oracle_sources(), c.oracle_receivers()
+    op = nl.nonlinearPropShotsGpu(c.vel_vector(), c.par, c.product_sources(nl), c.product_receivers(nl))
+    m = _nl_model(c)
+    d = np.zeros((1, or_[0].nDev, c.nts), np.float32)
+    op.forward(False, m, d)
+    of = O.nl_forward(g, c.vel, os_, or_, m, "f")
+    assert np.abs(of).max() > 0
+    assert rel_l2(d, of) <= TOL
+    dr = np.random.default_rng(8).standard_normal(d.shape).astype(np.float32)
+    ma = np.zeros_like(m)
+    op.adjoint(False, ma, dr)
+    af = O.nl_adjoint(g, c.vel, os_, or_, dr, 1, "f")
+    assert rel_l2(ma, af) <= TOL
+
+
+def test_add_semantics(mods):
+    nl, born, _ = mods
+    c = make_case(seed=9, nShot=2)
+    wav = c.wavelet[None, :].copy()
+    op = born.BornShotsGpu(c.vel_vector(), c.par, c.product_sources(nl), [wav], c.product_receivers(nl))
+    rng = np.random.default_rng(10)
+    m = rng.standard_normal((c.nx, c.nz)).astype(np.float32)
+    nrec = c.oracle_receivers()[0].nDev
+    d0 = rng.standard_normal((c.nShot, nrec, c.nts)).astype(np.float32)
+    d = d0.copy()
+    op.forward(True, m, d)
+    dz = np.zeros_like(d0)
+    op.forward(False, m, dz)
+    np.testing.assert_array_equal(d, d0 + dz)
+    m0 = rng.standard_normal(m.shape).astype(np.float32)
+    ma = m0.copy()
+    op.adjoint(True, ma, d0)
+    mz = np.zeros_like(m0)
+    op.adjoint(False, mz, d0)
+    np.testing.assert_array_equal(ma, m0 + mz)
+
+
+@pytest.mark.parametrize("name", ["nl", "born", "bornext"])
+def test_dot_product(mods, name):
+    """Operator::dotTest at the reference tolerance 1e-5 (F/include/operator.h:50-88)."""
+    nl, born, bext = mods
+    c = make_case(seed=11, nShot=2, nExt=3 if name == "bornext" else 1)
+    wav = c.wavelet[None, :].copy()
+    if name == "nl":
+        op = nl.nonlinearPropShotsGpu(c.vel_vector(), c.par, c.product_sources(nl), c.product_receivers(nl))
+    elif name == "born":
+        op = born.BornShotsGpu(c.vel_vector(), c.par, c.product_sources(nl), [wav], c.product_receivers(nl))
+    else:
+        op = bext.BornExtShotsGpu(c.vel_vector(), c.par, c.product_sources(nl), [wav], c.product_receivers(nl))
+    assert op.dotTest(True, 1e-5)
