@@ -1,0 +1,254 @@
+"""ctypes binding of the C-ABI in include/aiso.h (libaiso.so).
+
+Used where buffers already live on the device (torch tensors, the serving
+loop, bench.py): pointers are passed with ``mem=MEM_DEVICE``.  Raises
+ImportError if the library has not been built, and RuntimeError carrying
+``aiso_last_error()`` on any failed call.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib_path
+
+MEM_HOST, MEM_DEVICE = 0, 1
+EXT_NONE, EXT_OFFSET = 0, 1
+K_ALL, K_NL_FWD, K_NL_ADJ, K_SRC, K_BORN_SCAT, K_BORN_REC, K_EXT_SCAT, K_EXT_REC = range(-1, 7)
+
+
+class FdParams(C.Structure):
+    _fields_ = [("nz", C.c_int), ("nx", C.c_int), ("dz", C.c_float), ("dx", C.c_float),
+                ("oz", C.c_float), ("ox", C.c_float), ("nts", C.c_int), ("dts", C.c_float),
+                ("ots", C.c_float), ("sub", C.c_int), ("zPadMinus", C.c_int), ("zPadPlus", C.c_int),
+                ("xPadMinus", C.c_int), ("xPadPlus", C.c_int), ("fat", C.c_int), ("blockSize", C.c_int),
+                ("alphaCos", C.c_float), ("fMax", C.c_float), ("freeSurface", C.c_int),
+                ("nExt", C.c_int), ("extension", C.c_int)]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_F = C.c_float
+_PP = C.POINTER(C.c_void_p)
+
+_SIGS = {
+    "aiso_last_error": (C.c_char_p, []),
+    "aiso_ctx_create": (_I, [C.POINTER(FdParams), _I, _PP]),
+    "aiso_ctx_destroy": (_I, [_P]),
+    "aiso_ctx_set_velocity": (_I, [_P, _P, _I]),
+    "aiso_ctx_set_batch": (_I, [_P, _I]),
+    "aiso_ctx_set_tile": (_I, [_P, _I, _I]),
+    "aiso_ctx_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(C.c_size_t)]),
+    "aiso_ctx_stream": (_I, [_P, _PP]),
+    "aiso_ctx_profile": (_I, [_P, _I]),
+    "aiso_ctx_profile_read": (_I, [_P, _I, C.POINTER(C.c_longlong), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "aiso_ctx_batch_read": (_I, [_P, _P, _P, _I, C.POINTER(_I)]),
+    "aiso_device_create_coords": (_I, [_I, _P, _P, _I, _I, _F, _F, _F, _F, _I, _I, _F, _F, _PP]),
+    "aiso_device_create_ints": (_I, [_I, _P, _P, _I, _I, _I, _I, _I, _I, _PP]),
+    "aiso_device_create_grid": (_I, [_I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _PP]),
+    "aiso_device_destroy": (_I, [_P]),
+    "aiso_device_sizes": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    "aiso_device_reg_pos": (_I, [_P, _P]),
+    "aiso_device_corners": (_I, [_P, _P, _P]),
+    "aiso_device_forward": (_I, [_P, _I, _P, _P, _I]),
+    "aiso_device_adjoint": (_I, [_P, _I, _P, _P, _I]),
+    "aiso_nl_forward": (_I, [_P, _I, _P, _I, _P, _PP, _I, _PP, _I, _I]),
+    "aiso_nl_adjoint": (_I, [_P, _I, _P, _I, _P, _PP, _I, _PP, _I, _I]),
+    "aiso_born_forward": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
+    "aiso_born_adjoint": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
+    "aiso_bornext_forward": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
+    "aiso_bornext_adjoint": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
+    "aiso_pad_plus": (_I, [_I, _I, _I]),
+}
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        L = C.CDLL(lib_path())
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def _check(st):
+    if st != 0:
+        raise RuntimeError(lib().aiso_last_error().decode(errors="replace"))
+
+
+def _ptr(x):
+    """Raw pointer of a numpy array or a torch tensor (host or device)."""
+    if x is None:
+        return None
+    if hasattr(x, "data_ptr"):
+        return C.c_void_p(x.data_ptr())
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data_as(C.c_void_p)
+    return C.c_void_p(int(x))
+
+
+def pad_plus(n, pad, block=16):
+    return lib().aiso_pad_plus(n, pad, block)
+
+
+class Device:
+    """deviceGpu handle (aiso_device*)."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def grid(cls, nzDev, ozDev, dzDev, nxDev, oxDev, dxDev, nzVel, nxVel, nt, dipole=0, zShift=0, xShift=0):
+        h = C.c_void_p()
+        _check(lib().aiso_device_create_grid(nzDev, ozDev, dzDev, nxDev, oxDev, dxDev, nzVel, nxVel, nt,
+                                             dipole, zShift, xShift, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def ints(cls, z, x, nzVel, nxVel, nt, dipole=0, zShift=0, xShift=0):
+        z = np.ascontiguousarray(z, np.int32)
+        x = np.ascontiguousarray(x, np.int32)
+        h = C.c_void_p()
+        _check(lib().aiso_device_create_ints(len(z), _ptr(z), _ptr(x), nzVel, nxVel, nt, dipole, zShift, xShift,
+                                             C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def coords(cls, z, x, nzVel, nxVel, oz, dz, ox, dx, nt, dipole=0, zShift=0.0, xShift=0.0):
+        z = np.ascontiguousarray(z, np.float32)
+        x = np.ascontiguousarray(x, np.float32)
+        h = C.c_void_p()
+        _check(lib().aiso_device_create_coords(len(z), _ptr(z), _ptr(x), nzVel, nxVel, oz, dz, ox, dx, nt, dipole,
+                                               zShift, xShift, C.byref(h)))
+        return cls(h)
+
+    def sizes(self):
+        a, b, c = _I(), _I(), _I()
+        _check(lib().aiso_device_sizes(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    @property
+    def nDev(self):
+        return self.sizes()[0]
+
+    @property
+    def nReg(self):
+        return self.sizes()[1]
+
+    def reg_pos(self):
+        out = np.zeros(self.nReg, np.int32)
+        _check(lib().aiso_device_reg_pos(self.h, _ptr(out)))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.aiso_device_destroy(self.h)
+            self.h = None
+
+
+def _devarray(devs):
+    arr = (C.c_void_p * len(devs))(*[d.h for d in devs])
+    return C.cast(arr, _PP), arr
+
+
+class Context:
+    """aiso_ctx: one padded grid + velocity resident on one HIP device."""
+
+    def __init__(self, params: dict, device=0):
+        fp = FdParams()
+        defaults = dict(oz=0.0, ox=0.0, ots=0.0, fat=5, blockSize=16, alphaCos=0.99, fMax=1000.0,
+                        freeSurface=0, nExt=1, extension=EXT_NONE)
+        merged = {**defaults, **params}
+        for k, _ in FdParams._fields_:
+            setattr(fp, k, merged[k])
+        self.params = merged
+        self.h = C.c_void_p()
+        _check(lib().aiso_ctx_create(C.byref(fp), device, C.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().aiso_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        if _LIB is not None:
+            self.close()
+
+    def set_velocity(self, vel, mem=MEM_HOST):
+        if mem == MEM_HOST:
+            vel = np.ascontiguousarray(vel, np.float32)
+        _check(lib().aiso_ctx_set_velocity(self.h, _ptr(vel), mem))
+
+    def set_batch(self, n):
+        _check(lib().aiso_ctx_set_batch(self.h, n))
+
+    def set_tile(self, tz, tx):
+        _check(lib().aiso_ctx_set_tile(self.h, tz, tx))
+
+    def info(self):
+        a, b, c = _I(), _I(), C.c_size_t()
+        _check(lib().aiso_ctx_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(ntw=a.value, batch=b.value, bytes_per_shot=c.value)
+
+    def stream(self):
+        s = C.c_void_p()
+        _check(lib().aiso_ctx_stream(self.h, C.byref(s)))
+        return s.value
+
+    def profile(self, enable=True):
+        _check(lib().aiso_ctx_profile(self.h, int(enable)))
+
+    def profile_read(self, which=K_ALL):
+        n, ms, pts = C.c_longlong(), C.c_double(), C.c_double()
+        _check(lib().aiso_ctx_profile_read(self.h, which, C.byref(n), C.byref(ms), C.byref(pts)))
+        return dict(launches=n.value, ms=ms.value, point_updates=pts.value)
+
+    def batch_read(self):
+        n = _I()
+        _check(lib().aiso_ctx_batch_read(self.h, None, None, 0, C.byref(n)))
+        ms = np.zeros(max(n.value, 1), np.float32)
+        sh = np.zeros(max(n.value, 1), np.int32)
+        _check(lib().aiso_ctx_batch_read(self.h, _ptr(ms), _ptr(sh), n.value, C.byref(n)))
+        return ms[:n.value], sh[:n.value]
+
+    # operators ------------------------------------------------------------
+    def nl_forward(self, model, data, sources, receivers, add=False, mem=MEM_HOST):
+        s, _ks = _devarray(sources)
+        r, _kr = _devarray(receivers)
+        _check(lib().aiso_nl_forward(self.h, int(add), _ptr(model), model.shape[0], _ptr(data), s, len(sources),
+                                     r, len(receivers), mem))
+
+    def nl_adjoint(self, model, data, sources, receivers, add=False, mem=MEM_HOST):
+        s, _ks = _devarray(sources)
+        r, _kr = _devarray(receivers)
+        _check(lib().aiso_nl_adjoint(self.h, int(add), _ptr(model), model.shape[0], _ptr(data), s, len(sources),
+                                     r, len(receivers), mem))
+
+    def _born(self, fn, model, data, sources, wavelets, receivers, add, mem):
+        s, _ks = _devarray(sources)
+        r, _kr = _devarray(receivers)
+        w = np.ascontiguousarray(wavelets, np.float32)
+        nwav = w.shape[0] if w.ndim == 3 else 1
+        _check(fn(self.h, int(add), _ptr(model), _ptr(data), s, len(sources), _ptr(w), nwav, r, len(receivers), mem))
+
+    def born_forward(self, model, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
+        self._born(lib().aiso_born_forward, model, data, sources, wavelets, receivers, add, mem)
+
+    def born_adjoint(self, model, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
+        self._born(lib().aiso_born_adjoint, model, data, sources, wavelets, receivers, add, mem)
+
+    def bornext_forward(self, model, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
+        self._born(lib().aiso_bornext_forward, model, data, sources, wavelets, receivers, add, mem)
+
+    def bornext_adjoint(self, model, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
+        self._born(lib().aiso_bornext_adjoint, model, data, sources, wavelets, receivers, add, mem)

@@ -101,10 +101,14 @@ struct aiso_ctx {
     std::map<std::tuple<long, int, int>, std::unique_ptr<DevGeom>> geoms;
     // profiling of the step kernels
     bool profile = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-    std::vector<double> evPts;
-    long long launches = 0;
-    double ms = 0, pts = 0;
+    struct Timed { hipEvent_t e0, e1; int kind; double pts; };
+    std::vector<Timed> ev;
+    struct Batch { hipEvent_t e0, e1; int shots; };
+    std::vector<Batch> batchEv;
+    std::vector<std::pair<float, int>> batchDone;
+    std::vector<hipEvent_t> evPool;
+    long long launches[AISO_K_COUNT] = {0};
+    double ms[AISO_K_COUNT] = {0}, pts[AISO_K_COUNT] = {0};
 };
 
 namespace {
@@ -177,37 +181,55 @@ struct Request {
     int mem;
 };
 
-void launch_timed(aiso_ctx* c, unsigned mode, const StepArgs& a, int nshots)
+hipEvent_t pooled_event(aiso_ctx* c)
 {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!c->evPool.empty()) {
+        hipEvent_t e = c->evPool.back();
+        c->evPool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    HIPC(hipEventCreate(&e));
+    return e;
+}
+
+void launch_timed(aiso_ctx* c, unsigned mode, int kind, const StepArgs& a, int nshots)
+{
+    hipEvent_t e0 = nullptr;
     if (c->profile) {
-        HIPC(hipEventCreate(&e0));
-        HIPC(hipEventCreate(&e1));
+        e0 = pooled_event(c);
         HIPC(hipEventRecord(e0, c->stream));
     }
     HIPC(launch_step(mode, c->tz, c->tx, a, c->stream));
     if (c->profile) {
+        hipEvent_t e1 = pooled_event(c);
         HIPC(hipEventRecord(e1, c->stream));
-        c->ev.emplace_back(e0, e1);
-        c->evPts.push_back(double(c->p.interior()) * nshots);
+        c->ev.push_back({e0, e1, kind, double(c->p.interior()) * nshots});
     }
 }
 
 void drain_profile(aiso_ctx* c)
 {
-    if (c->ev.empty()) return;
+    if (c->ev.empty() && c->batchEv.empty()) return;
     HIPC(hipStreamSynchronize(c->stream));
-    for (size_t i = 0; i < c->ev.size(); ++i) {
-        float t = 0.f;
-        HIPC(hipEventElapsedTime(&t, c->ev[i].first, c->ev[i].second));
-        c->ms += t;
-        c->pts += c->evPts[i];
-        c->launches++;
-        (void)hipEventDestroy(c->ev[i].first);
-        (void)hipEventDestroy(c->ev[i].second);
+    for (auto& t : c->ev) {
+        float ms = 0.f;
+        HIPC(hipEventElapsedTime(&ms, t.e0, t.e1));
+        c->ms[t.kind] += ms;
+        c->pts[t.kind] += t.pts;
+        c->launches[t.kind]++;
+        c->evPool.push_back(t.e0);
+        c->evPool.push_back(t.e1);
     }
     c->ev.clear();
-    c->evPts.clear();
+    for (auto& b : c->batchEv) {
+        float ms = 0.f;
+        HIPC(hipEventElapsedTime(&ms, b.e0, b.e1));
+        c->batchDone.emplace_back(ms, b.shots);
+        c->evPool.push_back(b.e0);
+        c->evPool.push_back(b.e1);
+    }
+    c->batchEv.clear();
 }
 
 // Host signal chain for one Born shot: S = I_t(D_v(D2_t(R_src^T w)))
@@ -379,6 +401,11 @@ void run(aiso_ctx* c, const Request& q)
     std::vector<float> sigHost;
     for (int b0 = 0; b0 < q.nShot; b0 += B) {
         const int nb = std::min(B, q.nShot - b0);
+        hipEvent_t batch0 = nullptr;
+        if (c->profile) {
+            batch0 = pooled_event(c);
+            HIPC(hipEventRecord(batch0, s));
+        }
         // -- slot descriptors and source traces
         std::vector<ShotSlot> slots(nb);
         int maxSrcRing = 0, maxRecRing = 0;
@@ -445,10 +472,21 @@ void run(aiso_ctx* c, const Request& q)
             if (mr > 0) HIPC(launch_ring(mode, sa, mr, s));
         };
 
+        // Three consecutive fine slices hold everything a coarse sample needs
+        // when sub == 2, so captures and imaging are written once per coarse
+        // step instead of accumulated at every fine step.
+        const bool win = p.sub == 2;
+
         // -- source wavefield pass (all Born-type operators)
         if (is_born(q.op)) {
             HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
             a.cur = 1;
+            const unsigned mode = win ? MODE_SRC_WIN : MODE_SRC;
+            const int winOut = q.op == Op::BornFwd ? WIN_SS : WIN_W;
+            a.winOut = winOut;
+            a.wa = f[h + 1];
+            a.wb = f[h + 2 < 2 * h ? h + 2 : h + 1];
+            a.wc = f[1];
             for (int its = 0; its < p.nts - 1; ++its)
                 for (int it2 = 1; it2 <= p.sub; ++it2) {
                     a.its = its;
@@ -461,10 +499,15 @@ void run(aiso_ctx* c, const Request& q)
                     a.skipB = a.ff == 0.f;
                     a.assignA = its == 0 && it2 == 1;
                     a.assignB = it2 == 1;
-                    launch_timed(c, MODE_SRC, a, nb);
-                    ring(MODE_SRC, a);
+                    a.win = it2 == 1;
+                    launch_timed(c, mode, AISO_K_SRC, a, nb);
+                    ring(mode, a);
                     a.cur ^= 1;
                 }
+            if (win)   // W[nts-1] = p(last-1)*f[h+1] + p(last)*f[h+2]
+                HIPC(launch_win_tail(dSlots, nb, a.cur, winOut, f[h + 1], f[h + 2], (long)(p.nts - 1) * nzx, 0, 0,
+                                     mtil, p.nz, p.nx, s));
+            a.win = 0;
         }
 
         // -- second pass
@@ -472,7 +515,10 @@ void run(aiso_ctx* c, const Request& q)
         a.cur = 1;
         a.assignA = a.assignB = a.skipA = a.skipB = 0;
         if (q.op == Op::NlFwd || q.op == Op::BornFwd || q.op == Op::ExtFwd) {
-            const unsigned mode = q.op == Op::NlFwd ? MODE_NL_FWD : q.op == Op::BornFwd ? MODE_BORN_SCAT : MODE_EXT_SCAT;
+            unsigned mode = MODE_NL_FWD;
+            int kind = AISO_K_NL_FWD;
+            if (q.op == Op::BornFwd) { mode = win ? MODE_BORN_SCAT_PRE : MODE_BORN_SCAT; kind = AISO_K_BORN_SCAT; }
+            if (q.op == Op::ExtFwd) { mode = MODE_EXT_SCAT; kind = AISO_K_EXT_SCAT; }
             int L = 0, R = 1;
             if (q.op == Op::ExtFwd) HIPC(launch_ext_ss(dSlots, nb, L, 0, mtil, base.v2dt2, p.nz, p.nx, p.hExt, s));
             for (int its = 0; its < p.nts - 1; ++its) {
@@ -489,7 +535,7 @@ void run(aiso_ctx* c, const Request& q)
                     a.ssB = R;
                     a.fe = f[it2];
                     a.ff = f[h + it2];
-                    launch_timed(c, mode, a, nb);
+                    launch_timed(c, mode, kind, a, nb);
                     ring(mode, a);
                     a.cur ^= 1;
                 }
@@ -504,17 +550,24 @@ void run(aiso_ctx* c, const Request& q)
                                         q.add, s));
             }
         } else {
-            const unsigned mode = q.op == Op::NlAdj ? MODE_NL_ADJ : q.op == Op::BornAdj ? MODE_BORN_REC : MODE_EXT_REC;
+            unsigned mode = MODE_NL_ADJ;
+            int kind = AISO_K_NL_ADJ;
+            if (q.op == Op::BornAdj) { mode = win ? MODE_REC_WIN : MODE_BORN_REC; kind = AISO_K_BORN_REC; }
+            if (q.op == Op::ExtAdj) { mode = win ? MODE_REC_WIN : MODE_EXT_REC; kind = AISO_K_EXT_REC; }
             int L = 0, R = 1;
             if (ss) HIPC(launch_fill(ss, (long)nb * 2 * nzx, s));
             if (img) HIPC(launch_fill(img, (long)nb * nExt * nzx, s));
+            a.winOut = q.op == Op::BornAdj ? WIN_IMG : WIN_SLICE;
+            a.wa = f[1];
+            a.wb = f[0];
+            a.wc = f[h + 1];
             for (int its = p.nts - 2; its >= 0; --its) {
                 for (int it2 = p.sub - 1; it2 >= 0; --it2) {
                     a.its = its;
                     a.itwRec = its * p.sub + it2;
                     a.fc = f[it2 + 1];
                     a.fd = f[h + it2 + 1];
-                    a.ssA = L;
+                    a.ssA = win ? 0 : L;
                     a.ssB = R;
                     a.fe = f[it2];
                     a.ff = f[h + it2];
@@ -523,19 +576,26 @@ void run(aiso_ctx* c, const Request& q)
                     a.assignA = it2 == p.sub - 1;
                     a.assignB = 0;
                     a.image = (q.op == Op::BornAdj && it2 == 0) ? 1 : 0;
+                    a.win = it2 == 1;
                     a.wImg = (long)(its + 1) * nzx;
-                    launch_timed(c, mode, a, nb);
+                    launch_timed(c, mode, kind, a, nb);
                     ring(mode, a);
                     a.cur ^= 1;
                 }
                 if (q.op == Op::ExtAdj)
-                    HIPC(launch_ext_img(dSlots, nb, R, (long)(its + 1) * nzx, base.v2dt2, p.nz, p.nx, nExt, s));
+                    HIPC(launch_ext_img(dSlots, nb, win ? 0 : R, (long)(its + 1) * nzx, base.v2dt2, p.nz, p.nx, nExt, s));
                 std::swap(L, R);
             }
             if (q.op == Op::BornAdj) {
-                HIPC(launch_born_adj_finish(dSlots, nb, R, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
+                if (win) {   // img += W[0] * (q(1)*f[1] + q(0)*f[0]), then the scaling
+                    HIPC(launch_win_tail(dSlots, nb, a.cur, WIN_IMG, f[1], f[0], 0, 0, 0, nullptr, p.nz, p.nx, s));
+                    HIPC(launch_born_adj_finish(dSlots, nb, -1, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
+                } else {
+                    HIPC(launch_born_adj_finish(dSlots, nb, R, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
+                }
             } else if (q.op == Op::ExtAdj) {
-                HIPC(launch_ext_img(dSlots, nb, R, 0, base.v2dt2, p.nz, p.nx, nExt, s));
+                if (win) HIPC(launch_win_tail(dSlots, nb, a.cur, WIN_SLICE, f[1], f[0], 0, 0, 0, nullptr, p.nz, p.nx, s));
+                HIPC(launch_ext_img(dSlots, nb, win ? 0 : R, 0, base.v2dt2, p.nz, p.nx, nExt, s));
                 HIPC(launch_ext_adj_finish(dSlots, nb, c->refl.as<float>(), p.nz, p.nx, nExt, s));
             }
             if (needImg) {
@@ -562,6 +622,11 @@ void run(aiso_ctx* c, const Request& q)
                     }
                 }
             }
+        }
+        if (c->profile) {
+            hipEvent_t batch1 = pooled_event(c);
+            HIPC(hipEventRecord(batch1, s));
+            c->batchEv.push_back({batch0, batch1, nb});
         }
         drain_profile(c);
     }
@@ -640,7 +705,8 @@ aiso_status aiso_ctx_destroy(aiso_ctx* c)
         if (!c) return;
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
-        for (auto& e : c->ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+        for (auto& e : c->ev) { (void)hipEventDestroy(e.e0); (void)hipEventDestroy(e.e1); }
+        for (auto e : c->evPool) (void)hipEventDestroy(e);
         c->geoms.clear();
         hipStream_t s = c->stream;
         delete c;
@@ -713,20 +779,46 @@ aiso_status aiso_ctx_profile(aiso_ctx* c, int enable)
         if (!c) throw Error(AISO_ERR_PARAM, "null context");
         drain_profile(c);
         c->profile = enable != 0;
-        c->launches = 0;
-        c->ms = 0;
-        c->pts = 0;
+        for (int k = 0; k < AISO_K_COUNT; ++k) {
+            c->launches[k] = 0;
+            c->ms[k] = 0;
+            c->pts[k] = 0;
+        }
+        c->batchDone.clear();
     });
 }
 
-aiso_status aiso_ctx_profile_read(aiso_ctx* c, long long* launches, double* ms, double* pts)
+aiso_status aiso_ctx_batch_read(aiso_ctx* c, float* ms, int* shots, int maxBatches, int* count)
 {
     return guard([&] {
         if (!c) throw Error(AISO_ERR_PARAM, "null context");
         drain_profile(c);
-        if (launches) *launches = c->launches;
-        if (ms) *ms = c->ms;
-        if (pts) *pts = c->pts;
+        const int n = (int)c->batchDone.size();
+        for (int i = 0; i < std::min(n, maxBatches); ++i) {
+            if (ms) ms[i] = c->batchDone[i].first;
+            if (shots) shots[i] = c->batchDone[i].second;
+        }
+        if (count) *count = n;
+    });
+}
+
+aiso_status aiso_ctx_profile_read(aiso_ctx* c, int which, long long* launches, double* ms, double* pts)
+{
+    return guard([&] {
+        if (!c) throw Error(AISO_ERR_PARAM, "null context");
+        if (which < AISO_K_ALL || which >= AISO_K_COUNT) throw Error(AISO_ERR_PARAM, "bad kernel variant");
+        drain_profile(c);
+        long long l = 0;
+        double m = 0, p = 0;
+        for (int k = 0; k < AISO_K_COUNT; ++k)
+            if (which == AISO_K_ALL || which == k) {
+                l += c->launches[k];
+                m += c->ms[k];
+                p += c->pts[k];
+            }
+        if (launches) *launches = l;
+        if (ms) *ms = m;
+        if (pts) *pts = p;
     });
 }
 

@@ -91,6 +91,7 @@ __global__ __launch_bounds__(NT) void k_step(const StepArgs a)
     for (int j = 0; j < PX + 2 * FAT; ++j) w[j] = sC[(lx0 + j) * LZ + lane + FAT];
 
     float nv[PX], dmp[PX];
+    float ost[(MODE & M_WIN) ? PX : 1], cur[(MODE & M_WIN) ? PX : 1];
     bool ok[PX];
 #pragma unroll
     for (int j = 0; j < PX; ++j) {
@@ -114,13 +115,17 @@ __global__ __launch_bounds__(NT) void k_step(const StepArgs a)
         if (ok[j]) {
             const long gi = (long)gx * nz + gz;
             d = sponge(a.cd, gz, gx, nz, nx, a.minPad);
-            const float o = po[gi] * d;              // second damping of the old slice
+            const float os = po[gi];
+            const float o = os * d;                  // second damping of the old slice
             if constexpr (ADJ) {
                 const float c = pc[gi];
                 n = lap + c + c - o;
+                if constexpr (MODE & M_WIN) cur[j] = c;
             } else {
                 n = v2dt2[gi] * lap + cc + cc - o;
+                if constexpr (MODE & M_WIN) cur[j] = cc;
             }
+            if constexpr (MODE & M_WIN) ost[j] = os;
             if constexpr (MODE & M_SS_BORN) {
                 const float m = a.mtil[gi];
                 const float sa = m * S.W[a.wA + gi];
@@ -130,6 +135,11 @@ __global__ __launch_bounds__(NT) void k_step(const StepArgs a)
                 } else {
                     n += sa * a.fa;
                 }
+            }
+            if constexpr (MODE & M_SS_PRE) {
+                const float sa = S.W[a.wA + gi];
+                if (a.fb != 0.f) n += sa * a.fa + S.W[a.wB + gi] * a.fb;
+                else n += sa * a.fa;
             }
             if constexpr (MODE & M_SS_SLICE) {
                 const float sa = S.ss[a.ssA][gi];
@@ -195,6 +205,19 @@ __global__ __launch_bounds__(NT) void k_step(const StepArgs a)
             }
             if constexpr (MODE & M_IMAGE) {
                 if (a.image) S.img[gi] += S.W[a.wImg + gi] * bnew;
+            }
+        }
+        if constexpr (MODE & M_WIN) {
+            if (a.win) {
+                // Coarse sample from three consecutive fine slices, accumulated
+                // in the reference's order (old, current, new).
+                const float v = (ost[j] * a.wa + cur[j] * a.wb) + n * a.wc;
+                switch (a.winOut) {
+                case WIN_W: S.W[a.wA + gi] = v; break;
+                case WIN_SS: S.W[a.wA + gi] = a.mtil[gi] * v; break;
+                case WIN_IMG: S.img[gi] += S.W[a.wImg + gi] * v; break;
+                default: S.ss[a.ssA][gi] = v; break;
+                }
             }
         }
         po[gi] = n;
@@ -302,7 +325,8 @@ __global__ void k_born_adj_finish(const ShotSlot* slots, int ssR, const float* _
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nzx; i += (long)gridDim.x * blockDim.x) {
         const int iz = int(i % nz), ix = int(i / nz);
         if (iz < FAT || iz >= nz - FAT || ix < FAT || ix >= nx - FAT) continue;
-        float m = S.img[i] + S.W[i] * S.ss[ssR][i];
+        float m = S.img[i];
+        if (ssR >= 0) m += S.W[i] * S.ss[ssR][i];
         S.img[i] = m * (v2dt2[i] * refl[i]);
     }
 }
@@ -415,6 +439,27 @@ __global__ void k_dev_adjoint(float* __restrict__ reg, const float* __restrict__
     }
 }
 
+// Last window capture after a pass: V = o*wa + c*wb from the final pair.
+__global__ void k_win_tail(const ShotSlot* slots, int cur, int winOut, float wa, float wb, long wA,
+                           long wImg, int ssA, const float* __restrict__ mtil, int nz, int nx)
+{
+    const ShotSlot& S = slots[blockIdx.y];
+    const long nzx = (long)nz * nx;
+    const float* c = S.p[cur];
+    const float* o = S.p[cur ^ 1];
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nzx; i += (long)gridDim.x * blockDim.x) {
+        const int iz = int(i % nz), ix = int(i / nz);
+        if (iz < FAT || iz >= nz - FAT || ix < FAT || ix >= nx - FAT) continue;
+        const float v = o[i] * wa + c[i] * wb;
+        switch (winOut) {
+        case WIN_W: S.W[wA + i] = v; break;
+        case WIN_SS: S.W[wA + i] = mtil[i] * v; break;
+        case WIN_IMG: S.img[i] += S.W[wImg + i] * v; break;
+        default: S.ss[ssA][i] = v; break;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers (declared in kernels.hpp).
 
@@ -445,6 +490,9 @@ static StepKernel pick_step(unsigned mode, int tz, int tx)
     case MODE_BORN_REC: return pick_tile<MODE_BORN_REC>(tz, tx);
     case MODE_EXT_SCAT: return pick_tile<MODE_EXT_SCAT>(tz, tx);
     case MODE_EXT_REC: return pick_tile<MODE_EXT_REC>(tz, tx);
+    case MODE_SRC_WIN: return pick_tile<MODE_SRC_WIN>(tz, tx);
+    case MODE_BORN_SCAT_PRE: return pick_tile<MODE_BORN_SCAT_PRE>(tz, tx);
+    case MODE_REC_WIN: return pick_tile<MODE_REC_WIN>(tz, tx);
     default: return nullptr;
     }
 }
@@ -557,6 +605,15 @@ hipError_t launch_dev_forward(const float* reg, float* out, const int* cornerReg
 {
     hipLaunchKernelGGL(k_dev_forward, flat_grid((long)nDev * nt), dim3(256), 0, s, reg, out, cornerReg,
                        weight, nDev, nb, nt, add);
+    return hipGetLastError();
+}
+
+hipError_t launch_win_tail(const ShotSlot* slots, int nshots, int cur, int winOut, float wa, float wb,
+                           long wA, long wImg, int ssA, const float* mtil, int nz, int nx, hipStream_t s)
+{
+    dim3 g = flat_grid((long)nz * nx);
+    g.y = nshots;
+    hipLaunchKernelGGL(k_win_tail, g, dim3(256), 0, s, slots, cur, winOut, wa, wb, wA, wImg, ssA, mtil, nz, nx);
     return hipGetLastError();
 }
 

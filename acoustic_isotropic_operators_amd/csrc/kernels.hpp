@@ -36,6 +36,16 @@ enum : unsigned {
     M_IMAGE = 1u << 6,      // img += Wimg * B (when args.image != 0)
     M_REC_DATA = 1u << 7,   // d[r][its] += n*fe, d[r][its+1] += n*ff at receivers
     M_REC_SIG = 1u << 8,    // sig[s][itwRec] += n at source points
+    M_WIN = 1u << 9,        // window capture V = (o*wa + c*wb) + n*wc when args.win != 0
+    M_SS_PRE = 1u << 10,    // + SS[a]*fa + SS[b]*fb, SS = m~*W stored in the W array
+};
+
+// Where a window capture V goes (StepArgs::winOut).
+enum : int {
+    WIN_W = 0,      // W[wA] = V                (source wavefield)
+    WIN_SS = 1,     // W[wA] = m~ * V           (Born secondary source)
+    WIN_IMG = 2,    // img += W[wImg] * V       (Born imaging condition)
+    WIN_SLICE = 3,  // ss[ssA] = V              (receiver wavefield for the offset imaging)
 };
 
 struct StepArgs {
@@ -56,7 +66,9 @@ struct StepArgs {
     int assignA, assignB;   // first touch of A / B: write instead of accumulate
     int skipA, skipB;       // zero weight: leave A / B untouched
     int image;              // apply M_IMAGE this step
-    long wImg;              // element offset of the W slice used by M_IMAGE
+    long wImg;              // element offset of the W slice used by M_IMAGE / WIN_IMG
+    int win, winOut;        // M_WIN: capture at this step, and its destination
+    float wa, wb, wc;       // M_WIN weights of the old, current and new slices
 };
 
 }  // namespace aiso
@@ -71,6 +83,10 @@ constexpr unsigned MODE_BORN_SCAT = M_SS_BORN | M_REC_DATA;
 constexpr unsigned MODE_BORN_REC = M_ADJ | M_INJ_DATA | M_CAPTURE | M_IMAGE;
 constexpr unsigned MODE_EXT_SCAT = M_SS_SLICE | M_REC_DATA;
 constexpr unsigned MODE_EXT_REC = M_ADJ | M_INJ_DATA | M_CAPTURE;
+// sub == 2 forms: coarse samples computed once from the old/current/new slices.
+constexpr unsigned MODE_SRC_WIN = M_INJ_SIG | M_WIN;
+constexpr unsigned MODE_BORN_SCAT_PRE = M_SS_PRE | M_REC_DATA;
+constexpr unsigned MODE_REC_WIN = M_ADJ | M_INJ_DATA | M_WIN;
 
 hipError_t launch_step(unsigned mode, int tz, int tx, const StepArgs& a, hipStream_t s);
 hipError_t launch_ring(unsigned mode, const StepArgs& a, int maxRing, hipStream_t s);
@@ -91,6 +107,8 @@ hipError_t launch_accumulate(float* acc, const float* img, long n, hipStream_t s
 hipError_t launch_add_to(float* out, const float* acc, long n, int add, hipStream_t s);
 hipError_t launch_dev_forward(const float* reg, float* out, const int* cornerReg, const float* weight,
                               int nDev, int nb, int nt, int add, hipStream_t s);
+hipError_t launch_win_tail(const ShotSlot* slots, int nshots, int cur, int winOut, float wa, float wb,
+                           long wA, long wImg, int ssA, const float* mtil, int nz, int nx, hipStream_t s);
 hipError_t launch_dev_adjoint(float* reg, const float* irreg, const int* adjPtr, const int* adjEnt,
                               const float* weight, int nReg, int nb, int nt, hipStream_t s);
 

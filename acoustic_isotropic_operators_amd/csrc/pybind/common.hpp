@@ -274,8 +274,11 @@ struct ShotsBase {
         }
         check(aiso_ctx_create(&q, gpu, &ctx));
         check(aiso_ctx_set_velocity(ctx, velHost.data(), AISO_MEM_HOST));
+        // Engine-specific keys (the reference ignores them): shots advanced
+        // per launch and the step-kernel tile.
         int batch = par.i("shotsPerLaunch", 0);
         if (batch > 0) check(aiso_ctx_set_batch(ctx, batch));
+        check(aiso_ctx_set_tile(ctx, par.i("tileZ", 64), par.i("tileX", 32)));
         fp = q;
         dirty = false;
     }

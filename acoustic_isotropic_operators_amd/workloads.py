@@ -1,0 +1,143 @@
+"""Seeded synthetic workloads (velocity, reflectivity, wavelet, geometry).
+
+* ``nonlinear_small``: one shot on a 200 x 400 grid (NL forward);
+* ``born_lsrtm``: the Born forward+adjoint benchmark, 80 shots on 500 x 1200.
+
+Grids are padded with the rule of F/src/padFileGpuMain.cpp:49-118 (pads
+edge-replicated, FAT ring left at zero), devices are placed like
+buildSourceGeometry/buildReceiversGeometry (PY/Acoustic_iso_float.py:237-456:
+iz = zSource-1+zPadMinus+fat).  The data are synthetic, generated here.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+FAT = 5
+
+
+def pad_plus(n, pad, block=16):
+    """padPlus of F/src/padFileGpuMain.cpp:49-66 (ceil in float)."""
+    total = 2 * pad + n
+    return int(np.ceil(np.float32(total) / np.float32(block))) * block - n - pad
+
+
+def pad_model(m, zPad, xPad, block=16):
+    """Edge-replicate a physical [nx][nz] model into the padded grid; FAT ring = 0."""
+    nx, nz = m.shape
+    zPlus, xPlus = pad_plus(nz, zPad, block), pad_plus(nx, xPad, block)
+    NZ, NX = nz + zPad + zPlus + 2 * FAT, nx + xPad + xPlus + 2 * FAT
+    out = np.zeros((NX, NZ), np.float32)
+    z0, x0 = FAT + zPad, FAT + xPad
+    out[x0:x0 + nx, z0:z0 + nz] = m
+    out[x0:x0 + nx, FAT:z0] = m[:, :1]
+    out[x0:x0 + nx, z0 + nz:z0 + nz + zPlus] = m[:, -1:]
+    out[FAT:x0, FAT:NZ - FAT] = out[x0:x0 + 1, FAT:NZ - FAT]
+    out[x0 + nx:x0 + nx + xPlus, FAT:NZ - FAT] = out[x0 + nx - 1:x0 + nx, FAT:NZ - FAT]
+    return out, zPlus, xPlus
+
+
+def ricker(nts, dts, f0, t0):
+    """Ricker wavelet, seis_utils/seis_utils_float/python/waveletFloatMain.py:95-99."""
+    t = np.arange(nts, dtype=np.float64) * dts - t0
+    a = (np.pi * f0 * t) ** 2
+    return ((1.0 - 2.0 * a) * np.exp(-a)).astype(np.float32)
+
+
+@dataclass
+class Workload:
+    name: str
+    params: dict          # aiso_fd_params fields
+    vel: np.ndarray       # padded [NX][NZ]
+    wavelet: np.ndarray   # [1][nts] (one point source per shot)
+    src: list             # per shot (iz, ix) on the padded grid
+    rec: tuple            # (nzDev, ozDev, dzDev, nxDev, oxDev, dxDev)
+    model: np.ndarray     # padded reflectivity [NX][NZ]
+
+    @property
+    def nShot(self):
+        return len(self.src)
+
+    @property
+    def interior(self):
+        return (self.params["nz"] - 2 * FAT) * (self.params["nx"] - 2 * FAT)
+
+    @property
+    def fine_steps(self):
+        return (self.params["nts"] - 1) * self.params["sub"]
+
+    def parfile(self):
+        """The reference's parfile keys for this workload."""
+        p = self.params
+        keys = dict(nts=p["nts"], dts=p["dts"], ots=0.0, sub=p["sub"], nz=p["nz"], nx=p["nx"], dz=p["dz"],
+                    dx=p["dx"], zPadMinus=p["zPadMinus"], zPadPlus=p["zPadPlus"], xPadMinus=p["xPadMinus"],
+                    xPadPlus=p["xPadPlus"], fat=5, blockSize=16, alphaCos=p["alphaCos"], fMax=p["fMax"],
+                    nShot=self.nShot, nGpu=1)
+        return [f"{k}={v}" for k, v in keys.items()]
+
+
+def _smooth(field, sigma):
+    from scipy.ndimage import gaussian_filter
+    return gaussian_filter(field, sigma, mode="nearest")
+
+
+def born_lsrtm(nShot=80, seed=2024, nts=2000, dts=0.002, sub=2):
+    """500 x 1200 grid at 10 m, 50-point sponge (alpha 0.95), 1500-4000 m/s
+    gradient with +/-5 % smoothed random perturbations (sigma 300 m),
+    nShot sources and 1200 receivers at 20 m depth, nts=2000 at 2 ms (sub 2,
+    Courant ~0.42), 15 Hz Ricker, reflectivity = sparse N(0,1) spikes on 10
+    dipping interfaces."""
+    rng = np.random.default_rng(seed)
+    nz, nx, d, pad = 500, 1200, 10.0, 50
+    z = np.arange(nz) / (nz - 1)
+    v = np.broadcast_to(1500.0 + 2500.0 * z, (nx, nz)).astype(np.float64)
+    pert = _smooth(rng.standard_normal((nx, nz)), 30.0)
+    pert /= np.abs(pert).max()
+    v = (v * (1.0 + 0.05 * pert)).astype(np.float32)
+    refl = np.zeros((nx, nz), np.float32)
+    xs = np.arange(nx)
+    for _ in range(10):
+        z0 = rng.uniform(0.15, 0.95) * nz
+        slope = rng.uniform(-0.15, 0.15)
+        zi = np.clip(np.round(z0 + slope * (xs - nx / 2)).astype(int), 0, nz - 1)
+        keep = rng.random(nx) < 0.5          # ~1 % of all model points carry a spike
+        refl[xs[keep], zi[keep]] = rng.standard_normal(int(keep.sum())).astype(np.float32)
+    vp, zPlus, xPlus = pad_model(v, pad, pad)
+    mp, _, _ = pad_model(refl, pad, pad)
+    # reflectivity lives only in the physical domain
+    mask = np.zeros_like(mp)
+    mask[FAT + pad:FAT + pad + nx, FAT + pad:FAT + pad + nz] = 1.0
+    mp *= mask
+    NX, NZ = vp.shape
+    params = dict(nz=NZ, nx=NX, dz=d, dx=d, nts=nts, dts=dts, sub=sub, zPadMinus=pad, zPadPlus=zPlus,
+                  xPadMinus=pad, xPadPlus=xPlus, alphaCos=0.95, fMax=37.5)
+    izs = 3 - 1 + pad + FAT                  # zSource=3 -> 20 m depth
+    spacing = nx // nShot
+    src = [(izs, FAT + pad + spacing // 2 + spacing * s) for s in range(nShot)]
+    rec = (1, izs, 1, nx, FAT + pad, 1)      # every column at 20 m depth
+    wav = ricker(nts, dts, 15.0, 0.1)[None, :]
+    return Workload("born_lsrtm_80x500x1200", params, vp, wav, src, rec, mp)
+
+
+def nonlinear_small(seed=7, nts=1000, dts=0.004, sub=4):
+    """200 x 400 grid at 10 m, 50-point sponge (alpha 0.95), 1500-3500 m/s
+    gradient plus one Gaussian anomaly (+/-300 m/s, sigma 200 m), one source
+    at 20 m depth in the centre, 400 receivers at 20 m every 10 m, nts=1000
+    at 4 ms (sub 4)."""
+    rng = np.random.default_rng(seed)
+    nz, nx, d, pad = 200, 400, 10.0, 50
+    zz, xx = np.meshgrid(np.arange(nz) * d, np.arange(nx) * d)
+    v = 1500.0 + 2000.0 * zz / ((nz - 1) * d)
+    cz, cx = rng.uniform(0.3, 0.7) * nz * d, rng.uniform(0.3, 0.7) * nx * d
+    amp = 300.0 * rng.choice([-1.0, 1.0])
+    v = (v + amp * np.exp(-((zz - cz) ** 2 + (xx - cx) ** 2) / (2 * 200.0 ** 2))).astype(np.float32)
+    vp, zPlus, xPlus = pad_model(v, pad, pad)
+    NX, NZ = vp.shape
+    params = dict(nz=NZ, nx=NX, dz=d, dx=d, nts=nts, dts=dts, sub=sub, zPadMinus=pad, zPadPlus=zPlus,
+                  xPadMinus=pad, xPadPlus=xPlus, alphaCos=0.95, fMax=37.5)
+    izs = 3 - 1 + pad + FAT
+    src = [(izs, FAT + pad + nx // 2)]
+    rec = (1, izs, 1, nx, FAT + pad, 1)
+    wav = ricker(nts, dts, 15.0, 0.1)[None, :]
+    return Workload("nonlinear_1x200x400", params, vp, wav, src, rec, np.zeros_like(vp))

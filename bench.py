@@ -1,0 +1,159 @@
+"""Headline benchmark: Born forward + adjoint (LSRTM pair), 80 shots, 500 x 1200.
+
+One step = one Born forward (model -> data) plus one Born adjoint
+(data -> image) over all shots, with model, data and image resident in HBM
+(``mem=DEVICE``).  Work unit: one interior point-update of one fine time step;
+the pair runs 4 propagations per shot (source + scattered, source + receiver),
+so a step is 4 * 3998 * 797,696 * 80 = 1.0205e12 point-updates.
+
+Prints one JSON line with the contract fields plus ``roofline`` (dominant step
+kernel, HIP-event timed over the timed region) and ``cpu_baseline`` (the CPU
+oracle on the box's host cores, one shot of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes per interior point and fine step of each step-kernel
+# variant: 16 B for the stencil (read c, o, v2dt2; write n) plus the
+# coarse-rate traffic of that pass spread over its `sub` fine steps:
+# source pass writes W (4 B), scattered pass reads W and m~ (8 B), receiver
+# pass reads W and updates the image (12 B).
+CO_BYTES = {"K_SRC": 4.0, "K_BORN_SCAT": 8.0, "K_BORN_REC": 12.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--shots", type=int, default=80)
+    ap.add_argument("--batch", type=int, default=0, help="shots per launch (0 = by HBM budget)")
+    ap.add_argument("--tile", type=str, default="64,32")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-shots", type=int, default=1)
+    return ap.parse_args()
+
+
+def cpu_baseline(w, nshots):
+    """Oracle (float, OpenMP over ix) on the host: Born fwd + adj of the first
+    `nshots` shots of the same workload, as Gpts/s."""
+    threads = min(16, len(os.sched_getaffinity(0)))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    from oracle import oracle as O
+    p = w.params
+    g = O.Grid(nz=p["nz"], nx=p["nx"], nts=p["nts"], sub=p["sub"], minPad=min(p["zPadMinus"], p["zPadPlus"], p["xPadMinus"], p["xPadPlus"]),
+               dz=p["dz"], dx=p["dx"], dts=p["dts"], alphaCos=p["alphaCos"])
+    srcs = [O.Device.grid(1, iz, 1, 1, ix, 1, p["nz"], p["nx"], p["nts"]) for iz, ix in w.src[:nshots]]
+    rec = [O.Device.grid(*w.rec, p["nz"], p["nx"], p["nts"])]
+    t0 = time.perf_counter()
+    d = O.born_forward(g, w.vel, srcs, [w.wavelet], rec, w.model, "f")
+    O.born_adjoint(g, w.vel, srcs, [w.wavelet], rec, d, "f")
+    t = time.perf_counter() - t0
+    pts = 4.0 * w.fine_steps * w.interior * nshots
+    return {"value": pts / t / 1e9, "unit": "Gpts/s", "cores": threads, "kind": "port",
+            "sample": f"{nshots} of {w.nShot} shots, full grid and nts, Born fwd+adj, {t:.1f} s"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(shots shard across GPUs with one image all-reduce, not implemented here)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from acoustic_isotropic_operators_amd import _capi as A
+    from acoustic_isotropic_operators_amd.workloads import born_lsrtm
+
+    w = born_lsrtm(nShot=a.shots)
+    p = w.params
+    ctx = A.Context(p)
+    ctx.set_velocity(w.vel)
+    tz, tx = (int(v) for v in a.tile.split(","))
+    ctx.set_tile(tz, tx)
+    if a.batch:
+        ctx.set_batch(a.batch)
+    srcs = [A.Device.grid(1, iz, 1, 1, ix, 1, p["nz"], p["nx"], p["nts"]) for iz, ix in w.src]
+    rec = [A.Device.grid(*w.rec, p["nz"], p["nx"], p["nts"])]
+    nrec = rec[0].nDev
+    dev = torch.device("cuda:0")
+    model = torch.from_numpy(w.model).to(dev)
+    data = torch.empty((w.nShot, nrec, p["nts"]), dtype=torch.float32, device=dev)
+    image = torch.empty_like(model)
+    torch.cuda.synchronize()
+
+    def step():
+        ctx.born_forward(model, data, srcs, w.wavelet, rec, mem=A.MEM_DEVICE)
+        ctx.born_adjoint(image, data, srcs, w.wavelet, rec, mem=A.MEM_DEVICE)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / a.steps
+    ok = bool(torch.isfinite(image).all().item()) and bool(torch.isfinite(data).all().item())
+
+    kinds = {name: getattr(A, name) for name in CO_BYTES}
+    stats = {name: ctx.profile_read(k) for name, k in kinds.items()}
+    dom = max(stats, key=lambda n: stats[n]["ms"])
+    st = stats[dom]
+    bpp = 16.0 + CO_BYTES[dom] / p["sub"]
+    avg_ms = st["ms"] / max(st["launches"], 1)
+    per_launch_bytes = bpp * st["point_updates"] / max(st["launches"], 1)
+    achieved = per_launch_bytes / (avg_ms / 1e3) / 1e9
+    bms, bsh = ctx.batch_read()
+    info = ctx.info()
+    nb = len(bms) // (2 * a.steps)
+    lat = []
+    for s in range(a.steps):
+        fw = bms[2 * nb * s: 2 * nb * s + nb]
+        ad = bms[2 * nb * s + nb: 2 * nb * (s + 1)]
+        for b in range(nb):
+            lat += [float(fw[b] + ad[b])] * int(bsh[2 * nb * s + b])
+    pts = 4.0 * w.fine_steps * w.interior * w.nShot
+    job_bytes = w.nShot * w.interior * (4 * w.fine_steps * 16.0 + p["nts"] * 28.0)
+    out = {
+        "metric": "Gpts/s, Born fwd+adj 80 shots 500x1200",
+        "value": pts / t / 1e9,
+        "unit": "Gpts/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": t * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "shots": w.nShot, "grid": "500x1200 (padded %dx%d)" % (p["nz"], p["nx"]),
+                   "nts": p["nts"], "sub": p["sub"], "shots_per_launch": info["batch"], "tile": [tz, tx],
+                   "inputs": "model/data/image resident in HBM"},
+        "job_roofline_frac": job_bytes / t / 1e9 / HBM_PEAK_GBS,
+        "shot_latency_ms": {"p50": float(np.percentile(lat, 50)) if lat else None,
+                            "p99": float(np.percentile(lat, 99)) if lat else None},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms,
+                     "launches": st["launches"], "bytes_per_point": bpp},
+        "finite": ok,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, a.cpu_shots)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -94,12 +94,31 @@ aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
 aiso_status aiso_ctx_info(const aiso_ctx* ctx, int* ntw, int* batchUsed, size_t* bytesPerShot);
 /* The HIP stream (hipStream_t) every kernel of this context runs on. */
 aiso_status aiso_ctx_stream(const aiso_ctx* ctx, void** stream);
-/* Per-launch timing of the time-step kernels with HIP events on that stream. */
+/* Variants of the fused time-step kernel, for profiling. */
+enum {
+    AISO_K_ALL = -1,
+    AISO_K_NL_FWD = 0,     /* nonlinear forward step                         */
+    AISO_K_NL_ADJ = 1,     /* nonlinear adjoint step                         */
+    AISO_K_SRC = 2,        /* source-wavefield step with coarse capture      */
+    AISO_K_BORN_SCAT = 3,  /* Born scattered-wavefield step                  */
+    AISO_K_BORN_REC = 4,   /* Born adjoint receiver step with imaging        */
+    AISO_K_EXT_SCAT = 5,   /* extended-Born scattered step                   */
+    AISO_K_EXT_REC = 6,    /* extended-Born adjoint receiver step            */
+    AISO_K_COUNT = 7
+};
+/* Per-launch timing of the time-step kernels with HIP events on that stream
+ * (enable != 0), reset of the counters. */
 aiso_status aiso_ctx_profile(aiso_ctx* ctx, int enable);
 /* Launch count, summed duration (ms) and summed interior point-updates of
- * the time-step kernels timed since profiling was enabled. */
-aiso_status aiso_ctx_profile_read(aiso_ctx* ctx, long long* launches, double* ms,
+ * the time-step kernels of variant `which` (AISO_K_*) timed since profiling
+ * was enabled. */
+aiso_status aiso_ctx_profile_read(aiso_ctx* ctx, int which, long long* launches, double* ms,
                                   double* pointUpdates);
+/* Wall time (ms, HIP events) of every shot batch run since profiling was
+ * enabled, with the number of shots in each: a shot's latency is the time of
+ * the batch that carried it.  Writes at most maxBatches entries; *count gets
+ * the number available. */
+aiso_status aiso_ctx_batch_read(aiso_ctx* ctx, float* ms, int* shots, int maxBatches, int* count);
 
 /* ------------------------------------------------------------ geometry --
  * deviceGpu.  nzVel/nxVel are the padded velocity sizes; nt the trace length. */
