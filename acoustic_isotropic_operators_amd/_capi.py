@@ -15,7 +15,7 @@ from . import lib_path
 
 MEM_HOST, MEM_DEVICE = 0, 1
 EXT_NONE, EXT_OFFSET = 0, 1
-K_ALL, K_NL_FWD, K_NL_ADJ, K_SRC, K_BORN_SCAT, K_BORN_REC, K_EXT_SCAT, K_EXT_REC = range(-1, 7)
+K_ALL, K_NL_FWD, K_NL_ADJ, K_SRC, K_BORN_SCAT, K_BORN_REC, K_EXT_SCAT, K_EXT_REC, K_BORN_PAIR = range(-1, 8)
 
 
 class FdParams(C.Structure):
@@ -39,6 +39,7 @@ _SIGS = {
     "aiso_ctx_set_velocity": (_I, [_P, _P, _I]),
     "aiso_ctx_set_batch": (_I, [_P, _I]),
     "aiso_ctx_set_tile": (_I, [_P, _I, _I]),
+    "aiso_ctx_set_option": (_I, [_P, C.c_char_p, _I]),
     "aiso_ctx_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(C.c_size_t)]),
     "aiso_ctx_stream": (_I, [_P, _PP]),
     "aiso_ctx_profile": (_I, [_P, _I]),
@@ -194,6 +195,9 @@ class Context:
 
     def set_tile(self, tz, tx):
         _check(lib().aiso_ctx_set_tile(self.h, tz, tx))
+
+    def set_option(self, key, value):
+        _check(lib().aiso_ctx_set_option(self.h, key.encode(), int(value)))
 
     def info(self):
         a, b, c = _I(), _I(), C.c_size_t()

@@ -94,10 +94,16 @@ struct aiso_ctx {
     bool haveVel = false;
     std::vector<float> vel, v2dt2h, reflh;
     DBuf v2dt2, refl, cd;
-    int batchReq = 0, batchUsed = 0, tz = 64, tx = 32;
+    int batchReq = 0, batchUsed = 0, tz = 32, tx = 32;
+    int streamNT = 1;
     size_t bytesPerShot = 0;
     // work buffers
-    DBuf pairs, W, ss, img, dreg, sig, slots, mtil, acc, modelDev, dataDev;
+    DBuf pairs, pairsQ, W, ss, img, dreg, sig, slots, slotsQ, mtil, acc, modelDev, dataDev;
+    int lockstep = 1;
+    int batchFwd = 0, batchAdj = 0;     // per-direction overrides of batchReq
+    int batchRec = 0;                   // receiver-pass group size within an adjoint batch
+    int spb = 1;                        // shots per workgroup of the step kernel
+    long cacheBudget = 192l << 20;      // bytes of hot slices per launch (Infinity Cache is 256 MB)
     std::map<std::tuple<long, int, int>, std::unique_ptr<DevGeom>> geoms;
     // profiling of the step kernels
     bool profile = false;
@@ -108,7 +114,7 @@ struct aiso_ctx {
     std::vector<std::pair<float, int>> batchDone;
     std::vector<hipEvent_t> evPool;
     long long launches[AISO_K_COUNT] = {0};
-    double ms[AISO_K_COUNT] = {0}, pts[AISO_K_COUNT] = {0};
+    double ms[AISO_K_COUNT] = {0}, pts[AISO_K_COUNT] = {0};   // pts: point-updates (both fields of a pair)
 };
 
 namespace {
@@ -193,14 +199,15 @@ hipEvent_t pooled_event(aiso_ctx* c)
     return e;
 }
 
-void launch_timed(aiso_ctx* c, unsigned mode, int kind, const StepArgs& a, int nshots)
+void launch_timed(aiso_ctx* c, unsigned mode, int kind, const StepArgs& a, int nshots, bool generic = false)
 {
     hipEvent_t e0 = nullptr;
     if (c->profile) {
         e0 = pooled_event(c);
         HIPC(hipEventRecord(e0, c->stream));
     }
-    HIPC(launch_step(mode, c->tz, c->tx, a, c->stream));
+    if (generic) HIPC(launch_step_generic(mode, c->tz, c->tx, a, c->stream));
+    else HIPC(launch_step(mode, c->tz, c->tx, c->spb, a, c->stream));
     if (c->profile) {
         hipEvent_t e1 = pooled_event(c);
         HIPC(hipEventRecord(e1, c->stream));
@@ -341,27 +348,46 @@ void run(aiso_ctx* c, const Request& q)
     }
 
     // ---- batch size from the HBM budget
+    // The Born forward with sub == 2 runs its two propagations in lockstep:
+    // the secondary source then lives in a 3-slice ring instead of nts slices.
+    const bool lockstep = q.op == Op::BornFwd && p.sub == 2 && c->lockstep;
+    const long wSlices = lockstep ? 3 : p.nts;
     const bool needW = is_born(q.op), needSS = is_ext(q.op) || q.op == Op::BornAdj;
     const bool needImg = q.op == Op::BornAdj || q.op == Op::ExtAdj;
     size_t perShot = 2 * nzx * 4 + (size_t)nRecRegMax * p.nts * 4 + (size_t)nSrcRegMax * p.ntw * 4;
-    if (needW) perShot += (size_t)p.nts * nzx * 4;
+    if (needW) perShot += (size_t)wSlices * nzx * 4;
+    if (lockstep) perShot += 2 * nzx * 4;
     if (needSS) perShot += 2 * nzx * 4;
     if (needImg) perShot += (size_t)nExt * nzx * 4;
     c->bytesPerShot = perShot;
-    int B = c->batchReq;
+    // Shots per launch: an explicit request, else as many as keep the
+    // slices every fine step touches (per shot) within the on-die cache
+    // budget; never more than device memory holds.
+    int B = is_adj(q.op) ? c->batchAdj : c->batchFwd;
+    if (B <= 0) B = c->batchReq;
     if (B <= 0) {
+        int hot = 2;                                  // wavefield pair
+        if (lockstep) hot = 7;                        // two pairs + secondary-source ring
+        else if (q.op == Op::BornAdj) hot = 3;        // + image
+        else if (q.op == Op::ExtFwd) hot = 4;         // + two secondary-source slices
+        else if (q.op == Op::ExtAdj) hot = 3 + nExt;  // + receiver slice + extended image
+        B = (int)std::max<long>(4, c->cacheBudget / ((long)hot * nzx * 4));
+        B = std::min(B, 32);
+    }
+    {
         size_t freeB = 0, totalB = 0;
         HIPC(hipMemGetInfo(&freeB, &totalB));
-        size_t have = c->pairs.n + c->W.n + c->ss.n + c->img.n + c->dreg.n + c->sig.n;
-        size_t budget = (size_t)((freeB + have) * 0.80);
-        B = (int)std::max<size_t>(1, budget / perShot);
-        B = std::min(B, 32);
+        size_t have = c->pairs.n + c->pairsQ.n + c->W.n + c->ss.n + c->img.n + c->dreg.n + c->sig.n;
+        size_t budget = (size_t)((freeB + have) * 0.85);
+        B = std::min(B, (int)std::max<size_t>(1, budget / perShot));
     }
     B = std::min(B, q.nShot);
     c->batchUsed = B;
 
     float* pairs = c->pairs.need((size_t)B * 2 * nzx * 4) ? c->pairs.as<float>() : nullptr;
-    float* W = needW ? (float*)c->W.need((size_t)B * p.nts * nzx * 4) : nullptr;
+    float* W = needW ? (float*)c->W.need((size_t)B * wSlices * nzx * 4) : nullptr;
+    float* pairsQ = lockstep ? (float*)c->pairsQ.need((size_t)B * 2 * nzx * 4) : nullptr;
+    ShotSlot* dSlotsQ = lockstep ? (ShotSlot*)c->slotsQ.need(sizeof(ShotSlot) * B) : nullptr;
     float* ss = needSS ? (float*)c->ss.need((size_t)B * 2 * nzx * 4) : nullptr;
     float* img = needImg ? (float*)c->img.need((size_t)B * nExt * nzx * 4) : nullptr;
     float* dreg = (float*)c->dreg.need((size_t)B * nRecRegMax * p.nts * 4);
@@ -397,6 +423,7 @@ void run(aiso_ctx* c, const Request& q)
     base.ntiles = base.tilesZ * tilesX(c);
     for (int k = 0; k < 6; ++k) { base.zc[k] = p.zc[k]; base.xc[k] = p.xc[k]; }
     base.wA = base.wB = base.wImg = 0;
+    base.ntW = c->streamNT;
 
     std::vector<float> sigHost;
     for (int b0 = 0; b0 < q.nShot; b0 += B) {
@@ -419,7 +446,11 @@ void run(aiso_ctx* c, const Request& q)
             std::memset(&S, 0, sizeof(S));
             S.p[0] = pairs + (size_t)k * 2 * nzx;
             S.p[1] = S.p[0] + nzx;
-            if (W) S.W = W + (size_t)k * p.nts * nzx;
+            if (W) S.W = W + (size_t)k * wSlices * nzx;
+            if (pairsQ) {
+                S.q[0] = pairsQ + (size_t)k * 2 * nzx;
+                S.q[1] = S.q[0] + nzx;
+            }
             if (ss) { S.ss[0] = ss + (size_t)k * 2 * nzx; S.ss[1] = S.ss[0] + nzx; }
             if (img) S.img = img + (size_t)k * nExt * nzx;
             S.dreg = dreg + (size_t)k * nRecRegMax * p.nts;
@@ -444,6 +475,12 @@ void run(aiso_ctx* c, const Request& q)
             if (!fine.empty()) std::memcpy(sigHost.data() + (size_t)k * nSrcRegMax * p.ntw, fine.data(), fine.size() * 4);
         }
         HIPC(hipMemcpyAsync(dSlots, slots.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
+        std::vector<ShotSlot> slotsQ;
+        if (lockstep) {   // the scattered field seen as "the" pair, for single-field launches
+            slotsQ = slots;
+            for (auto& S : slotsQ) { S.p[0] = S.q[0]; S.p[1] = S.q[1]; }
+            HIPC(hipMemcpyAsync(dSlotsQ, slotsQ.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
+        }
         if (q.op != Op::NlAdj) HIPC(hipMemcpyAsync(sig, sigHost.data(), sigHost.size() * 4, hipMemcpyHostToDevice, s));
         else HIPC(launch_fill(sig, (long)nb * nSrcRegMax * p.ntw, s));
 
@@ -476,9 +513,83 @@ void run(aiso_ctx* c, const Request& q)
         // when sub == 2, so captures and imaging are written once per coarse
         // step instead of accumulated at every fine step.
         const bool win = p.sub == 2;
+        // The extended-Born operators step on the unspecialized kernel: the
+        // specialized one ended a GPU run in a memory fault on this path with
+        // no cause found in its source (it is exact and in bounds when the
+        // same source runs on the host).
+        const bool generic = is_ext(q.op);
+
+        if (lockstep) {
+            HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
+            HIPC(launch_fill(pairsQ, (long)nb * 2 * nzx, s));
+            const int T = p.ntw - 1;   // fine steps of each propagation
+            auto ring3 = [&](long k) { return (k % 3) * nzx; };
+            int curS = 1, curQ = 1;
+            a.winOut = WIN_SS;
+            a.wa = f[h + 1];
+            a.wb = f[h + 2];
+            a.wc = f[1];
+            for (int t = 1; t <= T + 2; ++t) {
+                const bool doS = t <= T, doQ = t - 2 >= 1;
+                const int itsS = (t - 1) / p.sub, it2S = t - itsS * p.sub;
+                const int tq = t - 2, itsQ = doQ ? (tq - 1) / p.sub : 0, it2Q = doQ ? tq - itsQ * p.sub : 1;
+                StepArgs x = a;
+                x.cur = curS;
+                x.curQ = curQ;
+                x.itwInj = t - 1;
+                x.win = doS && it2S == 1;
+                x.wA = ring3(itsS);
+                x.its = itsQ;
+                x.fa = f[it2Q - 1];
+                x.fb = f[h + it2Q - 1];
+                x.wQa = ring3(itsQ);
+                x.wQb = ring3(itsQ + 1);
+                x.fe = f[it2Q];
+                x.ff = f[h + it2Q];
+                if (doS && doQ) {
+                    hipEvent_t e0 = nullptr;
+                    if (c->profile) { e0 = pooled_event(c); HIPC(hipEventRecord(e0, s)); }
+                    HIPC(launch_fwd_pair(c->tz, c->tx, x, s));
+                    if (c->profile) {
+                        hipEvent_t e1 = pooled_event(c);
+                        HIPC(hipEventRecord(e1, s));
+                        c->ev.push_back({e0, e1, AISO_K_BORN_PAIR, 2.0 * double(p.interior()) * nb});
+                    }
+                    ring(MODE_SRC_WIN, x);
+                    StepArgs xq = x;
+                    xq.slots = dSlotsQ;
+                    xq.cur = curQ;
+                    ring(MODE_BORN_SCAT_PRE, xq);
+                } else if (doS) {
+                    launch_timed(c, MODE_SRC_WIN, AISO_K_SRC, x, nb);
+                    ring(MODE_SRC_WIN, x);
+                } else {
+                    StepArgs xq = x;
+                    xq.slots = dSlotsQ;
+                    xq.cur = curQ;
+                    xq.wA = x.wQa;
+                    xq.wB = x.wQb;
+                    launch_timed(c, MODE_BORN_SCAT_PRE, AISO_K_BORN_SCAT, xq, nb);
+                    ring(MODE_BORN_SCAT_PRE, xq);
+                }
+                if (doS) curS ^= 1;
+                if (doQ) curQ ^= 1;
+                if (t == T)   // last coarse sample of the source wavefield
+                    HIPC(launch_win_tail(dSlots, nb, curS, WIN_SS, f[h + 1], f[h + 2], ring3(p.nts - 1), 0, 0,
+                                         mtil, p.nz, p.nx, s));
+            }
+            for (int k = 0; k < nb; ++k) {
+                const int shot = b0 + k;
+                const aiso_device* rd = q.rec[q.nRec == 1 ? 0 : shot];
+                DevGeom* rg = recG[q.nRec == 1 ? 0 : shot];
+                HIPC(launch_dev_forward(dreg + (size_t)k * nRecRegMax * p.nts, dDataOut + (size_t)shot * nRecDev * p.nts,
+                                        rg->cornerReg.as<int>(), rg->weight.as<float>(), nRecDev, rd->g.nb, p.nts,
+                                        q.add, s));
+            }
+        }
 
         // -- source wavefield pass (all Born-type operators)
-        if (is_born(q.op)) {
+        if (is_born(q.op) && !lockstep) {
             HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
             a.cur = 1;
             const unsigned mode = win ? MODE_SRC_WIN : MODE_SRC;
@@ -500,7 +611,7 @@ void run(aiso_ctx* c, const Request& q)
                     a.assignA = its == 0 && it2 == 1;
                     a.assignB = it2 == 1;
                     a.win = it2 == 1;
-                    launch_timed(c, mode, AISO_K_SRC, a, nb);
+                    launch_timed(c, mode, AISO_K_SRC, a, nb, generic);
                     ring(mode, a);
                     a.cur ^= 1;
                 }
@@ -511,6 +622,7 @@ void run(aiso_ctx* c, const Request& q)
         }
 
         // -- second pass
+        if (!lockstep) {
         HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
         a.cur = 1;
         a.assignA = a.assignB = a.skipA = a.skipB = 0;
@@ -535,7 +647,7 @@ void run(aiso_ctx* c, const Request& q)
                     a.ssB = R;
                     a.fe = f[it2];
                     a.ff = f[h + it2];
-                    launch_timed(c, mode, kind, a, nb);
+                    launch_timed(c, mode, kind, a, nb, generic);
                     ring(mode, a);
                     a.cur ^= 1;
                 }
@@ -554,50 +666,64 @@ void run(aiso_ctx* c, const Request& q)
             int kind = AISO_K_NL_ADJ;
             if (q.op == Op::BornAdj) { mode = win ? MODE_REC_WIN : MODE_BORN_REC; kind = AISO_K_BORN_REC; }
             if (q.op == Op::ExtAdj) { mode = win ? MODE_REC_WIN : MODE_EXT_REC; kind = AISO_K_EXT_REC; }
-            int L = 0, R = 1;
             if (ss) HIPC(launch_fill(ss, (long)nb * 2 * nzx, s));
             if (img) HIPC(launch_fill(img, (long)nb * nExt * nzx, s));
             a.winOut = q.op == Op::BornAdj ? WIN_IMG : WIN_SLICE;
             a.wa = f[1];
             a.wb = f[0];
             a.wc = f[h + 1];
-            for (int its = p.nts - 2; its >= 0; --its) {
-                for (int it2 = p.sub - 1; it2 >= 0; --it2) {
-                    a.its = its;
-                    a.itwRec = its * p.sub + it2;
-                    a.fc = f[it2 + 1];
-                    a.fd = f[h + it2 + 1];
-                    a.ssA = win ? 0 : L;
-                    a.ssB = R;
-                    a.fe = f[it2];
-                    a.ff = f[h + it2];
-                    a.skipA = a.fe == 0.f;
-                    a.skipB = a.ff == 0.f;
-                    a.assignA = it2 == p.sub - 1;
-                    a.assignB = 0;
-                    a.image = (q.op == Op::BornAdj && it2 == 0) ? 1 : 0;
-                    a.win = it2 == 1;
-                    a.wImg = (long)(its + 1) * nzx;
-                    launch_timed(c, mode, kind, a, nb);
-                    ring(mode, a);
-                    a.cur ^= 1;
+            // The stored source wavefield lets the receiver pass run the batch
+            // in smaller groups, whose per-shot slices (pair + image) stay in
+            // the on-die cache.
+            const int subB = c->batchRec > 0 ? std::min(c->batchRec, nb) : nb;
+            for (int k0 = 0; k0 < nb; k0 += subB) {
+                const int nbr = std::min(subB, nb - k0);
+                ShotSlot* sl = dSlots + k0;
+                StepArgs r = a;
+                r.slots = sl;
+                r.nshots = nbr;
+                r.gridLogical = r.ntiles * nbr;
+                r.cur = 1;
+                int L = 0, R = 1;
+                for (int its = p.nts - 2; its >= 0; --its) {
+                    for (int it2 = p.sub - 1; it2 >= 0; --it2) {
+                        r.its = its;
+                        r.itwRec = its * p.sub + it2;
+                        r.fc = f[it2 + 1];
+                        r.fd = f[h + it2 + 1];
+                        r.ssA = win ? 0 : L;
+                        r.ssB = R;
+                        r.fe = f[it2];
+                        r.ff = f[h + it2];
+                        r.skipA = r.fe == 0.f;
+                        r.skipB = r.ff == 0.f;
+                        r.assignA = it2 == p.sub - 1;
+                        r.assignB = 0;
+                        r.image = (q.op == Op::BornAdj && it2 == 0) ? 1 : 0;
+                        r.win = it2 == 1;
+                        r.wImg = (long)(its + 1) * nzx;
+                        launch_timed(c, mode, kind, r, nbr, generic);
+                        ring(mode, r);
+                        r.cur ^= 1;
+                    }
+                    if (q.op == Op::ExtAdj)
+                        HIPC(launch_ext_img(sl, nbr, win ? 0 : R, (long)(its + 1) * nzx, base.v2dt2, p.nz, p.nx, nExt, s));
+                    std::swap(L, R);
                 }
-                if (q.op == Op::ExtAdj)
-                    HIPC(launch_ext_img(dSlots, nb, win ? 0 : R, (long)(its + 1) * nzx, base.v2dt2, p.nz, p.nx, nExt, s));
-                std::swap(L, R);
-            }
-            if (q.op == Op::BornAdj) {
-                if (win) {   // img += W[0] * (q(1)*f[1] + q(0)*f[0]), then the scaling
-                    HIPC(launch_win_tail(dSlots, nb, a.cur, WIN_IMG, f[1], f[0], 0, 0, 0, nullptr, p.nz, p.nx, s));
-                    HIPC(launch_born_adj_finish(dSlots, nb, -1, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
-                } else {
-                    HIPC(launch_born_adj_finish(dSlots, nb, R, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
+                if (q.op == Op::BornAdj) {
+                    if (win) {   // img += W[0] * (q(1)*f[1] + q(0)*f[0]), then the scaling
+                        HIPC(launch_win_tail(sl, nbr, r.cur, WIN_IMG, f[1], f[0], 0, 0, 0, nullptr, p.nz, p.nx, s));
+                        HIPC(launch_born_adj_finish(sl, nbr, -1, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
+                    } else {
+                        HIPC(launch_born_adj_finish(sl, nbr, R, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
+                    }
+                } else if (q.op == Op::ExtAdj) {
+                    if (win) HIPC(launch_win_tail(sl, nbr, r.cur, WIN_SLICE, f[1], f[0], 0, 0, 0, nullptr, p.nz, p.nx, s));
+                    HIPC(launch_ext_img(sl, nbr, win ? 0 : R, 0, base.v2dt2, p.nz, p.nx, nExt, s));
+                    HIPC(launch_ext_adj_finish(sl, nbr, c->refl.as<float>(), p.nz, p.nx, nExt, s));
                 }
-            } else if (q.op == Op::ExtAdj) {
-                if (win) HIPC(launch_win_tail(dSlots, nb, a.cur, WIN_SLICE, f[1], f[0], 0, 0, 0, nullptr, p.nz, p.nx, s));
-                HIPC(launch_ext_img(dSlots, nb, win ? 0 : R, 0, base.v2dt2, p.nz, p.nx, nExt, s));
-                HIPC(launch_ext_adj_finish(dSlots, nb, c->refl.as<float>(), p.nz, p.nx, nExt, s));
             }
+        }   // !lockstep
             if (needImg) {
                 for (int k = 0; k < nb; ++k) HIPC(launch_accumulate(acc, img + (size_t)k * nExt * nzx, nzx * nExt, s));
             }
@@ -748,10 +874,29 @@ aiso_status aiso_ctx_set_tile(aiso_ctx* c, int tz, int tx)
 {
     return guard([&] {
         if (!c) throw Error(AISO_ERR_PARAM, "null context");
-        if ((tz != 32 && tz != 64) || (tx != 8 && tx != 16 && tx != 32))
-            throw Error(AISO_ERR_PARAM, "tile must be tz in {32,64}, tx in {8,16,32}");
+        if ((tz != 32 && tz != 64) || (tx != 16 && tx != 32))
+            throw Error(AISO_ERR_PARAM, "tile must be tz in {32,64}, tx in {16,32}");
         c->tz = tz;
         c->tx = tx;
+    });
+}
+
+aiso_status aiso_ctx_set_option(aiso_ctx* c, const char* key, int value)
+{
+    return guard([&] {
+        if (!c || !key) throw Error(AISO_ERR_PARAM, "null argument");
+        std::string k(key);
+        if (k == "streamNT") c->streamNT = value != 0;
+        else if (k == "lockstep") c->lockstep = value != 0;
+        else if (k == "batchFwd") c->batchFwd = std::max(0, value);
+        else if (k == "batchAdj") c->batchAdj = std::max(0, value);
+        else if (k == "batchRec") c->batchRec = std::max(0, value);
+        else if (k == "shotsPerBlock") {
+            if (value != 1 && value != 2) throw Error(AISO_ERR_PARAM, "shotsPerBlock must be 1 or 2");
+            c->spb = value;
+        }
+        else if (k == "cacheBudgetMB") c->cacheBudget = (long)std::max(1, value) << 20;
+        else throw Error(AISO_ERR_PARAM, "unknown option " + k);
     });
 }
 

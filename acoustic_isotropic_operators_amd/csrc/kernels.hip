@@ -36,6 +36,18 @@ __device__ __forceinline__ int xcd_logical(int bid, int gridPadded)
     return (bid & 7) * per + (bid >> 3);
 }
 
+// Streams touched once per coarse step (W, SS, image) can bypass the caches
+// so they do not evict the wavefield slices reused at every fine step.
+__device__ __forceinline__ float ld_stream(const float* p, int nt)
+{
+    return nt ? __builtin_nontemporal_load(p) : *p;
+}
+__device__ __forceinline__ void st_stream(float* p, float v, int nt)
+{
+    if (nt) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
 __device__ __forceinline__ float sponge(const float* __restrict__ cd, int gz, int gx,
                                         int nz, int nx, int minPad)
 {
@@ -43,8 +55,488 @@ __device__ __forceinline__ float sponge(const float* __restrict__ cd, int gz, in
     return dist < minPad ? cd[dist] : 1.0f;
 }
 
-template <int TZ, int TX, unsigned MODE>
+// Global-memory views: pointers read from the ShotSlot table are generic to
+// the compiler; casting them lets it emit global (not flat) loads and stores
+// with a uniform 64-bit base and a 32-bit per-lane offset.
+#define GBL __attribute__((address_space(1)))
+using gf = GBL float;
+using cgf = const GBL float;
+__device__ __forceinline__ gf* G(float* p) { return (gf*)p; }
+__device__ __forceinline__ cgf* G(const float* p) { return (cgf*)p; }
+
+__device__ __forceinline__ float ldg_s(cgf* p, int nt) { return nt ? __builtin_nontemporal_load(p) : *p; }
+__device__ __forceinline__ void stg_s(gf* p, float v, int nt)
+{
+    if (nt) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <int TZ, int TX>
+struct TileShape {
+    static constexpr int NXW = NT / TZ, PX = TX / NXW, LZ = TZ + 2 * FAT, LX = TX + 2 * FAT;
+    static constexpr int KX = (LX + NXW - 1) / NXW, KZ = (LZ + TZ - 1) / TZ;
+    static_assert(PX >= 1 && PX * NXW == TX, "tile shape");
+};
+
+// Stage a (TZ+10) x (TX+10) window of src (times mul, when given) into LDS:
+// all loads first, then the stores.  With FULL the window lies inside the
+// grid and no bounds are tested.
+template <int TZ, int TX, bool FULL, bool MUL>
+__device__ __forceinline__ void stage_tile(float* sT, cgf* src, const float* sMul, unsigned soff, int nz,
+                                           int nx, int x0, int z0, int lane, int wy)
+{
+    using T = TileShape<TZ, TX>;
+    float tv[T::KX][T::KZ];
+#pragma unroll
+    for (int kx = 0; kx < T::KX; ++kx)
+#pragma unroll
+        for (int kz = 0; kz < T::KZ; ++kz) {
+            const int lx = wy + kx * T::NXW, lz = lane + kz * TZ;
+            bool in = (kx < T::KX - 1 || lx < T::LX) && (kz < T::KZ - 1 || lz < T::LZ);
+            if (!FULL) in = in && (x0 - FAT + lx < nx) && (z0 - FAT + lz < nz);
+            tv[kx][kz] = in ? src[soff + (unsigned)(kx * T::NXW * nz + kz * TZ)] : 0.f;
+        }
+#pragma unroll
+    for (int kx = 0; kx < T::KX; ++kx)
+#pragma unroll
+        for (int kz = 0; kz < T::KZ; ++kz) {
+            const int lx = wy + kx * T::NXW, lz = lane + kz * TZ;
+            if ((kx < T::KX - 1 || lx < T::LX) && (kz < T::KZ - 1 || lz < T::LZ)) {
+                if constexpr (MUL) sT[lx * T::LZ + lz] = tv[kx][kz] * sMul[lx * T::LZ + lz];
+                else sT[lx * T::LZ + lz] = tv[kx][kz];
+            }
+        }
+}
+
+// 10th-order Laplacian at the PX points of a thread, in the reference's
+// summation order (z terms, then x terms).  w holds the x-window.
+template <int TZ, int TX>
+__device__ __forceinline__ float lap_at(const float* sT, const float* w, int j, int lx0, int lane,
+                                        const StepArgs& a)
+{
+    using T = TileShape<TZ, TX>;
+    const float* s = &sT[(lx0 + j + FAT) * T::LZ + lane + FAT];
+    const float cc = w[j + FAT];
+    return a.zc[0] * cc
+         + a.zc[1] * (s[-1] + s[1])
+         + a.zc[2] * (s[-2] + s[2])
+         + a.zc[3] * (s[-3] + s[3])
+         + a.zc[4] * (s[-4] + s[4])
+         + a.zc[5] * (s[-5] + s[5])
+         + a.xc[0] * cc
+         + a.xc[1] * (w[j + 6] + w[j + 4])
+         + a.xc[2] * (w[j + 7] + w[j + 3])
+         + a.xc[3] * (w[j + 8] + w[j + 2])
+         + a.xc[4] * (w[j + 9] + w[j + 1])
+         + a.xc[5] * (w[j + 10] + w[j + 0]);
+}
+
+// Sponge factors of a thread's points; distZ is constant along the thread's
+// x-run.  FULL: every point is interior.
+template <int PX, bool FULL, bool SPONGE>
+__device__ __forceinline__ void tile_masks(bool (&ok)[PX], float (&dmp)[PX], const StepArgs& a, int gz, int gx0)
+{
+    const int nz = a.nz, nx = a.nx;
+    const int distZ = imin(gz - FAT, nz - 1 - FAT - gz);
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+        const int gx = gx0 + j;
+        ok[j] = FULL || (gz < nz - FAT && gx < nx - FAT);
+        float d = 1.f;
+        if (SPONGE && ok[j]) {
+            const int dist = imin(distZ, imin(gx - FAT, nx - 1 - FAT - gx));
+            if (dist < a.minPad) d = a.cd[dist];
+        }
+        dmp[j] = d;
+    }
+}
+
+template <int TZ, int TX, unsigned MODE, int SPB, bool FULL, bool SPONGE>
+__device__ __forceinline__ void step_block(const StepArgs& a, int group, int tile, int z0, int x0, float* sC,
+                                           float* sV, float* sN)
+{
+    using T = TileShape<TZ, TX>;
+    constexpr int PX = T::PX, NXW = T::NXW;
+    constexpr bool ADJ = MODE & M_ADJ;
+    constexpr bool PRE_SPARSE = MODE & (M_INJ_SIG | M_INJ_DATA);
+    constexpr bool POST_SPARSE = MODE & (M_REC_DATA | M_REC_SIG);
+    constexpr bool WIN = MODE & M_WIN;
+    constexpr bool SSB = MODE & M_SS_BORN, SSP = MODE & M_SS_PRE, SSS = MODE & M_SS_SLICE;
+    constexpr bool HAS_SS = SSB || SSP || SSS;
+
+    const int nz = a.nz, nx = a.nx;
+    const int lane = threadIdx.x % TZ, wy = threadIdx.x / TZ;
+    const int gz = z0 + lane;
+    const int lx0 = wy * PX;
+    const unsigned off0 = (unsigned)((x0 + lx0) * nz + gz);                 // thread's first point
+    const unsigned soff = (unsigned)((x0 - FAT + wy) * nz + (z0 - FAT + lane));  // first staged element
+    const bool useB = a.fb != 0.f;
+    cgf* V = G(a.v2dt2);
+
+    bool ok[PX];
+    float dmp[PX], vc[ADJ ? 1 : PX];
+    tile_masks<PX, FULL, SPONGE>(ok, dmp, a, gz, x0 + lx0);
+    if constexpr (!ADJ) {
+#pragma unroll
+        for (int j = 0; j < PX; ++j) vc[j] = ok[j] ? V[off0 + j * nz] : 0.f;
+    } else {
+        // v2dt2 window shared by the shots; each thread reads back only the
+        // entries it wrote itself.
+        stage_tile<TZ, TX, FULL, false>(sV, V, nullptr, soff, nz, nx, x0, z0, lane, wy);
+    }
+
+    for (int si = 0; si < SPB; ++si) {
+        const int shot = group * SPB + si;
+        if (shot >= a.nshots) break;                  // block-uniform
+        const ShotSlot& S = a.slots[shot];
+        cgf* pc = G(S.p[a.cur]);
+        gf* po = G(S.p[a.cur ^ 1]);
+
+        // Per-point operands first, so their latency overlaps the staging.
+        float os[PX], pv[ADJ ? PX : 1];
+        float sA[HAS_SS ? PX : 1], sB[HAS_SS ? PX : 1], mm[SSB ? PX : 1];
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            const unsigned o = off0 + j * nz;
+            os[j] = ok[j] ? po[o] : 0.f;
+            if constexpr (ADJ) pv[j] = ok[j] ? pc[o] : 0.f;
+            if (ok[j]) {
+                if constexpr (SSB) {
+                    mm[j] = G(a.mtil)[o];
+                    sA[j] = G(S.W + a.wA)[o];
+                    sB[j] = useB ? G(S.W + a.wB)[o] : 0.f;
+                }
+                if constexpr (SSP) {
+                    sA[j] = ldg_s(G(S.W + a.wA) + o, a.ntW);
+                    sB[j] = useB ? ldg_s(G(S.W + a.wB) + o, a.ntW) : 0.f;
+                }
+                if constexpr (SSS) {
+                    sA[j] = G(S.ss[a.ssA])[o];
+                    sB[j] = useB ? G(S.ss[a.ssB])[o] : 0.f;
+                }
+            }
+        }
+        stage_tile<TZ, TX, FULL, ADJ>(sC, pc, sV, soff, nz, nx, x0, z0, lane, wy);
+        __syncthreads();
+
+        float w[PX + 2 * FAT];
+#pragma unroll
+        for (int j = 0; j < PX + 2 * FAT; ++j) w[j] = sC[(lx0 + j) * T::LZ + lane + FAT];
+
+        float nv[PX];
+        float cur[WIN ? PX : 1];
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            const float lap = lap_at<TZ, TX>(sC, w, j, lx0, lane, a);
+            const float cc = w[j + FAT];
+            float n = 0.f;
+            if (ok[j]) {
+                const float o = SPONGE ? os[j] * dmp[j] : os[j];   // second damping of the old slice
+                if constexpr (ADJ) {
+                    const float c = pv[j];
+                    n = lap + c + c - o;
+                    if constexpr (WIN) cur[j] = c;
+                } else {
+                    n = vc[j] * lap + cc + cc - o;
+                    if constexpr (WIN) cur[j] = cc;
+                }
+                if constexpr (SSB) {
+                    const float ssa = mm[j] * sA[j];
+                    if (useB) n += ssa * a.fa + (mm[j] * sB[j]) * a.fb;
+                    else n += ssa * a.fa;
+                }
+                if constexpr (SSP || SSS) {
+                    if (useB) n += sA[j] * a.fa + sB[j] * a.fb;
+                    else n += sA[j] * a.fa;
+                }
+            }
+            nv[j] = n;
+        }
+
+        if constexpr (PRE_SPARSE) {
+            const int* tp = (MODE & M_INJ_SIG) ? S.srcTile : S.recTile;
+            const int2* te = (MODE & M_INJ_SIG) ? S.srcEnt : S.recEnt;
+            const int e0 = tp[tile], e1 = tp[tile + 1];
+            if (e1 > e0) {                               // block-uniform
+#pragma unroll
+                for (int j = 0; j < PX; ++j) sN[(lx0 + j) * TZ + lane] = nv[j];
+                __syncthreads();
+                for (int e = e0 + (int)threadIdx.x; e < e1; e += NT) {
+                    const int2 en = te[e];
+                    float v;
+                    if constexpr (MODE & M_INJ_SIG) {
+                        v = G(S.sig)[(long)en.y * a.ntw + a.itwInj];
+                    } else {
+                        cgf* dr = G(S.dreg) + ((long)en.y * a.nts + a.its);
+                        v = dr[0] * a.fc + dr[1] * a.fd;
+                    }
+                    sN[en.x] += v;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < PX; ++j) nv[j] = sN[(lx0 + j) * TZ + lane];
+                __syncthreads();
+            }
+        }
+
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            if (!ok[j]) continue;
+            const unsigned gi = off0 + j * nz;
+            const float n = SPONGE ? nv[j] * dmp[j] : nv[j];
+            nv[j] = n;
+            if constexpr (MODE & M_CAPTURE) {
+                // Forward passes capture the coarse source wavefield W; the
+                // adjoint receiver pass accumulates into the two ss slices.
+                gf* A = ADJ ? G(S.ss[a.ssA]) : G(S.W + a.wA);
+                gf* B = ADJ ? G(S.ss[a.ssB]) : G(S.W + a.wB);
+                // A zero weight leaves the slice as it is (x + n*0 == x); a
+                // first touch writes instead of accumulating into a zeroed slice.
+                if (!a.skipA) A[gi] = a.assignA ? n * a.fe : A[gi] + n * a.fe;
+                else if (a.assignA) A[gi] = 0.f;
+                float bnew = 0.f;
+                if (!a.skipB) {
+                    bnew = a.assignB ? n * a.ff : B[gi] + n * a.ff;
+                    B[gi] = bnew;
+                } else if (a.assignB) {
+                    B[gi] = 0.f;
+                } else if constexpr (MODE & M_IMAGE) {
+                    bnew = B[gi];
+                }
+                if constexpr (MODE & M_IMAGE) {
+                    if (a.image) G(S.img)[gi] += G(S.W + a.wImg)[gi] * bnew;
+                }
+            }
+            if constexpr (WIN) {
+                if (a.win) {
+                    // Coarse sample from three consecutive fine slices, accumulated
+                    // in the reference's order (old, current, new).
+                    const float v = (os[j] * a.wa + cur[j] * a.wb) + n * a.wc;
+                    switch (a.winOut) {
+                    case WIN_W: stg_s(G(S.W + a.wA) + gi, v, a.ntW); break;
+                    case WIN_SS: stg_s(G(S.W + a.wA) + gi, G(a.mtil)[gi] * v, a.ntW); break;
+                    case WIN_IMG: G(S.img)[gi] += ldg_s(G(S.W + a.wImg) + gi, a.ntW) * v; break;
+                    default: G(S.ss[a.ssA])[gi] = v; break;
+                    }
+                }
+            }
+            po[gi] = n;
+        }
+
+        if constexpr (POST_SPARSE) {
+            const int* tp = (MODE & M_REC_SIG) ? S.srcTile : S.recTile;
+            const int2* te = (MODE & M_REC_SIG) ? S.srcEnt : S.recEnt;
+            const int e0 = tp[tile], e1 = tp[tile + 1];
+            if (e1 > e0) {                               // block-uniform
+#pragma unroll
+                for (int j = 0; j < PX; ++j) sN[(lx0 + j) * TZ + lane] = nv[j];
+                __syncthreads();
+                for (int e = e0 + (int)threadIdx.x; e < e1; e += NT) {
+                    const int2 en = te[e];
+                    const float n = sN[en.x];
+                    if constexpr (MODE & M_REC_SIG) {
+                        G(S.sig)[(long)en.y * a.ntw + a.itwRec] += n;
+                    } else {
+                        gf* dr = G(S.dreg) + ((long)en.y * a.nts + a.its);
+                        dr[0] += n * a.fe;
+                        dr[1] += n * a.ff;
+                    }
+                }
+            }
+        }
+        if (SPB > 1) __syncthreads();                 // sC / sN are reused by the next shot
+    }
+}
+
+// Tile position, interior / sponge classification of the workgroup.
+struct TileInfo {
+    int group, tile, z0, x0;
+    bool full, sponge;
+};
+
+template <int TZ, int TX>
+__device__ __forceinline__ bool tile_info(const StepArgs& a, TileInfo& t)
+{
+    const int logical = xcd_logical(blockIdx.x, gridDim.x);
+    if (logical >= a.gridLogical) return false;      // padding blocks (block-uniform)
+    t.group = logical / a.ntiles;
+    t.tile = logical - t.group * a.ntiles;
+    const int tz = t.tile % a.tilesZ, tx = t.tile / a.tilesZ;
+    t.z0 = FAT + tz * TZ;
+    t.x0 = FAT + tx * TX;
+    t.full = t.z0 + TZ <= a.nz - FAT && t.x0 + TX <= a.nx - FAT;
+    const int edge = imin(imin(t.z0 - FAT, a.nz - FAT - (t.z0 + TZ)), imin(t.x0 - FAT, a.nx - FAT - (t.x0 + TX)));
+    t.sponge = !t.full || edge < a.minPad;
+    return true;
+}
+
+template <int TZ, int TX, unsigned MODE, int SPB>
 __global__ __launch_bounds__(NT) void k_step(const StepArgs a)
+{
+    using T = TileShape<TZ, TX>;
+    constexpr bool ADJ = MODE & M_ADJ;
+    constexpr bool SPARSE = MODE & (M_INJ_SIG | M_INJ_DATA | M_REC_DATA | M_REC_SIG);
+    __shared__ float sC[T::LX * T::LZ];
+    __shared__ float sV[ADJ ? T::LX * T::LZ : 1];   // v2dt2 window (adjoint), shared by the shots
+    __shared__ float sN[SPARSE ? TX * TZ : 1];
+    TileInfo t;
+    if (!tile_info<TZ, TX>(a, t)) return;
+    if (t.full && !t.sponge) step_block<TZ, TX, MODE, SPB, true, false>(a, t.group, t.tile, t.z0, t.x0, sC, sV, sN);
+    else if (t.full) step_block<TZ, TX, MODE, SPB, true, true>(a, t.group, t.tile, t.z0, t.x0, sC, sV, sN);
+    else step_block<TZ, TX, MODE, SPB, false, true>(a, t.group, t.tile, t.z0, t.x0, sC, sV, sN);
+}
+
+// Lockstep Born forward, sub == 2.  One launch advances the source field
+// (pair S.p) by fine step t and the scattered field (pair S.q) by fine step
+// t-2.  The source step writes the secondary source SS[k] = m~ * W[k] of the
+// coarse sample it completes into a 3-slice ring (S.W); the scattered step
+// reads the two ring slices of its coarse interval, which earlier launches
+// completed.  Velocity, sponge factor and tile bookkeeping are shared by the
+// two fields; the arithmetic per field is that of k_step (MODE_SRC_WIN with
+// WIN_SS, and MODE_BORN_SCAT_PRE).
+template <int TZ, int TX, bool FULL, bool SPONGE>
+__device__ __forceinline__ void pair_block(const StepArgs& a, int shot, int tile, int z0, int x0, float* sA, float* sB,
+                                           float* sN)
+{
+    using T = TileShape<TZ, TX>;
+    constexpr int PX = T::PX;
+    const int nz = a.nz, nx = a.nx;
+    const int lane = threadIdx.x % TZ, wy = threadIdx.x / TZ;
+    const int gz = z0 + lane;
+    const int lx0 = wy * PX;
+    const unsigned off0 = (unsigned)((x0 + lx0) * nz + gz);
+    const unsigned soff = (unsigned)((x0 - FAT + wy) * nz + (z0 - FAT + lane));
+    const bool useB = a.fb != 0.f;
+    const ShotSlot& S = a.slots[shot];
+    cgf* pcA = G(S.p[a.cur]);
+    gf* poA = G(S.p[a.cur ^ 1]);
+    cgf* pcB = G(S.q[a.curQ]);
+    gf* poB = G(S.q[a.curQ ^ 1]);
+    cgf* V = G(a.v2dt2);
+    cgf* Qa = G(S.W + a.wQa);
+    cgf* Qb = G(S.W + a.wQb);
+
+    bool ok[PX];
+    float dmp[PX];
+    tile_masks<PX, FULL, SPONGE>(ok, dmp, a, gz, x0 + lx0);
+    float oa[PX], ob[PX], v2[PX], qa[PX], qb[PX], mm[PX];
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+        const unsigned o = off0 + j * nz;
+        float o1 = 0.f, o2 = 0.f, v = 0.f, s1 = 0.f, s2 = 0.f, m = 0.f;
+        if (ok[j]) {
+            o1 = poA[o];
+            o2 = poB[o];
+            v = V[o];
+            s1 = ldg_s(Qa + o, a.ntW);
+            if (useB) s2 = ldg_s(Qb + o, a.ntW);
+            if (a.win) m = G(a.mtil)[o];
+        }
+        oa[j] = o1;
+        ob[j] = o2;
+        v2[j] = v;
+        qa[j] = s1;
+        qb[j] = s2;
+        mm[j] = m;
+    }
+    stage_tile<TZ, TX, FULL, false>(sA, pcA, nullptr, soff, nz, nx, x0, z0, lane, wy);
+    stage_tile<TZ, TX, FULL, false>(sB, pcB, nullptr, soff, nz, nx, x0, z0, lane, wy);
+    __syncthreads();
+
+    float wA[PX + 2 * FAT], wB[PX + 2 * FAT];
+#pragma unroll
+    for (int j = 0; j < PX + 2 * FAT; ++j) {
+        wA[j] = sA[(lx0 + j) * T::LZ + lane + FAT];
+        wB[j] = sB[(lx0 + j) * T::LZ + lane + FAT];
+    }
+    float nA[PX], nB[PX];
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+        const float lapA = lap_at<TZ, TX>(sA, wA, j, lx0, lane, a);
+        const float lapB = lap_at<TZ, TX>(sB, wB, j, lx0, lane, a);
+        const float ca = wA[j + FAT], cb = wB[j + FAT];
+        float na = 0.f, nb = 0.f;
+        if (ok[j]) {
+            na = v2[j] * lapA + ca + ca - (SPONGE ? oa[j] * dmp[j] : oa[j]);
+            nb = v2[j] * lapB + cb + cb - (SPONGE ? ob[j] * dmp[j] : ob[j]);
+            if (useB) nb += qa[j] * a.fa + qb[j] * a.fb;
+            else nb += qa[j] * a.fa;
+        }
+        nA[j] = na;
+        nB[j] = nb;
+    }
+
+    // source injection into the source field (before damping)
+    {
+        const int e0 = S.srcTile[tile], e1 = S.srcTile[tile + 1];
+        if (e1 > e0) {
+#pragma unroll
+            for (int j = 0; j < PX; ++j) sN[(lx0 + j) * TZ + lane] = nA[j];
+            __syncthreads();
+            for (int e = e0 + (int)threadIdx.x; e < e1; e += NT) {
+                const int2 en = S.srcEnt[e];
+                sN[en.x] += G(S.sig)[(long)en.y * a.ntw + a.itwInj];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < PX; ++j) nA[j] = sN[(lx0 + j) * TZ + lane];
+            __syncthreads();
+        }
+    }
+
+    gf* Wd = G(S.W + a.wA);
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+        if (!ok[j]) continue;
+        const unsigned gi = off0 + j * nz;
+        const float na = SPONGE ? nA[j] * dmp[j] : nA[j];
+        const float nb = SPONGE ? nB[j] * dmp[j] : nB[j];
+        nB[j] = nb;
+        if (a.win) {
+            const float v = (oa[j] * a.wa + wA[j + FAT] * a.wb) + na * a.wc;
+            stg_s(Wd + gi, mm[j] * v, a.ntW);
+        }
+        poA[gi] = na;
+        poB[gi] = nb;
+    }
+
+    // data record from the scattered field (after damping)
+    {
+        const int e0 = S.recTile[tile], e1 = S.recTile[tile + 1];
+        if (e1 > e0) {
+#pragma unroll
+            for (int j = 0; j < PX; ++j) sN[(lx0 + j) * TZ + lane] = nB[j];
+            __syncthreads();
+            for (int e = e0 + (int)threadIdx.x; e < e1; e += NT) {
+                const int2 en = S.recEnt[e];
+                const float n = sN[en.x];
+                gf* dr = G(S.dreg) + ((long)en.y * a.nts + a.its);
+                dr[0] += n * a.fe;
+                dr[1] += n * a.ff;
+            }
+        }
+    }
+}
+
+template <int TZ, int TX>
+__global__ __launch_bounds__(NT) void k_fwd_pair(const StepArgs a)
+{
+    using T = TileShape<TZ, TX>;
+    __shared__ float sA[T::LX * T::LZ];
+    __shared__ float sB[T::LX * T::LZ];
+    __shared__ float sN[TX * TZ];
+    TileInfo t;
+    if (!tile_info<TZ, TX>(a, t)) return;
+    if (t.full && !t.sponge) pair_block<TZ, TX, true, false>(a, t.group, t.tile, t.z0, t.x0, sA, sB, sN);
+    else if (t.full) pair_block<TZ, TX, true, true>(a, t.group, t.tile, t.z0, t.x0, sA, sB, sN);
+    else pair_block<TZ, TX, false, true>(a, t.group, t.tile, t.z0, t.x0, sA, sB, sN);
+}
+
+// Unspecialized form of the fused step: one shot per workgroup, every
+// operand addressed through generic pointers, bounds and sponge tested at
+// every point.  The extended-Born operators run on it.
+template <int TZ, int TX, unsigned MODE>
+__global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
 {
     constexpr bool ADJ = MODE & M_ADJ;
     constexpr bool PRE_SPARSE = MODE & (M_INJ_SIG | M_INJ_DATA);
@@ -465,41 +957,88 @@ __global__ void k_win_tail(const ShotSlot* slots, int cur, int winOut, float wa,
 
 using StepKernel = void (*)(const StepArgs);
 
-template <unsigned MODE>
-static StepKernel pick_tile(int tz, int tx)
+template <unsigned MODE, int SPB>
+static StepKernel pick_tile_spb(int tz, int tx)
 {
-    if (tz == 64) {
-        if (tx == 8) return k_step<64, 8, MODE>;
-        if (tx == 16) return k_step<64, 16, MODE>;
-        if (tx == 32) return k_step<64, 32, MODE>;
-    } else if (tz == 32) {
-        if (tx == 8) return k_step<32, 8, MODE>;
-        if (tx == 16) return k_step<32, 16, MODE>;
-        if (tx == 32) return k_step<32, 32, MODE>;
-    }
+    if (tz == 64 && tx == 16) return k_step<64, 16, MODE, SPB>;
+    if (tz == 64 && tx == 32) return k_step<64, 32, MODE, SPB>;
+    if (tz == 32 && tx == 16) return k_step<32, 16, MODE, SPB>;
+    if (tz == 32 && tx == 32) return k_step<32, 32, MODE, SPB>;
     return nullptr;
 }
 
-static StepKernel pick_step(unsigned mode, int tz, int tx)
+template <unsigned MODE>
+static StepKernel pick_tile(int tz, int tx, int spb)
+{
+    if (spb == 2) return pick_tile_spb<MODE, 2>(tz, tx);
+    return pick_tile_spb<MODE, 1>(tz, tx);
+}
+
+static StepKernel pick_step(unsigned mode, int tz, int tx, int spb)
 {
     switch (mode) {
-    case MODE_NL_FWD: return pick_tile<MODE_NL_FWD>(tz, tx);
-    case MODE_NL_ADJ: return pick_tile<MODE_NL_ADJ>(tz, tx);
-    case MODE_SRC: return pick_tile<MODE_SRC>(tz, tx);
-    case MODE_BORN_SCAT: return pick_tile<MODE_BORN_SCAT>(tz, tx);
-    case MODE_BORN_REC: return pick_tile<MODE_BORN_REC>(tz, tx);
-    case MODE_EXT_SCAT: return pick_tile<MODE_EXT_SCAT>(tz, tx);
-    case MODE_EXT_REC: return pick_tile<MODE_EXT_REC>(tz, tx);
-    case MODE_SRC_WIN: return pick_tile<MODE_SRC_WIN>(tz, tx);
-    case MODE_BORN_SCAT_PRE: return pick_tile<MODE_BORN_SCAT_PRE>(tz, tx);
-    case MODE_REC_WIN: return pick_tile<MODE_REC_WIN>(tz, tx);
+    case MODE_NL_FWD: return pick_tile<MODE_NL_FWD>(tz, tx, spb);
+    case MODE_NL_ADJ: return pick_tile<MODE_NL_ADJ>(tz, tx, spb);
+    case MODE_SRC: return pick_tile<MODE_SRC>(tz, tx, spb);
+    case MODE_BORN_SCAT: return pick_tile<MODE_BORN_SCAT>(tz, tx, spb);
+    case MODE_BORN_REC: return pick_tile<MODE_BORN_REC>(tz, tx, spb);
+    case MODE_EXT_SCAT: return pick_tile<MODE_EXT_SCAT>(tz, tx, spb);
+    case MODE_EXT_REC: return pick_tile<MODE_EXT_REC>(tz, tx, spb);
+    case MODE_SRC_WIN: return pick_tile<MODE_SRC_WIN>(tz, tx, spb);
+    case MODE_BORN_SCAT_PRE: return pick_tile<MODE_BORN_SCAT_PRE>(tz, tx, spb);
+    case MODE_REC_WIN: return pick_tile<MODE_REC_WIN>(tz, tx, spb);
     default: return nullptr;
     }
 }
 
-hipError_t launch_step(unsigned mode, int tz, int tx, const StepArgs& a, hipStream_t s)
+hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a, hipStream_t s)
 {
-    StepKernel k = pick_step(mode, tz, tx);
+    StepKernel k = pick_step(mode, tz, tx, spb);
+    if (!k) return hipErrorInvalidValue;
+    // One workgroup per (tile, group of spb shots).
+    StepArgs b = a;
+    b.gridLogical = a.ntiles * ((a.nshots + spb - 1) / spb);
+    const int grid = (b.gridLogical + 7) / 8 * 8;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, b);
+    return hipGetLastError();
+}
+
+template <unsigned MODE>
+static StepKernel pick_generic_tile(int tz, int tx)
+{
+    if (tz == 64 && tx == 16) return k_step_generic<64, 16, MODE>;
+    if (tz == 64 && tx == 32) return k_step_generic<64, 32, MODE>;
+    if (tz == 32 && tx == 16) return k_step_generic<32, 16, MODE>;
+    if (tz == 32 && tx == 32) return k_step_generic<32, 32, MODE>;
+    return nullptr;
+}
+
+hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a, hipStream_t s)
+{
+    StepKernel k = nullptr;
+    switch (mode) {
+    case MODE_SRC: k = pick_generic_tile<MODE_SRC>(tz, tx); break;
+    case MODE_SRC_WIN: k = pick_generic_tile<MODE_SRC_WIN>(tz, tx); break;
+    case MODE_EXT_SCAT: k = pick_generic_tile<MODE_EXT_SCAT>(tz, tx); break;
+    case MODE_EXT_REC: k = pick_generic_tile<MODE_EXT_REC>(tz, tx); break;
+    case MODE_REC_WIN: k = pick_generic_tile<MODE_REC_WIN>(tz, tx); break;
+    default: break;
+    }
+    if (!k) return hipErrorInvalidValue;
+    StepArgs b = a;
+    b.gridLogical = a.ntiles * a.nshots;               // one workgroup per (tile, shot)
+    const int grid = (b.gridLogical + 7) / 8 * 8;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_fwd_pair(int tz, int tx, const StepArgs& a, hipStream_t s)
+{
+    StepKernel k = nullptr;
+    if (tz == 64 && tx == 16) k = k_fwd_pair<64, 16>;
+    else if (tz == 64 && tx == 32) k = k_fwd_pair<64, 32>;
+    else if (tz == 32 && tx == 16) k = k_fwd_pair<32, 16>;
+    else if (tz == 32 && tx == 32) k = k_fwd_pair<32, 32>;
     if (!k) return hipErrorInvalidValue;
     const int grid = (a.gridLogical + 7) / 8 * 8;
     hipLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, a);

@@ -11,6 +11,7 @@ namespace aiso {
 // step kernel reads or updates.  Unused members are null.
 struct ShotSlot {
     float* p[2];          // wavefield pair [nx][nz]; the "current" one is picked by parity
+    float* q[2];          // second pair (scattered field of the lockstep Born forward)
     float* W;             // coarse source wavefield [nts][nx][nz]
     float* ss[2];         // two coarse-rate accumulators / secondary sources [nx][nz]
     float* img;           // per-shot image [nExt][nx][nz]
@@ -69,6 +70,11 @@ struct StepArgs {
     long wImg;              // element offset of the W slice used by M_IMAGE / WIN_IMG
     int win, winOut;        // M_WIN: capture at this step, and its destination
     float wa, wb, wc;       // M_WIN weights of the old, current and new slices
+    int ntW;                // non-temporal accesses to the W / SS stream
+    // lockstep Born forward (k_fwd_pair): scattered-field parity and the
+    // secondary-source slices it reads
+    int curQ;
+    long wQa, wQb;
 };
 
 }  // namespace aiso
@@ -88,7 +94,11 @@ constexpr unsigned MODE_SRC_WIN = M_INJ_SIG | M_WIN;
 constexpr unsigned MODE_BORN_SCAT_PRE = M_SS_PRE | M_REC_DATA;
 constexpr unsigned MODE_REC_WIN = M_ADJ | M_INJ_DATA | M_WIN;
 
-hipError_t launch_step(unsigned mode, int tz, int tx, const StepArgs& a, hipStream_t s);
+// spb: shots advanced per workgroup (1 or 2), sharing the tile's v2dt2 and sponge.
+hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a, hipStream_t s);
+// The unspecialized step kernel (modes of the extended-Born operators:
+// MODE_SRC, MODE_SRC_WIN, MODE_EXT_SCAT, MODE_EXT_REC, MODE_REC_WIN).
+hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a, hipStream_t s);
 hipError_t launch_ring(unsigned mode, const StepArgs& a, int maxRing, hipStream_t s);
 hipError_t launch_fill(float* p, long n, hipStream_t s);
 hipError_t launch_scale_model(const float* m, float* out, const float* v2dt2, const float* refl,
@@ -107,6 +117,9 @@ hipError_t launch_accumulate(float* acc, const float* img, long n, hipStream_t s
 hipError_t launch_add_to(float* out, const float* acc, long n, int add, hipStream_t s);
 hipError_t launch_dev_forward(const float* reg, float* out, const int* cornerReg, const float* weight,
                               int nDev, int nb, int nt, int add, hipStream_t s);
+// One fine step of the source field and, two fine steps behind, the
+// scattered field of the Born forward in one launch (sub == 2).
+hipError_t launch_fwd_pair(int tz, int tx, const StepArgs& a, hipStream_t s);
 hipError_t launch_win_tail(const ShotSlot* slots, int nshots, int cur, int winOut, float wa, float wb,
                            long wA, long wImg, int ssA, const float* mtil, int nz, int nx, hipStream_t s);
 hipError_t launch_dev_adjoint(float* reg, const float* irreg, const int* adjPtr, const int* adjEnt,

@@ -278,7 +278,8 @@ struct ShotsBase {
         // per launch and the step-kernel tile.
         int batch = par.i("shotsPerLaunch", 0);
         if (batch > 0) check(aiso_ctx_set_batch(ctx, batch));
-        check(aiso_ctx_set_tile(ctx, par.i("tileZ", 64), par.i("tileX", 32)));
+        check(aiso_ctx_set_tile(ctx, par.i("tileZ", 32), par.i("tileX", 32)));
+        check(aiso_ctx_set_option(ctx, "lockstep", par.i("lockstep", 1)));
         fp = q;
         dirty = false;
     }

@@ -29,7 +29,9 @@ HBM_PEAK_GBS = 8000.0
 # coarse-rate traffic of that pass spread over its `sub` fine steps:
 # source pass writes W (4 B), scattered pass reads W and m~ (8 B), receiver
 # pass reads W and updates the image (12 B).
-CO_BYTES = {"K_SRC": 4.0, "K_BORN_SCAT": 8.0, "K_BORN_REC": 12.0}
+# The lockstep forward advances both forward fields per launch: its
+# point-updates count both, with (4 + 8) / 2 B of coarse traffic each.
+CO_BYTES = {"K_SRC": 4.0, "K_BORN_SCAT": 8.0, "K_BORN_REC": 12.0, "K_BORN_PAIR": 6.0}
 
 
 def parse():
@@ -39,9 +41,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--shots", type=int, default=80)
     ap.add_argument("--batch", type=int, default=0, help="shots per launch (0 = by HBM budget)")
-    ap.add_argument("--tile", type=str, default="64,32")
+    ap.add_argument("--tile", type=str, default="32,32")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-shots", type=int, default=1)
+    ap.add_argument("--opt", action="append", default=[], help="engine option key=value (aiso_ctx_set_option)")
     return ap.parse_args()
 
 
@@ -83,6 +86,9 @@ def main():
     ctx.set_tile(tz, tx)
     if a.batch:
         ctx.set_batch(a.batch)
+    for kv in a.opt:
+        k, v = kv.split("=")
+        ctx.set_option(k, int(v))
     srcs = [A.Device.grid(1, iz, 1, 1, ix, 1, p["nz"], p["nx"], p["nts"]) for iz, ix in w.src]
     rec = [A.Device.grid(*w.rec, p["nz"], p["nx"], p["nts"])]
     nrec = rec[0].nDev
@@ -109,6 +115,8 @@ def main():
 
     kinds = {name: getattr(A, name) for name in CO_BYTES}
     stats = {name: ctx.profile_read(k) for name, k in kinds.items()}
+    kern = {name: {"launches": v["launches"], "ms_per_step": v["ms"] / a.steps,
+                   "avg_us": 1e3 * v["ms"] / max(v["launches"], 1)} for name, v in stats.items() if v["launches"]}
     dom = max(stats, key=lambda n: stats[n]["ms"])
     st = stats[dom]
     bpp = 16.0 + CO_BYTES[dom] / p["sub"]
@@ -140,7 +148,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": w.name, "shots": w.nShot, "grid": "500x1200 (padded %dx%d)" % (p["nz"], p["nx"]),
-                   "nts": p["nts"], "sub": p["sub"], "shots_per_launch": info["batch"], "tile": [tz, tx],
+                   "nts": p["nts"], "sub": p["sub"], "shots_per_launch_adj": info["batch"], "tile": [tz, tx],
                    "inputs": "model/data/image resident in HBM"},
         "job_roofline_frac": job_bytes / t / 1e9 / HBM_PEAK_GBS,
         "shot_latency_ms": {"p50": float(np.percentile(lat, 50)) if lat else None,
@@ -148,6 +156,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms,
                      "launches": st["launches"], "bytes_per_point": bpp},
+        "kernels": kern,
         "finite": ok,
     }
     if not a.no_cpu:

@@ -88,8 +88,17 @@ aiso_status aiso_ctx_set_velocity(aiso_ctx* ctx, const float* vel, int mem);
 
 /* Shots advanced together by one launch (0 = automatic, by free HBM). */
 aiso_status aiso_ctx_set_batch(aiso_ctx* ctx, int shotsPerLaunch);
-/* Tile shape of the time-step kernel: tz in {32,64}, tx in {8,16,32}. */
+/* Tile shape of the time-step kernel: tz in {32,64}, tx in {16,32}. */
 aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
+/* Engine tuning knobs by name (no counterpart in the reference):
+ *   "streamNT" 0/1: non-temporal access to the coarse wavefield stream;
+ *   "lockstep" 0/1: Born forward with sub == 2 as one lockstep pass;
+ *   "batchFwd", "batchAdj": shots per launch of forward / adjoint calls
+ *       (0 = aiso_ctx_set_batch's value, or automatic);
+ *   "batchRec": group size of the adjoint receiver pass within a batch;
+ *   "shotsPerBlock" 1/2: shots advanced by one workgroup of the step kernel;
+ *   "cacheBudgetMB": on-die cache budget the automatic batch size fills. */
+aiso_status aiso_ctx_set_option(aiso_ctx* ctx, const char* key, int value);
 /* Derived sizes: ntw, sub, nShotsPerLaunch actually used by the last call. */
 aiso_status aiso_ctx_info(const aiso_ctx* ctx, int* ntw, int* batchUsed, size_t* bytesPerShot);
 /* The HIP stream (hipStream_t) every kernel of this context runs on. */
@@ -104,7 +113,8 @@ enum {
     AISO_K_BORN_REC = 4,   /* Born adjoint receiver step with imaging        */
     AISO_K_EXT_SCAT = 5,   /* extended-Born scattered step                   */
     AISO_K_EXT_REC = 6,    /* extended-Born adjoint receiver step            */
-    AISO_K_COUNT = 7
+    AISO_K_BORN_PAIR = 7,  /* lockstep Born forward: source + scattered step */
+    AISO_K_COUNT = 8
 };
 /* Per-launch timing of the time-step kernels with HIP events on that stream
  * (enable != 0), reset of the counters. */
