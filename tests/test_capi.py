@@ -1,0 +1,67 @@
+"""The C-ABI library: it loads without a GPU, exports every function that
+include/aiso.h declares, and its host-only entry points (padding rule,
+geometry, error reporting) behave as the reference's.  CPU only."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from acoustic_isotropic_operators_amd import _capi, lib_path
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "aiso.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(aiso_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_operator_api():
+    names = declared_functions()
+    for n in ("aiso_ctx_create", "aiso_born_forward", "aiso_born_adjoint", "aiso_nl_forward", "aiso_nl_adjoint",
+              "aiso_bornext_forward", "aiso_bornext_adjoint", "aiso_device_create_grid", "aiso_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(lib_path())
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_ctypes_binding_covers_the_header():
+    assert sorted(_capi.exported_symbols()) == declared_functions()
+
+
+@pytest.mark.parametrize("n,pad,expect", [(500, 50, 58), (1200, 50, 62), (200, 50, 54), (400, 50, 62), (304, 50, 62)])
+def test_pad_plus(n, pad, expect):
+    """padPlus = ceil((2*pad + n) / 16) * 16 - n - pad (F/src/padFileGpuMain.cpp:49-66):
+    the padded interior n + pad + padPlus is the smallest multiple of 16 that
+    holds n + 2*pad."""
+    got = _capi.lib().aiso_pad_plus(n, pad, 16)
+    assert got == expect
+    assert (n + pad + got) % 16 == 0 and pad <= got < pad + 16
+
+
+def test_geometry_out_of_bounds_is_an_error_not_an_exit():
+    with pytest.raises(RuntimeError, match="out of bounds"):
+        _capi.Device.grid(1, 30, 1, 1, 8, 1, 26, 42, 8)
+    msg = _capi.lib().aiso_last_error().decode()
+    assert "out of bounds" in msg
+
+
+def test_device_grid_sizes_and_positions():
+    d = _capi.Device.grid(1, 57, 1, 4, 55, 1, 618, 1322, 10)
+    assert d.nDev == 4 and d.nReg == 10
+    b = 55 * 618 + 57
+    assert list(d.reg_pos()[:4]) == [b, b + 1, b + 618, b + 619]
+
+
+def test_ctx_create_rejects_bad_parameters_on_the_host():
+    """fdParam's checks run before any device work: fat != 5 is refused."""
+    with pytest.raises(RuntimeError):
+        _capi.Context(dict(nz=58, nx=74, dz=10.0, dx=10.0, nts=60, dts=0.004, sub=4, zPadMinus=10, zPadPlus=10,
+                           xPadMinus=10, xPadPlus=10, fat=4))
