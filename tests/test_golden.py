@@ -1,0 +1,90 @@
+"""Geometry of the two benchmark configurations against the committed
+fixtures (tests/golden/make_golden.py, restated from the reference's rules
+with numpy scalars): the engine's host code, the oracle and the workload
+generator must all give the same integers and float weights, bit for bit."""
+import os
+
+import numpy as np
+import pytest
+
+from acoustic_isotropic_operators_amd import workloads
+from acoustic_isotropic_operators_amd.sep import FloatVector, hypercube
+from oracle import oracle as O
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CONFIGS = {
+    "nonlinear_1x200x400": dict(nz=200, nx=400, rec=(1, 57, 1, 400, 55, 1)),
+    "born_lsrtm_80x500x1200": dict(nz=500, nx=1200, rec=(1, 57, 1, 1200, 55, 1)),
+}
+
+
+@pytest.fixture(scope="module")
+def nl():
+    import pyAcoustic_iso_float_nl as m
+    return m
+
+
+def _load(name):
+    return dict(np.load(os.path.join(HERE, f"geometry_{name}.npz")))
+
+
+def _vel(NZ, NX):
+    return FloatVector(hypercube(ns=[NZ, NX], os_=[0.0, 0.0], ds=[10.0, 10.0]), np.full((NX, NZ), 2000.0, np.float32))
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_padded_sizes(name):
+    g, c = _load(name), CONFIGS[name]
+    NZ = c["nz"] + 50 + workloads.pad_plus(c["nz"], 50) + 10
+    NX = c["nx"] + 50 + workloads.pad_plus(c["nx"], 50) + 10
+    assert (NZ, NX) == (int(g["NZ"]), int(g["NX"]))
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_grid_receivers(nl, name):
+    g, c = _load(name), CONFIGS[name]
+    NZ, NX = int(g["NZ"]), int(g["NX"])
+    d = nl.deviceGpu(*c["rec"], _vel(NZ, NX), 10, 0, 0, 0)
+    np.testing.assert_array_equal(np.array(d.getRegPos()), g["rec_gridIdx"])
+    np.testing.assert_array_equal(np.array(d.getWeights(), np.float32), g["rec_weight"])
+    np.testing.assert_array_equal(np.array(d.getRegPosUnique()), g["rec_regPos"])
+    o = O.Device.grid(*c["rec"], NZ, NX, 10)
+    np.testing.assert_array_equal(o.gridIdx, g["rec_gridIdx"])
+    np.testing.assert_array_equal(o.regPos, g["rec_regPos"])
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_coordinate_receivers(nl, name):
+    g = _load(name)
+    NZ, NX = int(g["NZ"]), int(g["NX"])
+    d = nl.deviceGpu(g["coord_z"], g["coord_x"], _vel(NZ, NX), 10, 0, 0.0, 0.0)
+    np.testing.assert_array_equal(np.array(d.getRegPos()), g["coord_gridIdx"])
+    np.testing.assert_array_equal(np.array(d.getWeights(), np.float32).view(np.int32), g["coord_weight"].view(np.int32))
+    np.testing.assert_array_equal(np.array(d.getRegPosUnique()), g["coord_regPos"])
+    o = O.Device.coords(g["coord_z"], g["coord_x"], NZ, NX, 0.0, 10.0, 0.0, 10.0, 10)
+    np.testing.assert_array_equal(o.gridIdx, g["coord_gridIdx"])
+    np.testing.assert_array_equal(np.asarray(o.weight, np.float32).view(np.int32), g["coord_weight"].view(np.int32))
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_dipole_receivers(nl, name):
+    g = _load(name)
+    NZ, NX = int(g["NZ"]), int(g["NX"])
+    z, x = g["coord_z"][:50], g["coord_x"][:50]
+    d = nl.deviceGpu(z, x, _vel(NZ, NX), 10, 1, 20.0, 0.0)
+    np.testing.assert_array_equal(np.array(d.getRegPos()), g["dip_gridIdx"])
+    np.testing.assert_array_equal(np.array(d.getWeights(), np.float32).view(np.int32), g["dip_weight"].view(np.int32))
+    np.testing.assert_array_equal(np.array(d.getRegPosUnique()), g["dip_regPos"])
+    o = O.Device.coords(z, x, NZ, NX, 0.0, 10.0, 0.0, 10.0, 10, dipole=1, zShift=20.0, xShift=0.0)
+    np.testing.assert_array_equal(o.regPos, g["dip_regPos"])
+
+
+def test_benchmark_workload_sources():
+    g = _load("born_lsrtm_80x500x1200")
+    w = workloads.born_lsrtm(nShot=80, nts=20)
+    assert (w.params["nz"], w.params["nx"]) == (int(g["NZ"]), int(g["NX"]))
+    for s, (iz, ix) in enumerate(w.src):
+        o = O.Device.grid(1, iz, 1, 1, ix, 1, w.params["nz"], w.params["nx"], 10)
+        np.testing.assert_array_equal(o.regPos, g["src_regPos"][s])
+    o = O.Device.grid(*w.rec, w.params["nz"], w.params["nx"], 10)
+    np.testing.assert_array_equal(o.regPos, g["rec_regPos"])
