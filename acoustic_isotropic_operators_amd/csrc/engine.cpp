@@ -100,6 +100,7 @@ struct aiso_ctx {
     // work buffers
     DBuf pairs, pairsQ, W, ss, img, dreg, sig, slots, slotsQ, mtil, acc, modelDev, dataDev;
     int lockstep = 1;
+    int pair2 = 1;                      // two fine steps per launch in the lockstep Born forward
     int batchFwd = 0, batchAdj = 0;     // per-direction overrides of batchReq
     int batchRec = 0;                   // receiver-pass group size within an adjoint batch
     int spb = 1;                        // shots per workgroup of the step kernel
@@ -164,6 +165,58 @@ DevGeom* dev_geom(aiso_ctx* c, const aiso_device* d)
     return out;
 }
 
+// Tiling of the two-step kernel: output tiles of OZ x OX points, each
+// computing step A on the EZ x EX region around it.
+constexpr int kOZ2 = kPair2EZ - 2 * kFat, kOX2 = kPair2EX - 2 * kFat;
+int tilesZ2(const aiso_ctx* c) { return (c->p.nz - 2 * kFat + kOZ2 - 1) / kOZ2; }
+int tilesX2(const aiso_ctx* c) { return (c->p.nx - 2 * kFat + kOX2 - 1) / kOX2; }
+
+// Sparse lists of a geometry for the two-step kernel.  With `region` every
+// tile whose EZ x EX region holds a point lists it (injections feed step A on
+// the whole region); otherwise only the owning tile does (records).  Local
+// indices are ex * EZ + ez in the region frame.  Points in the FAT ring are
+// not listed: the caller checks there are none.
+DevGeom* dev_geom_pair2(aiso_ctx* c, const aiso_device* d, bool region)
+{
+    auto key = std::make_tuple(d->id, region ? -1 : -2, kPair2EZ * 1000 + kPair2EX);
+    auto it = c->geoms.find(key);
+    if (it != c->geoms.end()) return it->second.get();
+    auto dg = std::make_unique<DevGeom>();
+    const Geometry& g = d->g;
+    const int nz = c->p.nz, nx = c->p.nx, tZ = tilesZ2(c), tX = tilesX2(c), nt = tZ * tX;
+    std::vector<std::vector<int2>> buckets(nt);
+    for (int r = 0; r < (int)g.regPos.size(); ++r) {
+        const int pos = g.regPos[r], iz = pos % nz, ix = pos / nz;
+        if (!(iz >= kFat && iz < nz - kFat && ix >= kFat && ix < nx - kFat)) continue;
+        int zlo = (iz - kFat) / kOZ2, zhi = zlo, xlo = (ix - kFat) / kOX2, xhi = xlo;
+        if (region) {   // tiles t with t*O <= i < t*O + E (region origin t*O)
+            zlo = std::max(0, (iz - kPair2EZ + kOZ2) / kOZ2);
+            zhi = std::min(tZ - 1, iz / kOZ2);
+            xlo = std::max(0, (ix - kPair2EX + kOX2) / kOX2);
+            xhi = std::min(tX - 1, ix / kOX2);
+        }
+        for (int tx = xlo; tx <= xhi; ++tx)
+            for (int tz = zlo; tz <= zhi; ++tz) {
+                const int lz = iz - tz * kOZ2, lx = ix - tx * kOX2;
+                if (lz < 0 || lz >= kPair2EZ || lx < 0 || lx >= kPair2EX) continue;
+                buckets[tz + tZ * tx].push_back(make_int2(lx * kPair2EZ + lz, r));
+            }
+    }
+    std::vector<int> ptr(nt + 1, 0);
+    std::vector<int2> ent;
+    for (int t = 0; t < nt; ++t) {
+        ptr[t + 1] = ptr[t] + (int)buckets[t].size();
+        ent.insert(ent.end(), buckets[t].begin(), buckets[t].end());
+    }
+    upload(dg->tilePtr, ptr, c->stream);
+    upload(dg->ent, ent, c->stream);
+    dg->nReg = (int)g.regPos.size();
+    HIPC(hipStreamSynchronize(c->stream));
+    DevGeom* out = dg.get();
+    c->geoms.emplace(key, std::move(dg));
+    return out;
+}
+
 enum class Op { NlFwd, NlAdj, BornFwd, BornAdj, ExtFwd, ExtAdj };
 
 bool is_adj(Op op) { return op == Op::NlAdj || op == Op::BornAdj || op == Op::ExtAdj; }
@@ -201,18 +254,14 @@ hipEvent_t pooled_event(aiso_ctx* c)
 
 void launch_timed(aiso_ctx* c, unsigned mode, int kind, const StepArgs& a, int nshots, bool generic = false)
 {
-    hipEvent_t e0 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->profile) {
         e0 = pooled_event(c);
-        HIPC(hipEventRecord(e0, c->stream));
-    }
-    if (generic) HIPC(launch_step_generic(mode, c->tz, c->tx, a, c->stream));
-    else HIPC(launch_step(mode, c->tz, c->tx, c->spb, a, c->stream));
-    if (c->profile) {
-        hipEvent_t e1 = pooled_event(c);
-        HIPC(hipEventRecord(e1, c->stream));
+        e1 = pooled_event(c);
         c->ev.push_back({e0, e1, kind, double(c->p.interior()) * nshots});
     }
+    if (generic) HIPC(launch_step_generic(mode, c->tz, c->tx, a, c->stream, e0, e1));
+    else HIPC(launch_step(mode, c->tz, c->tx, c->spb, a, c->stream, e0, e1));
 }
 
 void drain_profile(aiso_ctx* c)
@@ -351,12 +400,22 @@ void run(aiso_ctx* c, const Request& q)
     // The Born forward with sub == 2 runs its two propagations in lockstep:
     // the secondary source then lives in a 3-slice ring instead of nts slices.
     const bool lockstep = q.op == Op::BornFwd && p.sub == 2 && c->lockstep;
+    // Two fine steps per launch (k_fwd_pair2) when no source or receiver
+    // point lies in the FAT ring, whose values that kernel takes as 0.
+    // The Born adjoint with sub == 2 likewise runs both of its passes two
+    // fine steps per launch (k_fwd_pair2 as source pass, k_rec2).
+    bool noRing = true;
+    for (int i = 0; noRing && i < q.nShot; ++i) noRing = srcG[i]->nRing == 0;
+    for (int i = 0; noRing && i < q.nRec; ++i) noRing = recG[i]->nRing == 0;
+    const bool pair2 = lockstep && c->pair2 && noRing;
+    const bool adj2 = q.op == Op::BornAdj && p.sub == 2 && c->pair2 && noRing;
+    const int pairSlices = pair2 || adj2 ? 4 : 2;
     const long wSlices = lockstep ? 3 : p.nts;
     const bool needW = is_born(q.op), needSS = is_ext(q.op) || q.op == Op::BornAdj;
     const bool needImg = q.op == Op::BornAdj || q.op == Op::ExtAdj;
-    size_t perShot = 2 * nzx * 4 + (size_t)nRecRegMax * p.nts * 4 + (size_t)nSrcRegMax * p.ntw * 4;
+    size_t perShot = pairSlices * nzx * 4 + (size_t)nRecRegMax * p.nts * 4 + (size_t)nSrcRegMax * p.ntw * 4;
     if (needW) perShot += (size_t)wSlices * nzx * 4;
-    if (lockstep) perShot += 2 * nzx * 4;
+    if (lockstep) perShot += pairSlices * nzx * 4;
     if (needSS) perShot += 2 * nzx * 4;
     if (needImg) perShot += (size_t)nExt * nzx * 4;
     c->bytesPerShot = perShot;
@@ -384,10 +443,10 @@ void run(aiso_ctx* c, const Request& q)
     B = std::min(B, q.nShot);
     c->batchUsed = B;
 
-    float* pairs = c->pairs.need((size_t)B * 2 * nzx * 4) ? c->pairs.as<float>() : nullptr;
+    float* pairs = c->pairs.need((size_t)B * pairSlices * nzx * 4) ? c->pairs.as<float>() : nullptr;
     float* W = needW ? (float*)c->W.need((size_t)B * wSlices * nzx * 4) : nullptr;
-    float* pairsQ = lockstep ? (float*)c->pairsQ.need((size_t)B * 2 * nzx * 4) : nullptr;
-    ShotSlot* dSlotsQ = lockstep ? (ShotSlot*)c->slotsQ.need(sizeof(ShotSlot) * B) : nullptr;
+    float* pairsQ = lockstep ? (float*)c->pairsQ.need((size_t)B * pairSlices * nzx * 4) : nullptr;
+    ShotSlot* dSlotsQ = lockstep || adj2 ? (ShotSlot*)c->slotsQ.need(sizeof(ShotSlot) * B) : nullptr;
     float* ss = needSS ? (float*)c->ss.need((size_t)B * 2 * nzx * 4) : nullptr;
     float* img = needImg ? (float*)c->img.need((size_t)B * nExt * nzx * 4) : nullptr;
     float* dreg = (float*)c->dreg.need((size_t)B * nRecRegMax * p.nts * 4);
@@ -444,13 +503,10 @@ void run(aiso_ctx* c, const Request& q)
             DevGeom* rg = recG[q.nRec == 1 ? 0 : shot];
             ShotSlot& S = slots[k];
             std::memset(&S, 0, sizeof(S));
-            S.p[0] = pairs + (size_t)k * 2 * nzx;
-            S.p[1] = S.p[0] + nzx;
+            for (int i = 0; i < pairSlices; ++i) S.p[i] = pairs + ((size_t)k * pairSlices + i) * nzx;
             if (W) S.W = W + (size_t)k * wSlices * nzx;
-            if (pairsQ) {
-                S.q[0] = pairsQ + (size_t)k * 2 * nzx;
-                S.q[1] = S.q[0] + nzx;
-            }
+            if (pairsQ)
+                for (int i = 0; i < pairSlices; ++i) S.q[i] = pairsQ + ((size_t)k * pairSlices + i) * nzx;
             if (ss) { S.ss[0] = ss + (size_t)k * 2 * nzx; S.ss[1] = S.ss[0] + nzx; }
             if (img) S.img = img + (size_t)k * nExt * nzx;
             S.dreg = dreg + (size_t)k * nRecRegMax * p.nts;
@@ -519,7 +575,71 @@ void run(aiso_ctx* c, const Request& q)
         // same source runs on the host).
         const bool generic = is_ext(q.op);
 
-        if (lockstep) {
+        if (pair2) {
+            // The scattered field lags the source field by one launch (two
+            // fine steps); SS[k] lives in a 3-slice ring of W.
+            std::vector<ShotSlot> s2 = slots;
+            for (int k = 0; k < nb; ++k) {
+                const int shot = b0 + k;
+                DevGeom* sgE = dev_geom_pair2(c, q.src[shot], true);
+                DevGeom* rgT = dev_geom_pair2(c, q.rec[q.nRec == 1 ? 0 : shot], false);
+                s2[k].srcTile = sgE->tilePtr.as<int>();
+                s2[k].srcEnt = sgE->ent.as<int2>();
+                s2[k].recTile = rgT->tilePtr.as<int>();
+                s2[k].recEnt = rgT->ent.as<int2>();
+            }
+            HIPC(hipMemcpyAsync(dSlotsQ, s2.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
+            HIPC(launch_fill(pairs, (long)nb * 4 * nzx, s));
+            HIPC(launch_fill(pairsQ, (long)nb * 4 * nzx, s));
+            auto ring3 = [&](long k) { return (k % 3) * nzx; };
+            Pair2Args a2{};
+            a2.slots = dSlotsQ;
+            a2.v2dt2 = base.v2dt2;
+            a2.cd = base.cd;
+            a2.mtil = mtil;
+            a2.nz = p.nz; a2.nx = p.nx; a2.minPad = p.minPad; a2.nts = p.nts; a2.ntw = p.ntw;
+            a2.tilesZ = tilesZ2(c);
+            a2.ntiles = a2.tilesZ * tilesX2(c);
+            a2.gridLogical = a2.ntiles * nb;
+            for (int k = 0; k < 6; ++k) { a2.zc[k] = p.zc[k]; a2.xc[k] = p.xc[k]; }
+            a2.wa = f[h + 1]; a2.wb = f[h + 2]; a2.wc = f[1];
+            a2.ta = f[h + 1]; a2.tb = f[h + 2];
+            a2.faA = f[0]; a2.faB = f[1]; a2.fbB = f[h + 1];
+            a2.feA = f[1]; a2.ffA = f[h + 1]; a2.feB = f[2]; a2.ffB = f[h + 2];
+            a2.ntW = c->streamNT;
+            a2.ssTail = ring3(p.nts - 1);
+            int rs[4] = {0, 1, 2, 3}, rq[4] = {0, 1, 2, 3};   // current, old, new A, new B
+            const int M = (p.ntw - 1) / 2;
+            for (int m = 0; m <= M; ++m) {
+                a2.doS = m < M;
+                a2.doQ = m >= 1;
+                a2.tailS = m == M - 1;
+                a2.sC = rs[0]; a2.sO = rs[1]; a2.sA = rs[2]; a2.sB = rs[3];
+                a2.qC = rq[0]; a2.qO = rq[1]; a2.qA = rq[2]; a2.qB = rq[3];
+                a2.injA = 2 * m;
+                a2.injB = 2 * m + 1;
+                a2.ssNew = ring3(m);
+                a2.ssPrev = m >= 1 ? ring3(m - 1) : 0;
+                a2.its = m - 1;
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (c->profile) {
+                    e0 = pooled_event(c);
+                    e1 = pooled_event(c);
+                    c->ev.push_back({e0, e1, AISO_K_BORN_PAIR, 2.0 * (a2.doS + a2.doQ) * double(p.interior()) * nb});
+                }
+                HIPC(launch_fwd_pair2(a2, s, e0, e1));
+                if (a2.doS) { int t[4] = {rs[3], rs[2], rs[0], rs[1]}; std::copy(t, t + 4, rs); }
+                if (a2.doQ) { int t[4] = {rq[3], rq[2], rq[0], rq[1]}; std::copy(t, t + 4, rq); }
+            }
+            for (int k = 0; k < nb; ++k) {
+                const int shot = b0 + k;
+                const aiso_device* rd = q.rec[q.nRec == 1 ? 0 : shot];
+                DevGeom* rg = recG[q.nRec == 1 ? 0 : shot];
+                HIPC(launch_dev_forward(dreg + (size_t)k * nRecRegMax * p.nts, dDataOut + (size_t)shot * nRecDev * p.nts,
+                                        rg->cornerReg.as<int>(), rg->weight.as<float>(), nRecDev, rd->g.nb, p.nts,
+                                        q.add, s));
+            }
+        } else if (lockstep) {
             HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
             HIPC(launch_fill(pairsQ, (long)nb * 2 * nzx, s));
             const int T = p.ntw - 1;   // fine steps of each propagation
@@ -547,14 +667,13 @@ void run(aiso_ctx* c, const Request& q)
                 x.fe = f[it2Q];
                 x.ff = f[h + it2Q];
                 if (doS && doQ) {
-                    hipEvent_t e0 = nullptr;
-                    if (c->profile) { e0 = pooled_event(c); HIPC(hipEventRecord(e0, s)); }
-                    HIPC(launch_fwd_pair(c->tz, c->tx, x, s));
+                    hipEvent_t e0 = nullptr, e1 = nullptr;
                     if (c->profile) {
-                        hipEvent_t e1 = pooled_event(c);
-                        HIPC(hipEventRecord(e1, s));
+                        e0 = pooled_event(c);
+                        e1 = pooled_event(c);
                         c->ev.push_back({e0, e1, AISO_K_BORN_PAIR, 2.0 * double(p.interior()) * nb});
                     }
+                    HIPC(launch_fwd_pair(c->tz, c->tx, x, s, e0, e1));
                     ring(MODE_SRC_WIN, x);
                     StepArgs xq = x;
                     xq.slots = dSlotsQ;
@@ -588,8 +707,91 @@ void run(aiso_ctx* c, const Request& q)
             }
         }
 
+        if (adj2) {
+            // Source pass (W[k] for every coarse k) then receiver pass with
+            // imaging, two fine steps per launch each.
+            std::vector<ShotSlot> s2 = slots;
+            for (int k = 0; k < nb; ++k) {
+                const int shot = b0 + k;
+                DevGeom* sgE = dev_geom_pair2(c, q.src[shot], true);
+                DevGeom* rgE = dev_geom_pair2(c, q.rec[q.nRec == 1 ? 0 : shot], true);
+                s2[k].srcTile = sgE->tilePtr.as<int>();
+                s2[k].srcEnt = sgE->ent.as<int2>();
+                s2[k].recTile = rgE->tilePtr.as<int>();
+                s2[k].recEnt = rgE->ent.as<int2>();
+            }
+            HIPC(hipMemcpyAsync(dSlotsQ, s2.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
+            HIPC(launch_fill(pairs, (long)nb * 4 * nzx, s));
+            Pair2Args a2{};
+            a2.slots = dSlotsQ;
+            a2.v2dt2 = base.v2dt2;
+            a2.cd = base.cd;
+            a2.mtil = nullptr;                            // raw W
+            a2.nz = p.nz; a2.nx = p.nx; a2.minPad = p.minPad; a2.nts = p.nts; a2.ntw = p.ntw;
+            a2.tilesZ = tilesZ2(c);
+            a2.ntiles = a2.tilesZ * tilesX2(c);
+            a2.gridLogical = a2.ntiles * nb;
+            for (int k = 0; k < 6; ++k) { a2.zc[k] = p.zc[k]; a2.xc[k] = p.xc[k]; }
+            a2.wa = f[h + 1]; a2.wb = f[h + 2]; a2.wc = f[1];
+            a2.ta = f[h + 1]; a2.tb = f[h + 2];
+            a2.ntW = c->streamNT;
+            a2.ssTail = (long)(p.nts - 1) * nzx;
+            a2.doS = 1;
+            int rs[4] = {0, 1, 2, 3};
+            const int M = (p.ntw - 1) / 2;
+            for (int m = 0; m < M; ++m) {
+                a2.tailS = m == M - 1;
+                a2.sC = rs[0]; a2.sO = rs[1]; a2.sA = rs[2]; a2.sB = rs[3];
+                a2.injA = 2 * m;
+                a2.injB = 2 * m + 1;
+                a2.ssNew = (long)m * nzx;
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (c->profile) {
+                    e0 = pooled_event(c);
+                    e1 = pooled_event(c);
+                    c->ev.push_back({e0, e1, AISO_K_SRC, 2.0 * double(p.interior()) * nb});
+                }
+                HIPC(launch_fwd_pair2(a2, s, e0, e1));
+                int t[4] = {rs[3], rs[2], rs[0], rs[1]};
+                std::copy(t, t + 4, rs);
+            }
+            HIPC(launch_fill(pairs, (long)nb * 4 * nzx, s));
+            HIPC(launch_fill(img, (long)nb * nExt * nzx, s));
+            Rec2Args r2{};
+            r2.slots = dSlotsQ;
+            r2.v2dt2 = base.v2dt2;
+            r2.cd = base.cd;
+            r2.nz = p.nz; r2.nx = p.nx; r2.minPad = p.minPad; r2.nts = p.nts;
+            r2.tilesZ = a2.tilesZ;
+            r2.ntiles = a2.ntiles;
+            r2.gridLogical = a2.gridLogical;
+            for (int k = 0; k < 6; ++k) { r2.zc[k] = p.zc[k]; r2.xc[k] = p.xc[k]; }
+            r2.fcA = f[2]; r2.fdA = f[h + 2];               // it2 = 1
+            r2.fcB = f[1]; r2.fdB = f[h + 1];               // it2 = 0
+            r2.wa = f[1]; r2.wb = f[0]; r2.wc = f[h + 1];
+            r2.ta = f[1]; r2.tb = f[0];
+            r2.ntW = c->streamNT;
+            int rr[4] = {0, 1, 2, 3};
+            for (int its = p.nts - 2; its >= 0; --its) {
+                r2.sC = rr[0]; r2.sO = rr[1]; r2.sA = rr[2]; r2.sB = rr[3];
+                r2.its = its;
+                r2.wImg = (long)(its + 1) * nzx;
+                r2.tail = its == 0;
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (c->profile) {
+                    e0 = pooled_event(c);
+                    e1 = pooled_event(c);
+                    c->ev.push_back({e0, e1, AISO_K_BORN_REC, 2.0 * double(p.interior()) * nb});
+                }
+                HIPC(launch_rec2(r2, s, e0, e1));
+                int t[4] = {rr[3], rr[2], rr[0], rr[1]};
+                std::copy(t, t + 4, rr);
+            }
+            HIPC(launch_born_adj_finish(dSlots, nb, -1, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
+        }
+
         // -- source wavefield pass (all Born-type operators)
-        if (is_born(q.op) && !lockstep) {
+        if (is_born(q.op) && !lockstep && !adj2) {
             HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
             a.cur = 1;
             const unsigned mode = win ? MODE_SRC_WIN : MODE_SRC;
@@ -622,130 +824,130 @@ void run(aiso_ctx* c, const Request& q)
         }
 
         // -- second pass
-        if (!lockstep) {
-        HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
-        a.cur = 1;
-        a.assignA = a.assignB = a.skipA = a.skipB = 0;
-        if (q.op == Op::NlFwd || q.op == Op::BornFwd || q.op == Op::ExtFwd) {
-            unsigned mode = MODE_NL_FWD;
-            int kind = AISO_K_NL_FWD;
-            if (q.op == Op::BornFwd) { mode = win ? MODE_BORN_SCAT_PRE : MODE_BORN_SCAT; kind = AISO_K_BORN_SCAT; }
-            if (q.op == Op::ExtFwd) { mode = MODE_EXT_SCAT; kind = AISO_K_EXT_SCAT; }
-            int L = 0, R = 1;
-            if (q.op == Op::ExtFwd) HIPC(launch_ext_ss(dSlots, nb, L, 0, mtil, base.v2dt2, p.nz, p.nx, p.hExt, s));
-            for (int its = 0; its < p.nts - 1; ++its) {
-                if (q.op == Op::ExtFwd)
-                    HIPC(launch_ext_ss(dSlots, nb, R, (long)(its + 1) * nzx, mtil, base.v2dt2, p.nz, p.nx, p.hExt, s));
-                for (int it2 = 1; it2 <= p.sub; ++it2) {
-                    a.its = its;
-                    a.itwInj = its * p.sub + it2 - 1;
-                    a.fa = f[it2 - 1];
-                    a.fb = f[h + it2 - 1];
-                    a.wA = (long)its * nzx;
-                    a.wB = (long)(its + 1) * nzx;
-                    a.ssA = L;
-                    a.ssB = R;
-                    a.fe = f[it2];
-                    a.ff = f[h + it2];
-                    launch_timed(c, mode, kind, a, nb, generic);
-                    ring(mode, a);
-                    a.cur ^= 1;
-                }
-                std::swap(L, R);
-            }
-            for (int k = 0; k < nb; ++k) {
-                const int shot = b0 + k;
-                const aiso_device* rd = q.rec[q.nRec == 1 ? 0 : shot];
-                DevGeom* rg = recG[q.nRec == 1 ? 0 : shot];
-                HIPC(launch_dev_forward(dreg + (size_t)k * nRecRegMax * p.nts, dDataOut + (size_t)shot * nRecDev * p.nts,
-                                        rg->cornerReg.as<int>(), rg->weight.as<float>(), nRecDev, rd->g.nb, p.nts,
-                                        q.add, s));
-            }
-        } else {
-            unsigned mode = MODE_NL_ADJ;
-            int kind = AISO_K_NL_ADJ;
-            if (q.op == Op::BornAdj) { mode = win ? MODE_REC_WIN : MODE_BORN_REC; kind = AISO_K_BORN_REC; }
-            if (q.op == Op::ExtAdj) { mode = win ? MODE_REC_WIN : MODE_EXT_REC; kind = AISO_K_EXT_REC; }
-            if (ss) HIPC(launch_fill(ss, (long)nb * 2 * nzx, s));
-            if (img) HIPC(launch_fill(img, (long)nb * nExt * nzx, s));
-            a.winOut = q.op == Op::BornAdj ? WIN_IMG : WIN_SLICE;
-            a.wa = f[1];
-            a.wb = f[0];
-            a.wc = f[h + 1];
-            // The stored source wavefield lets the receiver pass run the batch
-            // in smaller groups, whose per-shot slices (pair + image) stay in
-            // the on-die cache.
-            const int subB = c->batchRec > 0 ? std::min(c->batchRec, nb) : nb;
-            for (int k0 = 0; k0 < nb; k0 += subB) {
-                const int nbr = std::min(subB, nb - k0);
-                ShotSlot* sl = dSlots + k0;
-                StepArgs r = a;
-                r.slots = sl;
-                r.nshots = nbr;
-                r.gridLogical = r.ntiles * nbr;
-                r.cur = 1;
+        if (!lockstep && !adj2) {
+            HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
+            a.cur = 1;
+            a.assignA = a.assignB = a.skipA = a.skipB = 0;
+            if (q.op == Op::NlFwd || q.op == Op::BornFwd || q.op == Op::ExtFwd) {
+                unsigned mode = MODE_NL_FWD;
+                int kind = AISO_K_NL_FWD;
+                if (q.op == Op::BornFwd) { mode = win ? MODE_BORN_SCAT_PRE : MODE_BORN_SCAT; kind = AISO_K_BORN_SCAT; }
+                if (q.op == Op::ExtFwd) { mode = MODE_EXT_SCAT; kind = AISO_K_EXT_SCAT; }
                 int L = 0, R = 1;
-                for (int its = p.nts - 2; its >= 0; --its) {
-                    for (int it2 = p.sub - 1; it2 >= 0; --it2) {
-                        r.its = its;
-                        r.itwRec = its * p.sub + it2;
-                        r.fc = f[it2 + 1];
-                        r.fd = f[h + it2 + 1];
-                        r.ssA = win ? 0 : L;
-                        r.ssB = R;
-                        r.fe = f[it2];
-                        r.ff = f[h + it2];
-                        r.skipA = r.fe == 0.f;
-                        r.skipB = r.ff == 0.f;
-                        r.assignA = it2 == p.sub - 1;
-                        r.assignB = 0;
-                        r.image = (q.op == Op::BornAdj && it2 == 0) ? 1 : 0;
-                        r.win = it2 == 1;
-                        r.wImg = (long)(its + 1) * nzx;
-                        launch_timed(c, mode, kind, r, nbr, generic);
-                        ring(mode, r);
-                        r.cur ^= 1;
+                if (q.op == Op::ExtFwd) HIPC(launch_ext_ss(dSlots, nb, L, 0, mtil, base.v2dt2, p.nz, p.nx, p.hExt, s));
+                for (int its = 0; its < p.nts - 1; ++its) {
+                    if (q.op == Op::ExtFwd)
+                        HIPC(launch_ext_ss(dSlots, nb, R, (long)(its + 1) * nzx, mtil, base.v2dt2, p.nz, p.nx, p.hExt, s));
+                    for (int it2 = 1; it2 <= p.sub; ++it2) {
+                        a.its = its;
+                        a.itwInj = its * p.sub + it2 - 1;
+                        a.fa = f[it2 - 1];
+                        a.fb = f[h + it2 - 1];
+                        a.wA = (long)its * nzx;
+                        a.wB = (long)(its + 1) * nzx;
+                        a.ssA = L;
+                        a.ssB = R;
+                        a.fe = f[it2];
+                        a.ff = f[h + it2];
+                        launch_timed(c, mode, kind, a, nb, generic);
+                        ring(mode, a);
+                        a.cur ^= 1;
                     }
-                    if (q.op == Op::ExtAdj)
-                        HIPC(launch_ext_img(sl, nbr, win ? 0 : R, (long)(its + 1) * nzx, base.v2dt2, p.nz, p.nx, nExt, s));
                     std::swap(L, R);
                 }
-                if (q.op == Op::BornAdj) {
-                    if (win) {   // img += W[0] * (q(1)*f[1] + q(0)*f[0]), then the scaling
-                        HIPC(launch_win_tail(sl, nbr, r.cur, WIN_IMG, f[1], f[0], 0, 0, 0, nullptr, p.nz, p.nx, s));
-                        HIPC(launch_born_adj_finish(sl, nbr, -1, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
-                    } else {
-                        HIPC(launch_born_adj_finish(sl, nbr, R, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
-                    }
-                } else if (q.op == Op::ExtAdj) {
-                    if (win) HIPC(launch_win_tail(sl, nbr, r.cur, WIN_SLICE, f[1], f[0], 0, 0, 0, nullptr, p.nz, p.nx, s));
-                    HIPC(launch_ext_img(sl, nbr, win ? 0 : R, 0, base.v2dt2, p.nz, p.nx, nExt, s));
-                    HIPC(launch_ext_adj_finish(sl, nbr, c->refl.as<float>(), p.nz, p.nx, nExt, s));
-                }
-            }
-        }   // !lockstep
-            if (needImg) {
-                for (int k = 0; k < nb; ++k) HIPC(launch_accumulate(acc, img + (size_t)k * nExt * nzx, nzx * nExt, s));
-            }
-            if (q.op == Op::NlAdj) {
-                std::vector<float> fineAll((size_t)nb * nSrcRegMax * p.ntw);
-                HIPC(hipMemcpyAsync(fineAll.data(), sig, fineAll.size() * 4, hipMemcpyDeviceToHost, s));
-                HIPC(hipStreamSynchronize(s));
                 for (int k = 0; k < nb; ++k) {
                     const int shot = b0 + k;
-                    const Geometry& sgm = q.src[shot]->g;
-                    const int nReg = (int)sgm.regPos.size();
-                    std::vector<float> reg((size_t)nReg * p.nts);
-                    interp_time_adj(p, nReg, reg.data(), fineAll.data() + (size_t)k * nSrcRegMax * p.ntw);
-                    scale_source(p, c->vel.data(), sgm.regPos, p.nts, reg.data());
-                    if (q.nShotSig == 1) {
-                        sgm.forward(true, reg.data(), nlAcc.data(), p.nts);
-                    } else {
-                        std::vector<float> sl((size_t)nSrcDev * p.nts);
-                        sgm.forward(false, reg.data(), sl.data(), p.nts);
-                        float* dst = hModel.data() + (size_t)shot * nSrcDev * p.nts;
-                        for (size_t i = 0; i < sl.size(); ++i) dst[i] += sl[i];
+                    const aiso_device* rd = q.rec[q.nRec == 1 ? 0 : shot];
+                    DevGeom* rg = recG[q.nRec == 1 ? 0 : shot];
+                    HIPC(launch_dev_forward(dreg + (size_t)k * nRecRegMax * p.nts, dDataOut + (size_t)shot * nRecDev * p.nts,
+                                            rg->cornerReg.as<int>(), rg->weight.as<float>(), nRecDev, rd->g.nb, p.nts,
+                                            q.add, s));
+                }
+            } else {
+                unsigned mode = MODE_NL_ADJ;
+                int kind = AISO_K_NL_ADJ;
+                if (q.op == Op::BornAdj) { mode = win ? MODE_REC_WIN : MODE_BORN_REC; kind = AISO_K_BORN_REC; }
+                if (q.op == Op::ExtAdj) { mode = win ? MODE_REC_WIN : MODE_EXT_REC; kind = AISO_K_EXT_REC; }
+                if (ss) HIPC(launch_fill(ss, (long)nb * 2 * nzx, s));
+                if (img) HIPC(launch_fill(img, (long)nb * nExt * nzx, s));
+                a.winOut = q.op == Op::BornAdj ? WIN_IMG : WIN_SLICE;
+                a.wa = f[1];
+                a.wb = f[0];
+                a.wc = f[h + 1];
+                // The stored source wavefield lets the receiver pass run the batch
+                // in smaller groups, whose per-shot slices (pair + image) stay in
+                // the on-die cache.
+                const int subB = c->batchRec > 0 ? std::min(c->batchRec, nb) : nb;
+                for (int k0 = 0; k0 < nb; k0 += subB) {
+                    const int nbr = std::min(subB, nb - k0);
+                    ShotSlot* sl = dSlots + k0;
+                    StepArgs r = a;
+                    r.slots = sl;
+                    r.nshots = nbr;
+                    r.gridLogical = r.ntiles * nbr;
+                    r.cur = 1;
+                    int L = 0, R = 1;
+                    for (int its = p.nts - 2; its >= 0; --its) {
+                        for (int it2 = p.sub - 1; it2 >= 0; --it2) {
+                            r.its = its;
+                            r.itwRec = its * p.sub + it2;
+                            r.fc = f[it2 + 1];
+                            r.fd = f[h + it2 + 1];
+                            r.ssA = win ? 0 : L;
+                            r.ssB = R;
+                            r.fe = f[it2];
+                            r.ff = f[h + it2];
+                            r.skipA = r.fe == 0.f;
+                            r.skipB = r.ff == 0.f;
+                            r.assignA = it2 == p.sub - 1;
+                            r.assignB = 0;
+                            r.image = (q.op == Op::BornAdj && it2 == 0) ? 1 : 0;
+                            r.win = it2 == 1;
+                            r.wImg = (long)(its + 1) * nzx;
+                            launch_timed(c, mode, kind, r, nbr, generic);
+                            ring(mode, r);
+                            r.cur ^= 1;
+                        }
+                        if (q.op == Op::ExtAdj)
+                            HIPC(launch_ext_img(sl, nbr, win ? 0 : R, (long)(its + 1) * nzx, base.v2dt2, p.nz, p.nx, nExt, s));
+                        std::swap(L, R);
                     }
+                    if (q.op == Op::BornAdj) {
+                        if (win) {   // img += W[0] * (q(1)*f[1] + q(0)*f[0]), then the scaling
+                            HIPC(launch_win_tail(sl, nbr, r.cur, WIN_IMG, f[1], f[0], 0, 0, 0, nullptr, p.nz, p.nx, s));
+                            HIPC(launch_born_adj_finish(sl, nbr, -1, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
+                        } else {
+                            HIPC(launch_born_adj_finish(sl, nbr, R, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
+                        }
+                    } else if (q.op == Op::ExtAdj) {
+                        if (win) HIPC(launch_win_tail(sl, nbr, r.cur, WIN_SLICE, f[1], f[0], 0, 0, 0, nullptr, p.nz, p.nx, s));
+                        HIPC(launch_ext_img(sl, nbr, win ? 0 : R, 0, base.v2dt2, p.nz, p.nx, nExt, s));
+                        HIPC(launch_ext_adj_finish(sl, nbr, c->refl.as<float>(), p.nz, p.nx, nExt, s));
+                    }
+                }
+            }
+        }
+        if (needImg) {
+            for (int k = 0; k < nb; ++k) HIPC(launch_accumulate(acc, img + (size_t)k * nExt * nzx, nzx * nExt, s));
+        }
+        if (q.op == Op::NlAdj) {
+            std::vector<float> fineAll((size_t)nb * nSrcRegMax * p.ntw);
+            HIPC(hipMemcpyAsync(fineAll.data(), sig, fineAll.size() * 4, hipMemcpyDeviceToHost, s));
+            HIPC(hipStreamSynchronize(s));
+            for (int k = 0; k < nb; ++k) {
+                const int shot = b0 + k;
+                const Geometry& sgm = q.src[shot]->g;
+                const int nReg = (int)sgm.regPos.size();
+                std::vector<float> reg((size_t)nReg * p.nts);
+                interp_time_adj(p, nReg, reg.data(), fineAll.data() + (size_t)k * nSrcRegMax * p.ntw);
+                scale_source(p, c->vel.data(), sgm.regPos, p.nts, reg.data());
+                if (q.nShotSig == 1) {
+                    sgm.forward(true, reg.data(), nlAcc.data(), p.nts);
+                } else {
+                    std::vector<float> sl((size_t)nSrcDev * p.nts);
+                    sgm.forward(false, reg.data(), sl.data(), p.nts);
+                    float* dst = hModel.data() + (size_t)shot * nSrcDev * p.nts;
+                    for (size_t i = 0; i < sl.size(); ++i) dst[i] += sl[i];
                 }
             }
         }
@@ -888,6 +1090,7 @@ aiso_status aiso_ctx_set_option(aiso_ctx* c, const char* key, int value)
         std::string k(key);
         if (k == "streamNT") c->streamNT = value != 0;
         else if (k == "lockstep") c->lockstep = value != 0;
+        else if (k == "pair2") c->pair2 = value != 0;
         else if (k == "batchFwd") c->batchFwd = std::max(0, value);
         else if (k == "batchAdj") c->batchAdj = std::max(0, value);
         else if (k == "batchRec") c->batchRec = std::max(0, value);

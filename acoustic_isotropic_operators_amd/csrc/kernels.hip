@@ -17,6 +17,7 @@
 // interior points; 256 threads = TZ lanes along z x (256/TZ) lanes along x,
 // each thread computing PX consecutive x-points with an x-window in
 // registers.  The tile and its 5-point halo are staged in LDS.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -110,9 +111,8 @@ __device__ __forceinline__ void stage_tile(float* sT, cgf* src, const float* sMu
 
 // 10th-order Laplacian at the PX points of a thread, in the reference's
 // summation order (z terms, then x terms).  w holds the x-window.
-template <int TZ, int TX>
-__device__ __forceinline__ float lap_at(const float* sT, const float* w, int j, int lx0, int lane,
-                                        const StepArgs& a)
+template <int TZ, int TX, class A>
+__device__ __forceinline__ float lap_at(const float* sT, const float* w, int j, int lx0, int lane, const A& a)
 {
     using T = TileShape<TZ, TX>;
     const float* s = &sT[(lx0 + j + FAT) * T::LZ + lane + FAT];
@@ -530,6 +530,396 @@ __global__ __launch_bounds__(NT) void k_fwd_pair(const StepArgs a)
     if (t.full && !t.sponge) pair_block<TZ, TX, true, false>(a, t.group, t.tile, t.z0, t.x0, sA, sB, sN);
     else if (t.full) pair_block<TZ, TX, true, true>(a, t.group, t.tile, t.z0, t.x0, sA, sB, sN);
     else pair_block<TZ, TX, false, true>(a, t.group, t.tile, t.z0, t.x0, sA, sB, sN);
+}
+
+// Stage the (EZ+10) x (EX+10) window of src whose first element is grid
+// point (rz0, rx0); points outside the grid read as 0.
+template <int EZ, int EX>
+__device__ __forceinline__ void stage_region(float* sT, cgf* src, int nz, int nx, int rz0, int rx0, int lane, int wy)
+{
+    using T = TileShape<EZ, EX>;
+    float tv[T::KX][T::KZ];
+#pragma unroll
+    for (int kx = 0; kx < T::KX; ++kx)
+#pragma unroll
+        for (int kz = 0; kz < T::KZ; ++kz) {
+            const int lx = wy + kx * T::NXW, lz = lane + kz * EZ;
+            const int gx = rx0 + lx, gz = rz0 + lz;
+            const bool in = lx < T::LX && lz < T::LZ && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+            tv[kx][kz] = in ? src[(unsigned)(gx * nz + gz)] : 0.f;
+        }
+#pragma unroll
+    for (int kx = 0; kx < T::KX; ++kx)
+#pragma unroll
+        for (int kz = 0; kz < T::KZ; ++kz) {
+            const int lx = wy + kx * T::NXW, lz = lane + kz * EZ;
+            if (lx < T::LX && lz < T::LZ) sT[lx * T::LZ + lz] = tv[kx][kz];
+        }
+}
+
+// Adds the traces of the sparse entries [e0, e1) of a tile (local index in the
+// EZ x EX region, trace row) to the thread's values v: column `col` of sig.
+template <int EZ, int PX>
+__device__ __forceinline__ void inject_region(float (&v)[PX], float* sN, const int2* ent, int e0, int e1,
+                                              cgf* sig, int ntw, int col, int lx0, int lane)
+{
+    if (e1 <= e0) return;                              // block-uniform
+#pragma unroll
+    for (int j = 0; j < PX; ++j) sN[(lx0 + j) * EZ + lane] = v[j];
+    __syncthreads();
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += NT) {
+        const int2 en = ent[e];
+        sN[en.x] += sig[(long)en.y * ntw + col];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PX; ++j) v[j] = sN[(lx0 + j) * EZ + lane];
+    __syncthreads();
+}
+
+// d[r][its] += v*fe, d[r][its+1] += v*ff at the receiver points the tile owns.
+template <int EZ, int PX>
+__device__ __forceinline__ void record_region(const float (&v)[PX], float* sN, const int2* ent, int e0, int e1,
+                                              gf* dreg, int nts, int its, float fe, float ff, int lx0, int lane)
+{
+    if (e1 <= e0) return;                              // block-uniform
+#pragma unroll
+    for (int j = 0; j < PX; ++j) sN[(lx0 + j) * EZ + lane] = v[j];
+    __syncthreads();
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += NT) {
+        const int2 en = ent[e];
+        const float n = sN[en.x];
+        gf* dr = dreg + ((long)en.y * nts + its);
+        dr[0] += n * fe;
+        dr[1] += n * ff;
+    }
+    __syncthreads();
+}
+
+// Lockstep Born forward, two fine steps per field per launch (sub == 2).
+// With mtil == nullptr and doQ == 0 it is the source pass of the Born
+// adjoint: W[m] itself is stored (x * 1 == x).
+//
+// A workgroup owns an (EZ-10) x (EX-10) output tile and computes step A on
+// the EZ x EX region around it (the tile plus the 5-point halo step B
+// needs), then step B on the tile from step A's region kept in LDS.  Each
+// field therefore reads its current slice (with a 10-point halo) and its old
+// slice once per two steps and writes both new slices, out of place: the old
+// slice is read with a halo too.  Per point and step the arithmetic is that
+// of k_fwd_pair: damping of the old slice on read, injection before damping,
+// SS[m] from the old/current/new slices of step A, records after damping.
+// Points of the FAT ring are never computed and read as 0 (callers use this
+// kernel only when no source or receiver lies in the ring).
+template <int EZ, int EX>
+__global__ __launch_bounds__(NT) void k_fwd_pair2(const Pair2Args a)
+{
+    using T = TileShape<EZ, EX>;
+    constexpr int PX = T::PX, OZ = EZ - 2 * FAT, OX = EX - 2 * FAT;
+    __shared__ float sX[T::LX * T::LZ];   // current slice, region plus 5-point halo
+    __shared__ float sY[T::LX * T::LZ];   // step A's new slice, same frame
+    __shared__ float sN[EX * EZ];         // sparse injection / record staging
+
+    const int logical = xcd_logical(blockIdx.x, gridDim.x);
+    if (logical >= a.gridLogical) return;             // padding blocks (block-uniform)
+    const int shot = logical / a.ntiles, tile = logical - shot * a.ntiles;
+    const int z0 = FAT + (tile % a.tilesZ) * OZ, x0 = FAT + (tile / a.tilesZ) * OX;
+    const int ez0 = z0 - FAT, ex0 = x0 - FAT;         // region origin
+    const int lane = threadIdx.x % EZ, wy = threadIdx.x / EZ, lx0 = wy * PX;
+    const int nz = a.nz, nx = a.nx, gz = ez0 + lane;
+    const ShotSlot& S = a.slots[shot];
+    cgf* V = G(a.v2dt2);
+
+    bool inA[PX], inT[PX];
+    unsigned gi[PX];
+    float dmp[PX], v2[PX];
+    {
+        const bool zin = gz >= FAT && gz < nz - FAT;
+        const bool zT = lane >= FAT && lane < EZ - FAT;
+        const int distZ = imin(gz - FAT, nz - 1 - FAT - gz);
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            const int gx = ex0 + lx0 + j;
+            inA[j] = zin && gx >= FAT && gx < nx - FAT;
+            inT[j] = inA[j] && zT && lx0 + j >= FAT && lx0 + j < EX - FAT;
+            gi[j] = inA[j] ? (unsigned)(gx * nz + gz) : 0u;
+            float d = 1.f;
+            if (inA[j]) {
+                const int dist = imin(distZ, imin(gx - FAT, nx - 1 - FAT - gx));
+                if (dist < a.minPad) d = a.cd[dist];
+            }
+            dmp[j] = d;
+            v2[j] = inA[j] ? V[gi[j]] : 0.f;
+        }
+    }
+    const int e0s = S.srcTile[tile], e1s = S.srcTile[tile + 1];
+    const int e0r = S.recTile[tile], e1r = S.recTile[tile + 1];
+    gf* W = G(S.W);
+
+    float ssm[PX];                                     // SS[m] at the tile's points
+    if (a.doS) {                                       // block-uniform
+        cgf* pC = G(S.p[a.sC]);
+        cgf* pO = G(S.p[a.sO]);
+        float o[PX], mm[PX];
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            o[j] = inA[j] ? pO[gi[j]] : 0.f;
+            mm[j] = inT[j] ? (a.mtil ? G(a.mtil)[gi[j]] : 1.f) : 0.f;   // raw W without m~
+        }
+        stage_region<EZ, EX>(sX, pC, nz, nx, ez0 - FAT, ex0 - FAT, lane, wy);
+        __syncthreads();
+        float w[PX + 2 * FAT];
+#pragma unroll
+        for (int j = 0; j < PX + 2 * FAT; ++j) w[j] = sX[(lx0 + j) * T::LZ + lane + FAT];
+        float nA[PX];
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            const float lap = lap_at<EZ, EX>(sX, w, j, lx0, lane, a);
+            const float cc = w[j + FAT];
+            nA[j] = inA[j] ? v2[j] * lap + cc + cc - o[j] * dmp[j] : 0.f;
+        }
+        inject_region<EZ, PX>(nA, sN, S.srcEnt, e0s, e1s, G(S.sig), a.ntw, a.injA, lx0, lane);
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            nA[j] = nA[j] * dmp[j];
+            ssm[j] = 0.f;
+            if (inT[j]) {
+                const float v = (o[j] * a.wa + w[j + FAT] * a.wb) + nA[j] * a.wc;
+                ssm[j] = mm[j] * v;
+                stg_s(W + a.ssNew + gi[j], ssm[j], a.ntW);
+            }
+            sY[(lx0 + j + FAT) * T::LZ + lane + FAT] = nA[j];
+        }
+        __syncthreads();
+        float wB[PX + 2 * FAT];
+#pragma unroll
+        for (int j = 0; j < PX + 2 * FAT; ++j) wB[j] = sY[(lx0 + j) * T::LZ + lane + FAT];
+        float nB[PX];
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            const float lap = lap_at<EZ, EX>(sY, wB, j, lx0, lane, a);
+            const float cc = wB[j + FAT];
+            nB[j] = inT[j] ? v2[j] * lap + cc + cc - w[j + FAT] * dmp[j] : 0.f;
+        }
+        inject_region<EZ, PX>(nB, sN, S.srcEnt, e0s, e1s, G(S.sig), a.ntw, a.injB, lx0, lane);
+        gf* pA = G(S.p[a.sA]);
+        gf* pB = G(S.p[a.sB]);
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            if (!inT[j]) continue;
+            const float nb = nB[j] * dmp[j];
+            pA[gi[j]] = nA[j];
+            pB[gi[j]] = nb;
+            if (a.tailS) stg_s(W + a.ssTail + gi[j], mm[j] * (nA[j] * a.ta + nb * a.tb), a.ntW);
+        }
+        __syncthreads();                               // sX, sY reused by the scattered field
+    }
+
+    if (a.doQ) {                                       // block-uniform
+        cgf* qC = G(S.q[a.qC]);
+        cgf* qO = G(S.q[a.qO]);
+        float o[PX], sp[PX];
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            o[j] = inA[j] ? qO[gi[j]] : 0.f;
+            sp[j] = inA[j] ? ldg_s(W + a.ssPrev + gi[j], a.ntW) : 0.f;
+            // without a source step in this launch SS[m] was completed earlier
+            if (!a.doS) ssm[j] = inT[j] ? ldg_s(W + a.ssNew + gi[j], a.ntW) : 0.f;
+        }
+        stage_region<EZ, EX>(sX, qC, nz, nx, ez0 - FAT, ex0 - FAT, lane, wy);
+        __syncthreads();
+        float w[PX + 2 * FAT];
+#pragma unroll
+        for (int j = 0; j < PX + 2 * FAT; ++j) w[j] = sX[(lx0 + j) * T::LZ + lane + FAT];
+        float nA[PX];
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            const float lap = lap_at<EZ, EX>(sX, w, j, lx0, lane, a);
+            const float cc = w[j + FAT];
+            float n = 0.f;
+            if (inA[j]) {
+                n = v2[j] * lap + cc + cc - o[j] * dmp[j];
+                n += sp[j] * a.faA;
+            }
+            nA[j] = n * dmp[j];
+            sY[(lx0 + j + FAT) * T::LZ + lane + FAT] = nA[j];
+        }
+        record_region<EZ, PX>(nA, sN, S.recEnt, e0r, e1r, G(S.dreg), a.nts, a.its, a.feA, a.ffA, lx0, lane);
+        __syncthreads();
+        float wB[PX + 2 * FAT];
+#pragma unroll
+        for (int j = 0; j < PX + 2 * FAT; ++j) wB[j] = sY[(lx0 + j) * T::LZ + lane + FAT];
+        float nB[PX];
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            const float lap = lap_at<EZ, EX>(sY, wB, j, lx0, lane, a);
+            const float cc = wB[j + FAT];
+            float n = 0.f;
+            if (inT[j]) {
+                n = v2[j] * lap + cc + cc - w[j + FAT] * dmp[j];
+                n += sp[j] * a.faB + ssm[j] * a.fbB;
+            }
+            nB[j] = n * dmp[j];
+        }
+        record_region<EZ, PX>(nB, sN, S.recEnt, e0r, e1r, G(S.dreg), a.nts, a.its, a.feB, a.ffB, lx0, lane);
+        gf* qA = G(S.q[a.qA]);
+        gf* qB = G(S.q[a.qB]);
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            if (!inT[j]) continue;
+            qA[gi[j]] = nA[j];
+            qB[gi[j]] = nB[j];
+        }
+    }
+}
+
+// Stage the (EZ+10) x (EX+10) window of src * mul (element-wise) whose first
+// element is grid point (rz0, rx0); points outside the grid read as 0.
+template <int EZ, int EX>
+__device__ __forceinline__ void stage_region_mul(float* sT, cgf* src, cgf* mul, int nz, int nx, int rz0, int rx0,
+                                                 int lane, int wy)
+{
+    using T = TileShape<EZ, EX>;
+    float tv[T::KX][T::KZ], tm[T::KX][T::KZ];
+#pragma unroll
+    for (int kx = 0; kx < T::KX; ++kx)
+#pragma unroll
+        for (int kz = 0; kz < T::KZ; ++kz) {
+            const int lx = wy + kx * T::NXW, lz = lane + kz * EZ;
+            const int gx = rx0 + lx, gz = rz0 + lz;
+            const bool in = lx < T::LX && lz < T::LZ && gx >= 0 && gx < nx && gz >= 0 && gz < nz;
+            const unsigned g = in ? (unsigned)(gx * nz + gz) : 0u;
+            tv[kx][kz] = in ? src[g] : 0.f;
+            tm[kx][kz] = in ? mul[g] : 0.f;
+        }
+#pragma unroll
+    for (int kx = 0; kx < T::KX; ++kx)
+#pragma unroll
+        for (int kz = 0; kz < T::KZ; ++kz) {
+            const int lx = wy + kx * T::NXW, lz = lane + kz * EZ;
+            if (lx < T::LX && lz < T::LZ) sT[lx * T::LZ + lz] = tv[kx][kz] * tm[kx][kz];
+        }
+}
+
+// d[r][its]*fc + d[r][its+1]*fd added at the receiver points of the region.
+template <int EZ, int PX>
+__device__ __forceinline__ void inject_data_region(float (&v)[PX], float* sN, const int2* ent, int e0, int e1,
+                                                   cgf* dreg, int nts, int its, float fc, float fd, int lx0,
+                                                   int lane)
+{
+    if (e1 <= e0) return;                              // block-uniform
+#pragma unroll
+    for (int j = 0; j < PX; ++j) sN[(lx0 + j) * EZ + lane] = v[j];
+    __syncthreads();
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += NT) {
+        const int2 en = ent[e];
+        cgf* dr = dreg + ((long)en.y * nts + its);
+        sN[en.x] += dr[0] * fc + dr[1] * fd;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PX; ++j) v[j] = sN[(lx0 + j) * EZ + lane];
+    __syncthreads();
+}
+
+// Born adjoint receiver pass, two fine steps per launch (sub == 2).  Same
+// tiling as k_fwd_pair2; the adjoint stepper takes the Laplacian of
+// v2dt2 * current, so step A stages that product with a 10-point halo and
+// step B stages v2dt2 * (step A's new slice) from registers.  Per point and
+// step the arithmetic is that of k_step in MODE_REC_WIN.
+template <int EZ, int EX>
+__global__ __launch_bounds__(NT) void k_rec2(const Rec2Args a)
+{
+    using T = TileShape<EZ, EX>;
+    constexpr int PX = T::PX, OZ = EZ - 2 * FAT, OX = EX - 2 * FAT;
+    __shared__ float sX[T::LX * T::LZ];
+    __shared__ float sY[T::LX * T::LZ];
+    __shared__ float sN[EX * EZ];
+
+    const int logical = xcd_logical(blockIdx.x, gridDim.x);
+    if (logical >= a.gridLogical) return;             // padding blocks (block-uniform)
+    const int shot = logical / a.ntiles, tile = logical - shot * a.ntiles;
+    const int z0 = FAT + (tile % a.tilesZ) * OZ, x0 = FAT + (tile / a.tilesZ) * OX;
+    const int ez0 = z0 - FAT, ex0 = x0 - FAT;
+    const int lane = threadIdx.x % EZ, wy = threadIdx.x / EZ, lx0 = wy * PX;
+    const int nz = a.nz, nx = a.nx, gz = ez0 + lane;
+    const ShotSlot& S = a.slots[shot];
+    cgf* V = G(a.v2dt2);
+
+    bool inA[PX], inT[PX];
+    unsigned gi[PX];
+    float dmp[PX], v2[PX];
+    {
+        const bool zin = gz >= FAT && gz < nz - FAT;
+        const bool zT = lane >= FAT && lane < EZ - FAT;
+        const int distZ = imin(gz - FAT, nz - 1 - FAT - gz);
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            const int gx = ex0 + lx0 + j;
+            inA[j] = zin && gx >= FAT && gx < nx - FAT;
+            inT[j] = inA[j] && zT && lx0 + j >= FAT && lx0 + j < EX - FAT;
+            gi[j] = inA[j] ? (unsigned)(gx * nz + gz) : 0u;
+            float d = 1.f;
+            if (inA[j]) {
+                const int dist = imin(distZ, imin(gx - FAT, nx - 1 - FAT - gx));
+                if (dist < a.minPad) d = a.cd[dist];
+            }
+            dmp[j] = d;
+            v2[j] = inA[j] ? V[gi[j]] : 0.f;
+        }
+    }
+    const int e0 = S.recTile[tile], e1 = S.recTile[tile + 1];
+    cgf* pC = G(S.p[a.sC]);
+    cgf* pO = G(S.p[a.sO]);
+    float o[PX], c[PX];
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+        o[j] = inA[j] ? pO[gi[j]] : 0.f;
+        c[j] = inA[j] ? pC[gi[j]] : 0.f;
+    }
+    stage_region_mul<EZ, EX>(sX, pC, V, nz, nx, ez0 - FAT, ex0 - FAT, lane, wy);
+    __syncthreads();
+    float w[PX + 2 * FAT];
+#pragma unroll
+    for (int j = 0; j < PX + 2 * FAT; ++j) w[j] = sX[(lx0 + j) * T::LZ + lane + FAT];
+    float nA[PX];
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+        const float lap = lap_at<EZ, EX>(sX, w, j, lx0, lane, a);
+        nA[j] = inA[j] ? lap + c[j] + c[j] - o[j] * dmp[j] : 0.f;
+    }
+    inject_data_region<EZ, PX>(nA, sN, S.recEnt, e0, e1, G(S.dreg), a.nts, a.its, a.fcA, a.fdA, lx0, lane);
+    gf* img = G(S.img);
+    cgf* W = G(S.W);
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+        nA[j] = nA[j] * dmp[j];
+        if (inT[j]) {
+            const float v = (o[j] * a.wa + c[j] * a.wb) + nA[j] * a.wc;
+            img[gi[j]] += ldg_s(W + a.wImg + gi[j], a.ntW) * v;
+        }
+        sY[(lx0 + j + FAT) * T::LZ + lane + FAT] = nA[j] * v2[j];
+    }
+    __syncthreads();
+    float wB[PX + 2 * FAT];
+#pragma unroll
+    for (int j = 0; j < PX + 2 * FAT; ++j) wB[j] = sY[(lx0 + j) * T::LZ + lane + FAT];
+    float nB[PX];
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+        const float lap = lap_at<EZ, EX>(sY, wB, j, lx0, lane, a);
+        nB[j] = inT[j] ? lap + nA[j] + nA[j] - c[j] * dmp[j] : 0.f;
+    }
+    inject_data_region<EZ, PX>(nB, sN, S.recEnt, e0, e1, G(S.dreg), a.nts, a.its, a.fcB, a.fdB, lx0, lane);
+    gf* pA = G(S.p[a.sA]);
+    gf* pB = G(S.p[a.sB]);
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+        if (!inT[j]) continue;
+        const float nb = nB[j] * dmp[j];
+        pA[gi[j]] = nA[j];
+        pB[gi[j]] = nb;
+        if (a.tail) img[gi[j]] += ldg_s(W + gi[j], a.ntW) * (nA[j] * a.ta + nb * a.tb);
+    }
 }
 
 // Unspecialized form of the fused step: one shot per workgroup, every
@@ -991,16 +1381,24 @@ static StepKernel pick_step(unsigned mode, int tz, int tx, int spb)
     }
 }
 
-hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a, hipStream_t s)
+// With a start/stop event pair the dispatch itself carries the timestamps
+// (hipExtLaunchKernelGGL), so the elapsed time is the kernel's execution.
+static hipError_t launch_k(StepKernel k, int grid, const StepArgs& b, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+{
+    if (e0 && e1) hipExtLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, e0, e1, 0, b);
+    else hipLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a, hipStream_t s, hipEvent_t e0,
+                       hipEvent_t e1)
 {
     StepKernel k = pick_step(mode, tz, tx, spb);
     if (!k) return hipErrorInvalidValue;
     // One workgroup per (tile, group of spb shots).
     StepArgs b = a;
     b.gridLogical = a.ntiles * ((a.nshots + spb - 1) / spb);
-    const int grid = (b.gridLogical + 7) / 8 * 8;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, b);
-    return hipGetLastError();
+    return launch_k(k, (b.gridLogical + 7) / 8 * 8, b, s, e0, e1);
 }
 
 template <unsigned MODE>
@@ -1013,7 +1411,8 @@ static StepKernel pick_generic_tile(int tz, int tx)
     return nullptr;
 }
 
-hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a, hipStream_t s)
+hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a, hipStream_t s, hipEvent_t e0,
+                               hipEvent_t e1)
 {
     StepKernel k = nullptr;
     switch (mode) {
@@ -1027,12 +1426,10 @@ hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a,
     if (!k) return hipErrorInvalidValue;
     StepArgs b = a;
     b.gridLogical = a.ntiles * a.nshots;               // one workgroup per (tile, shot)
-    const int grid = (b.gridLogical + 7) / 8 * 8;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, b);
-    return hipGetLastError();
+    return launch_k(k, (b.gridLogical + 7) / 8 * 8, b, s, e0, e1);
 }
 
-hipError_t launch_fwd_pair(int tz, int tx, const StepArgs& a, hipStream_t s)
+hipError_t launch_fwd_pair(int tz, int tx, const StepArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
     StepKernel k = nullptr;
     if (tz == 64 && tx == 16) k = k_fwd_pair<64, 16>;
@@ -1040,8 +1437,24 @@ hipError_t launch_fwd_pair(int tz, int tx, const StepArgs& a, hipStream_t s)
     else if (tz == 32 && tx == 16) k = k_fwd_pair<32, 16>;
     else if (tz == 32 && tx == 32) k = k_fwd_pair<32, 32>;
     if (!k) return hipErrorInvalidValue;
+    return launch_k(k, (a.gridLogical + 7) / 8 * 8, a, s, e0, e1);
+}
+
+hipError_t launch_fwd_pair2(const Pair2Args& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+{
     const int grid = (a.gridLogical + 7) / 8 * 8;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, a);
+    auto k = k_fwd_pair2<kPair2EZ, kPair2EX>;
+    if (e0 && e1) hipExtLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, e0, e1, 0, a);
+    else hipLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_rec2(const Rec2Args& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+{
+    const int grid = (a.gridLogical + 7) / 8 * 8;
+    auto k = k_rec2<kPair2EZ, kPair2EX>;
+    if (e0 && e1) hipExtLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, e0, e1, 0, a);
+    else hipLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, a);
     return hipGetLastError();
 }
 

@@ -10,8 +10,9 @@ namespace aiso {
 // One shot in flight: its wavefield pair and the per-shot buffers the fused
 // step kernel reads or updates.  Unused members are null.
 struct ShotSlot {
-    float* p[2];          // wavefield pair [nx][nz]; the "current" one is picked by parity
-    float* q[2];          // second pair (scattered field of the lockstep Born forward)
+    float* p[4];          // wavefield slices [nx][nz]: the pair p[0], p[1] ("current" picked by parity),
+                          // plus p[2], p[3] for the two-step kernel's out-of-place outputs
+    float* q[4];          // same for the scattered field of the lockstep Born forward
     float* W;             // coarse source wavefield [nts][nx][nz]
     float* ss[2];         // two coarse-rate accumulators / secondary sources [nx][nz]
     float* img;           // per-shot image [nExt][nx][nz]
@@ -77,6 +78,53 @@ struct StepArgs {
     long wQa, wQb;
 };
 
+// Two fine steps of both fields of the lockstep Born forward (sub == 2) per
+// launch (k_fwd_pair2).  Launch m advances the source field by steps 2m+1,
+// 2m+2 and the scattered field by steps 2m-1, 2m.  Each field reads its
+// current and old slices and writes the two new ones to the other two slices.
+struct Pair2Args {
+    const ShotSlot* slots;  // srcTile/srcEnt: source points of each tile's extended region,
+                            // recTile/recEnt: receiver points the tile owns
+    const float* v2dt2;
+    const float* cd;
+    const float* mtil;
+    int nz, nx, minPad, nts, ntw;
+    int tilesZ, ntiles, gridLogical;
+    float zc[6], xc[6];
+    int sC, sO, sA, sB;     // source field: current, old, new after step A, new after step B
+    int qC, qO, qA, qB;     // scattered field: same roles in ShotSlot::q
+    int doS, doQ, tailS;    // advance the source / scattered field; write SS[nts-1] after step B
+    int injA, injB;         // source-trace columns of steps A and B
+    long ssNew, ssPrev, ssTail;  // element offsets in W of SS[m], SS[m-1], SS[nts-1]
+    float wa, wb, wc;       // SS[m] = m~ * ((old*wa + current*wb) + newA*wc)
+    float ta, tb;           // SS[nts-1] = m~ * (newA*ta + newB*tb)
+    int its;                // coarse index of the scattered steps
+    float faA, faB, fbB;    // step A: + SS[m-1]*faA;  step B: + (SS[m-1]*faB + SS[m]*fbB)
+    float feA, ffA, feB, ffB;  // data record weights of steps A and B
+    int ntW;
+};
+
+// Born adjoint receiver pass, two fine steps per launch (sub == 2), backward
+// in time: launch m runs the steps (its, it2 = 1) and (its, it2 = 0) of
+// coarse step its = nts - 2 - m on the adjoint stepper, with the data
+// injection of each step and the imaging of the coarse sample at it2 = 1.
+struct Rec2Args {
+    const ShotSlot* slots;  // recTile/recEnt: receiver points of each tile's extended region
+    const float* v2dt2;
+    const float* cd;
+    int nz, nx, minPad, nts;
+    int tilesZ, ntiles, gridLogical;
+    float zc[6], xc[6];
+    int sC, sO, sA, sB;     // slices: current, old, new after step A, new after step B
+    int its;
+    float fcA, fdA, fcB, fdB;  // data injection: d[its]*fc + d[its+1]*fd
+    float wa, wb, wc;       // img += W[its+1] * ((old*wa + current*wb) + newA*wc)
+    long wImg;
+    int tail;               // last launch: img += W[0] * (newA*ta + newB*tb)
+    float ta, tb;
+    int ntW;
+};
+
 }  // namespace aiso
 
 namespace aiso {
@@ -95,10 +143,13 @@ constexpr unsigned MODE_BORN_SCAT_PRE = M_SS_PRE | M_REC_DATA;
 constexpr unsigned MODE_REC_WIN = M_ADJ | M_INJ_DATA | M_WIN;
 
 // spb: shots advanced per workgroup (1 or 2), sharing the tile's v2dt2 and sponge.
-hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a, hipStream_t s);
+// e0/e1 (optional): events timestamped by the dispatch itself.
+hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a, hipStream_t s,
+                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // The unspecialized step kernel (modes of the extended-Born operators:
 // MODE_SRC, MODE_SRC_WIN, MODE_EXT_SCAT, MODE_EXT_REC, MODE_REC_WIN).
-hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a, hipStream_t s);
+hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a, hipStream_t s,
+                               hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 hipError_t launch_ring(unsigned mode, const StepArgs& a, int maxRing, hipStream_t s);
 hipError_t launch_fill(float* p, long n, hipStream_t s);
 hipError_t launch_scale_model(const float* m, float* out, const float* v2dt2, const float* refl,
@@ -119,7 +170,12 @@ hipError_t launch_dev_forward(const float* reg, float* out, const int* cornerReg
                               int nDev, int nb, int nt, int add, hipStream_t s);
 // One fine step of the source field and, two fine steps behind, the
 // scattered field of the Born forward in one launch (sub == 2).
-hipError_t launch_fwd_pair(int tz, int tx, const StepArgs& a, hipStream_t s);
+hipError_t launch_fwd_pair(int tz, int tx, const StepArgs& a, hipStream_t s, hipEvent_t e0 = nullptr,
+                           hipEvent_t e1 = nullptr);
+// Extended region (EZ x EX) of k_fwd_pair2; the output tile is (EZ-10) x (EX-10).
+constexpr int kPair2EZ = 64, kPair2EX = 32;
+hipError_t launch_fwd_pair2(const Pair2Args& a, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+hipError_t launch_rec2(const Rec2Args& a, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 hipError_t launch_win_tail(const ShotSlot* slots, int nshots, int cur, int winOut, float wa, float wb,
                            long wA, long wImg, int ssA, const float* mtil, int nz, int nx, hipStream_t s);
 hipError_t launch_dev_adjoint(float* reg, const float* irreg, const int* adjPtr, const int* adjEnt,

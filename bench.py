@@ -29,8 +29,9 @@ HBM_PEAK_GBS = 8000.0
 # coarse-rate traffic of that pass spread over its `sub` fine steps:
 # source pass writes W (4 B), scattered pass reads W and m~ (8 B), receiver
 # pass reads W and updates the image (12 B).
-# The lockstep forward advances both forward fields per launch: its
-# point-updates count both, with (4 + 8) / 2 B of coarse traffic each.
+# The lockstep forward advances both forward fields (two fine steps each per
+# launch): its point-updates count all of them, with (4 + 8) / 2 B of coarse
+# traffic each.
 CO_BYTES = {"K_SRC": 4.0, "K_BORN_SCAT": 8.0, "K_BORN_REC": 12.0, "K_BORN_PAIR": 6.0}
 
 
@@ -45,7 +46,24 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-shots", type=int, default=1)
     ap.add_argument("--opt", action="append", default=[], help="engine option key=value (aiso_ctx_set_option)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed steps, copy 1 GiB device-to-device 3 times (HBM counter calibration)")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fwd_pair2_born_lsrtm_80x500x1200_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the lockstep Born forward kernel, from the committed
+    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("TRAFFIC_BYTES_PER_LAUNCH="):
+                return float(line.split("=", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(w, nshots):
@@ -112,6 +130,13 @@ def main():
     torch.cuda.synchronize()
     t = (time.perf_counter() - t0) / a.steps
     ok = bool(torch.isfinite(image).all().item()) and bool(torch.isfinite(data).all().item())
+    if a.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     kinds = {name: getattr(A, name) for name in CO_BYTES}
     stats = {name: ctx.profile_read(k) for name, k in kinds.items()}
@@ -154,8 +179,10 @@ def main():
         "shot_latency_ms": {"p50": float(np.percentile(lat, 50)) if lat else None,
                             "p99": float(np.percentile(lat, 99)) if lat else None},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms,
-                     "launches": st["launches"], "bytes_per_point": bpp},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if dom == "K_BORN_PAIR" else None, "avg_launch_ms": avg_ms,
+                     "launches": st["launches"], "bytes_per_point": bpp,
+                     "point_updates_per_launch": st["point_updates"] / max(st["launches"], 1)},
         "kernels": kern,
         "finite": ok,
     }
