@@ -24,6 +24,10 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              # keep hipcc from emitting packed FP32 instructions
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             # no fused multiply-adds: every kernel variant then rounds every
+             # product and sum the same way, as the reference-order CPU
+             # restatement does, so results do not depend on the variant
+             "-ffp-contract=off",
              "-I" + INC, "-I" + CSRC]
 
 

@@ -94,13 +94,21 @@ struct aiso_ctx {
     bool haveVel = false;
     std::vector<float> vel, v2dt2h, reflh;
     DBuf v2dt2, refl, cd;
-    int batchReq = 0, batchUsed = 0, tz = 32, tx = 32;
+    int batchReq = 0, batchUsed = 0;
+    int tz = 32, tx = 32;               // tile of the call in progress
+    // Step-kernel tile per direction: the forward's pair kernel runs best on
+    // 32 x 16 tiles, the adjoint passes on 32 x 32 (MI355X measurements).
+    int tzFwd = 32, txFwd = 16, tzAdj = 32, txAdj = 32;
     int streamNT = 1;
     size_t bytesPerShot = 0;
     // work buffers
     DBuf pairs, pairsQ, W, ss, img, dreg, sig, slots, slotsQ, mtil, acc, modelDev, dataDev;
     int lockstep = 1;
-    int pair2 = 1;                      // two fine steps per launch in the lockstep Born forward
+    // Two fine steps per launch in the Born forward and adjoint (sub == 2).
+    // They move half the HBM bytes of the one-step kernels but run more
+    // instructions per point (step A covers the tile plus halo), which on
+    // MI355X costs more than it saves, so they are opt-in.
+    int pair2 = 0;
     int batchFwd = 0, batchAdj = 0;     // per-direction overrides of batchReq
     int batchRec = 0;                   // receiver-pass group size within an adjoint batch
     int spb = 1;                        // shots per workgroup of the step kernel
@@ -317,6 +325,8 @@ std::vector<float> nl_source(const aiso_ctx* c, const Geometry& src, const float
 void run(aiso_ctx* c, const Request& q)
 {
     const FdParams& p = c->p;
+    c->tz = is_adj(q.op) ? c->tzAdj : c->tzFwd;
+    c->tx = is_adj(q.op) ? c->txAdj : c->txFwd;
     if (!c->haveVel) throw Error(AISO_ERR_STATE, "velocity not set");
     if (q.nShot < 1) throw Error(AISO_ERR_PARAM, "nShot must be >= 1");
     if (q.nRec != 1 && q.nRec != q.nShot) throw Error(AISO_ERR_PARAM, "receiver geometry count must be 1 or nShot");
@@ -1078,8 +1088,8 @@ aiso_status aiso_ctx_set_tile(aiso_ctx* c, int tz, int tx)
         if (!c) throw Error(AISO_ERR_PARAM, "null context");
         if ((tz != 32 && tz != 64) || (tx != 16 && tx != 32))
             throw Error(AISO_ERR_PARAM, "tile must be tz in {32,64}, tx in {16,32}");
-        c->tz = tz;
-        c->tx = tx;
+        c->tzFwd = c->tzAdj = tz;
+        c->txFwd = c->txAdj = tx;
     });
 }
 
@@ -1091,6 +1101,13 @@ aiso_status aiso_ctx_set_option(aiso_ctx* c, const char* key, int value)
         if (k == "streamNT") c->streamNT = value != 0;
         else if (k == "lockstep") c->lockstep = value != 0;
         else if (k == "pair2") c->pair2 = value != 0;
+        else if (k == "tileFwd" || k == "tileAdj") {   // tz * 100 + tx
+            const int tz = value / 100, tx = value % 100;
+            if ((tz != 32 && tz != 64) || (tx != 16 && tx != 32))
+                throw Error(AISO_ERR_PARAM, "tile must be tz in {32,64}, tx in {16,32} (value tz*100+tx)");
+            if (k == "tileFwd") { c->tzFwd = tz; c->txFwd = tx; }
+            else { c->tzAdj = tz; c->txAdj = tx; }
+        }
         else if (k == "batchFwd") c->batchFwd = std::max(0, value);
         else if (k == "batchAdj") c->batchAdj = std::max(0, value);
         else if (k == "batchRec") c->batchRec = std::max(0, value);

@@ -278,8 +278,10 @@ struct ShotsBase {
         // per launch and the step-kernel tile.
         int batch = par.i("shotsPerLaunch", 0);
         if (batch > 0) check(aiso_ctx_set_batch(ctx, batch));
-        check(aiso_ctx_set_tile(ctx, par.i("tileZ", 32), par.i("tileX", 32)));
+        const int tz = par.i("tileZ", 0), tx = par.i("tileX", 0);
+        if (tz > 0 || tx > 0) check(aiso_ctx_set_tile(ctx, tz > 0 ? tz : 32, tx > 0 ? tx : 32));
         check(aiso_ctx_set_option(ctx, "lockstep", par.i("lockstep", 1)));
+        check(aiso_ctx_set_option(ctx, "pair2", par.i("twoStep", 0)));
         fp = q;
         dirty = false;
     }

@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--shots", type=int, default=80)
     ap.add_argument("--batch", type=int, default=0, help="shots per launch (0 = by HBM budget)")
-    ap.add_argument("--tile", type=str, default="32,32")
+    ap.add_argument("--tile", type=str, default="", help="tz,tx for every call (default: engine's per-direction tiles)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-shots", type=int, default=1)
     ap.add_argument("--opt", action="append", default=[], help="engine option key=value (aiso_ctx_set_option)")
@@ -51,7 +51,7 @@ def parse():
     return ap.parse_args()
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fwd_pair2_born_lsrtm_80x500x1200_hbm_traffic.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fwd_pair_born_lsrtm_80x500x1200_hbm_traffic.md")
 
 
 def measured_traffic():
@@ -100,8 +100,9 @@ def main():
     p = w.params
     ctx = A.Context(p)
     ctx.set_velocity(w.vel)
-    tz, tx = (int(v) for v in a.tile.split(","))
-    ctx.set_tile(tz, tx)
+    if a.tile:
+        tz, tx = (int(v) for v in a.tile.split(","))
+        ctx.set_tile(tz, tx)
     if a.batch:
         ctx.set_batch(a.batch)
     for kv in a.opt:
@@ -173,7 +174,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": w.name, "shots": w.nShot, "grid": "500x1200 (padded %dx%d)" % (p["nz"], p["nx"]),
-                   "nts": p["nts"], "sub": p["sub"], "shots_per_launch_adj": info["batch"], "tile": [tz, tx],
+                   "nts": p["nts"], "sub": p["sub"], "shots_per_launch_adj": info["batch"], "tile": a.tile or "32x16 forward, 32x32 adjoint",
                    "inputs": "model/data/image resident in HBM"},
         "job_roofline_frac": job_bytes / t / 1e9 / HBM_PEAK_GBS,
         "shot_latency_ms": {"p50": float(np.percentile(lat, 50)) if lat else None,

@@ -88,11 +88,15 @@ aiso_status aiso_ctx_set_velocity(aiso_ctx* ctx, const float* vel, int mem);
 
 /* Shots advanced together by one launch (0 = automatic, by free HBM). */
 aiso_status aiso_ctx_set_batch(aiso_ctx* ctx, int shotsPerLaunch);
-/* Tile shape of the time-step kernel: tz in {32,64}, tx in {16,32}. */
+/* Tile shape of the time-step kernel for forward and adjoint calls: tz in
+ * {32,64}, tx in {16,32} (default 32 x 16 forward, 32 x 32 adjoint). */
 aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
 /* Engine tuning knobs by name (no counterpart in the reference):
  *   "streamNT" 0/1: non-temporal access to the coarse wavefield stream;
  *   "lockstep" 0/1: Born forward with sub == 2 as one lockstep pass;
+ *   "tileFwd", "tileAdj": tile of forward / adjoint calls, tz * 100 + tx;
+ *   "pair2" 0/1: two fine steps per launch in the Born forward (lockstep)
+ *       and adjoint when sub == 2 and no device lies in the FAT ring;
  *   "batchFwd", "batchAdj": shots per launch of forward / adjoint calls
  *       (0 = aiso_ctx_set_batch's value, or automatic);
  *   "batchRec": group size of the adjoint receiver pass within a batch;
