@@ -56,6 +56,7 @@ _SIGS = {
     "aiso_device_adjoint": (_I, [_P, _I, _P, _P, _I]),
     "aiso_nl_forward": (_I, [_P, _I, _P, _I, _P, _PP, _I, _PP, _I, _I]),
     "aiso_nl_adjoint": (_I, [_P, _I, _P, _I, _P, _PP, _I, _PP, _I, _I]),
+    "aiso_nl_velocity_gradient": (_I, [_P, _P, _I, _P, _P, _PP, _I, _PP, _I, _I, _I]),
     "aiso_born_forward": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_born_adjoint": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_bornext_forward": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
@@ -237,6 +238,16 @@ class Context:
         r, _kr = _devarray(receivers)
         _check(lib().aiso_nl_adjoint(self.h, int(add), _ptr(model), model.shape[0], _ptr(data), s, len(sources),
                                      r, len(receivers), mem))
+
+    def nl_velocity_gradient(self, model, data_grad, vel_grad, sources, receivers, checkpoint_every=0,
+                             mem=MEM_HOST):
+        """dJ/d(velocity) [nx][nz] given dJ/d(data) of nl_forward(model)."""
+        s, _ks = _devarray(sources)
+        r, _kr = _devarray(receivers)
+        if mem == MEM_HOST:
+            model = np.ascontiguousarray(model, np.float32)
+        _check(lib().aiso_nl_velocity_gradient(self.h, _ptr(model), model.shape[0], _ptr(data_grad), _ptr(vel_grad),
+                                               s, len(sources), r, len(receivers), int(checkpoint_every), mem))
 
     def _born(self, fn, model, data, sources, wavelets, receivers, add, mem):
         s, _ks = _devarray(sources)

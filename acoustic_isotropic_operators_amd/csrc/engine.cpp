@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <cstring>
 #include <map>
@@ -103,6 +104,8 @@ struct aiso_ctx {
     size_t bytesPerShot = 0;
     // work buffers
     DBuf pairs, pairsQ, W, ss, img, dreg, sig, slots, slotsQ, mtil, acc, modelDev, dataDev;
+    DBuf ck;                            // checkpoints of the velocity-gradient pass
+    hipEvent_t evIn = nullptr;          // orders device inputs after the default stream's work
     int lockstep = 1;
     // Two fine steps per launch in the Born forward and adjoint (sub == 2).
     // They move half the HBM bytes of the one-step kernels but run more
@@ -260,6 +263,16 @@ hipEvent_t pooled_event(aiso_ctx* c)
     return e;
 }
 
+// Device-memory operands may have been written by the caller on the legacy
+// default stream (PyTorch's default): the context's non-blocking stream waits
+// for that work before it reads them.
+void wait_default_stream(aiso_ctx* c)
+{
+    if (!c->evIn) HIPC(hipEventCreateWithFlags(&c->evIn, hipEventDisableTiming));
+    HIPC(hipEventRecord(c->evIn, nullptr));
+    HIPC(hipStreamWaitEvent(c->stream, c->evIn, 0));
+}
+
 void launch_timed(aiso_ctx* c, unsigned mode, int kind, const StepArgs& a, int nshots, bool generic = false)
 {
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -351,6 +364,7 @@ void run(aiso_ctx* c, const Request& q)
             if (v >= p.nzx()) throw Error(AISO_ERR_GEOM, "receiver outside the velocity grid");
 
     hipStream_t s = c->stream;
+    if (q.mem == AISO_MEM_DEVICE) wait_default_stream(c);
     const long nzx = p.nzx();
     const int nExt = is_ext(q.op) ? p.nExt : 1;
     const long modelN = is_born(q.op) ? nzx * nExt : (long)q.nShotSig * nSrcDev * p.nts;
@@ -1000,6 +1014,257 @@ aiso_status guard(F&& f)
     }
 }
 
+// Velocity gradient of the nonlinear operator by the adjoint-state method
+// (no counterpart in the reference, whose FWI takes the Born adjoint as its
+// gradient, PY/fwiMain.py:123-142).  The forward recursion of one fine step,
+//   p_t = D (v2dt2 * L p_{t-1} + 2 p_{t-1} - D p_{t-2} + s_t),
+// has as transpose the nonlinear adjoint propagation; driven by dJ/dd its
+// damped slices nu_t give dJ/dv2dt2 = sum_t (L p_{t-1}) * nu_t, plus at the
+// source points sum_t nu_t * u_t for the v2dt2 factor of the injected source
+// s = v2dt2 * u.  The forward slices are recomputed backward in segments of
+// K steps from checkpoints (the pair of slices every K steps).  Per batch of
+// shots: one forward pass writing checkpoints, then per segment, last first,
+// a forward recompute into W (K slices) and the K adjoint steps, each
+// followed by the imaging kernel.
+void run_nl_grad(aiso_ctx* c, const float* model, int nShotSig, const float* dataGrad, float* velGrad,
+                 aiso_device* const* srcs, int nShot, aiso_device* const* recs, int nRec, int every, int mem)
+{
+    const FdParams& p = c->p;
+    c->tz = c->tzAdj;
+    c->tx = c->txAdj;
+    if (!c->haveVel) throw Error(AISO_ERR_STATE, "velocity not set");
+    if (nShot < 1) throw Error(AISO_ERR_PARAM, "nShot must be >= 1");
+    if (nRec != 1 && nRec != nShot) throw Error(AISO_ERR_PARAM, "receiver geometry count must be 1 or nShot");
+    if (nShotSig != 1 && nShotSig != nShot) throw Error(AISO_ERR_PARAM, "model shot axis must be 1 or nShot");
+    if (p.sub >= kSubMax) throw Error(AISO_ERR_PARAM, "**** ERROR: Subsampling parameter too high ****");
+    if (every < 0) throw Error(AISO_ERR_PARAM, "checkpoint interval must be >= 0");
+    const int nRecDev = recs[0]->g.nDev, nSrcDev = srcs[0]->g.nDev;
+    for (int i = 0; i < nRec; ++i)
+        if (recs[i]->g.nDev != nRecDev) throw Error(AISO_ERR_GEOM, "receivers per shot must be constant");
+    for (int i = 0; i < nShot; ++i)
+        if (srcs[i]->g.nDev != nSrcDev) throw Error(AISO_ERR_GEOM, "sources per shot must be constant");
+
+    hipStream_t s = c->stream;
+    if (mem == AISO_MEM_DEVICE) wait_default_stream(c);
+    const long nzx = p.nzx();
+    const int h = p.h(), T = p.ntw - 1;
+    const float* f = p.filt.data();
+    const int K = every > 0 ? std::min(every, T) : std::max(1, (int)std::ceil(std::sqrt((double)T)));
+    const int nCk = (T - 1) / K + 1;                   // checkpoints at t = 0, K, 2K, ... < T
+
+    std::vector<float> hModel((size_t)nShotSig * nSrcDev * p.nts);
+    if (mem == AISO_MEM_DEVICE) HIPC(hipMemcpy(hModel.data(), model, hModel.size() * 4, hipMemcpyDeviceToHost));
+    else std::memcpy(hModel.data(), model, hModel.size() * 4);
+    const long dataN = (long)nShot * nRecDev * p.nts;
+    const float* dGrad = dataGrad;
+    if (mem != AISO_MEM_DEVICE) {
+        float* d = (float*)c->dataDev.need(dataN * 4);
+        HIPC(hipMemcpyAsync(d, dataGrad, dataN * 4, hipMemcpyHostToDevice, s));
+        dGrad = d;
+    }
+    std::vector<DevGeom*> srcG(nShot), recG(nRec);
+    int nSrcRegMax = 1, nRecRegMax = 1;
+    for (int i = 0; i < nShot; ++i) {
+        srcG[i] = dev_geom(c, srcs[i]);
+        nSrcRegMax = std::max(nSrcRegMax, srcG[i]->nReg);
+    }
+    for (int i = 0; i < nRec; ++i) {
+        recG[i] = dev_geom(c, recs[i]);
+        nRecRegMax = std::max(nRecRegMax, recG[i]->nReg);
+    }
+    int maxSrcRing = 0, maxRecRing = 0;
+    for (auto* g : srcG) maxSrcRing = std::max(maxSrcRing, g->nRing);
+    for (auto* g : recG) maxRecRing = std::max(maxRecRing, g->nRing);
+
+    // per shot: forward pair, adjoint pair, K segment slices, gradient, checkpoints
+    const size_t perShot = (size_t)(2 + 2 + K + 1 + 2 * nCk) * nzx * 4 + (size_t)nRecRegMax * p.nts * 4 +
+                           2 * (size_t)nSrcRegMax * p.ntw * 4;
+    int B = c->batchAdj > 0 ? c->batchAdj : (c->batchReq > 0 ? c->batchReq : 8);
+    {
+        size_t freeB = 0, totalB = 0;
+        HIPC(hipMemGetInfo(&freeB, &totalB));
+        size_t have = c->pairs.n + c->pairsQ.n + c->W.n + c->img.n + c->dreg.n + c->sig.n + c->ss.n + c->ck.n;
+        B = std::min(B, (int)std::max<size_t>(1, (size_t)((freeB + have) * 0.85) / perShot));
+    }
+    B = std::min(B, nShot);
+    c->batchUsed = B;
+    float* pairs = (float*)c->pairs.need((size_t)B * 2 * nzx * 4);
+    float* pairsA = (float*)c->pairsQ.need((size_t)B * 2 * nzx * 4);
+    float* W = (float*)c->W.need((size_t)B * K * nzx * 4);
+    float* img = (float*)c->img.need((size_t)B * nzx * 4);
+    float* ck = (float*)c->ck.need((size_t)B * 2 * nCk * nzx * 4);
+    float* dreg = (float*)c->dreg.need((size_t)B * nRecRegMax * p.nts * 4);
+    float* sig = (float*)c->sig.need((size_t)B * nSrcRegMax * p.ntw * 4);
+    float* sigA = (float*)c->ss.need((size_t)B * nSrcRegMax * p.ntw * 4);
+    ShotSlot* dF = (ShotSlot*)c->slots.need(sizeof(ShotSlot) * B);
+    ShotSlot* dA = (ShotSlot*)c->slotsQ.need(sizeof(ShotSlot) * B);
+    float* acc = (float*)c->acc.need(nzx * 4);
+    HIPC(launch_fill(acc, nzx, s));
+
+    StepArgs base{};
+    base.v2dt2 = c->v2dt2.as<float>();
+    base.cd = c->cd.as<float>();
+    base.nz = p.nz; base.nx = p.nx; base.minPad = p.minPad; base.nts = p.nts; base.ntw = p.ntw;
+    base.tilesZ = tilesZ(c);
+    base.ntiles = base.tilesZ * tilesX(c);
+    for (int k = 0; k < 6; ++k) { base.zc[k] = p.zc[k]; base.xc[k] = p.xc[k]; }
+    base.ntW = c->streamNT;
+    GradArgs ga{};
+    ga.nz = p.nz; ga.nx = p.nx;
+    ga.tilesZ = base.tilesZ;
+    ga.ntiles = base.ntiles;
+    for (int k = 0; k < 6; ++k) { ga.zc[k] = p.zc[k]; ga.xc[k] = p.xc[k]; }
+
+    // source-scaling terms: per shot, per regular source point
+    std::vector<std::vector<std::pair<int, double>>> srcTerms(nShot);
+    for (int b0 = 0; b0 < nShot; b0 += B) {
+        const int nb = std::min(B, nShot - b0);
+        std::vector<ShotSlot> fs(nb), as(nb);
+        std::vector<float> sigHost((size_t)nb * nSrcRegMax * p.ntw, 0.f);
+        std::vector<std::vector<float>> uHost(nb);
+        for (int k = 0; k < nb; ++k) {
+            const int shot = b0 + k;
+            DevGeom* sg = srcG[shot];
+            DevGeom* rg = recG[nRec == 1 ? 0 : shot];
+            ShotSlot& F = fs[k];
+            std::memset(&F, 0, sizeof(F));
+            F.p[0] = pairs + (size_t)k * 2 * nzx;
+            F.p[1] = F.p[0] + nzx;
+            F.W = W + (size_t)k * K * nzx;
+            F.img = img + (size_t)k * nzx;
+            F.dreg = dreg + (size_t)k * nRecRegMax * p.nts;
+            F.sig = sig + (size_t)k * nSrcRegMax * p.ntw;
+            F.srcTile = sg->tilePtr.as<int>();
+            F.srcEnt = sg->ent.as<int2>();
+            F.srcRing = sg->ring.as<int2>();
+            F.nSrcRing = sg->nRing;
+            F.recTile = rg->tilePtr.as<int>();
+            F.recEnt = rg->ent.as<int2>();
+            F.recRing = rg->ring.as<int2>();
+            F.nRecRing = rg->nRing;
+            ShotSlot& A = as[k];
+            A = F;
+            A.p[0] = pairsA + (size_t)k * 2 * nzx;
+            A.p[1] = A.p[0] + nzx;
+            A.sig = sigA + (size_t)k * nSrcRegMax * p.ntw;
+            const Geometry& sgm = srcs[shot]->g;
+            const float* m = hModel.data() + (size_t)(nShotSig == 1 ? 0 : shot) * nSrcDev * p.nts;
+            std::vector<float> fine = nl_source(c, sgm, m);
+            std::memcpy(sigHost.data() + (size_t)k * nSrcRegMax * p.ntw, fine.data(), fine.size() * 4);
+            const int nReg = (int)sgm.regPos.size();
+            std::vector<float> reg((size_t)nReg * p.nts);
+            uHost[k].resize((size_t)nReg * p.ntw);
+            sgm.adjoint(false, reg.data(), m, p.nts);
+            interp_time_fwd(p, nReg, reg.data(), uHost[k].data());   // unscaled source
+            HIPC(launch_dev_adjoint(dreg + (size_t)k * nRecRegMax * p.nts, dGrad + (size_t)shot * nRecDev * p.nts,
+                                    rg->adjPtr.as<int>(), rg->adjEnt.as<int>(), rg->weight.as<float>(), rg->nReg,
+                                    recs[nRec == 1 ? 0 : shot]->g.nb, p.nts, s));
+        }
+        HIPC(hipMemcpyAsync(dF, fs.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
+        HIPC(hipMemcpyAsync(dA, as.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
+        HIPC(hipMemcpyAsync(sig, sigHost.data(), sigHost.size() * 4, hipMemcpyHostToDevice, s));
+        HIPC(launch_fill(sigA, (long)nb * nSrcRegMax * p.ntw, s));
+        HIPC(launch_fill(img, (long)nb * nzx, s));
+        HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
+        HIPC(launch_fill(pairsA, (long)nb * 2 * nzx, s));
+        // The recompute writes interior points only; the imaging kernel's
+        // Laplacian reads the FAT ring of these slices, which must hold 0.
+        HIPC(launch_fill(W, (long)nb * K * nzx, s));
+
+        StepArgs a = base;
+        a.nshots = nb;
+        a.gridLogical = a.ntiles * nb;
+        ga.gridLogical = a.gridLogical;
+        auto ckSlice = [&](int k, int j, int which) { return ck + (((size_t)k * nCk + j) * 2 + which) * nzx; };
+        auto save = [&](int j, int cur) {
+            for (int k = 0; k < nb; ++k) {
+                HIPC(hipMemcpyAsync(ckSlice(k, j, 0), fs[k].p[cur], nzx * 4, hipMemcpyDeviceToDevice, s));
+                HIPC(hipMemcpyAsync(ckSlice(k, j, 1), fs[k].p[cur ^ 1], nzx * 4, hipMemcpyDeviceToDevice, s));
+            }
+        };
+        // forward: one fine step = MODE_SRC without capture (step, injection, sponge)
+        auto fwd_step = [&](int t, int cur, long capture) {
+            StepArgs x = a;
+            x.slots = dF;
+            x.cur = cur;
+            x.itwInj = t - 1;
+            x.fe = 1.f;
+            x.ff = 0.f;
+            x.assignA = capture >= 0;
+            x.skipA = capture < 0;
+            x.wA = capture >= 0 ? capture : 0;
+            x.assignB = 0;
+            x.skipB = 1;
+            launch_timed(c, MODE_SRC, AISO_K_SRC, x, nb);
+            if (maxSrcRing > 0) HIPC(launch_ring(M_INJ_SIG, x, maxSrcRing, s));
+        };
+        int cur = 1;
+        save(0, cur);
+        for (int t = 1; t < T; ++t) {
+            fwd_step(t, cur, -1);
+            cur ^= 1;
+            if (t % K == 0) save(t / K, cur);
+        }
+        // backward, segment by segment
+        StepArgs r = a;
+        r.slots = dA;
+        r.cur = 1;
+        ga.slots = dA;
+        for (int j = nCk - 1; j >= 0; --j) {
+            const int t0 = j * K, t1 = std::min(t0 + K, T);
+            for (int k = 0; k < nb; ++k) {
+                HIPC(hipMemcpyAsync(fs[k].p[1], ckSlice(k, j, 0), nzx * 4, hipMemcpyDeviceToDevice, s));
+                HIPC(hipMemcpyAsync(fs[k].p[0], ckSlice(k, j, 1), nzx * 4, hipMemcpyDeviceToDevice, s));
+                HIPC(hipMemcpyAsync(fs[k].W, ckSlice(k, j, 0), nzx * 4, hipMemcpyDeviceToDevice, s));
+            }
+            int fc = 1;
+            for (int t = t0 + 1; t < t1; ++t) {          // W[t - t0] = p_t
+                fwd_step(t, fc, (long)(t - t0) * nzx);
+                fc ^= 1;
+            }
+            for (int t = t1; t > t0; --t) {              // nu_t, then img += L(p_{t-1}) nu_t
+                const int itw = t - 1, its = itw / p.sub, it2 = itw % p.sub;
+                r.its = its;
+                r.itwRec = itw;
+                r.fc = f[it2 + 1];
+                r.fd = f[h + it2 + 1];
+                launch_timed(c, MODE_NL_ADJ, AISO_K_NL_ADJ, r, nb);
+                if (maxSrcRing > 0 || maxRecRing > 0)
+                    HIPC(launch_ring(M_INJ_DATA | M_REC_SIG, r, std::max(maxSrcRing, maxRecRing), s));
+                r.cur ^= 1;
+                ga.wF = (long)(t - 1 - t0) * nzx;
+                ga.nu = r.cur;
+                HIPC(launch_grad(c->tz, c->tx, ga, s));
+            }
+        }
+        for (int k = 0; k < nb; ++k) HIPC(launch_accumulate(acc, img + (size_t)k * nzx, nzx, s));
+        std::vector<float> sa((size_t)nb * nSrcRegMax * p.ntw);
+        HIPC(hipMemcpyAsync(sa.data(), sigA, sa.size() * 4, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        for (int k = 0; k < nb; ++k) {
+            const Geometry& sgm = srcs[b0 + k]->g;
+            for (int rr = 0; rr < (int)sgm.regPos.size(); ++rr) {
+                double sum = 0.0;
+                const float* x = sa.data() + (size_t)k * nSrcRegMax * p.ntw + (size_t)rr * p.ntw;
+                const float* u = uHost[k].data() + (size_t)rr * p.ntw;
+                for (int i = 0; i < p.ntw; ++i) sum += (double)x[i] * (double)u[i];
+                srcTerms[b0 + k].push_back({sgm.regPos[rr], sum});
+            }
+        }
+        drain_profile(c);
+    }
+    std::vector<float> G2(nzx);
+    HIPC(hipMemcpyAsync(G2.data(), acc, nzx * 4, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    for (auto& shot : srcTerms)
+        for (auto& t : shot) G2[t.first] += (float)t.second;
+    const float dtw = p.dtw;
+    for (long i = 0; i < nzx; ++i) G2[i] = G2[i] * (((2.f * c->vel[i]) * dtw) * dtw);
+    if (mem == AISO_MEM_DEVICE) HIPC(hipMemcpyAsync(velGrad, G2.data(), nzx * 4, hipMemcpyHostToDevice, s));
+    else std::memcpy(velGrad, G2.data(), nzx * 4);
+    HIPC(hipStreamSynchronize(s));
+}
+
 aiso_status op_call(aiso_ctx* c, Op op, int add, const float* mIn, float* mOut, const float* dIn, float* dOut,
                     int nShotSig, aiso_device* const* src, int nShot, const float* wav, int nWav,
                     aiso_device* const* rec, int nRec, int mem)
@@ -1045,6 +1310,7 @@ aiso_status aiso_ctx_destroy(aiso_ctx* c)
         (void)hipStreamSynchronize(c->stream);
         for (auto& e : c->ev) { (void)hipEventDestroy(e.e0); (void)hipEventDestroy(e.e1); }
         for (auto e : c->evPool) (void)hipEventDestroy(e);
+        if (c->evIn) (void)hipEventDestroy(c->evIn);
         c->geoms.clear();
         hipStream_t s = c->stream;
         delete c;
@@ -1282,6 +1548,18 @@ aiso_status aiso_nl_adjoint(aiso_ctx* c, int add, float* model, int nShotSig, co
                             aiso_device* const* src, int nShot, aiso_device* const* rec, int nRec, int mem)
 {
     return op_call(c, Op::NlAdj, add, nullptr, model, data, nullptr, nShotSig, src, nShot, nullptr, 1, rec, nRec, mem);
+}
+
+aiso_status aiso_nl_velocity_gradient(aiso_ctx* c, const float* model, int nShotSig, const float* dataGrad,
+                                      float* velGrad, aiso_device* const* src, int nShot, aiso_device* const* rec,
+                                      int nRec, int checkpointEvery, int mem)
+{
+    return guard([&] {
+        if (!c) throw Error(AISO_ERR_STATE, "null context");
+        if (!src || !rec || !model || !dataGrad || !velGrad) throw Error(AISO_ERR_PARAM, "null argument");
+        HIPC(hipSetDevice(c->device));
+        run_nl_grad(c, model, nShotSig, dataGrad, velGrad, src, nShot, rec, nRec, checkpointEvery, mem);
+    });
 }
 
 aiso_status aiso_born_forward(aiso_ctx* c, int add, const float* model, float* data, aiso_device* const* src,

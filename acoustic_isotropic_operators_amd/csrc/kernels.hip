@@ -922,6 +922,40 @@ __global__ __launch_bounds__(NT) void k_rec2(const Rec2Args a)
     }
 }
 
+// Velocity-gradient imaging of the nonlinear operator, one (tile, shot) per
+// workgroup: img += L(p_{t-1}) * nu_t at the tile's points, the Laplacian in
+// the stepper's order on an LDS-staged window of the forward slice.
+template <int TZ, int TX>
+__global__ __launch_bounds__(NT) void k_grad(const GradArgs a)
+{
+    using T = TileShape<TZ, TX>;
+    constexpr int PX = T::PX;
+    __shared__ float sC[T::LX * T::LZ];
+    const int logical = xcd_logical(blockIdx.x, gridDim.x);
+    if (logical >= a.gridLogical) return;             // padding blocks (block-uniform)
+    const int shot = logical / a.ntiles, tile = logical - shot * a.ntiles;
+    const int z0 = FAT + (tile % a.tilesZ) * TZ, x0 = FAT + (tile / a.tilesZ) * TX;
+    const int nz = a.nz, nx = a.nx;
+    const int lane = threadIdx.x % TZ, wy = threadIdx.x / TZ, lx0 = wy * PX, gz = z0 + lane;
+    const ShotSlot& S = a.slots[shot];
+    const unsigned soff = (unsigned)((x0 - FAT + wy) * nz + (z0 - FAT + lane));
+    stage_tile<TZ, TX, false, false>(sC, G(S.W + a.wF), nullptr, soff, nz, nx, x0, z0, lane, wy);
+    __syncthreads();
+    float w[PX + 2 * FAT];
+#pragma unroll
+    for (int j = 0; j < PX + 2 * FAT; ++j) w[j] = sC[(lx0 + j) * T::LZ + lane + FAT];
+    cgf* nu = G(S.p[a.nu]);
+    gf* img = G(S.img);
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+        const int gx = x0 + lx0 + j;
+        if (gz >= nz - FAT || gx >= nx - FAT) continue;
+        const unsigned gi = (unsigned)(gx * nz + gz);
+        const float lap = lap_at<TZ, TX>(sC, w, j, lx0, lane, a);
+        img[gi] += lap * nu[gi];
+    }
+}
+
 // Unspecialized form of the fused step: one shot per workgroup, every
 // operand addressed through generic pointers, bounds and sponge tested at
 // every point.  The extended-Born operators run on it.
@@ -1455,6 +1489,19 @@ hipError_t launch_rec2(const Rec2Args& a, hipStream_t s, hipEvent_t e0, hipEvent
     auto k = k_rec2<kPair2EZ, kPair2EX>;
     if (e0 && e1) hipExtLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, e0, e1, 0, a);
     else hipLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad(int tz, int tx, const GradArgs& a, hipStream_t s)
+{
+    using GradKernel = void (*)(const GradArgs);
+    GradKernel g = nullptr;
+    if (tz == 64 && tx == 16) g = k_grad<64, 16>;
+    else if (tz == 64 && tx == 32) g = k_grad<64, 32>;
+    else if (tz == 32 && tx == 16) g = k_grad<32, 16>;
+    else if (tz == 32 && tx == 32) g = k_grad<32, 32>;
+    if (!g) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(g, dim3((a.gridLogical + 7) / 8 * 8), dim3(NT), 0, s, a);
     return hipGetLastError();
 }
 

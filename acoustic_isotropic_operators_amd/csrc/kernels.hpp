@@ -125,6 +125,16 @@ struct Rec2Args {
     int ntW;
 };
 
+// Velocity-gradient imaging of the nonlinear operator: img += L(p_{t-1}) * nu_t,
+// with p_{t-1} a forward slice stored in W and nu_t the adjoint slice p[nu].
+struct GradArgs {
+    const ShotSlot* slots;
+    int nz, nx, tilesZ, ntiles, gridLogical;
+    float zc[6], xc[6];
+    long wF;                // element offset in W of p_{t-1}
+    int nu;                 // ShotSlot::p index of nu_t
+};
+
 }  // namespace aiso
 
 namespace aiso {
@@ -176,6 +186,7 @@ hipError_t launch_fwd_pair(int tz, int tx, const StepArgs& a, hipStream_t s, hip
 constexpr int kPair2EZ = 64, kPair2EX = 32;
 hipError_t launch_fwd_pair2(const Pair2Args& a, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 hipError_t launch_rec2(const Rec2Args& a, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+hipError_t launch_grad(int tz, int tx, const GradArgs& a, hipStream_t s);
 hipError_t launch_win_tail(const ShotSlot* slots, int nshots, int cur, int winOut, float wa, float wb,
                            long wA, long wImg, int ssA, const float* mtil, int nz, int nx, hipStream_t s);
 hipError_t launch_dev_adjoint(float* reg, const float* irreg, const int* adjPtr, const int* adjEnt,

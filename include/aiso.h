@@ -23,7 +23,10 @@
  *     [iShot|0][iSrcDevice][its]; extended models [iExt][ix][iz];
  *   - add == 0 overwrites the output, add != 0 accumulates into it;
  *   - every call returns AISO_OK or an error code; aiso_last_error() gives the
- *     message (the reference prints it and throws std::runtime_error).
+ *     message (the reference prints it and throws std::runtime_error);
+ *   - with AISO_MEM_DEVICE, work the caller queued on the legacy default
+ *     stream before the call is ordered before the call's reads (work on other
+ *     streams must be complete); outputs are complete when the call returns.
  * Each context owns all of its state (no globals), so contexts are independent;
  * calls on one context must not overlap.
  */
@@ -176,6 +179,21 @@ aiso_status aiso_nl_forward(aiso_ctx* ctx, int add, const float* model, int nSho
 aiso_status aiso_nl_adjoint(aiso_ctx* ctx, int add, float* model, int nShotSig,
                             const float* data, aiso_device* const* sources, int nShot,
                             aiso_device* const* receivers, int nRecGeom, int mem);
+
+/* Velocity gradient of a data misfit through the nonlinear operator, by the
+ * adjoint-state method with checkpointed forward recomputation.  No
+ * counterpart in the reference (its FWI takes the Born adjoint as the
+ * gradient, PY/fwiMain.py:123-142); it serves torch.autograd and FWI callers.
+ * model: the source signals of aiso_nl_forward ([nShotSig][nSrcDev][nts]);
+ * dataGrad: dJ/d(data) [nShot][nRec][nts]; velGrad (out): dJ/d(velocity)
+ * on the padded grid [nx][nz], overwritten.  checkpointEvery: fine steps
+ * between checkpoints (0 = ceil(sqrt(ntw - 1))).  mem applies to dataGrad
+ * and velGrad (the model is read through the host either way). */
+aiso_status aiso_nl_velocity_gradient(aiso_ctx* ctx, const float* model, int nShotSig,
+                                      const float* dataGrad, float* velGrad,
+                                      aiso_device* const* sources, int nShot,
+                                      aiso_device* const* receivers, int nRecGeom,
+                                      int checkpointEvery, int mem);
 
 /* BornShotsGpu::forward/adjoint (F/src/BornShotsGpu.cpp:60-143, 245-339).
  * model [nx][nz]. */

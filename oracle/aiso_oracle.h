@@ -48,6 +48,9 @@ void orc_nl_fwd##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* sig,     
 void orc_nl_adj##SFX(const orc_grid* g, const REAL* v2dt2, REAL* sig,               \
                      const int* srcPos, int nSrc, const int* recPos, int nRec,       \
                      const REAL* dreg);                                              \
+void orc_nl_vgrad##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* sig,       \
+                       const int* srcPos, int nSrc, const int* recPos, int nRec,     \
+                       const REAL* dres, REAL* grad, REAL* sigAdj);                  \
 void orc_born_fwd##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* refl,      \
                        const REAL* sig, const int* srcPos, int nSrc,                 \
                        const int* recPos, int nRec, const REAL* model, REAL* dreg,   \

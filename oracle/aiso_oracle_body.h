@@ -235,6 +235,77 @@ void FN(orc_nl_adj)(const orc_grid* g, const REAL* v2dt2, REAL* sig,
     free(p0); free(p1); free(tmp);
 }
 
+/* 10th-order Laplacian of c at interior point i, in the reference's order. */
+static REAL FN(lap_at)(const REAL* zc, const REAL* xc, const REAL* c, long i, long nz)
+{
+    return zc[0] * c[i]
+         + zc[1] * (c[i - 1] + c[i + 1])
+         + zc[2] * (c[i - 2] + c[i + 2])
+         + zc[3] * (c[i - 3] + c[i + 3])
+         + zc[4] * (c[i - 4] + c[i + 4])
+         + zc[5] * (c[i - 5] + c[i + 5])
+         + xc[0] * c[i]
+         + xc[1] * (c[i + nz] + c[i - nz])
+         + xc[2] * (c[i + 2 * nz] + c[i - 2 * nz])
+         + xc[3] * (c[i + 3 * nz] + c[i - 3 * nz])
+         + xc[4] * (c[i + 4 * nz] + c[i - 4 * nz])
+         + xc[5] * (c[i + 5 * nz] + c[i - 5 * nz]);
+}
+
+/* Adjoint-state gradient of J(v2dt2) = <r, d(v2dt2)> for the nonlinear
+ * operator of one shot, where d is the data orc_nl_fwd records and dres = dJ/dd
+ * on the regular receiver grid.  The forward recursion is
+ *   p_t = D (v2dt2 * L p_{t-1} + 2 p_{t-1} - D p_{t-2} + s_t)
+ * (step, injection before the sponge D, the old slice damped twice), whose
+ * transpose is the adjoint propagation of orc_nl_adj driven by dres; its
+ * damped slices nu_t give
+ *   dJ/dv2dt2 += sum_t (L p_{t-1}) * nu_t          (grad, accumulated, t descending)
+ * and sigAdj (nSrc x ntw) the adjoint recorded at the source points, which
+ * pairs with the unscaled source for the v2dt2 factor of the injection.
+ * Not a reference function: the reference's FWI uses the Born adjoint as its
+ * gradient (PY/fwiMain.py:123-142). */
+void FN(orc_nl_vgrad)(const orc_grid* g, const REAL* v2dt2, const REAL* sig,
+                      const int* srcPos, int nSrc, const int* recPos, int nRec,
+                      const REAL* dres, REAL* grad, REAL* sigAdj)
+{
+    REAL zc[6], xc[6], f[2 * 101], cd[512];
+    FN(orc_tables)(g, zc, xc, f, cd);
+    const int nts = g->nts, sub = g->sub, ntw = (nts - 1) * sub + 1, h = sub + 1, T = ntw - 1;
+    const long nz = g->nz, nzx = (long)g->nz * g->nx;
+    REAL* P = FN(zalloc)((long)T * nzx);      /* P[t] = p_t, t = 0 .. T-1 (p_0 = 0) */
+    REAL* p0 = FN(zalloc)(nzx);
+    REAL* p1 = FN(zalloc)(nzx);
+    for (int its = 0; its < nts - 1; ++its) {
+        for (int it2 = 1; it2 < sub + 1; ++it2) {
+            int itw = its * sub + it2;
+            FN(orc_step_fwd)(g, zc, xc, p0, p1, p0, v2dt2);
+            FN(inject_source)(ntw, sig, p0, itw - 1, srcPos, nSrc);
+            FN(orc_damp)(g, cd, p0, p1);
+            if (itw < T) memcpy(P + (long)itw * nzx, p0, sizeof(REAL) * (size_t)nzx);
+            FN(swap)(&p0, &p1);
+        }
+    }
+    memset(p0, 0, sizeof(REAL) * (size_t)nzx);
+    memset(p1, 0, sizeof(REAL) * (size_t)nzx);
+    REAL* tmp = FN(zalloc)(nzx);
+    memset(sigAdj, 0, sizeof(REAL) * (size_t)nSrc * ntw);
+    for (int its = nts - 2; its > -1; --its) {
+        for (int it2 = sub - 1; it2 > -1; --it2) {
+            int itw = its * sub + it2;                /* produces nu_t, t = itw + 1 */
+            FN(orc_step_adj)(g, zc, xc, p0, p1, p0, v2dt2, tmp);
+            FN(interp_inject)(nts, h, f, dres, p0, its, it2, recPos, nRec);
+            FN(orc_damp)(g, cd, p0, p1);
+            FN(record_source)(ntw, p0, sigAdj, itw, srcPos, nSrc);
+            const REAL* pf = P + (long)itw * nzx;     /* p_{t-1} */
+            INTERIOR_BEGIN(g)
+                grad[i] += FN(lap_at)(zc, xc, pf, i, nz) * p0[i];
+            INTERIOR_END
+            FN(swap)(&p0, &p1);
+        }
+    }
+    free(P); free(p0); free(p1); free(tmp);
+}
+
 /* Source-wavefield pass shared by every Born-type driver, e.g.
  * F/src/BornShotsGpuFunctions.cu:281-321: W (nts slices) is zeroed, then
  * step -> injectSource -> damp -> interpWavefield -> swap. */

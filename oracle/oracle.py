@@ -275,6 +275,18 @@ def nl_adj_reg(g, v2dt2, dreg, srcPos, recPos, prec="f"):
     return sig
 
 
+def nl_vgrad_reg(g, v2dt2, sig, srcPos, recPos, dres, prec="f"):
+    """One shot: wave-equation part of dJ/dv2dt2 and the adjoint recorded at
+    the source points (orc_nl_vgrad)."""
+    dt = _dt(prec)
+    grad = np.zeros((g.nx, g.nz), dt)
+    sigAdj = np.zeros((len(srcPos), g.ntw), dt)
+    gc = g.c()
+    _fn("orc_nl_vgrad", prec)(C.byref(gc), _p(v2dt2), _p(sig), _p(srcPos), len(srcPos),
+                              _p(recPos), len(recPos), _p(np.ascontiguousarray(dres, dt)), _p(grad), _p(sigAdj))
+    return grad, sigAdj
+
+
 def _born_call(name, g, v2dt2, refl, sig, srcPos, recPos, model, dreg, prec):
     W = np.zeros((g.nts, g.nx, g.nz), _dt(prec))
     gc = g.c()
@@ -329,6 +341,39 @@ def nl_forward(g, vel, sources, receivers, model, prec="f"):
         sl = rec.forward(dreg, prec=prec)
         data[s] += sl
     return data
+
+
+def nl_velocity_gradient(g, vel, sources, receivers, model, dgrad, prec="f"):
+    """dJ/dvel of a misfit J whose derivative with respect to the data of
+    nl_forward(vel) is dgrad [nShot][nRec][nts] (for J = |d - d_obs|^2 / 2,
+    dgrad = d - d_obs).  Procedure (the engine follows it step for step):
+    per shot the wave-equation part (orc_nl_vgrad) summed in shot order; then,
+    shot by shot and source point by source point, the source-scaling part
+    sum_t sigAdj * u (u: the unscaled fine source signal, in double, rounded
+    once); finally the chain factor d(v^2 dtw^2)/dv = 2 v dtw^2."""
+    dt = _dt(prec)
+    vel = np.ascontiguousarray(vel, dt)
+    v2, _ = vel_scales(g, vel, prec)
+    G2 = np.zeros((g.nx, g.nz), dt)
+    terms = []
+    for s in range(len(sources)):
+        src, rec = sources[s], _shot_geom(receivers, s)
+        m = model[0] if model.shape[0] == 1 else model[s]
+        reg = src.adjoint(m, prec=prec)
+        u = interp_fwd(g, reg.copy(), prec=prec)
+        scale_source(g, vel, src.regPos, reg, prec=prec)
+        fine = interp_fwd(g, reg, prec=prec)
+        dres = rec.adjoint(np.ascontiguousarray(dgrad[s], dt), prec=prec)
+        gs, sigAdj = nl_vgrad_reg(g, v2, fine, src.regPos, rec.regPos, dres, prec)
+        G2 += gs
+        terms.append([(int(pos), float(np.dot(sigAdj[r].astype(np.float64), u[r].astype(np.float64))))
+                      for r, pos in enumerate(src.regPos)])
+    flat = G2.reshape(-1)
+    for shot in terms:
+        for pos, val in shot:
+            flat[pos] += dt(val)
+    dtw = dt(g.dts) / dt(g.sub)
+    return G2 * ((dt(2.0) * vel) * dtw * dtw)
 
 
 def nl_adjoint(g, vel, sources, receivers, data, nShotSig=1, prec="f"):

@@ -3,6 +3,13 @@ import sys
 
 import pytest
 
+# PyTorch ships its own HIP runtime; it must be loaded before libaiso.so (which
+# links the system ROCm runtime) for both to see the GPU in one process.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (ROOT, os.path.join(ROOT, "acoustic_isotropic_operators_amd")):
     if p not in sys.path:

@@ -1,0 +1,89 @@
+"""The oracle's adjoint-state velocity gradient of the nonlinear operator,
+pinned by central finite differences of the misfit in FP64.  CPU only.
+
+J(v) = |nl_forward(v) - d_obs|^2 / 2 on a small grid; for seeded directions
+dv, <dJ/dv, dv> must match (J(v + e dv) - J(v - e dv)) / (2 e).  One direction
+is supported only on the source's grid points, where the velocity enters
+through the source scaling v^2 dtw^2 rather than through the wave equation.
+"""
+import numpy as np
+import pytest
+
+from cases import make_case
+from oracle import oracle as O
+
+FAT = 5
+
+
+@pytest.fixture(scope="module")
+def setup():
+    c = make_case(seed=41, nShot=2, nts=40, sub=2, dts=0.002)
+    g = c.grid
+    os_, or_ = c.oracle_sources(), c.oracle_receivers()
+    m = np.zeros((1, 1, c.nts), np.float64)
+    m[0, 0] = c.wavelet
+    v0 = c.vel.astype(np.float64)
+    rng = np.random.default_rng(42)
+    # observed data from a model with a smooth anomaly
+    zz, xx = np.meshgrid(np.arange(c.nz), np.arange(c.nx))
+    bump = 80.0 * np.exp(-((zz - 30) ** 2 + (xx - 40) ** 2) / (2 * 6.0 ** 2))
+    dobs = O.nl_forward(g, v0 + bump, os_, or_, m, "d")
+    return c, g, os_, or_, m, v0, dobs, rng
+
+
+def _misfit(g, vel, os_, or_, m, dobs):
+    d = O.nl_forward(g, vel, os_, or_, m, "d")
+    r = d - dobs
+    return 0.5 * float(np.sum(r * r)), r
+
+
+def _smooth_direction(rng, nz, nx):
+    d = rng.standard_normal((nx, nz))
+    k = np.ones(7) / 7.0
+    for _ in range(3):
+        d = np.apply_along_axis(lambda r: np.convolve(r, k, mode="same"), 0, d)
+        d = np.apply_along_axis(lambda r: np.convolve(r, k, mode="same"), 1, d)
+    d[:FAT, :] = d[-FAT:, :] = 0.0
+    d[:, :FAT] = d[:, -FAT:] = 0.0
+    return d / np.abs(d).max()
+
+
+def test_gradient_matches_finite_differences(setup):
+    c, g, os_, or_, m, v0, dobs, rng = setup
+    J0, r = _misfit(g, v0, os_, or_, m, dobs)
+    assert J0 > 0
+    grad = O.nl_velocity_gradient(g, v0, os_, or_, m, r, "d")
+    eps = 1e-2                                   # m/s, against v ~ 2000 m/s
+    for k in range(4):
+        dv = _smooth_direction(rng, c.nz, c.nx)
+        jp, _ = _misfit(g, v0 + eps * dv, os_, or_, m, dobs)
+        jm, _ = _misfit(g, v0 - eps * dv, os_, or_, m, dobs)
+        fd = (jp - jm) / (2 * eps)
+        an = float(np.sum(grad * dv))
+        print(f"direction {k}: finite difference {fd:.10e}, adjoint state {an:.10e}, rel {abs(fd - an) / abs(fd):.2e}")
+        assert abs(fd - an) <= 1e-6 * abs(fd)
+
+
+def test_source_scaling_term(setup):
+    """A perturbation on the source's own grid points only."""
+    c, g, os_, or_, m, v0, dobs, _ = setup
+    J0, r = _misfit(g, v0, os_, or_, m, dobs)
+    grad = O.nl_velocity_gradient(g, v0, os_, or_, m, r, "d")
+    dv = np.zeros_like(v0)
+    for s in os_:
+        dv.reshape(-1)[s.regPos] = 1.0
+    eps = 1e-2
+    fd = (_misfit(g, v0 + eps * dv, os_, or_, m, dobs)[0] - _misfit(g, v0 - eps * dv, os_, or_, m, dobs)[0]) / (2 * eps)
+    an = float(np.sum(grad * dv))
+    print(f"source points: finite difference {fd:.10e}, adjoint state {an:.10e}")
+    assert abs(fd - an) <= 1e-6 * abs(fd)
+
+
+def test_float_gradient_close_to_fp64(setup):
+    c, g, os_, or_, m, v0, dobs, _ = setup
+    _, r = _misfit(g, v0, os_, or_, m, dobs)
+    gd = O.nl_velocity_gradient(g, v0, os_, or_, m, r, "d")
+    gf = O.nl_velocity_gradient(g, v0.astype(np.float32), os_, or_, m.astype(np.float32), r.astype(np.float32), "f")
+    rel = np.linalg.norm(gf - gd) / np.linalg.norm(gd)
+    print(f"float vs FP64 gradient relL2 {rel:.2e}")
+    assert rel < 1e-4
