@@ -87,3 +87,24 @@ def test_float_gradient_close_to_fp64(setup):
     rel = np.linalg.norm(gf - gd) / np.linalg.norm(gd)
     print(f"float vs FP64 gradient relL2 {rel:.2e}")
     assert rel < 1e-4
+
+
+def test_close_to_the_born_adjoint_gradient():
+    """The reference's FWI takes the Born adjoint of the residual as the
+    gradient (PY/fwiMain.py:123-142).  It differs from the exact gradient by
+    the source-scaling term and the Born source's coarse second derivative,
+    so the check is a stated tolerance: cosine >= 0.99, norms within 5 %."""
+    c = make_case(seed=41, nShot=3, nts=80, sub=2, dts=0.002)
+    g, os_, or_ = c.grid, c.oracle_sources(), c.oracle_receivers()
+    m = c.wavelet[None, None, :].astype(np.float64)
+    zz, xx = np.meshgrid(np.arange(c.nz), np.arange(c.nx))
+    vtrue = c.vel + 80.0 * np.exp(-((zz - 30) ** 2 + (xx - 40) ** 2) / (2 * 6.0 ** 2))
+    dobs = O.nl_forward(g, vtrue, os_, or_, m, "d")
+    v0 = c.vel.astype(np.float64)
+    r = O.nl_forward(g, v0, os_, or_, m, "d") - dobs
+    gn = O.nl_velocity_gradient(g, v0, os_, or_, m, r, "d").ravel()
+    gb = O.born_adjoint(g, v0, os_, [c.wavelet[None, :].astype(np.float64)], or_, r, "d").ravel()
+    cos = float(gn @ gb / np.linalg.norm(gn) / np.linalg.norm(gb))
+    ratio = float(np.linalg.norm(gb) / np.linalg.norm(gn))
+    print(f"cosine {cos:.4f}, |Born| / |exact| {ratio:.4f}")
+    assert cos >= 0.99 and abs(ratio - 1) <= 0.05

@@ -115,3 +115,31 @@ def test_torch_autograd_function(where):
     np.testing.assert_array_equal(v.grad.cpu().numpy(), ref)
     of = O.nl_forward(c.grid, c.vel, c.oracle_sources(), c.oracle_receivers(), c.wavelet[None, None, :], "f")
     np.testing.assert_array_equal(d.detach().cpu().numpy(), of)
+
+
+def test_engine_gradient_close_to_engine_born_adjoint():
+    """Same comparison on the GPU: the engine's exact gradient against the
+    engine's Born adjoint of the same residual (cosine >= 0.99)."""
+    import pyAcoustic_iso_float_born as born
+    import pyAcoustic_iso_float_nl as nl
+    c = make_case(seed=58, nShot=3, nts=80, sub=2, dts=0.002)
+    A, ctx, srcs, recs = _engine(c)
+    m = c.wavelet[None, None, :].astype(np.float32)
+    zz, xx = np.meshgrid(np.arange(c.nz), np.arange(c.nx))
+    ctx.set_velocity((c.vel + 80.0 * np.exp(-((zz - 30) ** 2 + (xx - 40) ** 2) / 72.0)).astype(np.float32))
+    dobs = np.zeros((c.nShot, recs[0].nDev, c.nts), np.float32)
+    ctx.nl_forward(m, dobs, srcs, recs)
+    ctx.set_velocity(c.vel)
+    d = np.zeros_like(dobs)
+    ctx.nl_forward(m, d, srcs, recs)
+    r = d - dobs
+    gn = np.zeros((c.nx, c.nz), np.float32)
+    ctx.nl_velocity_gradient(m, r, gn, srcs, recs)
+    op = born.BornShotsGpu(c.vel_vector(), c.par, c.product_sources(nl), [c.wavelet[None, :].copy()],
+                           c.product_receivers(nl))
+    gb = np.zeros((c.nx, c.nz), np.float32)
+    op.adjoint(False, gb, r)
+    a, b = gn.ravel().astype(np.float64), gb.ravel().astype(np.float64)
+    cos = float(a @ b / np.linalg.norm(a) / np.linalg.norm(b))
+    print(f"cosine {cos:.4f}")
+    assert cos >= 0.99
