@@ -106,6 +106,16 @@ struct aiso_ctx {
     DBuf pairs, pairsQ, W, ss, img, dreg, sig, slots, slotsQ, mtil, acc, modelDev, dataDev;
     DBuf ck;                            // checkpoints of the velocity-gradient pass
     hipEvent_t evIn = nullptr;          // orders device inputs after the default stream's work
+    // Lanes: the shots of a batch split over streams whose kernels run
+    // concurrently, so one lane's launch gaps and tails are filled by the
+    // other's.  On MI355X two lanes gain about 1 % on the Born benchmark, so
+    // the option is off by default.
+    int lanes = 1;
+    int extBatch = 16;                  // coarse samples per offset-imaging / secondary-source launch
+    int curLanes = 1;                   // lanes of the call in progress
+    std::vector<hipStream_t> lane;
+    hipEvent_t forkEv = nullptr;
+    std::vector<hipEvent_t> joinEv;
     int lockstep = 1;
     // Two fine steps per launch in the Born forward and adjoint (sub == 2).
     // They move half the HBM bytes of the one-step kernels but run more
@@ -119,6 +129,8 @@ struct aiso_ctx {
     std::map<std::tuple<long, int, int>, std::unique_ptr<DevGeom>> geoms;
     // profiling of the step kernels
     bool profile = false;
+    // With lanes > 1 the dispatches of different lanes overlap in time, so
+    // their durations are those of concurrent kernels.
     struct Timed { hipEvent_t e0, e1; int kind; double pts; };
     std::vector<Timed> ev;
     struct Batch { hipEvent_t e0, e1; int shots; };
@@ -273,16 +285,73 @@ void wait_default_stream(aiso_ctx* c)
     HIPC(hipStreamWaitEvent(c->stream, c->evIn, 0));
 }
 
+// Shots [k0, k0 + cnt) of a launch over `nshots` that lane l advances.
+void lane_range(int nshots, int lanes, int l, int& k0, int& cnt)
+{
+    const int per = (nshots + lanes - 1) / lanes;
+    k0 = l * per;
+    cnt = std::max(0, std::min(per, nshots - k0));
+}
+
+StepArgs lane_args(const StepArgs& a, int k0, int cnt)
+{
+    StepArgs y = a;
+    y.slots = a.slots + k0;
+    y.nshots = cnt;
+    y.gridLogical = a.ntiles * cnt;
+    return y;
+}
+
+hipStream_t lane_stream(const aiso_ctx* c, int l) { return c->curLanes > 1 ? c->lane[l] : c->stream; }
+
+void ensure_lanes(aiso_ctx* c)
+{
+    while ((int)c->lane.size() < c->lanes) {
+        hipStream_t st = nullptr;
+        HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));   // default priority
+        c->lane.push_back(st);
+        hipEvent_t e = nullptr;
+        HIPC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->joinEv.push_back(e);
+    }
+    if (!c->forkEv) HIPC(hipEventCreateWithFlags(&c->forkEv, hipEventDisableTiming));
+}
+
+// Lane streams start after the work queued on the context's stream...
+void fork_lanes(aiso_ctx* c)
+{
+    if (c->curLanes <= 1) return;
+    HIPC(hipEventRecord(c->forkEv, c->stream));
+    for (int l = 0; l < c->curLanes; ++l) HIPC(hipStreamWaitEvent(c->lane[l], c->forkEv, 0));
+}
+
+// ...and the context's stream continues after all of them.
+void join_lanes(aiso_ctx* c)
+{
+    if (c->curLanes <= 1) return;
+    for (int l = 0; l < c->curLanes; ++l) {
+        HIPC(hipEventRecord(c->joinEv[l], c->lane[l]));
+        HIPC(hipStreamWaitEvent(c->stream, c->joinEv[l], 0));
+    }
+}
+
 void launch_timed(aiso_ctx* c, unsigned mode, int kind, const StepArgs& a, int nshots, bool generic = false)
 {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->profile) {
-        e0 = pooled_event(c);
-        e1 = pooled_event(c);
-        c->ev.push_back({e0, e1, kind, double(c->p.interior()) * nshots});
+    (void)nshots;
+    for (int l = 0; l < c->curLanes; ++l) {
+        int k0, cnt;
+        lane_range(a.nshots, c->curLanes, l, k0, cnt);
+        if (cnt == 0) continue;
+        const StepArgs y = c->curLanes > 1 ? lane_args(a, k0, cnt) : a;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (c->profile) {
+            e0 = pooled_event(c);
+            e1 = pooled_event(c);
+            c->ev.push_back({e0, e1, kind, double(c->p.interior()) * cnt});
+        }
+        if (generic) HIPC(launch_step_generic(mode, c->tz, c->tx, y, lane_stream(c, l), e0, e1));
+        else HIPC(launch_step(mode, c->tz, c->tx, c->spb, y, lane_stream(c, l), e0, e1));
     }
-    if (generic) HIPC(launch_step_generic(mode, c->tz, c->tx, a, c->stream, e0, e1));
-    else HIPC(launch_step(mode, c->tz, c->tx, c->spb, a, c->stream, e0, e1));
 }
 
 void drain_profile(aiso_ctx* c)
@@ -436,11 +505,17 @@ void run(aiso_ctx* c, const Request& q)
     const int pairSlices = pair2 || adj2 ? 4 : 2;
     const long wSlices = lockstep ? 3 : p.nts;
     const bool needW = is_born(q.op), needSS = is_ext(q.op) || q.op == Op::BornAdj;
+    // The offset-extended passes keep a ring of coarse-rate slices per shot:
+    // extK samples batched per imaging / secondary-source launch, plus the
+    // one the time steps are writing or reading.
+    const int extK = std::min(c->extBatch, p.nts);
+    const int ssSlices = is_ext(q.op) ? extK + 1 : 2;
+    auto ssSlot = [&](int k) { return (long)(k % ssSlices) * nzx; };
     const bool needImg = q.op == Op::BornAdj || q.op == Op::ExtAdj;
     size_t perShot = pairSlices * nzx * 4 + (size_t)nRecRegMax * p.nts * 4 + (size_t)nSrcRegMax * p.ntw * 4;
     if (needW) perShot += (size_t)wSlices * nzx * 4;
     if (lockstep) perShot += pairSlices * nzx * 4;
-    if (needSS) perShot += 2 * nzx * 4;
+    if (needSS) perShot += (size_t)ssSlices * nzx * 4;
     if (needImg) perShot += (size_t)nExt * nzx * 4;
     c->bytesPerShot = perShot;
     // Shots per launch: an explicit request, else as many as keep the
@@ -453,7 +528,7 @@ void run(aiso_ctx* c, const Request& q)
         if (lockstep) hot = 7;                        // two pairs + secondary-source ring
         else if (q.op == Op::BornAdj) hot = 3;        // + image
         else if (q.op == Op::ExtFwd) hot = 4;         // + two secondary-source slices
-        else if (q.op == Op::ExtAdj) hot = 3 + nExt;  // + receiver slice + extended image
+        else if (q.op == Op::ExtAdj) hot = 3;         // + receiver slice (the image is touched once per extK samples)
         B = (int)std::max<long>(4, c->cacheBudget / ((long)hot * nzx * 4));
         B = std::min(B, 32);
     }
@@ -465,13 +540,14 @@ void run(aiso_ctx* c, const Request& q)
         B = std::min(B, (int)std::max<size_t>(1, budget / perShot));
     }
     B = std::min(B, q.nShot);
+    B = (q.nShot + (q.nShot + B - 1) / B - 1) / ((q.nShot + B - 1) / B);   // same batch count, balanced sizes
     c->batchUsed = B;
 
     float* pairs = c->pairs.need((size_t)B * pairSlices * nzx * 4) ? c->pairs.as<float>() : nullptr;
     float* W = needW ? (float*)c->W.need((size_t)B * wSlices * nzx * 4) : nullptr;
     float* pairsQ = lockstep ? (float*)c->pairsQ.need((size_t)B * pairSlices * nzx * 4) : nullptr;
     ShotSlot* dSlotsQ = lockstep || adj2 ? (ShotSlot*)c->slotsQ.need(sizeof(ShotSlot) * B) : nullptr;
-    float* ss = needSS ? (float*)c->ss.need((size_t)B * 2 * nzx * 4) : nullptr;
+    float* ss = needSS ? (float*)c->ss.need((size_t)B * ssSlices * nzx * 4) : nullptr;
     float* img = needImg ? (float*)c->img.need((size_t)B * nExt * nzx * 4) : nullptr;
     float* dreg = (float*)c->dreg.need((size_t)B * nRecRegMax * p.nts * 4);
     float* sig = (float*)c->sig.need((size_t)B * nSrcRegMax * p.ntw * 4);
@@ -531,7 +607,7 @@ void run(aiso_ctx* c, const Request& q)
             if (W) S.W = W + (size_t)k * wSlices * nzx;
             if (pairsQ)
                 for (int i = 0; i < pairSlices; ++i) S.q[i] = pairsQ + ((size_t)k * pairSlices + i) * nzx;
-            if (ss) { S.ss[0] = ss + (size_t)k * 2 * nzx; S.ss[1] = S.ss[0] + nzx; }
+            if (ss) { S.ss[0] = ss + (size_t)k * ssSlices * nzx; S.ss[1] = S.ss[0] + nzx; }
             if (img) S.img = img + (size_t)k * nExt * nzx;
             S.dreg = dreg + (size_t)k * nRecRegMax * p.nts;
             S.sig = sig + (size_t)k * nSrcRegMax * p.ntw;
@@ -586,8 +662,15 @@ void run(aiso_ctx* c, const Request& q)
             int mr = 0;
             if (mode & (M_INJ_SIG | M_REC_SIG)) mr = std::max(mr, maxSrcRing);
             if (mode & (M_INJ_DATA | M_REC_DATA)) mr = std::max(mr, maxRecRing);
-            if (mr > 0) HIPC(launch_ring(mode, sa, mr, s));
+            if (mr <= 0) return;
+            for (int l = 0; l < c->curLanes; ++l) {
+                int k0, cnt;
+                lane_range(sa.nshots, c->curLanes, l, k0, cnt);
+                if (cnt > 0) HIPC(launch_ring(mode, c->curLanes > 1 ? lane_args(sa, k0, cnt) : sa, mr, lane_stream(c, l)));
+            }
         };
+        c->curLanes = is_ext(q.op) || pair2 || adj2 ? 1 : std::max(1, std::min(c->lanes, nb));
+        if (c->curLanes > 1) ensure_lanes(c);
 
         // Three consecutive fine slices hold everything a coarse sample needs
         // when sub == 2, so captures and imaging are written once per coarse
@@ -673,6 +756,7 @@ void run(aiso_ctx* c, const Request& q)
             a.wa = f[h + 1];
             a.wb = f[h + 2];
             a.wc = f[1];
+            fork_lanes(c);
             for (int t = 1; t <= T + 2; ++t) {
                 const bool doS = t <= T, doQ = t - 2 >= 1;
                 const int itsS = (t - 1) / p.sub, it2S = t - itsS * p.sub;
@@ -691,13 +775,19 @@ void run(aiso_ctx* c, const Request& q)
                 x.fe = f[it2Q];
                 x.ff = f[h + it2Q];
                 if (doS && doQ) {
-                    hipEvent_t e0 = nullptr, e1 = nullptr;
-                    if (c->profile) {
-                        e0 = pooled_event(c);
-                        e1 = pooled_event(c);
-                        c->ev.push_back({e0, e1, AISO_K_BORN_PAIR, 2.0 * double(p.interior()) * nb});
+                    for (int l = 0; l < c->curLanes; ++l) {
+                        int k0, cnt;
+                        lane_range(nb, c->curLanes, l, k0, cnt);
+                        if (cnt == 0) continue;
+                        hipEvent_t e0 = nullptr, e1 = nullptr;
+                        if (c->profile) {
+                            e0 = pooled_event(c);
+                            e1 = pooled_event(c);
+                            c->ev.push_back({e0, e1, AISO_K_BORN_PAIR, 2.0 * double(p.interior()) * cnt});
+                        }
+                        HIPC(launch_fwd_pair(c->tz, c->tx, c->curLanes > 1 ? lane_args(x, k0, cnt) : x, lane_stream(c, l),
+                                             e0, e1));
                     }
-                    HIPC(launch_fwd_pair(c->tz, c->tx, x, s, e0, e1));
                     ring(MODE_SRC_WIN, x);
                     StepArgs xq = x;
                     xq.slots = dSlotsQ;
@@ -718,9 +808,15 @@ void run(aiso_ctx* c, const Request& q)
                 if (doS) curS ^= 1;
                 if (doQ) curQ ^= 1;
                 if (t == T)   // last coarse sample of the source wavefield
-                    HIPC(launch_win_tail(dSlots, nb, curS, WIN_SS, f[h + 1], f[h + 2], ring3(p.nts - 1), 0, 0,
-                                         mtil, p.nz, p.nx, s));
+                    for (int l = 0; l < c->curLanes; ++l) {
+                        int k0, cnt;
+                        lane_range(nb, c->curLanes, l, k0, cnt);
+                        if (cnt > 0)
+                            HIPC(launch_win_tail(dSlots + k0, cnt, curS, WIN_SS, f[h + 1], f[h + 2], ring3(p.nts - 1),
+                                                 0, 0, mtil, p.nz, p.nx, lane_stream(c, l)));
+                    }
             }
+            join_lanes(c);
             for (int k = 0; k < nb; ++k) {
                 const int shot = b0 + k;
                 const aiso_device* rd = q.rec[q.nRec == 1 ? 0 : shot];
@@ -824,6 +920,7 @@ void run(aiso_ctx* c, const Request& q)
             a.wa = f[h + 1];
             a.wb = f[h + 2 < 2 * h ? h + 2 : h + 1];
             a.wc = f[1];
+            fork_lanes(c);
             for (int its = 0; its < p.nts - 1; ++its)
                 for (int it2 = 1; it2 <= p.sub; ++it2) {
                     a.its = its;
@@ -841,6 +938,7 @@ void run(aiso_ctx* c, const Request& q)
                     ring(mode, a);
                     a.cur ^= 1;
                 }
+            join_lanes(c);
             if (win)   // W[nts-1] = p(last-1)*f[h+1] + p(last)*f[h+2]
                 HIPC(launch_win_tail(dSlots, nb, a.cur, winOut, f[h + 1], f[h + 2], (long)(p.nts - 1) * nzx, 0, 0,
                                      mtil, p.nz, p.nx, s));
@@ -858,10 +956,33 @@ void run(aiso_ctx* c, const Request& q)
                 if (q.op == Op::BornFwd) { mode = win ? MODE_BORN_SCAT_PRE : MODE_BORN_SCAT; kind = AISO_K_BORN_SCAT; }
                 if (q.op == Op::ExtFwd) { mode = MODE_EXT_SCAT; kind = AISO_K_EXT_SCAT; }
                 int L = 0, R = 1;
-                if (q.op == Op::ExtFwd) HIPC(launch_ext_ss(dSlots, nb, L, 0, mtil, base.v2dt2, p.nz, p.nx, p.hExt, s));
+                // Secondary sources of the offset-extended forward, extK
+                // samples per launch into the ring, ahead of their use.
+                int nextSS = 0;
+                auto ext_ss_upto = [&](int upto) {
+                    while (nextSS <= upto) {
+                        ExtBatchArgs e{};
+                        e.slots = dSlots;
+                        e.v2dt2 = base.v2dt2;
+                        e.mext = mtil;
+                        e.nz = p.nz; e.nx = p.nx; e.hExt = p.hExt;
+                        e.nk = std::min(extK, p.nts - nextSS);
+                        for (int j = 0; j < e.nk; ++j) {
+                            e.ssOff[j] = ssSlot(nextSS + j);
+                            e.wOff[j] = (long)(nextSS + j) * nzx;
+                        }
+                        HIPC(launch_ext_ss_batch(e, nb, s));
+                        nextSS += e.nk;
+                    }
+                };
+                fork_lanes(c);
                 for (int its = 0; its < p.nts - 1; ++its) {
-                    if (q.op == Op::ExtFwd)
-                        HIPC(launch_ext_ss(dSlots, nb, R, (long)(its + 1) * nzx, mtil, base.v2dt2, p.nz, p.nx, p.hExt, s));
+                    if (q.op == Op::ExtFwd) {
+                        ext_ss_upto(its + 1);
+                        L = R = 0;
+                        a.ssOffA = ssSlot(its);
+                        a.ssOffB = ssSlot(its + 1);
+                    }
                     for (int it2 = 1; it2 <= p.sub; ++it2) {
                         a.its = its;
                         a.itwInj = its * p.sub + it2 - 1;
@@ -879,6 +1000,7 @@ void run(aiso_ctx* c, const Request& q)
                     }
                     std::swap(L, R);
                 }
+                join_lanes(c);
                 for (int k = 0; k < nb; ++k) {
                     const int shot = b0 + k;
                     const aiso_device* rd = q.rec[q.nRec == 1 ? 0 : shot];
@@ -892,7 +1014,7 @@ void run(aiso_ctx* c, const Request& q)
                 int kind = AISO_K_NL_ADJ;
                 if (q.op == Op::BornAdj) { mode = win ? MODE_REC_WIN : MODE_BORN_REC; kind = AISO_K_BORN_REC; }
                 if (q.op == Op::ExtAdj) { mode = win ? MODE_REC_WIN : MODE_EXT_REC; kind = AISO_K_EXT_REC; }
-                if (ss) HIPC(launch_fill(ss, (long)nb * 2 * nzx, s));
+                if (ss) HIPC(launch_fill(ss, (long)nb * ssSlices * nzx, s));
                 if (img) HIPC(launch_fill(img, (long)nb * nExt * nzx, s));
                 a.winOut = q.op == Op::BornAdj ? WIN_IMG : WIN_SLICE;
                 a.wa = f[1];
@@ -911,6 +1033,24 @@ void run(aiso_ctx* c, const Request& q)
                     r.gridLogical = r.ntiles * nbr;
                     r.cur = 1;
                     int L = 0, R = 1;
+                    // Completed receiver-wavefield samples awaiting the offset
+                    // imaging, newest last; imaged extK at a time.
+                    std::vector<int> pend;
+                    auto ext_img_flush = [&] {
+                        if (pend.empty()) return;
+                        ExtBatchArgs e{};
+                        e.slots = sl;
+                        e.v2dt2 = base.v2dt2;
+                        e.nz = p.nz; e.nx = p.nx; e.hExt = p.hExt;
+                        e.nk = (int)pend.size();
+                        for (int j = 0; j < e.nk; ++j) {
+                            e.ssOff[j] = ssSlot(pend[j]);
+                            e.wOff[j] = (long)pend[j] * nzx;
+                        }
+                        HIPC(launch_ext_img_batch(e, nbr, s));
+                        pend.clear();
+                    };
+                    fork_lanes(c);
                     for (int its = p.nts - 2; its >= 0; --its) {
                         for (int it2 = p.sub - 1; it2 >= 0; --it2) {
                             r.its = its;
@@ -919,6 +1059,11 @@ void run(aiso_ctx* c, const Request& q)
                             r.fd = f[h + it2 + 1];
                             r.ssA = win ? 0 : L;
                             r.ssB = R;
+                            if (q.op == Op::ExtAdj) {   // A: sample its (window: its + 1), B: sample its + 1
+                                r.ssA = r.ssB = 0;
+                                r.ssOffA = ssSlot(win ? its + 1 : its);
+                                r.ssOffB = ssSlot(its + 1);
+                            }
                             r.fe = f[it2];
                             r.ff = f[h + it2];
                             r.skipA = r.fe == 0.f;
@@ -932,10 +1077,13 @@ void run(aiso_ctx* c, const Request& q)
                             ring(mode, r);
                             r.cur ^= 1;
                         }
-                        if (q.op == Op::ExtAdj)
-                            HIPC(launch_ext_img(sl, nbr, win ? 0 : R, (long)(its + 1) * nzx, base.v2dt2, p.nz, p.nx, nExt, s));
+                        if (q.op == Op::ExtAdj) {
+                            pend.push_back(its + 1);
+                            if ((int)pend.size() == extK) ext_img_flush();
+                        }
                         std::swap(L, R);
                     }
+                    join_lanes(c);
                     if (q.op == Op::BornAdj) {
                         if (win) {   // img += W[0] * (q(1)*f[1] + q(0)*f[0]), then the scaling
                             HIPC(launch_win_tail(sl, nbr, r.cur, WIN_IMG, f[1], f[0], 0, 0, 0, nullptr, p.nz, p.nx, s));
@@ -944,8 +1092,11 @@ void run(aiso_ctx* c, const Request& q)
                             HIPC(launch_born_adj_finish(sl, nbr, R, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
                         }
                     } else if (q.op == Op::ExtAdj) {
-                        if (win) HIPC(launch_win_tail(sl, nbr, r.cur, WIN_SLICE, f[1], f[0], 0, 0, 0, nullptr, p.nz, p.nx, s));
-                        HIPC(launch_ext_img(sl, nbr, win ? 0 : R, 0, base.v2dt2, p.nz, p.nx, nExt, s));
+                        if (win)
+                            HIPC(launch_win_tail(sl, nbr, r.cur, WIN_SLICE, f[1], f[0], ssSlot(0), 0, 0, nullptr, p.nz,
+                                                 p.nx, s));
+                        pend.push_back(0);
+                        ext_img_flush();
                         HIPC(launch_ext_adj_finish(sl, nbr, c->refl.as<float>(), p.nz, p.nx, nExt, s));
                     }
                 }
@@ -1100,6 +1251,7 @@ void run_nl_grad(aiso_ctx* c, const float* model, int nShotSig, const float* dat
     ShotSlot* dA = (ShotSlot*)c->slotsQ.need(sizeof(ShotSlot) * B);
     float* acc = (float*)c->acc.need(nzx * 4);
     HIPC(launch_fill(acc, nzx, s));
+    c->curLanes = 1;
 
     StepArgs base{};
     base.v2dt2 = c->v2dt2.as<float>();
@@ -1311,6 +1463,12 @@ aiso_status aiso_ctx_destroy(aiso_ctx* c)
         for (auto& e : c->ev) { (void)hipEventDestroy(e.e0); (void)hipEventDestroy(e.e1); }
         for (auto e : c->evPool) (void)hipEventDestroy(e);
         if (c->evIn) (void)hipEventDestroy(c->evIn);
+        if (c->forkEv) (void)hipEventDestroy(c->forkEv);
+        for (auto e : c->joinEv) (void)hipEventDestroy(e);
+        for (auto l : c->lane) {
+            (void)hipStreamSynchronize(l);
+            (void)hipStreamDestroy(l);
+        }
         c->geoms.clear();
         hipStream_t s = c->stream;
         delete c;
@@ -1367,6 +1525,14 @@ aiso_status aiso_ctx_set_option(aiso_ctx* c, const char* key, int value)
         if (k == "streamNT") c->streamNT = value != 0;
         else if (k == "lockstep") c->lockstep = value != 0;
         else if (k == "pair2") c->pair2 = value != 0;
+        else if (k == "extBatch") {
+            if (value < 1 || value > kExtBatchMax) throw Error(AISO_ERR_PARAM, "extBatch must be 1..16");
+            c->extBatch = value;
+        }
+        else if (k == "lanes") {
+            if (value < 1 || value > 4) throw Error(AISO_ERR_PARAM, "lanes must be 1..4");
+            c->lanes = value;
+        }
         else if (k == "tileFwd" || k == "tileAdj") {   // tz * 100 + tx
             const int tz = value / 100, tx = value % 100;
             if ((tz != 32 && tz != 64) || (tx != 16 && tx != 32))

@@ -211,8 +211,8 @@ __device__ __forceinline__ void step_block(const StepArgs& a, int group, int til
                     sB[j] = useB ? ldg_s(G(S.W + a.wB) + o, a.ntW) : 0.f;
                 }
                 if constexpr (SSS) {
-                    sA[j] = G(S.ss[a.ssA])[o];
-                    sB[j] = useB ? G(S.ss[a.ssB])[o] : 0.f;
+                    sA[j] = G(S.ss[a.ssA] + a.ssOffA)[o];
+                    sB[j] = useB ? G(S.ss[a.ssB] + a.ssOffB)[o] : 0.f;
                 }
             }
         }
@@ -288,8 +288,8 @@ __device__ __forceinline__ void step_block(const StepArgs& a, int group, int til
             if constexpr (MODE & M_CAPTURE) {
                 // Forward passes capture the coarse source wavefield W; the
                 // adjoint receiver pass accumulates into the two ss slices.
-                gf* A = ADJ ? G(S.ss[a.ssA]) : G(S.W + a.wA);
-                gf* B = ADJ ? G(S.ss[a.ssB]) : G(S.W + a.wB);
+                gf* A = ADJ ? G(S.ss[a.ssA] + a.ssOffA) : G(S.W + a.wA);
+                gf* B = ADJ ? G(S.ss[a.ssB] + a.ssOffB) : G(S.W + a.wB);
                 // A zero weight leaves the slice as it is (x + n*0 == x); a
                 // first touch writes instead of accumulating into a zeroed slice.
                 if (!a.skipA) A[gi] = a.assignA ? n * a.fe : A[gi] + n * a.fe;
@@ -316,7 +316,7 @@ __device__ __forceinline__ void step_block(const StepArgs& a, int group, int til
                     case WIN_W: stg_s(G(S.W + a.wA) + gi, v, a.ntW); break;
                     case WIN_SS: stg_s(G(S.W + a.wA) + gi, G(a.mtil)[gi] * v, a.ntW); break;
                     case WIN_IMG: G(S.img)[gi] += ldg_s(G(S.W + a.wImg) + gi, a.ntW) * v; break;
-                    default: G(S.ss[a.ssA])[gi] = v; break;
+                    default: G(S.ss[a.ssA] + a.ssOffA)[gi] = v; break;
                     }
                 }
             }
@@ -1058,8 +1058,8 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
                 else n += sa * a.fa;
             }
             if constexpr (MODE & M_SS_SLICE) {
-                const float sa = S.ss[a.ssA][gi];
-                if (a.fb != 0.f) n += sa * a.fa + S.ss[a.ssB][gi] * a.fb;
+                const float sa = S.ss[a.ssA][a.ssOffA + gi];
+                if (a.fb != 0.f) n += sa * a.fa + S.ss[a.ssB][a.ssOffB + gi] * a.fb;
                 else n += sa * a.fa;
             }
         }
@@ -1104,8 +1104,8 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
         if constexpr (MODE & M_CAPTURE) {
             // Forward passes capture the coarse source wavefield W; the
             // adjoint receiver pass accumulates into the two ss slices.
-            float* A = ADJ ? S.ss[a.ssA] : S.W + a.wA;
-            float* B = ADJ ? S.ss[a.ssB] : S.W + a.wB;
+            float* A = ADJ ? S.ss[a.ssA] + a.ssOffA : S.W + a.wA;
+            float* B = ADJ ? S.ss[a.ssB] + a.ssOffB : S.W + a.wB;
             // A zero weight leaves the slice as it is (x + n*0 == x); a
             // first touch writes instead of accumulating into a zeroed slice.
             if (!a.skipA) A[gi] = a.assignA ? n * a.fe : A[gi] + n * a.fe;
@@ -1132,7 +1132,7 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
                 case WIN_W: S.W[a.wA + gi] = v; break;
                 case WIN_SS: S.W[a.wA + gi] = a.mtil[gi] * v; break;
                 case WIN_IMG: S.img[gi] += S.W[a.wImg + gi] * v; break;
-                default: S.ss[a.ssA][gi] = v; break;
+                default: S.ss[a.ssA][a.ssOffA + gi] = v; break;
                 }
             }
         }
@@ -1247,47 +1247,235 @@ __global__ void k_born_adj_finish(const ShotSlot* slots, int ssR, const float* _
     }
 }
 
+// ---------------------------------------------------------------------------
+// Offset-extended Born over a batch of coarse samples.  One launch handles
+// a.nk samples; every output element sees the same sequence of float
+// operations as one launch per sample taken in the listed order, so the
+// batching changes the HBM traffic and not the bits.
+//
+// Tiles are kExtTZ rows by kExtTX columns; 256 threads, each owning one row
+// and kExtCX consecutive columns.  HX >= hExt bounds the offsets at compile
+// time so that the per-thread offset loops index registers only.
+
+constexpr int kExtTZ = 64, kExtTX = 16, kExtCX = 4;
+static_assert(kExtTX == 4 * kExtCX, "four column groups of kExtCX columns");
+
 // imagingOffsetFwdGpu + scaleSecondarySourceFd (F/src/kernelsGpu.cu:381-396,
-// 270-275): ss[x] = v2dt2 * sum_h m~[h][x-h] * W[k][x-2h], written (not added)
-// into the slot's ss[dst].
-__global__ void k_ext_ss(const ShotSlot* slots, int dst, long wk, const float* __restrict__ mext,
-                         const float* __restrict__ v2dt2, int nz, int nx, int hExt)
+// 270-275): for each shot and sample k,
+//   ss_k[x] = v2dt2[x] * sum_{h = lo..hi} m~[h][x-h] * W_k[x-2h]
+//   lo = max(-hExt, x+1+FAT+hExt-nx),  hi = min(x-hExt-FAT, hExt)
+// written into ss[0] + ssOff[k].  A block walks (shot, sample) pairs of its
+// shot group; the scaled model values of the thread's columns stay in
+// registers for all of them, and the next pair's W tile is fetched into
+// registers while the current one is computed.
+template <int HX>
+__global__ __launch_bounds__(256) void k_ext_ss_tile(ExtBatchArgs a, int nshots, int spb)
 {
-    const ShotSlot& S = slots[blockIdx.y];
-    const long nzx = (long)nz * nx;
-    const int iz = FAT + blockIdx.x * blockDim.x + threadIdx.x;
-    const int ix = FAT + blockIdx.z;
-    if (iz >= nz - FAT) return;
-    const int lo = max(-hExt, ix + 1 + FAT + hExt - nx);
-    const int hi = min(ix - hExt - FAT, hExt);
-    const float* Wk = S.W + wk;
-    float acc = 0.f;
-    for (int e = lo; e <= hi; ++e)
-        acc += mext[nzx * (e + hExt) + (long)nz * (ix - e) + iz] * Wk[(long)(ix - 2 * e) * nz + iz];
-    const long i = (long)ix * nz + iz;
-    S.ss[dst][i] = acc * v2dt2[i];
+    constexpr int NE = 2 * HX + 1;
+    constexpr int LW = kExtTX + 4 * HX;        // LDS columns at hExt == HX
+    constexpr int WW = kExtCX + 4 * HX;        // register window of W_k columns
+    constexpr int PF = LW / 4;                 // tile columns fetched per thread (its own row)
+    static_assert(LW % 4 == 0, "four column groups fetch the tile");
+    __shared__ float Wl[LW * kExtTZ];
+    const int hExt = a.hExt, ne = 2 * hExt + 1, lw = kExtTX + 4 * hExt;
+    const long nz = a.nz, nzx = nz * a.nx;
+    const int tz = threadIdx.x, g = threadIdx.y;
+    const int z = FAT + blockIdx.z * kExtTZ + tz;
+    const bool zok = z < a.nz - FAT;
+    const int x0 = FAT + blockIdx.x * kExtTX;   // first output column of the tile
+    const int xb = x0 + kExtCX * g;
+    int lo[kExtCX], hi[kExtCX];
+    float mv[NE][kExtCX];
+#pragma unroll
+    for (int c = 0; c < kExtCX; ++c) {
+        const int x = xb + c;
+        lo[c] = max(-hExt, x + 1 + FAT + hExt - a.nx);
+        hi[c] = min(x - hExt - FAT, hExt);
+        if (!zok || x >= a.nx - FAT) hi[c] = lo[c] - 1;
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const int h = e - hExt;
+            mv[e][c] = (e < ne && h >= lo[c] && h <= hi[c]) ? a.mext[nzx * e + (long)(x - h) * nz + z] : 0.f;
+        }
+    }
+    const int sh0 = blockIdx.y * spb;
+    const int nIt = max(0, min(spb, nshots - sh0)) * a.nk;
+    // Wl column j holds global column x0 - 2*hExt + j; thread (g, tz)
+    // fetches columns j = 4u + g of its own row.
+    float pre[PF];
+    auto fetch = [&](int it) {
+        const float* W = a.slots[sh0 + it / a.nk].W + a.wOff[it % a.nk];
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int j = 4 * u + g, gx = x0 - 2 * hExt + j;
+            pre[u] = (zok && j < lw && gx >= 0 && gx < a.nx) ? W[(long)gx * nz + z] : 0.f;
+        }
+    };
+    // Window entry i holds column xb + i - 4*HX + 2*hExt, so W_k[x - 2h] of
+    // column c and offset index e is entry c - 2e + 4*HX.
+    const int jb = xb - x0 - 4 * HX + 4 * hExt;
+    if (nIt > 0) fetch(0);
+    for (int it = 0; it < nIt; ++it) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) Wl[(4 * u + g) * kExtTZ + tz] = pre[u];
+        __syncthreads();
+        if (it + 1 < nIt) fetch(it + 1);
+        float ww[WW];
+#pragma unroll
+        for (int i = 0; i < WW; ++i) {
+            const int j = jb + i;
+            ww[i] = (j >= 0 && j < lw) ? Wl[j * kExtTZ + tz] : 0.f;
+        }
+        __syncthreads();
+        if (zok) {
+            float* ss = a.slots[sh0 + it / a.nk].ss[0] + a.ssOff[it % a.nk];
+#pragma unroll
+            for (int c = 0; c < kExtCX; ++c) {
+                const int x = xb + c;
+                if (x >= a.nx - FAT) continue;
+                float acc = 0.f;
+#pragma unroll
+                for (int e = 0; e < NE; ++e) {
+                    const int h = e - hExt;
+                    if (e < ne && h >= lo[c] && h <= hi[c]) acc += mv[e][c] * ww[c - 2 * e + 4 * HX];
+                }
+                const long i = (long)x * nz + z;
+                ss[i] = acc * a.v2dt2[i];
+            }
+        }
+    }
 }
 
 // scaleSecondarySourceFd + imagingOffsetAdjGpu (F/src/kernelsGpu.cu:270-275,
-// 398-411): img[e][x] += (ss[x+h]*v2dt2[x+h]) * W[k][x-h], h = e - hExt,
-// for x in [FAT+hExt, nx-FAT-hExt).
-__global__ void k_ext_img(const ShotSlot* slots, int src, long wk, const float* __restrict__ v2dt2,
-                          int nz, int nx, int nExt)
+// 398-411): for each shot and each sample k in order,
+//   img[e][x] += (ss_k[x+h] * v2dt2[x+h]) * W_k[x-h],  h = e - hExt,
+// for x in [FAT+hExt, nx-FAT-hExt).  The thread's image values of a shot
+// stay in registers across the batch, so the extended image is read and
+// written once per batch; ss_k * v2dt2 and W_k pass through LDS, the next
+// pair's tiles fetched into registers while the current one is computed.
+template <int HX>
+__global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nshots, int spb)
 {
-    const ShotSlot& S = slots[blockIdx.y];
-    const int hExt = (nExt - 1) / 2;
-    const long nzx = (long)nz * nx;
+    constexpr int NE = 2 * HX + 1;
+    constexpr int LW = kExtTX + 2 * HX;
+    constexpr int RW = kExtCX + 2 * HX;        // register windows of R and W columns
+    constexpr int PF = (LW + 3) / 4;
+    __shared__ float Rl[LW * kExtTZ];
+    __shared__ float Wl[LW * kExtTZ];
+    const int hExt = a.hExt, ne = 2 * hExt + 1, lw = kExtTX + 2 * hExt;
+    const long nz = a.nz, nzx = nz * a.nx;
+    const int tz = threadIdx.x, g = threadIdx.y;
+    const int z = FAT + blockIdx.z * kExtTZ + tz;
+    const int x0 = FAT + hExt + blockIdx.x * kExtTX;
+    const int xEnd = a.nx - FAT - hExt;
+    const bool zok = z < a.nz - FAT;
+    const int xb = x0 + kExtCX * g;
+    const int sh0 = blockIdx.y * spb;
+    const int nIt = max(0, min(spb, nshots - sh0)) * a.nk;
+    // Rl/Wl column j holds global column x0 - hExt + j; thread (g, tz)
+    // fetches columns j = 4u + g of its own row.
+    float vv[PF], pr[PF], pw[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        const int j = 4 * u + g, gx = x0 - hExt + j;
+        vv[u] = (zok && j < lw && gx < a.nx) ? a.v2dt2[(long)gx * nz + z] : 0.f;
+    }
+    auto fetch = [&](int it) {
+        const ShotSlot& S = a.slots[sh0 + it / a.nk];
+        const float* ss = S.ss[0] + a.ssOff[it % a.nk];
+        const float* W = S.W + a.wOff[it % a.nk];
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int j = 4 * u + g, gx = x0 - hExt + j;
+            const bool in = zok && j < lw && gx < a.nx;
+            pr[u] = in ? ss[(long)gx * nz + z] : 0.f;
+            pw[u] = in ? W[(long)gx * nz + z] : 0.f;
+        }
+    };
+    const int jr = xb - x0;                      // R window entry i: column xb + i - hExt
+    const int jw = xb - x0 - 2 * HX + 2 * hExt;  // W window entry i: column xb + i - 2*HX + hExt
+    float m[NE][kExtCX];
+    if (nIt > 0) fetch(0);
+    for (int it = 0; it < nIt; ++it) {
+        float* img = a.slots[sh0 + it / a.nk].img;
+        if (it % a.nk == 0) {
+#pragma unroll
+            for (int e = 0; e < NE; ++e)
+#pragma unroll
+                for (int c = 0; c < kExtCX; ++c)
+                    m[e][c] = (e < ne && zok && xb + c < xEnd) ? img[nzx * e + (long)(xb + c) * nz + z] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            if (4 * u + g < LW) {
+                Rl[(4 * u + g) * kExtTZ + tz] = pr[u] * vv[u];
+                Wl[(4 * u + g) * kExtTZ + tz] = pw[u];
+            }
+        }
+        __syncthreads();
+        if (it + 1 < nIt) fetch(it + 1);
+        float rw[RW], ww[RW];
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+            const int j0 = jr + i, j1 = jw + i;
+            rw[i] = j0 < lw ? Rl[j0 * kExtTZ + tz] : 0.f;
+            ww[i] = j1 >= 0 ? Wl[j1 * kExtTZ + tz] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < NE; ++e)
+#pragma unroll
+            for (int c = 0; c < kExtCX; ++c)
+                // R at xb+c+h is entry c + e; W at xb+c-h is entry c - e + 2*HX
+                if (e < ne) m[e][c] += rw[c + e] * ww[c - e + 2 * HX];
+        if (it % a.nk == a.nk - 1) {
+#pragma unroll
+            for (int e = 0; e < NE; ++e)
+#pragma unroll
+                for (int c = 0; c < kExtCX; ++c)
+                    if (e < ne && zok && xb + c < xEnd) img[nzx * e + (long)(xb + c) * nz + z] = m[e][c];
+        }
+    }
+}
+
+// The same two operations for offsets beyond the register-tiled kernels:
+// one thread per grid point, operands read through the caches.
+__global__ void k_ext_ss_any(ExtBatchArgs a)
+{
+    const ShotSlot& S = a.slots[blockIdx.y];
+    const long nz = a.nz, nzx = nz * a.nx;
+    const int iz = FAT + blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = FAT + blockIdx.z;
+    if (iz >= a.nz - FAT) return;
+    const int hExt = a.hExt;
+    const int lo = max(-hExt, ix + 1 + FAT + hExt - a.nx);
+    const int hi = min(ix - hExt - FAT, hExt);
+    const long i = (long)ix * nz + iz;
+    for (int k = 0; k < a.nk; ++k) {
+        const float* W = S.W + a.wOff[k];
+        float acc = 0.f;
+        for (int h = lo; h <= hi; ++h)
+            acc += a.mext[nzx * (h + hExt) + (long)(ix - h) * nz + iz] * W[(long)(ix - 2 * h) * nz + iz];
+        S.ss[0][a.ssOff[k] + i] = acc * a.v2dt2[i];
+    }
+}
+
+__global__ void k_ext_img_any(ExtBatchArgs a)
+{
+    const ShotSlot& S = a.slots[blockIdx.y];
+    const long nz = a.nz, nzx = nz * a.nx;
+    const int hExt = a.hExt;
     const int iz = FAT + blockIdx.x * blockDim.x + threadIdx.x;
     const int ix = FAT + hExt + blockIdx.z;
-    if (iz >= nz - FAT || ix >= nx - FAT - hExt) return;
-    const float* ss = S.ss[src];
-    const float* Wk = S.W + wk;
-    for (int e = 0; e < nExt; ++e) {
+    if (iz >= a.nz - FAT || ix >= a.nx - FAT - hExt) return;
+    for (int e = 0; e <= 2 * hExt; ++e) {
         const int h = e - hExt;
-        const long ir = (long)(ix + h) * nz + iz;
-        const float r = ss[ir] * v2dt2[ir];
-        float* m = S.img + nzx * e + (long)ix * nz + iz;
-        *m += r * Wk[(long)(ix - h) * nz + iz];
+        const long ir = (long)(ix + h) * nz + iz, iw = (long)(ix - h) * nz + iz;
+        float* mp = S.img + nzx * e + (long)ix * nz + iz;
+        float m = *mp;
+        for (int k = 0; k < a.nk; ++k)
+            m += (S.ss[0][a.ssOff[k] + ir] * a.v2dt2[ir]) * S.W[a.wOff[k] + iw];
+        *mp = m;
     }
 }
 
@@ -1371,7 +1559,7 @@ __global__ void k_win_tail(const ShotSlot* slots, int cur, int winOut, float wa,
         case WIN_W: S.W[wA + i] = v; break;
         case WIN_SS: S.W[wA + i] = mtil[i] * v; break;
         case WIN_IMG: S.img[i] += S.W[wImg + i] * v; break;
-        default: S.ss[ssA][i] = v; break;
+        default: S.ss[ssA][wA + i] = v; break;   // WIN_SLICE: wA offsets the ss slice
         }
     }
 }
@@ -1559,22 +1747,43 @@ hipError_t launch_born_adj_finish(const ShotSlot* slots, int nshots, int ssR, co
     return hipGetLastError();
 }
 
-hipError_t launch_ext_ss(const ShotSlot* slots, int nshots, int dst, long wk, const float* mext,
-                         const float* v2dt2, int nz, int nx, int hExt, hipStream_t s)
+// Register-tiled variants by the bound HX >= hExt; larger offsets take the
+// untiled kernels.
+hipError_t launch_ext_ss_batch(const ExtBatchArgs& a, int nshots, hipStream_t s)
 {
-    dim3 g((nz - 2 * FAT + 63) / 64, nshots, nx - 2 * FAT);
-    hipLaunchKernelGGL(k_ext_ss, g, dim3(64), 0, s, slots, dst, wk, mext, v2dt2, nz, nx, hExt);
+    if (a.nk < 1 || a.nk > kExtBatchMax) return hipErrorInvalidValue;
+    const int nzi = a.nz - 2 * FAT, nxi = a.nx - 2 * FAT;
+    if (nzi <= 0 || nxi <= 0) return hipSuccess;
+    if (a.hExt <= 20) {
+        const int spb = 2;   // shots per block: the model registers serve 2 * nk tiles
+        dim3 g((nxi + kExtTX - 1) / kExtTX, (nshots + spb - 1) / spb, (nzi + kExtTZ - 1) / kExtTZ), b(kExtTZ, 4);
+        if (a.hExt <= 2) hipLaunchKernelGGL(k_ext_ss_tile<2>, g, b, 0, s, a, nshots, spb);
+        else if (a.hExt <= 5) hipLaunchKernelGGL(k_ext_ss_tile<5>, g, b, 0, s, a, nshots, spb);
+        else if (a.hExt <= 10) hipLaunchKernelGGL(k_ext_ss_tile<10>, g, b, 0, s, a, nshots, spb);
+        else hipLaunchKernelGGL(k_ext_ss_tile<20>, g, b, 0, s, a, nshots, spb);
+    } else {
+        dim3 g((nzi + 63) / 64, nshots, nxi);
+        hipLaunchKernelGGL(k_ext_ss_any, g, dim3(64), 0, s, a);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_ext_img(const ShotSlot* slots, int nshots, int src, long wk, const float* v2dt2,
-                          int nz, int nx, int nExt, hipStream_t s)
+hipError_t launch_ext_img_batch(const ExtBatchArgs& a, int nshots, hipStream_t s)
 {
-    const int hExt = (nExt - 1) / 2;
-    const int nxr = nx - 2 * FAT - 2 * hExt;
-    if (nxr <= 0) return hipSuccess;
-    dim3 g((nz - 2 * FAT + 63) / 64, nshots, nxr);
-    hipLaunchKernelGGL(k_ext_img, g, dim3(64), 0, s, slots, src, wk, v2dt2, nz, nx, nExt);
+    if (a.nk < 1 || a.nk > kExtBatchMax) return hipErrorInvalidValue;
+    const int nzi = a.nz - 2 * FAT, nxr = a.nx - 2 * FAT - 2 * a.hExt;
+    if (nzi <= 0 || nxr <= 0) return hipSuccess;
+    if (a.hExt <= 20) {
+        const int spb = 1;
+        dim3 g((nxr + kExtTX - 1) / kExtTX, (nshots + spb - 1) / spb, (nzi + kExtTZ - 1) / kExtTZ), b(kExtTZ, 4);
+        if (a.hExt <= 2) hipLaunchKernelGGL(k_ext_img_tile<2>, g, b, 0, s, a, nshots, spb);
+        else if (a.hExt <= 5) hipLaunchKernelGGL(k_ext_img_tile<5>, g, b, 0, s, a, nshots, spb);
+        else if (a.hExt <= 10) hipLaunchKernelGGL(k_ext_img_tile<10>, g, b, 0, s, a, nshots, spb);
+        else hipLaunchKernelGGL(k_ext_img_tile<20>, g, b, 0, s, a, nshots, spb);
+    } else {
+        dim3 g((nzi + 63) / 64, nshots, nxr);
+        hipLaunchKernelGGL(k_ext_img_any, g, dim3(64), 0, s, a);
+    }
     return hipGetLastError();
 }
 

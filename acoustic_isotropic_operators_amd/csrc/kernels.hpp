@@ -62,6 +62,7 @@ struct StepArgs {
     int its, itwInj, itwRec;
     long wA, wB;            // element offsets of the W slices used (capture / SS_BORN)
     int ssA, ssB;           // ShotSlot::ss indices used as A and B
+    long ssOffA, ssOffB;    // element offsets added to them (ring slots of the offset-extended passes)
     float fa, fb;           // pre-damp pointwise weights (b skipped when fb == 0)
     float fc, fd;           // data-injection weights
     float fe, ff;           // capture / record weights
@@ -76,6 +77,20 @@ struct StepArgs {
     // secondary-source slices it reads
     int curQ;
     long wQa, wQb;
+};
+
+// Offset-extended Born over a batch of coarse samples (k_ext_ss_tile,
+// k_ext_img_tile): sample k's receiver / secondary-source slice is
+// ShotSlot::ss[0] + ssOff[k] and its source-wavefield slice W + wOff[k].
+constexpr int kExtBatchMax = 16;
+struct ExtBatchArgs {
+    const ShotSlot* slots;
+    const float* v2dt2;
+    const float* mext;      // scaled extended model [nExt][nx][nz] (secondary sources)
+    int nz, nx, hExt;
+    int nk;                 // samples in the batch, 1..kExtBatchMax
+    long ssOff[kExtBatchMax];
+    long wOff[kExtBatchMax];
 };
 
 // Two fine steps of both fields of the lockstep Born forward (sub == 2) per
@@ -168,10 +183,8 @@ hipError_t launch_scale_ext(const float* m, float* out, const float* refl, int n
                             hipStream_t s);
 hipError_t launch_born_adj_finish(const ShotSlot* slots, int nshots, int ssR, const float* v2dt2,
                                   const float* refl, int nz, int nx, hipStream_t s);
-hipError_t launch_ext_ss(const ShotSlot* slots, int nshots, int dst, long wk, const float* mext,
-                         const float* v2dt2, int nz, int nx, int hExt, hipStream_t s);
-hipError_t launch_ext_img(const ShotSlot* slots, int nshots, int src, long wk, const float* v2dt2,
-                          int nz, int nx, int nExt, hipStream_t s);
+hipError_t launch_ext_ss_batch(const ExtBatchArgs& a, int nshots, hipStream_t s);
+hipError_t launch_ext_img_batch(const ExtBatchArgs& a, int nshots, hipStream_t s);
 hipError_t launch_ext_adj_finish(const ShotSlot* slots, int nshots, const float* refl, int nz, int nx,
                                  int nExt, hipStream_t s);
 hipError_t launch_accumulate(float* acc, const float* img, long n, hipStream_t s);

@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-shots", type=int, default=1)
     ap.add_argument("--opt", action="append", default=[], help="engine option key=value (aiso_ctx_set_option)")
+    ap.add_argument("--workload", default="born", choices=["born", "bornext21"],
+                    help="born: the headline Born fwd+adj; bornext21: subsurface-offset extended Born "
+                         "fwd+adj with 21 offsets on the same grid and shots")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy 1 GiB device-to-device 3 times (HBM counter calibration)")
     return ap.parse_args()
@@ -96,6 +99,8 @@ def main():
     from acoustic_isotropic_operators_amd import _capi as A
     from acoustic_isotropic_operators_amd.workloads import born_lsrtm
 
+    if a.workload == "bornext21":
+        return bench_ext(a, torch, A, born_lsrtm)
     w = born_lsrtm(nShot=a.shots)
     p = w.params
     ctx = A.Context(p)
@@ -189,6 +194,52 @@ def main():
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_shots)
+    print(json.dumps(out), flush=True)
+
+
+def bench_ext(a, torch, A, born_lsrtm, nExt=21):
+    """Extended Born (subsurface offset, nExt offsets) forward + adjoint on the
+    headline grid and shots: one JSON line, same fields as the headline where
+    they apply.  Point-updates count the four propagations per shot."""
+    w = born_lsrtm(nShot=a.shots)
+    p = dict(w.params, nExt=nExt, extension=A.EXT_OFFSET)
+    ctx = A.Context(p)
+    ctx.set_velocity(w.vel)
+    for kv in a.opt:
+        k, v = kv.split("=")
+        ctx.set_option(k, int(v))
+    srcs = [A.Device.grid(1, iz, 1, 1, ix, 1, p["nz"], p["nx"], p["nts"]) for iz, ix in w.src]
+    rec = [A.Device.grid(*w.rec, p["nz"], p["nx"], p["nts"])]
+    dev = torch.device("cuda:0")
+    mext = np.zeros((nExt,) + w.model.shape, np.float32)
+    mext[nExt // 2] = w.model
+    model = torch.from_numpy(mext).to(dev)
+    data = torch.empty((w.nShot, rec[0].nDev, p["nts"]), dtype=torch.float32, device=dev)
+    image = torch.empty_like(model)
+    torch.cuda.synchronize()
+
+    def step():
+        ctx.bornext_forward(model, data, srcs, w.wavelet, rec, mem=A.MEM_DEVICE)
+        ctx.bornext_adjoint(image, data, srcs, w.wavelet, rec, mem=A.MEM_DEVICE)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / a.steps
+    stats = {n: ctx.profile_read(getattr(A, n)) for n in ("K_SRC", "K_EXT_SCAT", "K_EXT_REC")}
+    pts = 4.0 * w.fine_steps * w.interior * w.nShot
+    out = {"metric": f"Gpts/s, extended Born fwd+adj nExt={nExt}, {w.nShot} shots 500x1200", "value": pts / t / 1e9,
+           "unit": "Gpts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": t * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"bornext{nExt}_{w.nShot}x500x1200", "nExt": nExt, "shots": w.nShot,
+                      "shots_per_launch_adj": ctx.info()["batch"]},
+           "step_kernels_ms_per_step": {n: v["ms"] / a.steps for n, v in stats.items() if v["launches"]},
+           "finite": bool(torch.isfinite(image).all().item())}
     print(json.dumps(out), flush=True)
 
 

@@ -104,7 +104,10 @@ aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
  *       (0 = aiso_ctx_set_batch's value, or automatic);
  *   "batchRec": group size of the adjoint receiver pass within a batch;
  *   "shotsPerBlock" 1/2: shots advanced by one workgroup of the step kernel;
- *   "cacheBudgetMB": on-die cache budget the automatic batch size fills. */
+ *   "cacheBudgetMB": on-die cache budget the automatic batch size fills;
+ *   "extBatch" 1..16: coarse samples per offset-imaging / secondary-source
+ *       launch of the extended Born operators (default 16);
+ *   "lanes" 1..4: streams the shots of a launch are split over (default 1). */
 aiso_status aiso_ctx_set_option(aiso_ctx* ctx, const char* key, int value);
 /* Derived sizes: ntw, sub, nShotsPerLaunch actually used by the last call. */
 aiso_status aiso_ctx_info(const aiso_ctx* ctx, int* ntw, int* batchUsed, size_t* bytesPerShot);
