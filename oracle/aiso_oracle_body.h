@@ -339,7 +339,7 @@ typedef void (*FN(ss_fn))(const orc_grid*, const void*, int, REAL*);
 static void FN(scat_pass)(const orc_grid* g, const REAL* zc, const REAL* xc,
                           const REAL* f, const REAL* cd, const REAL* v2dt2,
                           const int* recPos, int nRec, REAL* dreg,
-                          FN(ss_fn) make_ss, const void* ctx)
+                          FN(ss_fn) make_ss, const void* ctx, int clearData)
 {
     const int nts = g->nts, sub = g->sub, h = sub + 1;
     const long nzx = (long)g->nz * g->nx;
@@ -347,7 +347,7 @@ static void FN(scat_pass)(const orc_grid* g, const REAL* zc, const REAL* xc,
     REAL* p1 = FN(zalloc)(nzx);
     REAL* ssL = FN(zalloc)(nzx);
     REAL* ssR = FN(zalloc)(nzx);
-    memset(dreg, 0, sizeof(REAL) * (size_t)nRec * nts);
+    if (clearData) memset(dreg, 0, sizeof(REAL) * (size_t)nRec * nts);
     make_ss(g, ctx, 0, ssL);
     for (int its = 0; its < nts - 1; ++its) {
         make_ss(g, ctx, its + 1, ssR);
@@ -444,7 +444,7 @@ void FN(orc_born_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
     memcpy(m, model, sizeof(REAL) * (size_t)nzx);
     FN(scale_refl)(g, m, refl, v2dt2);
     FN(born_ctx) ctx = {m, W, v2dt2, refl};
-    FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(born_ss), &ctx);
+    FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(born_ss), &ctx, 1);
     free(m);
 }
 
@@ -536,7 +536,7 @@ void FN(orc_bornext_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
     memcpy(m, model, sizeof(REAL) * (size_t)nm);
     FN(scale_refl_ext)(g, m, refl);
     FN(born_ctx) ctx = {m, W, v2dt2, refl};
-    FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(ext_ss), &ctx);
+    FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(ext_ss), &ctx, 1);
     free(m);
 }
 
@@ -555,6 +555,238 @@ void FN(orc_bornext_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
     FN(rec_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(ext_img), &ctx);
     FN(scale_refl_ext)(g, model, refl);
 }
+
+/* ---------------------------------------------------------------------------
+ * Offset-extended tomographic operator (tomoExtShotsGpu, extension=offset).
+ */
+
+/* computeTomoSrcWfldDt2 (F/src/tomoExtShotsGpuFunctions.cu:1251-1360) with
+ * srcWfldSecondTimeDerivative (F/src/kernelsGpu.cu:188-197): the coarse
+ * source wavefield W from src_pass is replaced in place by
+ * cSide*(W[k-1] + W[k+1]) + cCenter*W[k], zero beyond both ends, where
+ * cSide = 1/dts^2 and cCenter = -2/dts^2 are rounded to REAL
+ * (F/src/tomoExtShotsGpuFunctions.cu:138-139). */
+static void FN(dt2_coarse)(const orc_grid* g, REAL* W)
+{
+    const int nts = g->nts;
+    const long nzx = (long)g->nz * g->nx;
+    const REAL dts = (REAL)g->dts;
+    const REAL d2 = dts * dts;
+    const REAL cs = (REAL)(1.0 / (double)d2), cc = (REAL)(-2.0 / (double)d2);
+    INTERIOR_BEGIN(g)
+        REAL prev = 0, cur = W[i];
+        for (int k = 0; k < nts; ++k) {
+            const REAL next = k + 1 < nts ? W[(long)(k + 1) * nzx + i] : (REAL)0;
+            W[(long)k * nzx + i] = cs * (prev + next) + cc * cur;
+            prev = cur;
+            cur = next;
+        }
+    INTERIOR_END
+}
+
+/* Scattered wavefield captured at the coarse rate: the first half of
+ * computeTomoLeg1OffsetFwd and computeTomoLeg2OffsetFwd
+ * (F/src/tomoExtShotsGpuFunctions.cu:2005-2062, 2227-2261).  P (nts slices)
+ * is zeroed, then per fine step: step -> injectSecondarySource -> damp ->
+ * interpWavefield. */
+static void FN(cap_pass)(const orc_grid* g, const REAL* zc, const REAL* xc,
+                         const REAL* f, const REAL* cd, const REAL* v2dt2,
+                         FN(ss_fn) make_ss, const void* ctx, REAL* P)
+{
+    const int nts = g->nts, sub = g->sub, h = sub + 1;
+    const long nzx = (long)g->nz * g->nx;
+    REAL* p0 = FN(zalloc)(nzx);
+    REAL* p1 = FN(zalloc)(nzx);
+    REAL* ssL = FN(zalloc)(nzx);
+    REAL* ssR = FN(zalloc)(nzx);
+    memset(P, 0, sizeof(REAL) * (size_t)nzx * nts);
+    make_ss(g, ctx, 0, ssL);
+    for (int its = 0; its < nts - 1; ++its) {
+        make_ss(g, ctx, its + 1, ssR);
+        for (int it2 = 1; it2 < sub + 1; ++it2) {
+            FN(orc_step_fwd)(g, zc, xc, p0, p1, p0, v2dt2);
+            FN(inject_secondary)(g, f, h, ssL, ssR, p0, it2 - 1);
+            FN(orc_damp)(g, cd, p0, p1);
+            FN(capture2)(g, f, h, p0, P + (long)its * nzx, P + (long)(its + 1) * nzx, it2);
+            FN(swap)(&p0, &p1);
+        }
+        FN(swap)(&ssL, &ssR);
+        memset(ssR, 0, sizeof(REAL) * (size_t)nzx);
+    }
+    free(p0); free(p1); free(ssL); free(ssR);
+}
+
+/* computeTomoRecWfld (F/src/tomoExtShotsGpuFunctions.cu:1481-1515): the
+ * receiver wavefield R (nts slices) of the adjoint propagation of the data,
+ * captured unscaled at the coarse rate. */
+static void FN(rec_wfld)(const orc_grid* g, const REAL* zc, const REAL* xc,
+                         const REAL* f, const REAL* cd, const REAL* v2dt2,
+                         const int* recPos, int nRec, const REAL* dreg, REAL* R)
+{
+    const int nts = g->nts, sub = g->sub, h = sub + 1;
+    const long nzx = (long)g->nz * g->nx;
+    REAL* p0 = FN(zalloc)(nzx);
+    REAL* p1 = FN(zalloc)(nzx);
+    REAL* tmp = FN(zalloc)(nzx);
+    memset(R, 0, sizeof(REAL) * (size_t)nzx * nts);
+    for (int its = nts - 2; its > -1; --its) {
+        for (int it2 = sub - 1; it2 > -1; --it2) {
+            FN(orc_step_adj)(g, zc, xc, p0, p1, p0, v2dt2, tmp);
+            FN(interp_inject)(nts, h, f, dreg, p0, its, it2, recPos, nRec);
+            FN(orc_damp)(g, cd, p0, p1);
+            FN(capture2)(g, f, h, p0, R + (long)its * nzx, R + (long)(its + 1) * nzx, it2);
+            FN(swap)(&p0, &p1);
+        }
+    }
+    free(p0); free(p1); free(tmp);
+}
+
+/* imagingOffsetTomoAdjGpu (F/src/kernelsGpu.cu:460-475): secondary source of
+ * leg 2 of the adjoint from the receiver wavefield R at coarse sample k,
+ * ss[x] += r[e][x+e] * R[k][x+2e] * v2dt2[x+2e] into a zeroed slice. */
+static void FN(tomo_adj_ss)(const orc_grid* g, const REAL* R, const REAL* r,
+                            const REAL* v2dt2, int k, REAL* ss)
+{
+    const int nz = g->nz, nx = g->nx, hExt = (g->nExt - 1) / 2;
+    const long nzx = (long)nz * nx;
+    INTERIOR_BEGIN(g)
+        int lo = -hExt, t = FAT + hExt - ix;
+        if (t > lo) lo = t;
+        int hi = nx - 1 - FAT - ix - hExt;
+        if (hExt < hi) hi = hExt;
+        for (int e = lo; e < hi + 1; ++e) {
+            long iModel = nzx * (e + hExt) + (long)nz * (ix + e) + iz;
+            long iRec = (long)(ix + 2 * e) * nz + iz;
+            ss[i] += r[iModel] * R[nzx * k + iRec] * v2dt2[iRec];
+        }
+    INTERIOR_END
+}
+
+/* imagingAdjGpu (F/src/kernelsGpu.cu:318-325): m += X[k] * slice. */
+static void FN(img_acc)(const orc_grid* g, REAL* m, const REAL* X, int k, const REAL* slice)
+{
+    const long nzx = (long)g->nz * g->nx;
+    const REAL* Xk = X + (long)k * nzx;
+    INTERIOR_BEGIN(g)
+        m[i] += Xk[i] * slice[i];
+    INTERIOR_END
+}
+
+/* tomoOffsetShotsFwdGpu (F/src/tomoExtShotsGpuFunctions.cu:467-572) with
+ * computeTomoLeg1OffsetFwd (:2005-2113) and computeTomoLeg2OffsetFwd
+ * (:2227-2317).  model: nz*nx velocity perturbation; extRefl: nExt*nz*nx
+ * extended reflectivity (both read only); W, P: nts*nz*nx scratch.
+ *   leg 1: s -> extended reflectivity -> P -> model -> data
+ *   leg 2: s -> model -> P -> extended reflectivity -> data */
+void FN(orc_tomo_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
+                      const REAL* sig, const int* srcPos, int nSrc,
+                      const int* recPos, int nRec, const REAL* model, const REAL* extRefl,
+                      int leg1, int leg2, REAL* dreg, REAL* W, REAL* P)
+{
+    REAL zc[6], xc[6], f[2 * 101], cd[512];
+    FN(orc_tables)(g, zc, xc, f, cd);
+    const long nzx = (long)g->nz * g->nx, nr = nzx * g->nExt;
+    FN(src_pass)(g, zc, xc, f, cd, v2dt2, sig, srcPos, nSrc, W);
+    FN(dt2_coarse)(g, W);
+    REAL* m = FN(zalloc)(nzx);
+    memcpy(m, model, sizeof(REAL) * (size_t)nzx);
+    FN(scale_refl)(g, m, refl, v2dt2);
+    REAL* r = FN(zalloc)(nr);
+    memcpy(r, extRefl, sizeof(REAL) * (size_t)nr);
+    FN(scale_refl_ext)(g, r, refl);
+    memset(dreg, 0, sizeof(REAL) * (size_t)nRec * g->nts);
+    FN(born_ctx) rW = {r, W, v2dt2, refl}, mW = {m, W, v2dt2, refl};
+    FN(born_ctx) rP = {r, P, v2dt2, refl}, mP = {m, P, v2dt2, refl};
+    if (leg1) {
+        FN(cap_pass)(g, zc, xc, f, cd, v2dt2, FN(ext_ss), &rW, P);
+        FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(born_ss), &mP, 0);
+    }
+    if (leg2) {
+        FN(cap_pass)(g, zc, xc, f, cd, v2dt2, FN(born_ss), &mW, P);
+        FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(ext_ss), &rP, 0);
+    }
+    free(m); free(r);
+}
+
+/* tomoOffsetShotsAdjGpu (F/src/tomoExtShotsGpuFunctions.cu:912-1020) with
+ * computeTomoLeg1OffsetAdj (:2788-2860) and computeTomoLeg2OffsetAdj
+ * (:2949-3020).  model (output, nz*nx) is overwritten with this shot's
+ * contribution; W holds the source wavefield's second time derivative and P
+ * the receiver wavefield. */
+void FN(orc_tomo_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
+                      const REAL* sig, const int* srcPos, int nSrc,
+                      const int* recPos, int nRec, REAL* model, const REAL* dreg,
+                      const REAL* extRefl, int leg1, int leg2, REAL* W, REAL* P)
+{
+    REAL zc[6], xc[6], f[2 * 101], cd[512];
+    FN(orc_tables)(g, zc, xc, f, cd);
+    const int nts = g->nts, sub = g->sub, h = sub + 1;
+    const long nzx = (long)g->nz * g->nx, nr = nzx * g->nExt;
+    FN(src_pass)(g, zc, xc, f, cd, v2dt2, sig, srcPos, nSrc, W);
+    FN(dt2_coarse)(g, W);
+    FN(rec_wfld)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, P);
+    REAL* r = FN(zalloc)(nr);
+    memcpy(r, extRefl, sizeof(REAL) * (size_t)nr);
+    FN(scale_refl_ext)(g, r, refl);
+    memset(model, 0, sizeof(REAL) * (size_t)nzx);
+    REAL* p0 = FN(zalloc)(nzx);
+    REAL* p1 = FN(zalloc)(nzx);
+    REAL* ssL = FN(zalloc)(nzx);
+    REAL* ssR = FN(zalloc)(nzx);
+    REAL* scL = FN(zalloc)(nzx);
+    REAL* scR = FN(zalloc)(nzx);
+    REAL* tmp = FN(zalloc)(nzx);
+    FN(born_ctx) rW = {r, W, v2dt2, refl};
+    if (leg1) {   /* forward in time from the extended scattering of W'' */
+        FN(ext_ss)(g, &rW, 0, ssL);
+        for (int its = 0; its < nts - 1; ++its) {
+            FN(ext_ss)(g, &rW, its + 1, ssR);
+            for (int it2 = 1; it2 < sub + 1; ++it2) {
+                FN(orc_step_fwd)(g, zc, xc, p0, p1, p0, v2dt2);
+                FN(inject_secondary)(g, f, h, ssL, ssR, p0, it2 - 1);
+                FN(orc_damp)(g, cd, p0, p1);
+                FN(capture2)(g, f, h, p0, scL, scR, it2);
+                FN(swap)(&p0, &p1);
+            }
+            FN(img_acc)(g, model, P, its, scL);
+            FN(swap)(&ssL, &ssR);
+            memset(ssR, 0, sizeof(REAL) * (size_t)nzx);
+            FN(swap)(&scL, &scR);
+            memset(scR, 0, sizeof(REAL) * (size_t)nzx);
+        }
+        FN(img_acc)(g, model, P, nts - 1, scL);
+    }
+    if (leg2) {   /* backward in time from the adjoint extended scattering of R */
+        memset(p0, 0, sizeof(REAL) * (size_t)nzx);
+        memset(p1, 0, sizeof(REAL) * (size_t)nzx);
+        memset(ssL, 0, sizeof(REAL) * (size_t)nzx);
+        memset(ssR, 0, sizeof(REAL) * (size_t)nzx);
+        memset(scL, 0, sizeof(REAL) * (size_t)nzx);
+        memset(scR, 0, sizeof(REAL) * (size_t)nzx);
+        FN(tomo_adj_ss)(g, P, r, v2dt2, nts - 1, ssR);
+        for (int its = nts - 2; its > -1; --its) {
+            FN(tomo_adj_ss)(g, P, r, v2dt2, its, ssL);
+            for (int it2 = sub - 1; it2 > -1; --it2) {
+                FN(orc_step_adj)(g, zc, xc, p0, p1, p0, v2dt2, tmp);
+                FN(inject_secondary)(g, f, h, ssL, ssR, p0, it2 + 1);
+                FN(orc_damp)(g, cd, p0, p1);
+                FN(capture2)(g, f, h, p0, scL, scR, it2);
+                FN(swap)(&p0, &p1);
+            }
+            FN(img_acc)(g, model, W, its + 1, scR);
+            FN(swap)(&ssR, &ssL);
+            memset(ssL, 0, sizeof(REAL) * (size_t)nzx);
+            FN(swap)(&scR, &scL);
+            memset(scL, 0, sizeof(REAL) * (size_t)nzx);
+        }
+        FN(img_acc)(g, model, W, 0, scR);
+    }
+    FN(scale_refl)(g, model, refl, v2dt2);
+    free(r); free(p0); free(p1); free(ssL); free(ssR); free(scL); free(scR); free(tmp);
+}
+
+/* Test hook for dt2_coarse: W (nts*nz*nx) in place. */
+void FN(orc_dt2_coarse)(const orc_grid* g, REAL* W) { FN(dt2_coarse)(g, W); }
 
 /* interpTimeLinTbb::forward, F/src/interpTimeLinTbb.cpp:9-62.
  * m: nDev x nts (coarse), d: nDev x ntw (fine). */

@@ -16,7 +16,9 @@ reference's host classes do:
   chain of F/include/seismicOperator2D.cpp:151-180 inside
   F/src/BornShotsGpu.cpp:60-143,245-339;
 * ``bornext_forward`` / ``bornext_adjoint``: F/src/BornExtGpu.cpp:33-134 inside
-  F/src/BornExtShotsGpu.cpp (same shot loop).
+  F/src/BornExtShotsGpu.cpp (same shot loop);
+* ``tomo_forward`` / ``tomo_adjoint``: F/src/tomoExtGpu.cpp:34-100 inside
+  F/src/tomoExtShotsGpu.cpp:62-145 (same shot loop), extension=offset.
 """
 from __future__ import annotations
 
@@ -310,6 +312,46 @@ def born_adj_reg(g, v2dt2, refl, sig, srcPos, recPos, dreg, prec="f", ext=False)
     return model
 
 
+def _tomo_fns(name, prec):
+    f = _fn(name, prec)
+    f.restype = None
+    return f
+
+
+def tomo_fwd_reg(g, v2dt2, refl, sig, srcPos, recPos, model, ext_refl, leg1=1, leg2=1, prec="f"):
+    dt = _dt(prec)
+    dreg = np.zeros((len(recPos), g.nts), dt)
+    W = np.zeros((g.nts, g.nx, g.nz), dt)
+    P = np.zeros_like(W)
+    gc = g.c()
+    _tomo_fns("orc_tomo_fwd", prec)(C.byref(gc), _p(v2dt2), _p(refl), _p(sig), _p(srcPos), len(srcPos),
+                                    _p(recPos), len(recPos), _p(np.ascontiguousarray(model, dt)),
+                                    _p(np.ascontiguousarray(ext_refl, dt)), int(leg1), int(leg2), _p(dreg),
+                                    _p(W), _p(P))
+    return dreg
+
+
+def tomo_adj_reg(g, v2dt2, refl, sig, srcPos, recPos, dreg, ext_refl, leg1=1, leg2=1, prec="f"):
+    dt = _dt(prec)
+    model = np.zeros((g.nx, g.nz), dt)
+    W = np.zeros((g.nts, g.nx, g.nz), dt)
+    P = np.zeros_like(W)
+    gc = g.c()
+    _tomo_fns("orc_tomo_adj", prec)(C.byref(gc), _p(v2dt2), _p(refl), _p(sig), _p(srcPos), len(srcPos),
+                                    _p(recPos), len(recPos), _p(model), _p(np.ascontiguousarray(dreg, dt)),
+                                    _p(np.ascontiguousarray(ext_refl, dt)), int(leg1), int(leg2), _p(W), _p(P))
+    return model
+
+
+def dt2_coarse(g, W, prec="f"):
+    """Second time derivative of a coarse wavefield [nts][nx][nz] (interior
+    points), zero beyond both ends, as the tomographic operator applies it."""
+    W = np.array(W, _dt(prec), copy=True, order="C")
+    gc = g.c()
+    _tomo_fns("orc_dt2_coarse", prec)(C.byref(gc), _p(W))
+    return W
+
+
 # ------------------------------------------------- multi-shot operators --
 def _shot_geom(lst, i):
     return lst[0] if len(lst) == 1 else lst[i]
@@ -436,3 +478,34 @@ def bornext_forward(g, vel, sources, wavelets, receivers, model, prec="f"):
 
 def bornext_adjoint(g, vel, sources, wavelets, receivers, data, prec="f"):
     return born_adjoint(g, vel, sources, wavelets, receivers, data, prec, ext=True)
+
+
+def tomo_forward(g, vel, sources, wavelets, receivers, model, ext_refl, leg1=1, leg2=1, prec="f"):
+    """Offset-extended tomographic operator (tomoExtShotsGpu, extension=offset):
+    velocity perturbation [nx][nz] -> data [nShot][nRec][nts], about the
+    extended reflectivity ext_refl [nExt][nx][nz]."""
+    dt = _dt(prec)
+    vel = np.ascontiguousarray(vel, dt)
+    v2, rf = vel_scales(g, vel, prec)
+    nShot = len(sources)
+    nRec = _shot_geom(receivers, 0).nDev
+    data = np.zeros((nShot, nRec, g.nts), dt)
+    for s in range(nShot):
+        src, rec = sources[s], _shot_geom(receivers, s)
+        fine = born_source_fine(g, vel, src, _shot_geom(wavelets, s), prec)
+        dreg = tomo_fwd_reg(g, v2, rf, fine, src.regPos, rec.regPos, model, ext_refl, leg1, leg2, prec)
+        data[s] += rec.forward(dreg, prec=prec)
+    return data
+
+
+def tomo_adjoint(g, vel, sources, wavelets, receivers, data, ext_refl, leg1=1, leg2=1, prec="f"):
+    dt = _dt(prec)
+    vel = np.ascontiguousarray(vel, dt)
+    v2, rf = vel_scales(g, vel, prec)
+    model = np.zeros((g.nx, g.nz), dt)
+    for s in range(len(sources)):
+        src, rec = sources[s], _shot_geom(receivers, s)
+        fine = born_source_fine(g, vel, src, _shot_geom(wavelets, s), prec)
+        dreg = rec.adjoint(data[s], prec=prec)
+        model += tomo_adj_reg(g, v2, rf, fine, src.regPos, rec.regPos, dreg, ext_refl, leg1, leg2, prec)
+    return model
