@@ -22,10 +22,12 @@ from .pyOperator import Operator
 
 try:  # the modules sit next to this file; the package dir may also be on sys.path
     from . import pyAcoustic_iso_float_nl, pyAcoustic_iso_float_born, pyAcoustic_iso_float_born_ext
+    from . import pyAcoustic_iso_float_tomo
 except ImportError:  # pragma: no cover
     import pyAcoustic_iso_float_nl
     import pyAcoustic_iso_float_born
     import pyAcoustic_iso_float_born_ext
+    import pyAcoustic_iso_float_tomo
 
 deviceGpu = pyAcoustic_iso_float_nl.deviceGpu
 parfile2pars = sep.parfile2pars
@@ -241,14 +243,8 @@ class BornShotsGpu(_Wrapped):
 
 
 # --------------------------------------------------------- extended Born --
-def BornExtOpInitFloat(args, client=None):
-    """Extended model [nExt][nx][nz] with the extension axis centred on zero
-    (offset: d = dx; PY/Acoustic_iso_float.py:973-1078)."""
-    _no_client(client)
-    par = create_parObj(args)
-    vel = _read_vel(par)
-    zAxis, xAxis = vel.getHyper().axes[0], vel.getHyper().axes[1]
-    time = _time_axis(par)
+def _ext_axis(par):
+    """Offset extension axis centred on zero, d = dx (PY/Acoustic_iso_float.py:1006-1030)."""
     extension = par.getString("extension", "noExtensionType")
     if extension == "noExtensionType":
         raise ValueError("**** ERROR: User did not provide extension type ****")
@@ -261,7 +257,18 @@ def BornExtOpInitFloat(args, client=None):
         raise ValueError("Length of extension axis must be an uneven number")
     dExt = par.getFloat("dx", -1.0)
     hExt = (nExt - 1) // 2
-    extAxis = sep.axis(n=nExt, o=-dExt * hExt, d=dExt)
+    return sep.axis(n=nExt, o=-dExt * hExt, d=dExt)
+
+
+def BornExtOpInitFloat(args, client=None):
+    """Extended model [nExt][nx][nz] with the extension axis centred on zero
+    (offset: d = dx; PY/Acoustic_iso_float.py:973-1078)."""
+    _no_client(client)
+    par = create_parObj(args)
+    vel = _read_vel(par)
+    zAxis, xAxis = vel.getHyper().axes[0], vel.getHyper().axes[1]
+    time = _time_axis(par)
+    extAxis = _ext_axis(par)
     signals = _read_sources(par)
     model = sep.getSepVector(axes=[zAxis, xAxis, extAxis])
     sources, srcAxis = buildSourceGeometry(par, vel)
@@ -287,3 +294,49 @@ class BornExtShotsGpu(_Wrapped):
 
     def getSecWfld(self):
         return self.pyOp.getSecWfld()
+
+
+# ------------------------------------------------------- tomographic op --
+def tomoExtOpInitFloat(args, client=None):
+    """(model, data, vel, parObject, sources, sourcesSignals, receivers,
+    reflectivityExt, modelLocal): model is a velocity perturbation on the
+    velocity's hypercube; reflectivityExt [nExt][nx][nz] is read from the
+    ``reflectivity`` parfile key, zero when absent
+    (PY/Acoustic_iso_float.py:1306-1422)."""
+    _no_client(client)
+    par = create_parObj(args)
+    vel = _read_vel(par)
+    zAxis, xAxis = vel.getHyper().axes[0], vel.getHyper().axes[1]
+    time = _time_axis(par)
+    extAxis = _ext_axis(par)
+    signals = _read_sources(par)
+    reflFile = par.getString("reflectivity", "None")
+    if reflFile == "None":
+        refl = sep.getSepVector(axes=[zAxis, xAxis, extAxis])
+        refl.scale(0.0)
+    else:
+        refl = sep.read(reflFile)
+    model = sep.getSepVector(vel.getHyper())
+    sources, srcAxis = buildSourceGeometry(par, vel)
+    receivers, recAxis = buildReceiversGeometry(par, vel)
+    data = _data_vector(time, recAxis, srcAxis)
+    return model, data, vel, par, sources, [signals], receivers, refl, model
+
+
+class tomoExtShotsGpu(_Wrapped):
+    """Subsurface-offset tomographic operator: velocity perturbation -> data
+    about a fixed extended reflectivity (PY/Acoustic_iso_float.py:1424-1510)."""
+
+    _mod = pyAcoustic_iso_float_tomo
+    _name = " TomoOp "
+
+    def __init__(self, domain, range, velocity, paramP, sourceVector, sourcesSignalsVector, receiversVector,
+                 reflectivityExt):
+        self.setDomainRange(domain, range)
+        sig = [_cpp(s) for s in sourcesSignalsVector]
+        self.pyOp = pyAcoustic_iso_float_tomo.tomoExtShotsGpu(_cpp(velocity), _cpp(paramP).param, sourceVector, sig,
+                                                              receiversVector, _cpp(reflectivityExt))
+
+    def setReflectivityExt(self, reflectivityExt):
+        with self._mod.ostream_redirect():
+            self.pyOp.setReflectivityExt(_cpp(reflectivityExt))

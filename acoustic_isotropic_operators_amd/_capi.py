@@ -15,7 +15,7 @@ from . import lib_path
 
 MEM_HOST, MEM_DEVICE = 0, 1
 EXT_NONE, EXT_OFFSET = 0, 1
-K_ALL, K_NL_FWD, K_NL_ADJ, K_SRC, K_BORN_SCAT, K_BORN_REC, K_EXT_SCAT, K_EXT_REC, K_BORN_PAIR = range(-1, 8)
+K_ALL, K_NL_FWD, K_NL_ADJ, K_SRC, K_BORN_SCAT, K_BORN_REC, K_EXT_SCAT, K_EXT_REC, K_BORN_PAIR, K_TOMO = range(-1, 9)
 
 
 class FdParams(C.Structure):
@@ -61,6 +61,9 @@ _SIGS = {
     "aiso_born_adjoint": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_bornext_forward": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_bornext_adjoint": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
+    "aiso_ctx_set_ext_reflectivity": (_I, [_P, _P, _I]),
+    "aiso_tomo_forward": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
+    "aiso_tomo_adjoint": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_pad_plus": (_I, [_I, _I, _I]),
 }
 
@@ -267,3 +270,16 @@ class Context:
 
     def bornext_adjoint(self, model, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
         self._born(lib().aiso_bornext_adjoint, model, data, sources, wavelets, receivers, add, mem)
+
+    def set_ext_reflectivity(self, refl, mem=MEM_HOST):
+        """Extended reflectivity [nExt][nx][nz] of the tomographic operator."""
+        if mem == MEM_HOST:
+            refl = np.ascontiguousarray(refl, np.float32)
+        self._ext_refl = refl
+        _check(lib().aiso_ctx_set_ext_reflectivity(self.h, _ptr(refl), mem))
+
+    def tomo_forward(self, model, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
+        self._born(lib().aiso_tomo_forward, model, data, sources, wavelets, receivers, add, mem)
+
+    def tomo_adjoint(self, model, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
+        self._born(lib().aiso_tomo_adjoint, model, data, sources, wavelets, receivers, add, mem)

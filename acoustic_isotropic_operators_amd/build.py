@@ -2,7 +2,7 @@
 
 * ``libaiso.so``: the HIP engine (kernels.hip, engine.cpp, host.cpp) for gfx950,
   exporting the C-ABI declared in ``include/aiso.h``;
-* the pybind11 modules ``pyAcoustic_iso_float_{nl,born,born_ext}`` that mirror the
+* the pybind11 modules ``pyAcoustic_iso_float_{nl,born,born_ext,tomo}`` that mirror the
   reference's module names on top of that C-ABI.
 
 Run ``python -m acoustic_isotropic_operators_amd.build`` (or ``build()``).
@@ -58,7 +58,8 @@ def build_pybind():
     ext = sysconfig.get_config_var("EXT_SUFFIX")
     inc = ["-I" + pybind11.get_include(), "-I" + sysconfig.get_paths()["include"], "-I" + INC, "-I" + CSRC]
     outs = []
-    for name in ("pyAcoustic_iso_float_nl", "pyAcoustic_iso_float_born", "pyAcoustic_iso_float_born_ext"):
+    for name in ("pyAcoustic_iso_float_nl", "pyAcoustic_iso_float_born", "pyAcoustic_iso_float_born_ext",
+                 "pyAcoustic_iso_float_tomo"):
         src = os.path.join(CSRC, "pybind", name + ".cpp")
         out = os.path.join(PKG, name + ext)
         _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", *inc, src,

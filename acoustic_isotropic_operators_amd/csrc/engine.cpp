@@ -105,6 +105,12 @@ struct aiso_ctx {
     // work buffers
     DBuf pairs, pairsQ, W, ss, img, dreg, sig, slots, slotsQ, mtil, acc, modelDev, dataDev;
     DBuf ck;                            // checkpoints of the velocity-gradient pass
+    // tomographic operator: the extended reflectivity it linearizes about
+    // (set once, [nExt][nx][nz]), its per-call 2/v^3 scaling, and the
+    // scattered / receiver wavefield and capture buffers
+    DBuf extRefl, rExt, P, cap, slotsC;
+    bool haveExtRefl = false;
+    int leg1 = 1, leg2 = 1;
     hipEvent_t evIn = nullptr;          // orders device inputs after the default stream's work
     // Lanes: the shots of a batch split over streams whose kernels run
     // concurrently, so one lane's launch gaps and tails are filled by the
@@ -240,11 +246,12 @@ DevGeom* dev_geom_pair2(aiso_ctx* c, const aiso_device* d, bool region)
     return out;
 }
 
-enum class Op { NlFwd, NlAdj, BornFwd, BornAdj, ExtFwd, ExtAdj };
+enum class Op { NlFwd, NlAdj, BornFwd, BornAdj, ExtFwd, ExtAdj, TomoFwd, TomoAdj };
 
-bool is_adj(Op op) { return op == Op::NlAdj || op == Op::BornAdj || op == Op::ExtAdj; }
+bool is_adj(Op op) { return op == Op::NlAdj || op == Op::BornAdj || op == Op::ExtAdj || op == Op::TomoAdj; }
 bool is_born(Op op) { return op != Op::NlFwd && op != Op::NlAdj; }
 bool is_ext(Op op) { return op == Op::ExtFwd || op == Op::ExtAdj; }
+bool is_tomo(Op op) { return op == Op::TomoFwd || op == Op::TomoAdj; }
 
 struct Request {
     Op op;
@@ -416,6 +423,13 @@ void run(aiso_ctx* c, const Request& q)
     if ((!is_born(q.op) || is_ext(q.op)) && p.minPad >= kPadMax) throw Error(AISO_ERR_PARAM, "**** ERROR: Padding value is too high ****");
     if (is_ext(q.op) && p.extension != AISO_EXT_OFFSET && p.nExt > 1)
         throw Error(AISO_ERR_PARAM, "extended Born needs extension=offset");
+    if (is_tomo(q.op)) {
+        if (p.extension != AISO_EXT_OFFSET)
+            throw Error(AISO_ERR_PARAM, "the tomographic operator needs extension=offset (time-lag is not supported)");
+        if (!c->haveExtRefl) throw Error(AISO_ERR_STATE, "extended reflectivity not set");
+        if (p.sub >= kSubMax) throw Error(AISO_ERR_PARAM, "**** ERROR: Subsampling parameter too high ****");
+        if (p.minPad >= kPadMax) throw Error(AISO_ERR_PARAM, "**** ERROR: Padding value is too high ****");
+    }
     if (is_born(q.op) && q.nWav != 1 && q.nWav != q.nShot) throw Error(AISO_ERR_PARAM, "wavelet count must be 1 or nShot");
     if (!is_born(q.op) && q.nShotSig != 1 && q.nShotSig != q.nShot)
         throw Error(AISO_ERR_PARAM, "model shot axis must be 1 or nShot");
@@ -504,19 +518,21 @@ void run(aiso_ctx* c, const Request& q)
     const bool adj2 = q.op == Op::BornAdj && p.sub == 2 && c->pair2 && noRing;
     const int pairSlices = pair2 || adj2 ? 4 : 2;
     const long wSlices = lockstep ? 3 : p.nts;
-    const bool needW = is_born(q.op), needSS = is_ext(q.op) || q.op == Op::BornAdj;
+    const bool needW = is_born(q.op), needSS = is_ext(q.op) || is_tomo(q.op) || q.op == Op::BornAdj;
     // The offset-extended passes keep a ring of coarse-rate slices per shot:
     // extK samples batched per imaging / secondary-source launch, plus the
     // one the time steps are writing or reading.
     const int extK = std::min(c->extBatch, p.nts);
-    const int ssSlices = is_ext(q.op) ? extK + 1 : 2;
+    const int ssSlices = is_ext(q.op) || is_tomo(q.op) ? extK + 1 : 2;
     auto ssSlot = [&](int k) { return (long)(k % ssSlices) * nzx; };
-    const bool needImg = q.op == Op::BornAdj || q.op == Op::ExtAdj;
+    const bool needImg = q.op == Op::BornAdj || q.op == Op::ExtAdj || q.op == Op::TomoAdj;
+    const bool tomo = is_tomo(q.op);
     size_t perShot = pairSlices * nzx * 4 + (size_t)nRecRegMax * p.nts * 4 + (size_t)nSrcRegMax * p.ntw * 4;
     if (needW) perShot += (size_t)wSlices * nzx * 4;
     if (lockstep) perShot += pairSlices * nzx * 4;
     if (needSS) perShot += (size_t)ssSlices * nzx * 4;
     if (needImg) perShot += (size_t)nExt * nzx * 4;
+    if (tomo) perShot += (size_t)(p.nts + 2) * nzx * 4;   // scattered / receiver wavefield + capture pair
     c->bytesPerShot = perShot;
     // Shots per launch: an explicit request, else as many as keep the
     // slices every fine step touches (per shot) within the on-die cache
@@ -529,13 +545,15 @@ void run(aiso_ctx* c, const Request& q)
         else if (q.op == Op::BornAdj) hot = 3;        // + image
         else if (q.op == Op::ExtFwd) hot = 4;         // + two secondary-source slices
         else if (q.op == Op::ExtAdj) hot = 3;         // + receiver slice (the image is touched once per extK samples)
+        else if (tomo) hot = 4;                       // + secondary source and captured slices
         B = (int)std::max<long>(4, c->cacheBudget / ((long)hot * nzx * 4));
         B = std::min(B, 32);
     }
     {
         size_t freeB = 0, totalB = 0;
         HIPC(hipMemGetInfo(&freeB, &totalB));
-        size_t have = c->pairs.n + c->pairsQ.n + c->W.n + c->ss.n + c->img.n + c->dreg.n + c->sig.n;
+        size_t have = c->pairs.n + c->pairsQ.n + c->W.n + c->ss.n + c->img.n + c->dreg.n + c->sig.n + c->P.n +
+                      c->cap.n;
         size_t budget = (size_t)((freeB + have) * 0.85);
         B = std::min(B, (int)std::max<size_t>(1, budget / perShot));
     }
@@ -546,7 +564,13 @@ void run(aiso_ctx* c, const Request& q)
     float* pairs = c->pairs.need((size_t)B * pairSlices * nzx * 4) ? c->pairs.as<float>() : nullptr;
     float* W = needW ? (float*)c->W.need((size_t)B * wSlices * nzx * 4) : nullptr;
     float* pairsQ = lockstep ? (float*)c->pairsQ.need((size_t)B * pairSlices * nzx * 4) : nullptr;
-    ShotSlot* dSlotsQ = lockstep || adj2 ? (ShotSlot*)c->slotsQ.need(sizeof(ShotSlot) * B) : nullptr;
+    ShotSlot* dSlotsQ = lockstep || adj2 || tomo ? (ShotSlot*)c->slotsQ.need(sizeof(ShotSlot) * B) : nullptr;
+    // Tomographic operator: P holds the scattered wavefield of the forward
+    // legs or the receiver wavefield of the adjoint (slots dSlotsQ see it as
+    // W); cap holds two captured coarse samples (slots dSlotsC).
+    float* Pw = tomo ? (float*)c->P.need((size_t)B * p.nts * nzx * 4) : nullptr;
+    float* cap = tomo ? (float*)c->cap.need((size_t)B * 2 * nzx * 4) : nullptr;
+    ShotSlot* dSlotsC = tomo ? (ShotSlot*)c->slotsC.need(sizeof(ShotSlot) * B) : nullptr;
     float* ss = needSS ? (float*)c->ss.need((size_t)B * ssSlices * nzx * 4) : nullptr;
     float* img = needImg ? (float*)c->img.need((size_t)B * nExt * nzx * 4) : nullptr;
     float* dreg = (float*)c->dreg.need((size_t)B * nRecRegMax * p.nts * 4);
@@ -564,6 +588,17 @@ void run(aiso_ctx* c, const Request& q)
         float* m = (float*)c->mtil.need(nzx * nExt * 4);
         HIPC(launch_scale_ext(dModelIn, m, c->refl.as<float>(), p.nz, p.nx, nExt, s));
         mtil = m;
+    }
+    const float* rExt = nullptr;   // tomographic operator: extended reflectivity scaled by 2/v^3
+    if (tomo) {
+        if (q.op == Op::TomoFwd) {
+            float* m = (float*)c->mtil.need(nzx * 4);
+            HIPC(launch_scale_model(dModelIn, m, c->v2dt2.as<float>(), c->refl.as<float>(), p.nz, p.nx, s));
+            mtil = m;
+        }
+        float* r = (float*)c->rExt.need(nzx * p.nExt * 4);
+        HIPC(launch_scale_ext(c->extRefl.as<float>(), r, c->refl.as<float>(), p.nz, p.nx, p.nExt, s));
+        rExt = r;
     }
     if (needImg) {
         acc = (float*)c->acc.need(nzx * nExt * 4);
@@ -631,6 +666,17 @@ void run(aiso_ctx* c, const Request& q)
             if (!fine.empty()) std::memcpy(sigHost.data() + (size_t)k * nSrcRegMax * p.ntw, fine.data(), fine.size() * 4);
         }
         HIPC(hipMemcpyAsync(dSlots, slots.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
+        std::vector<ShotSlot> slotsP, slotsCv;
+        if (tomo) {   // the same shots seeing P, or the capture pair, as W
+            slotsP = slots;
+            slotsCv = slots;
+            for (int k = 0; k < nb; ++k) {
+                slotsP[k].W = Pw + (size_t)k * p.nts * nzx;
+                slotsCv[k].W = cap + (size_t)k * 2 * nzx;
+            }
+            HIPC(hipMemcpyAsync(dSlotsQ, slotsP.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
+            HIPC(hipMemcpyAsync(dSlotsC, slotsCv.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
+        }
         std::vector<ShotSlot> slotsQ;
         if (lockstep) {   // the scattered field seen as "the" pair, for single-field launches
             slotsQ = slots;
@@ -669,7 +715,7 @@ void run(aiso_ctx* c, const Request& q)
                 if (cnt > 0) HIPC(launch_ring(mode, c->curLanes > 1 ? lane_args(sa, k0, cnt) : sa, mr, lane_stream(c, l)));
             }
         };
-        c->curLanes = is_ext(q.op) || pair2 || adj2 ? 1 : std::max(1, std::min(c->lanes, nb));
+        c->curLanes = is_ext(q.op) || tomo || pair2 || adj2 ? 1 : std::max(1, std::min(c->lanes, nb));
         if (c->curLanes > 1) ensure_lanes(c);
 
         // Three consecutive fine slices hold everything a coarse sample needs
@@ -680,7 +726,7 @@ void run(aiso_ctx* c, const Request& q)
         // specialized one ended a GPU run in a memory fault on this path with
         // no cause found in its source (it is exact and in bounds when the
         // same source runs on the host).
-        const bool generic = is_ext(q.op);
+        const bool generic = is_ext(q.op) || tomo;
 
         if (pair2) {
             // The scattered field lags the source field by one launch (two
@@ -945,8 +991,172 @@ void run(aiso_ctx* c, const Request& q)
             a.win = 0;
         }
 
+        // -- tomographic operator (extension=offset): both legs, forward or
+        // adjoint, after the source pass above (F/src/tomoExtShotsGpuFunctions.cu
+        // :467-572 and :912-1020)
+        if (tomo) {
+            // W <- its coarse second time derivative, in place
+            const float d2 = p.dts * p.dts;
+            HIPC(launch_dt2_coarse(dSlots, nb, p.nts, (float)(1.0 / (double)d2), (float)(-2.0 / (double)d2), p.nz,
+                                   p.nx, s));
+            enum { SS_EXT, SS_MUL, SS_ADJ };   // ring producers
+            auto produce = [&](int kind, const ShotSlot* from, int k0, int n) {
+                ExtBatchArgs e{};
+                e.slots = from;
+                e.v2dt2 = base.v2dt2;
+                e.nz = p.nz; e.nx = p.nx; e.hExt = p.hExt;
+                e.nk = n;
+                for (int j = 0; j < n; ++j) {
+                    e.ssOff[j] = ssSlot(k0 + j);
+                    e.wOff[j] = (long)(k0 + j) * nzx;
+                }
+                if (kind == SS_MUL) {
+                    e.mext = mtil;
+                    HIPC(launch_mul_batch(e, nb, s));
+                } else {
+                    e.mext = rExt;
+                    HIPC(kind == SS_EXT ? launch_ext_ss_batch(e, nb, s) : launch_tomo_adj_ss(e, nb, s));
+                }
+            };
+            // Forward in time.  ssKind >= 0: inject ring samples made from
+            // `from`'s W; capture: 1 = into the launch slots' W at the sample
+            // index (nts slices), 2 = into the capture pair; imageR: after each
+            // coarse step, img += R[its] * captured sample (adjoint leg 1).
+            auto forward_pass = [&](unsigned mode, const ShotSlot* on, int ssKind, const ShotSlot* from, int capture,
+                                    bool imageR) {
+                HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
+                StepArgs x = a;
+                x.slots = on;
+                x.cur = 1;
+                x.win = 0;
+                int next = 0;
+                auto capOff = [&](int k) { return capture == 1 ? (long)k * nzx : (long)(k % 2) * nzx; };
+                for (int its = 0; its < p.nts - 1; ++its) {
+                    while (ssKind >= 0 && next <= its + 1) {
+                        const int n = std::min(extK, p.nts - next);
+                        produce(ssKind, from, next, n);
+                        next += n;
+                    }
+                    for (int it2 = 1; it2 <= p.sub; ++it2) {
+                        x.its = its;
+                        x.fa = f[it2 - 1];
+                        x.fb = f[h + it2 - 1];
+                        x.fe = f[it2];
+                        x.ff = f[h + it2];
+                        if (ssKind >= 0) {
+                            x.ssA = x.ssB = 0;
+                            x.ssOffA = ssSlot(its);
+                            x.ssOffB = ssSlot(its + 1);
+                        }
+                        if (capture) {
+                            x.wA = capOff(its);
+                            x.wB = capOff(its + 1);
+                            x.skipA = x.fe == 0.f;
+                            x.skipB = x.ff == 0.f;
+                            x.assignA = its == 0 && it2 == 1;
+                            x.assignB = it2 == 1;
+                        } else {   // MODE_BORN_SCAT reads W[its], W[its + 1]
+                            x.wA = (long)its * nzx;
+                            x.wB = (long)(its + 1) * nzx;
+                            x.skipA = x.skipB = x.assignA = x.assignB = 0;
+                        }
+                        launch_timed(c, mode, AISO_K_TOMO, x, nb, true);
+                        ring(mode, x);
+                        x.cur ^= 1;
+                    }
+                    if (imageR) HIPC(launch_img_acc(dSlotsQ, (long)its * nzx, on, capOff(its), nb, p.nz, p.nx, s));
+                }
+                if (imageR)
+                    HIPC(launch_img_acc(dSlotsQ, (long)(p.nts - 1) * nzx, on, capOff(p.nts - 1), nb, p.nz, p.nx, s));
+            };
+            if (q.op == Op::TomoFwd) {
+                if (c->leg1) {   // source -> extended reflectivity -> P -> model -> data
+                    forward_pass(MODE_TOMO_CAP, dSlotsQ, SS_EXT, dSlots, 1, false);
+                    forward_pass(MODE_BORN_SCAT, dSlotsQ, -1, nullptr, 0, false);
+                }
+                if (c->leg2) {   // source -> model -> P -> extended reflectivity -> data
+                    forward_pass(MODE_TOMO_CAP, dSlotsQ, SS_MUL, dSlots, 1, false);
+                    forward_pass(MODE_EXT_SCAT, dSlotsQ, SS_EXT, dSlotsQ, 0, false);
+                }
+                for (int k = 0; k < nb; ++k) {
+                    const int shot = b0 + k;
+                    const aiso_device* rd = q.rec[q.nRec == 1 ? 0 : shot];
+                    DevGeom* rg = recG[q.nRec == 1 ? 0 : shot];
+                    HIPC(launch_dev_forward(dreg + (size_t)k * nRecRegMax * p.nts, dDataOut + (size_t)shot * nRecDev * p.nts,
+                                            rg->cornerReg.as<int>(), rg->weight.as<float>(), nRecDev, rd->g.nb, p.nts,
+                                            q.add, s));
+                }
+            } else {
+                HIPC(launch_fill(img, (long)nb * nzx, s));
+                // receiver wavefield R into P (computeTomoRecWfld, :1481-1515)
+                HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
+                StepArgs x = a;
+                x.slots = dSlotsQ;
+                x.cur = 1;
+                x.win = 0;
+                for (int its = p.nts - 2; its >= 0; --its)
+                    for (int it2 = p.sub - 1; it2 >= 0; --it2) {
+                        x.its = its;
+                        x.itwRec = its * p.sub + it2;
+                        x.fc = f[it2 + 1];
+                        x.fd = f[h + it2 + 1];
+                        x.wA = (long)its * nzx;
+                        x.wB = (long)(its + 1) * nzx;
+                        x.fe = f[it2];
+                        x.ff = f[h + it2];
+                        x.skipA = x.fe == 0.f;
+                        x.skipB = x.ff == 0.f;
+                        x.assignA = it2 == p.sub - 1;
+                        x.assignB = its == p.nts - 2 && it2 == p.sub - 1;
+                        launch_timed(c, MODE_TOMO_REC, AISO_K_TOMO, x, nb, true);
+                        ring(MODE_TOMO_REC, x);
+                        x.cur ^= 1;
+                    }
+                if (c->leg1)   // forward from the extended scattering of W'', imaged against R
+                    forward_pass(MODE_TOMO_CAP, dSlotsC, SS_EXT, dSlots, 2, true);
+                if (c->leg2) {   // backward from the adjoint extended scattering of R, imaged against W''
+                    HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
+                    HIPC(launch_fill(cap, (long)nb * 2 * nzx, s));
+                    StepArgs y = a;
+                    y.slots = dSlotsC;
+                    y.cur = 1;
+                    y.win = 0;
+                    int next = p.nts - 1;   // highest sample not yet in the ring
+                    for (int its = p.nts - 2; its >= 0; --its) {
+                        while (next >= its) {
+                            const int n = std::min(extK, next + 1);
+                            produce(SS_ADJ, dSlotsQ, next - n + 1, n);
+                            next -= n;
+                        }
+                        for (int it2 = p.sub - 1; it2 >= 0; --it2) {
+                            y.its = its;
+                            y.fa = f[it2 + 1];
+                            y.fb = f[h + it2 + 1];
+                            y.ssA = y.ssB = 0;
+                            y.ssOffA = ssSlot(its);
+                            y.ssOffB = ssSlot(its + 1);
+                            y.wA = (long)(its % 2) * nzx;
+                            y.wB = (long)((its + 1) % 2) * nzx;
+                            y.fe = f[it2];
+                            y.ff = f[h + it2];
+                            y.skipA = y.fe == 0.f;
+                            y.skipB = y.ff == 0.f;
+                            y.assignA = it2 == p.sub - 1;
+                            y.assignB = 0;
+                            launch_timed(c, MODE_TOMO_ADJ_CAP, AISO_K_TOMO, y, nb, true);
+                            y.cur ^= 1;
+                        }
+                        HIPC(launch_img_acc(dSlots, (long)(its + 1) * nzx, dSlotsC, (long)((its + 1) % 2) * nzx, nb,
+                                            p.nz, p.nx, s));
+                    }
+                    HIPC(launch_img_acc(dSlots, 0, dSlotsC, 0, nb, p.nz, p.nx, s));
+                }
+                HIPC(launch_born_adj_finish(dSlots, nb, -1, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
+            }
+        }
+
         // -- second pass
-        if (!lockstep && !adj2) {
+        if (!lockstep && !adj2 && !tomo) {
             HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
             a.cur = 1;
             a.assignA = a.assignB = a.skipA = a.skipB = 0;
@@ -1525,6 +1735,8 @@ aiso_status aiso_ctx_set_option(aiso_ctx* c, const char* key, int value)
         if (k == "streamNT") c->streamNT = value != 0;
         else if (k == "lockstep") c->lockstep = value != 0;
         else if (k == "pair2") c->pair2 = value != 0;
+        else if (k == "leg1") c->leg1 = value != 0;
+        else if (k == "leg2") c->leg2 = value != 0;
         else if (k == "extBatch") {
             if (value < 1 || value > kExtBatchMax) throw Error(AISO_ERR_PARAM, "extBatch must be 1..16");
             c->extBatch = value;
@@ -1750,6 +1962,36 @@ aiso_status aiso_bornext_adjoint(aiso_ctx* c, int add, float* model, const float
                                  int nShot, const float* wav, int nWav, aiso_device* const* rec, int nRec, int mem)
 {
     return op_call(c, Op::ExtAdj, add, nullptr, model, data, nullptr, 1, src, nShot, wav, nWav, rec, nRec, mem);
+}
+
+aiso_status aiso_ctx_set_ext_reflectivity(aiso_ctx* c, const float* refl, int mem)
+{
+    return guard([&] {
+        if (!c || !refl) throw Error(AISO_ERR_PARAM, "null argument");
+        HIPC(hipSetDevice(c->device));
+        const size_t n = (size_t)c->p.nzx() * c->p.nExt;
+        float* d = (float*)c->extRefl.need(n * 4);
+        if (mem == AISO_MEM_DEVICE) {
+            wait_default_stream(c);
+            HIPC(hipMemcpyAsync(d, refl, n * 4, hipMemcpyDeviceToDevice, c->stream));
+        } else {
+            HIPC(hipMemcpyAsync(d, refl, n * 4, hipMemcpyHostToDevice, c->stream));
+        }
+        HIPC(hipStreamSynchronize(c->stream));
+        c->haveExtRefl = true;
+    });
+}
+
+aiso_status aiso_tomo_forward(aiso_ctx* c, int add, const float* model, float* data, aiso_device* const* src,
+                              int nShot, const float* wav, int nWav, aiso_device* const* rec, int nRec, int mem)
+{
+    return op_call(c, Op::TomoFwd, add, model, nullptr, nullptr, data, 1, src, nShot, wav, nWav, rec, nRec, mem);
+}
+
+aiso_status aiso_tomo_adjoint(aiso_ctx* c, int add, float* model, const float* data, aiso_device* const* src,
+                              int nShot, const float* wav, int nWav, aiso_device* const* rec, int nRec, int mem)
+{
+    return op_call(c, Op::TomoAdj, add, nullptr, model, data, nullptr, 1, src, nShot, wav, nWav, rec, nRec, mem);
 }
 
 int aiso_pad_plus(int n, int pad, int blockSize) { return pad_plus(n, pad, blockSize); }

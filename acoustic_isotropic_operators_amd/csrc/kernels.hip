@@ -1104,8 +1104,9 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
         if constexpr (MODE & M_CAPTURE) {
             // Forward passes capture the coarse source wavefield W; the
             // adjoint receiver pass accumulates into the two ss slices.
-            float* A = ADJ ? S.ss[a.ssA] + a.ssOffA : S.W + a.wA;
-            float* B = ADJ ? S.ss[a.ssB] + a.ssOffB : S.W + a.wB;
+            constexpr bool CAP_SS = ADJ && !(MODE & M_CAP_W);
+            float* A = CAP_SS ? S.ss[a.ssA] + a.ssOffA : S.W + a.wA;
+            float* B = CAP_SS ? S.ss[a.ssB] + a.ssOffB : S.W + a.wB;
             // A zero weight leaves the slice as it is (x + n*0 == x); a
             // first touch writes instead of accumulating into a zeroed slice.
             if (!a.skipA) A[gi] = a.assignA ? n * a.fe : A[gi] + n * a.fe;
@@ -1493,6 +1494,79 @@ __global__ void k_ext_adj_finish(const ShotSlot* slots, const float* __restrict_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Offset-extended tomographic operator (F/src/tomoExtShotsGpuFunctions.cu).
+
+// computeTomoSrcWfldDt2 + srcWfldSecondTimeDerivative (:1251-1360,
+// F/src/kernelsGpu.cu:188-197): W[k] <- cSide*(W[k-1] + W[k+1]) +
+// cCenter*W[k], zero beyond both ends, in place; one thread walks the nts
+// samples of one interior point.
+__global__ void k_dt2_coarse(const ShotSlot* slots, int nts, float cSide, float cCenter, int nz, int nx)
+{
+    const ShotSlot& S = slots[blockIdx.y];
+    const long nzx = (long)nz * nx;
+    const int iz = FAT + blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = FAT + blockIdx.z;
+    if (iz >= nz - FAT) return;
+    float* w = S.W + (long)ix * nz + iz;
+    float prev = 0.f, cur = w[0];
+    for (int k = 0; k < nts; ++k) {
+        const float next = k + 1 < nts ? w[(long)(k + 1) * nzx] : 0.f;
+        w[(long)k * nzx] = cSide * (prev + next) + cCenter * cur;
+        prev = cur;
+        cur = next;
+    }
+}
+
+// imagingFwdGpu (F/src/kernelsGpu.cu:309-316) into ring slots:
+// ss[0][ssOff[k]] = m~ * W[wOff[k]] at interior points.
+__global__ void k_mul_batch(ExtBatchArgs a)
+{
+    const ShotSlot& S = a.slots[blockIdx.y];
+    const int iz = FAT + blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = FAT + blockIdx.z;
+    if (iz >= a.nz - FAT) return;
+    const long i = (long)ix * a.nz + iz;
+    for (int k = 0; k < a.nk; ++k) S.ss[0][a.ssOff[k] + i] = a.mext[i] * S.W[a.wOff[k] + i];
+}
+
+// imagingOffsetTomoAdjGpu (F/src/kernelsGpu.cu:460-475) into ring slots:
+// ss[x] = sum_e r[e][x+e] * R[k][x+2e] * v2dt2[x+2e], R in W[wOff[k]].
+__global__ void k_tomo_adj_ss(ExtBatchArgs a)
+{
+    const ShotSlot& S = a.slots[blockIdx.y];
+    const long nz = a.nz, nzx = nz * a.nx;
+    const int iz = FAT + blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = FAT + blockIdx.z;
+    if (iz >= a.nz - FAT) return;
+    const int hExt = a.hExt;
+    const int lo = max(-hExt, FAT + hExt - ix);
+    const int hi = min(a.nx - 1 - FAT - ix - hExt, hExt);
+    const long i = (long)ix * nz + iz;
+    for (int k = 0; k < a.nk; ++k) {
+        const float* R = S.W + a.wOff[k];
+        float acc = 0.f;
+        for (int e = lo; e <= hi; ++e) {
+            const long iRec = (long)(ix + 2 * e) * nz + iz;
+            acc += a.mext[nzx * (e + hExt) + (long)(ix + e) * nz + iz] * R[iRec] * a.v2dt2[iRec];
+        }
+        S.ss[0][a.ssOff[k] + i] = acc;
+    }
+}
+
+// imagingAdjGpu (F/src/kernelsGpu.cu:318-325): img += X.W[xOff] * Y.W[yOff]
+// at interior points, img from X's slot.
+__global__ void k_img_acc(const ShotSlot* X, long xOff, const ShotSlot* Y, long yOff, int nz, int nx)
+{
+    const ShotSlot& SX = X[blockIdx.y];
+    const ShotSlot& SY = Y[blockIdx.y];
+    const int iz = FAT + blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = FAT + blockIdx.z;
+    if (iz >= nz - FAT) return;
+    const long i = (long)ix * nz + iz;
+    SX.img[i] += SX.W[xOff + i] * SY.W[yOff + i];
+}
+
 // acc += img (one shot, in shot order).
 __global__ void k_accumulate(float* __restrict__ acc, const float* __restrict__ img, long n)
 {
@@ -1640,7 +1714,11 @@ hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a,
     switch (mode) {
     case MODE_SRC: k = pick_generic_tile<MODE_SRC>(tz, tx); break;
     case MODE_SRC_WIN: k = pick_generic_tile<MODE_SRC_WIN>(tz, tx); break;
+    case MODE_BORN_SCAT: k = pick_generic_tile<MODE_BORN_SCAT>(tz, tx); break;
     case MODE_EXT_SCAT: k = pick_generic_tile<MODE_EXT_SCAT>(tz, tx); break;
+    case MODE_TOMO_CAP: k = pick_generic_tile<MODE_TOMO_CAP>(tz, tx); break;
+    case MODE_TOMO_ADJ_CAP: k = pick_generic_tile<MODE_TOMO_ADJ_CAP>(tz, tx); break;
+    case MODE_TOMO_REC: k = pick_generic_tile<MODE_TOMO_REC>(tz, tx); break;
     case MODE_EXT_REC: k = pick_generic_tile<MODE_EXT_REC>(tz, tx); break;
     case MODE_REC_WIN: k = pick_generic_tile<MODE_REC_WIN>(tz, tx); break;
     default: break;
@@ -1784,6 +1862,36 @@ hipError_t launch_ext_img_batch(const ExtBatchArgs& a, int nshots, hipStream_t s
         dim3 g((nzi + 63) / 64, nshots, nxr);
         hipLaunchKernelGGL(k_ext_img_any, g, dim3(64), 0, s, a);
     }
+    return hipGetLastError();
+}
+
+static dim3 column_grid(int nz, int nx, int nshots) { return dim3((nz - 2 * FAT + 63) / 64, nshots, nx - 2 * FAT); }
+
+hipError_t launch_dt2_coarse(const ShotSlot* slots, int nshots, int nts, float cSide, float cCenter, int nz, int nx,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dt2_coarse, column_grid(nz, nx, nshots), dim3(64), 0, s, slots, nts, cSide, cCenter, nz, nx);
+    return hipGetLastError();
+}
+
+hipError_t launch_mul_batch(const ExtBatchArgs& a, int nshots, hipStream_t s)
+{
+    if (a.nk < 1 || a.nk > kExtBatchMax) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mul_batch, column_grid(a.nz, a.nx, nshots), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_tomo_adj_ss(const ExtBatchArgs& a, int nshots, hipStream_t s)
+{
+    if (a.nk < 1 || a.nk > kExtBatchMax) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_tomo_adj_ss, column_grid(a.nz, a.nx, nshots), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_img_acc(const ShotSlot* X, long xOff, const ShotSlot* Y, long yOff, int nshots, int nz, int nx,
+                          hipStream_t s)
+{
+    hipLaunchKernelGGL(k_img_acc, column_grid(nz, nx, nshots), dim3(64), 0, s, X, xOff, Y, yOff, nz, nx);
     return hipGetLastError();
 }
 

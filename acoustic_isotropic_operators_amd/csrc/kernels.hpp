@@ -40,6 +40,7 @@ enum : unsigned {
     M_REC_SIG = 1u << 8,    // sig[s][itwRec] += n at source points
     M_WIN = 1u << 9,        // window capture V = (o*wa + c*wb) + n*wc when args.win != 0
     M_SS_PRE = 1u << 10,    // + SS[a]*fa + SS[b]*fb, SS = m~*W stored in the W array
+    M_CAP_W = 1u << 11,     // adjoint stepper: capture into W[wA], W[wB] instead of ss
 };
 
 // Where a window capture V goes (StepArgs::winOut).
@@ -166,13 +167,19 @@ constexpr unsigned MODE_EXT_REC = M_ADJ | M_INJ_DATA | M_CAPTURE;
 constexpr unsigned MODE_SRC_WIN = M_INJ_SIG | M_WIN;
 constexpr unsigned MODE_BORN_SCAT_PRE = M_SS_PRE | M_REC_DATA;
 constexpr unsigned MODE_REC_WIN = M_ADJ | M_INJ_DATA | M_WIN;
+// Offset-extended tomographic operator: scattered fields captured at the
+// coarse rate (forward, or adjoint into W), and the receiver wavefield.
+constexpr unsigned MODE_TOMO_CAP = M_SS_SLICE | M_CAPTURE;
+constexpr unsigned MODE_TOMO_ADJ_CAP = M_ADJ | M_SS_SLICE | M_CAPTURE | M_CAP_W;
+constexpr unsigned MODE_TOMO_REC = M_ADJ | M_INJ_DATA | M_CAPTURE | M_CAP_W;
 
 // spb: shots advanced per workgroup (1 or 2), sharing the tile's v2dt2 and sponge.
 // e0/e1 (optional): events timestamped by the dispatch itself.
 hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a, hipStream_t s,
                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-// The unspecialized step kernel (modes of the extended-Born operators:
-// MODE_SRC, MODE_SRC_WIN, MODE_EXT_SCAT, MODE_EXT_REC, MODE_REC_WIN).
+// The unspecialized step kernel (modes of the extended-Born and tomographic
+// operators: MODE_SRC, MODE_SRC_WIN, MODE_BORN_SCAT, MODE_EXT_SCAT,
+// MODE_EXT_REC, MODE_REC_WIN, MODE_TOMO_*).
 hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a, hipStream_t s,
                                hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 hipError_t launch_ring(unsigned mode, const StepArgs& a, int maxRing, hipStream_t s);
@@ -187,6 +194,16 @@ hipError_t launch_ext_ss_batch(const ExtBatchArgs& a, int nshots, hipStream_t s)
 hipError_t launch_ext_img_batch(const ExtBatchArgs& a, int nshots, hipStream_t s);
 hipError_t launch_ext_adj_finish(const ShotSlot* slots, int nshots, const float* refl, int nz, int nx,
                                  int nExt, hipStream_t s);
+// Tomographic operator: W <- second time derivative of W (nts coarse
+// slices, in place); ss ring slots <- m~ * W[k] (a.mext is the 2-D m~);
+// ss ring slots <- adjoint offset scattering of the receiver wavefield in W
+// (a.mext is the scaled extended reflectivity); img += X.W[xOff] * Y.W[yOff].
+hipError_t launch_dt2_coarse(const ShotSlot* slots, int nshots, int nts, float cSide, float cCenter, int nz, int nx,
+                             hipStream_t s);
+hipError_t launch_mul_batch(const ExtBatchArgs& a, int nshots, hipStream_t s);
+hipError_t launch_tomo_adj_ss(const ExtBatchArgs& a, int nshots, hipStream_t s);
+hipError_t launch_img_acc(const ShotSlot* X, long xOff, const ShotSlot* Y, long yOff, int nshots, int nz, int nx,
+                          hipStream_t s);
 hipError_t launch_accumulate(float* acc, const float* img, long n, hipStream_t s);
 hipError_t launch_add_to(float* out, const float* acc, long n, int add, hipStream_t s);
 hipError_t launch_dev_forward(const float* reg, float* out, const int* cornerReg, const float* weight,

@@ -204,6 +204,7 @@ struct ShotsBase {
     std::vector<DevPtr> sources, receivers;
     aiso_ctx* ctx = nullptr;
     bool dirty = true;
+    int ctxGen = 0;                      // bumped each time ensure() makes a new context
     int nShot = 0, gpu = 0;
     aiso_fd_params fp{};
 
@@ -273,6 +274,7 @@ struct ShotsBase {
             ctx = nullptr;
         }
         check(aiso_ctx_create(&q, gpu, &ctx));
+        ++ctxGen;
         check(aiso_ctx_set_velocity(ctx, velHost.data(), AISO_MEM_HOST));
         // Engine-specific keys (the reference ignores them): shots advanced
         // per launch and the step-kernel tile.

@@ -107,7 +107,9 @@ aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
  *   "cacheBudgetMB": on-die cache budget the automatic batch size fills;
  *   "extBatch" 1..16: coarse samples per offset-imaging / secondary-source
  *       launch of the extended Born operators (default 16);
- *   "lanes" 1..4: streams the shots of a launch are split over (default 1). */
+ *   "lanes" 1..4: streams the shots of a launch are split over (default 1);
+ *   "leg1", "leg2" 0/1: legs of the tomographic operator (parfile keys
+ *       leg1/leg2 of F/src/tomoExtGpu.cpp:10-11, default 1). */
 aiso_status aiso_ctx_set_option(aiso_ctx* ctx, const char* key, int value);
 /* Derived sizes: ntw, sub, nShotsPerLaunch actually used by the last call. */
 aiso_status aiso_ctx_info(const aiso_ctx* ctx, int* ntw, int* batchUsed, size_t* bytesPerShot);
@@ -124,7 +126,8 @@ enum {
     AISO_K_EXT_SCAT = 5,   /* extended-Born scattered step                   */
     AISO_K_EXT_REC = 6,    /* extended-Born adjoint receiver step            */
     AISO_K_BORN_PAIR = 7,  /* lockstep Born forward: source + scattered step */
-    AISO_K_COUNT = 8
+    AISO_K_TOMO = 8,       /* tomographic operator: scattered / receiver steps */
+    AISO_K_COUNT = 9
 };
 /* Per-launch timing of the time-step kernels with HIP events on that stream
  * (enable != 0), reset of the counters. */
@@ -220,6 +223,26 @@ aiso_status aiso_bornext_adjoint(aiso_ctx* ctx, int add, float* model, const flo
                                  aiso_device* const* sources, int nShot,
                                  const float* wavelets, int nWavelet,
                                  aiso_device* const* receivers, int nRecGeom, int mem);
+
+/* ------------------------------------------- tomographic (extension=offset) */
+/* tomoExtShotsGpu (F/include/tomoExtShotsGpu.h:20-58): the extended
+ * reflectivity [nExt][nx][nz] the operator linearizes about, the constructor
+ * argument and setReflectivityExt of the reference (:37, :55).  Copied; the
+ * per-call 2/v^3 scaling follows the current velocity. */
+aiso_status aiso_ctx_set_ext_reflectivity(aiso_ctx* ctx, const float* reflectivityExt, int mem);
+/* tomoExtShotsGpu::forward / adjoint (F/src/tomoExtShotsGpu.cpp:62-145 and
+ * the adjoint below it; F/src/tomoExtGpu.cpp:34-100): velocity perturbation
+ * model [nx][nz] <-> data [nShot][nRec][nts], legs selected by the options
+ * "leg1" / "leg2".  Needs extension = AISO_EXT_OFFSET (the time-lag form
+ * is not supported) and aiso_ctx_set_ext_reflectivity first. */
+aiso_status aiso_tomo_forward(aiso_ctx* ctx, int add, const float* model, float* data,
+                              aiso_device* const* sources, int nShot,
+                              const float* wavelets, int nWavelet,
+                              aiso_device* const* receivers, int nRecGeom, int mem);
+aiso_status aiso_tomo_adjoint(aiso_ctx* ctx, int add, float* model, const float* data,
+                              aiso_device* const* sources, int nShot,
+                              const float* wavelets, int nWavelet,
+                              aiso_device* const* receivers, int nRecGeom, int mem);
 
 /* ---------------------------------------------------------------- misc */
 /* padFileGpuMain padding rule (F/src/padFileGpuMain.cpp:49-66, no free

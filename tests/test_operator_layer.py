@@ -127,3 +127,47 @@ def test_born_operator_class_matches_oracle(tmp_path):
     np.testing.assert_array_equal(d.getNdArray(), ref)
     assert op.dotTest(verb=True, maxError=1e-4)
     assert op.dotTestCpp(False, 1e-5)
+
+
+def test_tomo_opinit_shapes_and_reflectivity_file(tmp_path):
+    c = make_case(nShot=2)
+    args = _write_case_files(tmp_path, c, nExt=5)
+    m, d, vel, par, srcs, sig, recs, refl, _ = AI.tomoExtOpInitFloat(args)
+    assert m.getNdArray().shape == (c.nx, c.nz)
+    assert refl.getNdArray().shape == (5, c.nx, c.nz) and not np.any(refl.getNdArray())
+    r = np.random.default_rng(1).standard_normal((5, c.nx, c.nz)).astype(np.float32)
+    sep.write(str(tmp_path / "refl.H"), sep.FloatVector(refl.getHyper(), r))
+    *_, refl2, _ = AI.tomoExtOpInitFloat(args + [f"reflectivity={tmp_path / 'refl.H'}"])
+    np.testing.assert_array_equal(refl2.getNdArray(), r)
+
+
+@pytest.mark.gpu
+def test_tomo_operator_class_matches_oracle(tmp_path):
+    c = make_case(seed=35, nShot=2, sub=2, dts=0.002, nExt=5)
+    args = _write_case_files(tmp_path, c, nExt=5)
+    m, d, vel, par, srcs, sig, recs, refl, _ = AI.tomoExtOpInitFloat(args)
+    rng = np.random.default_rng(36)
+    refl.getNdArray()[...] = rng.standard_normal(refl.getNdArray().shape)
+    op = AI.tomoExtShotsGpu(m, d, vel, par, srcs, sig, recs, refl)
+    m.getNdArray()[...] = rng.standard_normal(m.getNdArray().shape)
+    op.forward(False, m, d)
+    g = c.grid
+    osrc = [O.Device.grid(1, 2 - 1 + c.pad + FAT, 1, 1, c.pad + FAT + 12 * s, 1, c.nz, c.nx, c.nts) for s in range(2)]
+    orec = [O.Device.grid(1, 2 - 1 + c.pad + FAT, 1, 20, 3 - 1 + c.pad + FAT, 2, c.nz, c.nx, c.nts)]
+    wav = sig[0].getNdArray()
+    ref = O.tomo_forward(g, c.vel, osrc, [wav], orec, m.getNdArray(), refl.getNdArray())
+    assert np.abs(ref).max() > 0
+    np.testing.assert_array_equal(d.getNdArray(), ref)
+    dr = d.clone()
+    dr.getNdArray()[...] = rng.standard_normal(dr.getNdArray().shape)
+    op.adjoint(False, m, dr)
+    np.testing.assert_array_equal(m.getNdArray(), O.tomo_adjoint(g, c.vel, osrc, [wav], orec, dr.getNdArray(),
+                                                                 refl.getNdArray()))
+    # a new reflectivity is taken by the next call
+    refl2 = refl.clone()
+    refl2.getNdArray()[...] *= 2.0
+    op.setReflectivityExt(refl2)
+    op.forward(False, m, d)
+    np.testing.assert_array_equal(d.getNdArray(), O.tomo_forward(g, c.vel, osrc, [wav], orec, m.getNdArray(),
+                                                                 refl2.getNdArray()))
+    assert op.dotTestCpp(False, 1e-5)
