@@ -77,6 +77,15 @@ void orc_tomo_adj##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* refl,  
                        const int* recPos, int nRec, REAL* model, const REAL* dreg,   \
                        const REAL* extRefl, int leg1, int leg2, REAL* W, REAL* P);   \
 void orc_dt2_coarse##SFX(const orc_grid* g, REAL* W);                                \
+void orc_wemva_fwd##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* refl,     \
+                        const REAL* sig, const int* srcPos, int nSrc,                \
+                        const int* recPos, int nRec, const REAL* model,              \
+                        const REAL* dobs, int leg1, int leg2, REAL* image,           \
+                        REAL* W, REAL* P);                                           \
+void orc_wemva_adj##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* refl,     \
+                        const REAL* sig, const int* srcPos, int nSrc,                \
+                        const int* recPos, int nRec, REAL* model, const REAL* image, \
+                        const REAL* dobs, int leg1, int leg2, REAL* W, REAL* P);     \
 void orc_interp_fwd##SFX(int nts, double dts, double ots, int sub, int nDev,        \
                          int add, const REAL* m, REAL* d);                           \
 void orc_interp_adj##SFX(int nts, double dts, double ots, int sub, int nDev,        \

@@ -708,6 +708,84 @@ void FN(orc_tomo_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
     free(m); free(r);
 }
 
+/* Forward in time from the extended scattering of W (ext_ss with the
+ * scaled extended reflectivity r), each captured coarse sample k imaged as
+ * model += X[k] * sample: leg 1 of the tomographic adjoint
+ * (computeTomoLeg1OffsetAdj, F/src/tomoExtShotsGpuFunctions.cu:2788-2860)
+ * and leg 2 of the WEMVA adjoint (computeWemvaLeg2OffsetAdj,
+ * F/src/wemvaExtShotsGpuFunctions.cu:1502-1571). */
+static void FN(adj_leg_fwd)(const orc_grid* g, const REAL* zc, const REAL* xc, const REAL* f,
+                            const REAL* cd, const REAL* v2dt2, const REAL* r, const REAL* W,
+                            const REAL* X, REAL* model)
+{
+    const int nts = g->nts, sub = g->sub, h = sub + 1;
+    const long nzx = (long)g->nz * g->nx;
+    REAL* p0 = FN(zalloc)(nzx);
+    REAL* p1 = FN(zalloc)(nzx);
+    REAL* ssL = FN(zalloc)(nzx);
+    REAL* ssR = FN(zalloc)(nzx);
+    REAL* scL = FN(zalloc)(nzx);
+    REAL* scR = FN(zalloc)(nzx);
+    FN(born_ctx) rW = {r, W, v2dt2, NULL};
+    FN(ext_ss)(g, &rW, 0, ssL);
+    for (int its = 0; its < nts - 1; ++its) {
+        FN(ext_ss)(g, &rW, its + 1, ssR);
+        for (int it2 = 1; it2 < sub + 1; ++it2) {
+            FN(orc_step_fwd)(g, zc, xc, p0, p1, p0, v2dt2);
+            FN(inject_secondary)(g, f, h, ssL, ssR, p0, it2 - 1);
+            FN(orc_damp)(g, cd, p0, p1);
+            FN(capture2)(g, f, h, p0, scL, scR, it2);
+            FN(swap)(&p0, &p1);
+        }
+        FN(img_acc)(g, model, X, its, scL);
+        FN(swap)(&ssL, &ssR);
+        memset(ssR, 0, sizeof(REAL) * (size_t)nzx);
+        FN(swap)(&scL, &scR);
+        memset(scR, 0, sizeof(REAL) * (size_t)nzx);
+    }
+    FN(img_acc)(g, model, X, nts - 1, scL);
+    free(p0); free(p1); free(ssL); free(ssR); free(scL); free(scR);
+}
+
+/* Backward in time from the adjoint extended scattering of the receiver
+ * wavefield R (tomo_adj_ss with r), each captured coarse sample k imaged as
+ * model += W[k] * sample: leg 2 of the tomographic adjoint
+ * (computeTomoLeg2OffsetAdj, F/src/tomoExtShotsGpuFunctions.cu:2949-3020)
+ * and leg 1 of the WEMVA adjoint (computeWemvaLeg1OffsetAdj,
+ * F/src/wemvaExtShotsGpuFunctions.cu:1428-1500). */
+static void FN(adj_leg_bwd)(const orc_grid* g, const REAL* zc, const REAL* xc, const REAL* f,
+                            const REAL* cd, const REAL* v2dt2, const REAL* r, const REAL* W,
+                            const REAL* R, REAL* model)
+{
+    const int nts = g->nts, sub = g->sub, h = sub + 1;
+    const long nzx = (long)g->nz * g->nx;
+    REAL* p0 = FN(zalloc)(nzx);
+    REAL* p1 = FN(zalloc)(nzx);
+    REAL* ssL = FN(zalloc)(nzx);
+    REAL* ssR = FN(zalloc)(nzx);
+    REAL* scL = FN(zalloc)(nzx);
+    REAL* scR = FN(zalloc)(nzx);
+    REAL* tmp = FN(zalloc)(nzx);
+    FN(tomo_adj_ss)(g, R, r, v2dt2, nts - 1, ssR);
+    for (int its = nts - 2; its > -1; --its) {
+        FN(tomo_adj_ss)(g, R, r, v2dt2, its, ssL);
+        for (int it2 = sub - 1; it2 > -1; --it2) {
+            FN(orc_step_adj)(g, zc, xc, p0, p1, p0, v2dt2, tmp);
+            FN(inject_secondary)(g, f, h, ssL, ssR, p0, it2 + 1);
+            FN(orc_damp)(g, cd, p0, p1);
+            FN(capture2)(g, f, h, p0, scL, scR, it2);
+            FN(swap)(&p0, &p1);
+        }
+        FN(img_acc)(g, model, W, its + 1, scR);
+        FN(swap)(&ssR, &ssL);
+        memset(ssL, 0, sizeof(REAL) * (size_t)nzx);
+        FN(swap)(&scR, &scL);
+        memset(scL, 0, sizeof(REAL) * (size_t)nzx);
+    }
+    FN(img_acc)(g, model, W, 0, scR);
+    free(p0); free(p1); free(ssL); free(ssR); free(scL); free(scR); free(tmp);
+}
+
 /* tomoOffsetShotsAdjGpu (F/src/tomoExtShotsGpuFunctions.cu:912-1020) with
  * computeTomoLeg1OffsetAdj (:2788-2860) and computeTomoLeg2OffsetAdj
  * (:2949-3020).  model (output, nz*nx) is overwritten with this shot's
@@ -720,7 +798,6 @@ void FN(orc_tomo_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
 {
     REAL zc[6], xc[6], f[2 * 101], cd[512];
     FN(orc_tables)(g, zc, xc, f, cd);
-    const int nts = g->nts, sub = g->sub, h = sub + 1;
     const long nzx = (long)g->nz * g->nx, nr = nzx * g->nExt;
     FN(src_pass)(g, zc, xc, f, cd, v2dt2, sig, srcPos, nSrc, W);
     FN(dt2_coarse)(g, W);
@@ -729,6 +806,56 @@ void FN(orc_tomo_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
     memcpy(r, extRefl, sizeof(REAL) * (size_t)nr);
     FN(scale_refl_ext)(g, r, refl);
     memset(model, 0, sizeof(REAL) * (size_t)nzx);
+    if (leg1) FN(adj_leg_fwd)(g, zc, xc, f, cd, v2dt2, r, W, P, model);
+    if (leg2) FN(adj_leg_bwd)(g, zc, xc, f, cd, v2dt2, r, W, P, model);
+    FN(scale_refl)(g, model, refl, v2dt2);
+    free(r);
+}
+
+/* ---------------------------------------------------------------------------
+ * Offset-extended WEMVA operator (wemvaExtShotsGpu, extension=offset):
+ * velocity perturbation <-> extended image, about the receiver wavefield of
+ * the observed data dobs.
+ */
+
+/* imagingOffsetWemvaScaleFwdGpu (F/src/kernelsGpu.cu:429-443):
+ * img[e][x] += scat[x-h] * R[k][x+h] * v2dt2[x+h], h = e - hExt. */
+static void FN(wemva_img_fwd)(const orc_grid* g, REAL* img, const REAL* scat, const REAL* R, int k,
+                              const REAL* v2dt2)
+{
+    const int nz = g->nz, nx = g->nx, nExt = g->nExt, hExt = (nExt - 1) / 2;
+    const long nzx = (long)nz * nx;
+#pragma omp parallel for schedule(static)
+    for (int ix = FAT + hExt; ix < nx - FAT - hExt; ++ix)
+        for (int e = 0; e < nExt; ++e)
+            for (int iz = FAT; iz < nz - FAT; ++iz) {
+                int sh = e - hExt;
+                long iRec = (long)(ix + sh) * nz + iz;
+                img[nzx * e + (long)ix * nz + iz] += scat[(long)(ix - sh) * nz + iz] * R[nzx * k + iRec] * v2dt2[iRec];
+            }
+}
+
+/* wemvaOffsetShotsFwdGpu (F/src/wemvaExtShotsGpuFunctions.cu:458-574) with
+ * computeWemvaLeg1OffsetFwd (:1123-1188) and computeWemvaLeg2OffsetFwd
+ * (:1190-1264).  model: nz*nx velocity perturbation (read only); dobs: the
+ * shot's observed data on the regular receiver grid; image (output,
+ * nExt*nz*nx) is overwritten with this shot's extended image. */
+void FN(orc_wemva_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
+                       const REAL* sig, const int* srcPos, int nSrc,
+                       const int* recPos, int nRec, const REAL* model, const REAL* dobs,
+                       int leg1, int leg2, REAL* image, REAL* W, REAL* P)
+{
+    REAL zc[6], xc[6], f[2 * 101], cd[512];
+    FN(orc_tables)(g, zc, xc, f, cd);
+    const int nts = g->nts, sub = g->sub, h = sub + 1;
+    const long nzx = (long)g->nz * g->nx;
+    FN(src_pass)(g, zc, xc, f, cd, v2dt2, sig, srcPos, nSrc, W);
+    FN(dt2_coarse)(g, W);
+    FN(rec_wfld)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dobs, P);
+    REAL* m = FN(zalloc)(nzx);
+    memcpy(m, model, sizeof(REAL) * (size_t)nzx);
+    FN(scale_refl)(g, m, refl, v2dt2);
+    memset(image, 0, sizeof(REAL) * (size_t)nzx * g->nExt);
     REAL* p0 = FN(zalloc)(nzx);
     REAL* p1 = FN(zalloc)(nzx);
     REAL* ssL = FN(zalloc)(nzx);
@@ -736,11 +863,11 @@ void FN(orc_tomo_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
     REAL* scL = FN(zalloc)(nzx);
     REAL* scR = FN(zalloc)(nzx);
     REAL* tmp = FN(zalloc)(nzx);
-    FN(born_ctx) rW = {r, W, v2dt2, refl};
-    if (leg1) {   /* forward in time from the extended scattering of W'' */
-        FN(ext_ss)(g, &rW, 0, ssL);
+    if (leg1) {   /* forward from m * W'', imaged against R */
+        FN(born_ctx) mW = {m, W, v2dt2, refl};
+        FN(born_ss)(g, &mW, 0, ssL);
         for (int its = 0; its < nts - 1; ++its) {
-            FN(ext_ss)(g, &rW, its + 1, ssR);
+            FN(born_ss)(g, &mW, its + 1, ssR);
             for (int it2 = 1; it2 < sub + 1; ++it2) {
                 FN(orc_step_fwd)(g, zc, xc, p0, p1, p0, v2dt2);
                 FN(inject_secondary)(g, f, h, ssL, ssR, p0, it2 - 1);
@@ -748,24 +875,25 @@ void FN(orc_tomo_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
                 FN(capture2)(g, f, h, p0, scL, scR, it2);
                 FN(swap)(&p0, &p1);
             }
-            FN(img_acc)(g, model, P, its, scL);
+            FN(wemva_img_fwd)(g, image, scL, P, its, v2dt2);
             FN(swap)(&ssL, &ssR);
-            memset(ssR, 0, sizeof(REAL) * (size_t)nzx);
             FN(swap)(&scL, &scR);
             memset(scR, 0, sizeof(REAL) * (size_t)nzx);
         }
-        FN(img_acc)(g, model, P, nts - 1, scL);
+        FN(wemva_img_fwd)(g, image, scL, P, nts - 1, v2dt2);
     }
-    if (leg2) {   /* backward in time from the adjoint extended scattering of R */
+    if (leg2) {   /* backward from m * R, imaged against W'' */
         memset(p0, 0, sizeof(REAL) * (size_t)nzx);
         memset(p1, 0, sizeof(REAL) * (size_t)nzx);
         memset(ssL, 0, sizeof(REAL) * (size_t)nzx);
         memset(ssR, 0, sizeof(REAL) * (size_t)nzx);
         memset(scL, 0, sizeof(REAL) * (size_t)nzx);
         memset(scR, 0, sizeof(REAL) * (size_t)nzx);
-        FN(tomo_adj_ss)(g, P, r, v2dt2, nts - 1, ssR);
+        FN(born_ctx) mR = {m, P, v2dt2, refl};
+        FN(born_ctx) iW = {image, W, v2dt2, refl};
+        FN(born_ss)(g, &mR, nts - 1, ssR);
         for (int its = nts - 2; its > -1; --its) {
-            FN(tomo_adj_ss)(g, P, r, v2dt2, its, ssL);
+            FN(born_ss)(g, &mR, its, ssL);
             for (int it2 = sub - 1; it2 > -1; --it2) {
                 FN(orc_step_adj)(g, zc, xc, p0, p1, p0, v2dt2, tmp);
                 FN(inject_secondary)(g, f, h, ssL, ssR, p0, it2 + 1);
@@ -773,16 +901,40 @@ void FN(orc_tomo_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
                 FN(capture2)(g, f, h, p0, scL, scR, it2);
                 FN(swap)(&p0, &p1);
             }
-            FN(img_acc)(g, model, W, its + 1, scR);
+            FN(ext_img)(g, &iW, its + 1, scR);   /* scR *= v2dt2, then the offset imaging */
             FN(swap)(&ssR, &ssL);
-            memset(ssL, 0, sizeof(REAL) * (size_t)nzx);
             FN(swap)(&scR, &scL);
             memset(scL, 0, sizeof(REAL) * (size_t)nzx);
         }
-        FN(img_acc)(g, model, W, 0, scR);
+        FN(ext_img)(g, &iW, 0, scR);
     }
+    FN(scale_refl_ext)(g, image, refl);
+    free(m); free(p0); free(p1); free(ssL); free(ssR); free(scL); free(scR); free(tmp);
+}
+
+/* wemvaOffsetShotsAdjGpu (F/src/wemvaExtShotsGpuFunctions.cu:689-808):
+ * image (read only, nExt*nz*nx) -> model (output, nz*nx), overwritten with
+ * this shot's contribution.  Leg 1 is the backward pass, leg 2 the forward
+ * one, in that order. */
+void FN(orc_wemva_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
+                       const REAL* sig, const int* srcPos, int nSrc,
+                       const int* recPos, int nRec, REAL* model, const REAL* image, const REAL* dobs,
+                       int leg1, int leg2, REAL* W, REAL* P)
+{
+    REAL zc[6], xc[6], f[2 * 101], cd[512];
+    FN(orc_tables)(g, zc, xc, f, cd);
+    const long nzx = (long)g->nz * g->nx, nr = nzx * g->nExt;
+    FN(src_pass)(g, zc, xc, f, cd, v2dt2, sig, srcPos, nSrc, W);
+    FN(dt2_coarse)(g, W);
+    FN(rec_wfld)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dobs, P);
+    REAL* r = FN(zalloc)(nr);
+    memcpy(r, image, sizeof(REAL) * (size_t)nr);
+    FN(scale_refl_ext)(g, r, refl);
+    memset(model, 0, sizeof(REAL) * (size_t)nzx);
+    if (leg1) FN(adj_leg_bwd)(g, zc, xc, f, cd, v2dt2, r, W, P, model);
+    if (leg2) FN(adj_leg_fwd)(g, zc, xc, f, cd, v2dt2, r, W, P, model);
     FN(scale_refl)(g, model, refl, v2dt2);
-    free(r); free(p0); free(p1); free(ssL); free(ssR); free(scL); free(scR); free(tmp);
+    free(r);
 }
 
 /* Test hook for dt2_coarse: W (nts*nz*nx) in place. */

@@ -18,7 +18,10 @@ reference's host classes do:
 * ``bornext_forward`` / ``bornext_adjoint``: F/src/BornExtGpu.cpp:33-134 inside
   F/src/BornExtShotsGpu.cpp (same shot loop);
 * ``tomo_forward`` / ``tomo_adjoint``: F/src/tomoExtGpu.cpp:34-100 inside
-  F/src/tomoExtShotsGpu.cpp:62-145 (same shot loop), extension=offset.
+  F/src/tomoExtShotsGpu.cpp:62-145 (same shot loop), extension=offset;
+* ``wemva_forward`` / ``wemva_adjoint``: F/src/wemvaExtGpu.cpp inside
+  F/src/wemvaExtShotsGpu.cpp (same shot loop, observed data per shot),
+  extension=offset.
 """
 from __future__ import annotations
 
@@ -343,6 +346,31 @@ def tomo_adj_reg(g, v2dt2, refl, sig, srcPos, recPos, dreg, ext_refl, leg1=1, le
     return model
 
 
+def wemva_fwd_reg(g, v2dt2, refl, sig, srcPos, recPos, model, dobs, leg1=1, leg2=1, prec="f"):
+    dt = _dt(prec)
+    image = np.zeros((g.nExt, g.nx, g.nz), dt)
+    W = np.zeros((g.nts, g.nx, g.nz), dt)
+    P = np.zeros_like(W)
+    gc = g.c()
+    _tomo_fns("orc_wemva_fwd", prec)(C.byref(gc), _p(v2dt2), _p(refl), _p(sig), _p(srcPos), len(srcPos),
+                                     _p(recPos), len(recPos), _p(np.ascontiguousarray(model, dt)),
+                                     _p(np.ascontiguousarray(dobs, dt)), int(leg1), int(leg2), _p(image), _p(W),
+                                     _p(P))
+    return image
+
+
+def wemva_adj_reg(g, v2dt2, refl, sig, srcPos, recPos, image, dobs, leg1=1, leg2=1, prec="f"):
+    dt = _dt(prec)
+    model = np.zeros((g.nx, g.nz), dt)
+    W = np.zeros((g.nts, g.nx, g.nz), dt)
+    P = np.zeros_like(W)
+    gc = g.c()
+    _tomo_fns("orc_wemva_adj", prec)(C.byref(gc), _p(v2dt2), _p(refl), _p(sig), _p(srcPos), len(srcPos),
+                                     _p(recPos), len(recPos), _p(model), _p(np.ascontiguousarray(image, dt)),
+                                     _p(np.ascontiguousarray(dobs, dt)), int(leg1), int(leg2), _p(W), _p(P))
+    return model
+
+
 def dt2_coarse(g, W, prec="f"):
     """Second time derivative of a coarse wavefield [nts][nx][nz] (interior
     points), zero beyond both ends, as the tomographic operator applies it."""
@@ -496,6 +524,35 @@ def tomo_forward(g, vel, sources, wavelets, receivers, model, ext_refl, leg1=1, 
         dreg = tomo_fwd_reg(g, v2, rf, fine, src.regPos, rec.regPos, model, ext_refl, leg1, leg2, prec)
         data[s] += rec.forward(dreg, prec=prec)
     return data
+
+
+def wemva_forward(g, vel, sources, wavelets, receivers, dobs, model, leg1=1, leg2=1, prec="f"):
+    """Offset-extended WEMVA operator (wemvaExtShotsGpu, extension=offset):
+    velocity perturbation [nx][nz] -> extended image [nExt][nx][nz], about
+    the receiver wavefields of the observed data dobs [nShot][nRec][nts]."""
+    dt = _dt(prec)
+    vel = np.ascontiguousarray(vel, dt)
+    v2, rf = vel_scales(g, vel, prec)
+    image = np.zeros((g.nExt, g.nx, g.nz), dt)
+    for s in range(len(sources)):
+        src, rec = sources[s], _shot_geom(receivers, s)
+        fine = born_source_fine(g, vel, src, _shot_geom(wavelets, s), prec)
+        dreg = rec.adjoint(dobs[s], prec=prec)
+        image += wemva_fwd_reg(g, v2, rf, fine, src.regPos, rec.regPos, model, dreg, leg1, leg2, prec)
+    return image
+
+
+def wemva_adjoint(g, vel, sources, wavelets, receivers, dobs, image, leg1=1, leg2=1, prec="f"):
+    dt = _dt(prec)
+    vel = np.ascontiguousarray(vel, dt)
+    v2, rf = vel_scales(g, vel, prec)
+    model = np.zeros((g.nx, g.nz), dt)
+    for s in range(len(sources)):
+        src, rec = sources[s], _shot_geom(receivers, s)
+        fine = born_source_fine(g, vel, src, _shot_geom(wavelets, s), prec)
+        dreg = rec.adjoint(dobs[s], prec=prec)
+        model += wemva_adj_reg(g, v2, rf, fine, src.regPos, rec.regPos, image, dreg, leg1, leg2, prec)
+    return model
 
 
 def tomo_adjoint(g, vel, sources, wavelets, receivers, data, ext_refl, leg1=1, leg2=1, prec="f"):
