@@ -22,12 +22,13 @@ from .pyOperator import Operator
 
 try:  # the modules sit next to this file; the package dir may also be on sys.path
     from . import pyAcoustic_iso_float_nl, pyAcoustic_iso_float_born, pyAcoustic_iso_float_born_ext
-    from . import pyAcoustic_iso_float_tomo
+    from . import pyAcoustic_iso_float_tomo, pyAcoustic_iso_float_wemva
 except ImportError:  # pragma: no cover
     import pyAcoustic_iso_float_nl
     import pyAcoustic_iso_float_born
     import pyAcoustic_iso_float_born_ext
     import pyAcoustic_iso_float_tomo
+    import pyAcoustic_iso_float_wemva
 
 deviceGpu = pyAcoustic_iso_float_nl.deviceGpu
 parfile2pars = sep.parfile2pars
@@ -340,3 +341,52 @@ class tomoExtShotsGpu(_Wrapped):
     def setReflectivityExt(self, reflectivityExt):
         with self._mod.ostream_redirect():
             self.pyOp.setReflectivityExt(_cpp(reflectivityExt))
+
+
+# ------------------------------------------------------------------ WEMVA --
+def wemvaExtOpInitFloat(args):
+    """(model, data, vel, parObject, sources, sourcesSignals, receivers,
+    receiversSignals): model is a velocity perturbation, data the extended
+    image [nExt][nx][nz]; receiversSignals are the observed data of each shot
+    [nReceiver][nts], read from the ``seismicData`` file [nShot][nReceiver][nts]
+    (PY/Acoustic_iso_float.py:1512-1603)."""
+    par = create_parObj(args)
+    vel = _read_vel(par)
+    sources, srcAxis = buildSourceGeometry(par, vel)
+    receivers, recAxis = buildReceiversGeometry(par, vel)
+    zAxis, xAxis = vel.getHyper().axes[0], vel.getHyper().axes[1]
+    time = _time_axis(par)
+    extAxis = _ext_axis(par)
+    signals = _read_sources(par)
+    dataFile = par.getString("seismicData", "noWemvaDataFile")
+    if dataFile == "noWemvaDataFile":
+        raise ValueError("**** ERROR: User did not provide wemva seismic data file ****")
+    allData = sep.read(dataFile).getNdArray()
+    recSignals = []
+    for s in range(srcAxis.n):
+        sl = sep.getSepVector(axes=[time, recAxis])
+        sl.getNdArray()[...] = allData[s]
+        recSignals.append(sl)
+    data = sep.getSepVector(axes=[zAxis, xAxis, extAxis])
+    model = sep.getSepVector(vel.getHyper())
+    return model, data, vel, par, sources, [signals], receivers, recSignals
+
+
+class wemvaExtShotsGpu(_Wrapped):
+    """Subsurface-offset WEMVA operator: velocity perturbation -> extended
+    image perturbation about the observed data (PY/Acoustic_iso_float.py:1605-1684)."""
+
+    _mod = pyAcoustic_iso_float_wemva
+    _name = " WemvaOp "
+
+    def __init__(self, domain, range, velocity, paramP, sourceVector, sourcesSignalsVector, receiversVector,
+                 receiversSignalsVector):
+        self.setDomainRange(domain, range)
+        sig = [_cpp(s) for s in sourcesSignalsVector]
+        rsig = [_cpp(s) for s in receiversSignalsVector]
+        self.pyOp = pyAcoustic_iso_float_wemva.wemvaExtShotsGpu(_cpp(velocity), _cpp(paramP).param, sourceVector, sig,
+                                                                receiversVector, rsig)
+
+    def setReceiversSignalsVector(self, receiversSignalsVector):
+        with self._mod.ostream_redirect():
+            self.pyOp.setReceiversSignalsVector([_cpp(s) for s in receiversSignalsVector])

@@ -64,6 +64,8 @@ _SIGS = {
     "aiso_ctx_set_ext_reflectivity": (_I, [_P, _P, _I]),
     "aiso_tomo_forward": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_tomo_adjoint": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
+    "aiso_wemva_forward": (_I, [_P, _I, _P, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
+    "aiso_wemva_adjoint": (_I, [_P, _I, _P, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_pad_plus": (_I, [_I, _I, _I]),
 }
 
@@ -283,3 +285,19 @@ class Context:
 
     def tomo_adjoint(self, model, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
         self._born(lib().aiso_tomo_adjoint, model, data, sources, wavelets, receivers, add, mem)
+
+    def _wemva(self, fn, model, image, data, sources, wavelets, receivers, add, mem):
+        s, _ks = _devarray(sources)
+        r, _kr = _devarray(receivers)
+        w = np.ascontiguousarray(wavelets, np.float32)
+        nwav = w.shape[0] if w.ndim == 3 else 1
+        _check(fn(self.h, int(add), _ptr(model), _ptr(image), _ptr(data), s, len(sources), _ptr(w), nwav, r,
+                  len(receivers), mem))
+
+    def wemva_forward(self, model, image, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
+        """Velocity perturbation [nx][nz] -> extended image [nExt][nx][nz] about
+        the observed data [nShot][nRec][nts]."""
+        self._wemva(lib().aiso_wemva_forward, model, image, data, sources, wavelets, receivers, add, mem)
+
+    def wemva_adjoint(self, model, image, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
+        self._wemva(lib().aiso_wemva_adjoint, model, image, data, sources, wavelets, receivers, add, mem)

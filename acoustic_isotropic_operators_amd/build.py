@@ -2,7 +2,7 @@
 
 * ``libaiso.so``: the HIP engine (kernels.hip, engine.cpp, host.cpp) for gfx950,
   exporting the C-ABI declared in ``include/aiso.h``;
-* the pybind11 modules ``pyAcoustic_iso_float_{nl,born,born_ext,tomo}`` that mirror the
+* the pybind11 modules ``pyAcoustic_iso_float_{nl,born,born_ext,tomo,wemva}`` that mirror the
   reference's module names on top of that C-ABI.
 
 Run ``python -m acoustic_isotropic_operators_amd.build`` (or ``build()``).
@@ -59,7 +59,7 @@ def build_pybind():
     inc = ["-I" + pybind11.get_include(), "-I" + sysconfig.get_paths()["include"], "-I" + INC, "-I" + CSRC]
     outs = []
     for name in ("pyAcoustic_iso_float_nl", "pyAcoustic_iso_float_born", "pyAcoustic_iso_float_born_ext",
-                 "pyAcoustic_iso_float_tomo"):
+                 "pyAcoustic_iso_float_tomo", "pyAcoustic_iso_float_wemva"):
         src = os.path.join(CSRC, "pybind", name + ".cpp")
         out = os.path.join(PKG, name + ext)
         _run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", *inc, src,

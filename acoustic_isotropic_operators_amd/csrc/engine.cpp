@@ -17,6 +17,7 @@
 #include <memory>
 #include <string>
 #include <tuple>
+#include <functional>
 #include <vector>
 
 #include "aiso.h"
@@ -108,7 +109,7 @@ struct aiso_ctx {
     // tomographic operator: the extended reflectivity it linearizes about
     // (set once, [nExt][nx][nz]), its per-call 2/v^3 scaling, and the
     // scattered / receiver wavefield and capture buffers
-    DBuf extRefl, rExt, P, cap, slotsC;
+    DBuf extRefl, rExt, P, cap, slotsC, slotsI, slotsJ, imgDev;
     bool haveExtRefl = false;
     int leg1 = 1, leg2 = 1;
     hipEvent_t evIn = nullptr;          // orders device inputs after the default stream's work
@@ -246,12 +247,18 @@ DevGeom* dev_geom_pair2(aiso_ctx* c, const aiso_device* d, bool region)
     return out;
 }
 
-enum class Op { NlFwd, NlAdj, BornFwd, BornAdj, ExtFwd, ExtAdj, TomoFwd, TomoAdj };
+enum class Op { NlFwd, NlAdj, BornFwd, BornAdj, ExtFwd, ExtAdj, TomoFwd, TomoAdj, WemvaFwd, WemvaAdj };
 
-bool is_adj(Op op) { return op == Op::NlAdj || op == Op::BornAdj || op == Op::ExtAdj || op == Op::TomoAdj; }
+// The adjoint of each operator writes the model side (for WEMVA, the
+// velocity perturbation); both WEMVA directions read observed data.
+bool is_adj(Op op)
+{
+    return op == Op::NlAdj || op == Op::BornAdj || op == Op::ExtAdj || op == Op::TomoAdj || op == Op::WemvaAdj;
+}
 bool is_born(Op op) { return op != Op::NlFwd && op != Op::NlAdj; }
 bool is_ext(Op op) { return op == Op::ExtFwd || op == Op::ExtAdj; }
 bool is_tomo(Op op) { return op == Op::TomoFwd || op == Op::TomoAdj; }
+bool is_wemva(Op op) { return op == Op::WemvaFwd || op == Op::WemvaAdj; }
 
 struct Request {
     Op op;
@@ -423,10 +430,11 @@ void run(aiso_ctx* c, const Request& q)
     if ((!is_born(q.op) || is_ext(q.op)) && p.minPad >= kPadMax) throw Error(AISO_ERR_PARAM, "**** ERROR: Padding value is too high ****");
     if (is_ext(q.op) && p.extension != AISO_EXT_OFFSET && p.nExt > 1)
         throw Error(AISO_ERR_PARAM, "extended Born needs extension=offset");
-    if (is_tomo(q.op)) {
+    if (is_tomo(q.op) || is_wemva(q.op)) {
         if (p.extension != AISO_EXT_OFFSET)
-            throw Error(AISO_ERR_PARAM, "the tomographic operator needs extension=offset (time-lag is not supported)");
-        if (!c->haveExtRefl) throw Error(AISO_ERR_STATE, "extended reflectivity not set");
+            throw Error(AISO_ERR_PARAM, is_tomo(q.op) ? "the tomographic operator needs extension=offset (time-lag is not supported)"
+                                                      : "the WEMVA operator needs extension=offset (time-lag is not supported)");
+        if (is_tomo(q.op) && !c->haveExtRefl) throw Error(AISO_ERR_STATE, "extended reflectivity not set");
         if (p.sub >= kSubMax) throw Error(AISO_ERR_PARAM, "**** ERROR: Subsampling parameter too high ****");
         if (p.minPad >= kPadMax) throw Error(AISO_ERR_PARAM, "**** ERROR: Padding value is too high ****");
     }
@@ -451,6 +459,13 @@ void run(aiso_ctx* c, const Request& q)
     const long nzx = p.nzx();
     const int nExt = is_ext(q.op) ? p.nExt : 1;
     const long modelN = is_born(q.op) ? nzx * nExt : (long)q.nShotSig * nSrcDev * p.nts;
+    // WEMVA: the velocity perturbation and the extended image are its two
+    // sides (model in / out differ in size); both directions read data.
+    const bool wemva = is_wemva(q.op);
+    const long mInN = q.op == Op::WemvaAdj ? nzx * p.nExt : modelN;
+    const long mOutN = q.op == Op::WemvaFwd ? nzx * p.nExt : modelN;
+    const int nImg = is_ext(q.op) || q.op == Op::WemvaFwd ? p.nExt : 1;   // slices of the per-shot image
+    const bool dataInput = is_adj(q.op) || wemva;
     const long dataN = (long)q.nShot * nRecDev * p.nts;
     const int h = p.h();
     const float* f = p.filt.data();
@@ -475,6 +490,13 @@ void run(aiso_ctx* c, const Request& q)
     } else if (q.mem == AISO_MEM_DEVICE) {
         dModelIn = q.modelIn;
         dModelOut = q.modelOut;
+    } else if (wemva) {
+        float* mi = (float*)c->modelDev.need(mInN * 4);
+        float* mo = (float*)c->imgDev.need(mOutN * 4);
+        HIPC(hipMemcpyAsync(mi, q.modelIn, mInN * 4, hipMemcpyHostToDevice, s));
+        if (q.add) HIPC(hipMemcpyAsync(mo, q.modelOut, mOutN * 4, hipMemcpyHostToDevice, s));
+        dModelIn = mi;
+        dModelOut = mo;
     } else {
         float* m = c->modelDev.need(modelN * 4) ? c->modelDev.as<float>() : nullptr;
         if (!is_adj(q.op) || q.add) HIPC(hipMemcpyAsync(m, is_adj(q.op) ? q.modelOut : q.modelIn, modelN * 4, hipMemcpyHostToDevice, s));
@@ -486,7 +508,7 @@ void run(aiso_ctx* c, const Request& q)
         dDataOut = q.dataOut;
     } else {
         float* d = c->dataDev.need(dataN * 4) ? c->dataDev.as<float>() : nullptr;
-        if (is_adj(q.op) || q.add) HIPC(hipMemcpyAsync(d, is_adj(q.op) ? q.dataIn : q.dataOut, dataN * 4, hipMemcpyHostToDevice, s));
+        if (dataInput || q.add) HIPC(hipMemcpyAsync(d, dataInput ? q.dataIn : q.dataOut, dataN * 4, hipMemcpyHostToDevice, s));
         dDataIn = d;
         dDataOut = d;
     }
@@ -518,21 +540,22 @@ void run(aiso_ctx* c, const Request& q)
     const bool adj2 = q.op == Op::BornAdj && p.sub == 2 && c->pair2 && noRing;
     const int pairSlices = pair2 || adj2 ? 4 : 2;
     const long wSlices = lockstep ? 3 : p.nts;
-    const bool needW = is_born(q.op), needSS = is_ext(q.op) || is_tomo(q.op) || q.op == Op::BornAdj;
+    const bool needW = is_born(q.op), needSS = is_ext(q.op) || is_tomo(q.op) || wemva || q.op == Op::BornAdj;
     // The offset-extended passes keep a ring of coarse-rate slices per shot:
     // extK samples batched per imaging / secondary-source launch, plus the
     // one the time steps are writing or reading.
     const int extK = std::min(c->extBatch, p.nts);
-    const int ssSlices = is_ext(q.op) || is_tomo(q.op) ? extK + 1 : 2;
+    const int ssSlices = is_ext(q.op) || is_tomo(q.op) || wemva ? extK + 1 : 2;
     auto ssSlot = [&](int k) { return (long)(k % ssSlices) * nzx; };
-    const bool needImg = q.op == Op::BornAdj || q.op == Op::ExtAdj || q.op == Op::TomoAdj;
-    const bool tomo = is_tomo(q.op);
+    const bool needImg = q.op == Op::BornAdj || q.op == Op::ExtAdj || q.op == Op::TomoAdj || wemva;
+    const bool tomo = is_tomo(q.op) || wemva;   // the tomographic and WEMVA passes share the machinery below
+    const int capSlices = wemva ? extK + 1 : 2;     // captured coarse samples kept per shot
     size_t perShot = pairSlices * nzx * 4 + (size_t)nRecRegMax * p.nts * 4 + (size_t)nSrcRegMax * p.ntw * 4;
     if (needW) perShot += (size_t)wSlices * nzx * 4;
     if (lockstep) perShot += pairSlices * nzx * 4;
     if (needSS) perShot += (size_t)ssSlices * nzx * 4;
-    if (needImg) perShot += (size_t)nExt * nzx * 4;
-    if (tomo) perShot += (size_t)(p.nts + 2) * nzx * 4;   // scattered / receiver wavefield + capture pair
+    if (needImg) perShot += (size_t)nImg * nzx * 4;
+    if (tomo) perShot += (size_t)(p.nts + capSlices) * nzx * 4;   // scattered / receiver wavefield + captures
     c->bytesPerShot = perShot;
     // Shots per launch: an explicit request, else as many as keep the
     // slices every fine step touches (per shot) within the on-die cache
@@ -553,7 +576,7 @@ void run(aiso_ctx* c, const Request& q)
         size_t freeB = 0, totalB = 0;
         HIPC(hipMemGetInfo(&freeB, &totalB));
         size_t have = c->pairs.n + c->pairsQ.n + c->W.n + c->ss.n + c->img.n + c->dreg.n + c->sig.n + c->P.n +
-                      c->cap.n;
+                      c->cap.n + c->imgDev.n;
         size_t budget = (size_t)((freeB + have) * 0.85);
         B = std::min(B, (int)std::max<size_t>(1, budget / perShot));
     }
@@ -569,10 +592,14 @@ void run(aiso_ctx* c, const Request& q)
     // legs or the receiver wavefield of the adjoint (slots dSlotsQ see it as
     // W); cap holds two captured coarse samples (slots dSlotsC).
     float* Pw = tomo ? (float*)c->P.need((size_t)B * p.nts * nzx * 4) : nullptr;
-    float* cap = tomo ? (float*)c->cap.need((size_t)B * 2 * nzx * 4) : nullptr;
+    float* cap = tomo ? (float*)c->cap.need((size_t)B * capSlices * nzx * 4) : nullptr;
     ShotSlot* dSlotsC = tomo ? (ShotSlot*)c->slotsC.need(sizeof(ShotSlot) * B) : nullptr;
+    // WEMVA forward's batched offset imaging: slots that see (ss: captures,
+    // W: W'') and (ss: the receiver wavefield, W: captures)
+    ShotSlot* dSlotsI = q.op == Op::WemvaFwd ? (ShotSlot*)c->slotsI.need(sizeof(ShotSlot) * B) : nullptr;
+    ShotSlot* dSlotsJ = q.op == Op::WemvaFwd ? (ShotSlot*)c->slotsJ.need(sizeof(ShotSlot) * B) : nullptr;
     float* ss = needSS ? (float*)c->ss.need((size_t)B * ssSlices * nzx * 4) : nullptr;
-    float* img = needImg ? (float*)c->img.need((size_t)B * nExt * nzx * 4) : nullptr;
+    float* img = needImg ? (float*)c->img.need((size_t)B * nImg * nzx * 4) : nullptr;
     float* dreg = (float*)c->dreg.need((size_t)B * nRecRegMax * p.nts * 4);
     float* sig = (float*)c->sig.need((size_t)B * nSrcRegMax * p.ntw * 4);
     ShotSlot* dSlots = (ShotSlot*)c->slots.need(sizeof(ShotSlot) * B);
@@ -589,8 +616,8 @@ void run(aiso_ctx* c, const Request& q)
         HIPC(launch_scale_ext(dModelIn, m, c->refl.as<float>(), p.nz, p.nx, nExt, s));
         mtil = m;
     }
-    const float* rExt = nullptr;   // tomographic operator: extended reflectivity scaled by 2/v^3
-    if (tomo) {
+    const float* rExt = nullptr;   // extended reflectivity (tomographic) or image (WEMVA adjoint) scaled by 2/v^3
+    if (is_tomo(q.op)) {
         if (q.op == Op::TomoFwd) {
             float* m = (float*)c->mtil.need(nzx * 4);
             HIPC(launch_scale_model(dModelIn, m, c->v2dt2.as<float>(), c->refl.as<float>(), p.nz, p.nx, s));
@@ -599,10 +626,18 @@ void run(aiso_ctx* c, const Request& q)
         float* r = (float*)c->rExt.need(nzx * p.nExt * 4);
         HIPC(launch_scale_ext(c->extRefl.as<float>(), r, c->refl.as<float>(), p.nz, p.nx, p.nExt, s));
         rExt = r;
+    } else if (q.op == Op::WemvaFwd) {
+        float* m = (float*)c->mtil.need(nzx * 4);
+        HIPC(launch_scale_model(dModelIn, m, c->v2dt2.as<float>(), c->refl.as<float>(), p.nz, p.nx, s));
+        mtil = m;
+    } else if (q.op == Op::WemvaAdj) {   // the extended image in, scaled by 2/v^3
+        float* r = (float*)c->rExt.need(nzx * p.nExt * 4);
+        HIPC(launch_scale_ext(dModelIn, r, c->refl.as<float>(), p.nz, p.nx, p.nExt, s));
+        rExt = r;
     }
     if (needImg) {
-        acc = (float*)c->acc.need(nzx * nExt * 4);
-        HIPC(launch_fill(acc, nzx * nExt, s));
+        acc = (float*)c->acc.need(nzx * nImg * 4);
+        HIPC(launch_fill(acc, nzx * nImg, s));
     }
     std::vector<float> nlAcc;
     if (q.op == Op::NlAdj && q.nShotSig == 1) nlAcc.assign((size_t)nSrcDev * p.nts, 0.f);
@@ -643,7 +678,7 @@ void run(aiso_ctx* c, const Request& q)
             if (pairsQ)
                 for (int i = 0; i < pairSlices; ++i) S.q[i] = pairsQ + ((size_t)k * pairSlices + i) * nzx;
             if (ss) { S.ss[0] = ss + (size_t)k * ssSlices * nzx; S.ss[1] = S.ss[0] + nzx; }
-            if (img) S.img = img + (size_t)k * nExt * nzx;
+            if (img) S.img = img + (size_t)k * nImg * nzx;
             S.dreg = dreg + (size_t)k * nRecRegMax * p.nts;
             S.sig = sig + (size_t)k * nSrcRegMax * p.ntw;
             S.srcTile = sg->tilePtr.as<int>();
@@ -667,15 +702,26 @@ void run(aiso_ctx* c, const Request& q)
         }
         HIPC(hipMemcpyAsync(dSlots, slots.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
         std::vector<ShotSlot> slotsP, slotsCv;
-        if (tomo) {   // the same shots seeing P, or the capture pair, as W
+        std::vector<ShotSlot> slotsIv, slotsJv;
+        if (tomo) {   // the same shots seeing P, or the captured samples, as W
             slotsP = slots;
             slotsCv = slots;
             for (int k = 0; k < nb; ++k) {
                 slotsP[k].W = Pw + (size_t)k * p.nts * nzx;
-                slotsCv[k].W = cap + (size_t)k * 2 * nzx;
+                slotsCv[k].W = cap + (size_t)k * capSlices * nzx;
             }
             HIPC(hipMemcpyAsync(dSlotsQ, slotsP.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
             HIPC(hipMemcpyAsync(dSlotsC, slotsCv.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
+        }
+        if (q.op == Op::WemvaFwd) {
+            slotsIv = slots;
+            slotsJv = slotsCv;
+            for (int k = 0; k < nb; ++k) {
+                slotsIv[k].ss[0] = cap + (size_t)k * capSlices * nzx;
+                slotsJv[k].ss[0] = Pw + (size_t)k * p.nts * nzx;
+            }
+            HIPC(hipMemcpyAsync(dSlotsI, slotsIv.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
+            HIPC(hipMemcpyAsync(dSlotsJ, slotsJv.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
         }
         std::vector<ShotSlot> slotsQ;
         if (lockstep) {   // the scattered field seen as "the" pair, for single-field launches
@@ -690,8 +736,8 @@ void run(aiso_ctx* c, const Request& q)
         a.nshots = nb;
         a.gridLogical = a.ntiles * nb;
 
-        // -- adjoint: data traces to the regular receiver grid
-        if (is_adj(q.op)) {
+        // -- adjoint (and WEMVA): data traces to the regular receiver grid
+        if (dataInput) {
             for (int k = 0; k < nb; ++k) {
                 const int shot = b0 + k;
                 const aiso_device* rd = q.rec[q.nRec == 1 ? 0 : shot];
@@ -991,50 +1037,54 @@ void run(aiso_ctx* c, const Request& q)
             a.win = 0;
         }
 
-        // -- tomographic operator (extension=offset): both legs, forward or
-        // adjoint, after the source pass above (F/src/tomoExtShotsGpuFunctions.cu
-        // :467-572 and :912-1020)
+        // -- tomographic and WEMVA operators (extension=offset): both legs,
+        // forward or adjoint, after the source pass above
+        // (F/src/tomoExtShotsGpuFunctions.cu:467-572, 912-1020;
+        // F/src/wemvaExtShotsGpuFunctions.cu:458-574, 689-808)
         if (tomo) {
+            const bool haveAll = ss && Pw && cap && dSlotsQ && dSlotsC && (!needImg || img) &&
+                                 (q.op != Op::WemvaFwd || (dSlotsI && dSlotsJ)) &&
+                                 ((q.op != Op::TomoFwd && q.op != Op::WemvaFwd) || mtil) &&
+                                 ((!is_tomo(q.op) && q.op != Op::WemvaAdj) || rExt);
+            if (!haveAll) throw Error(AISO_ERR_STATE, "internal: a work buffer of the tomographic / WEMVA passes is missing");
             // W <- its coarse second time derivative, in place
             const float d2 = p.dts * p.dts;
             HIPC(launch_dt2_coarse(dSlots, nb, p.nts, (float)(1.0 / (double)d2), (float)(-2.0 / (double)d2), p.nz,
                                    p.nx, s));
             enum { SS_EXT, SS_MUL, SS_ADJ };   // ring producers
-            auto produce = [&](int kind, const ShotSlot* from, int k0, int n) {
+            auto produce = [&](int kind, const ShotSlot* from, const float* scale, int k0, int n) {
                 ExtBatchArgs e{};
                 e.slots = from;
                 e.v2dt2 = base.v2dt2;
+                e.mext = scale;
                 e.nz = p.nz; e.nx = p.nx; e.hExt = p.hExt;
                 e.nk = n;
                 for (int j = 0; j < n; ++j) {
                     e.ssOff[j] = ssSlot(k0 + j);
                     e.wOff[j] = (long)(k0 + j) * nzx;
                 }
-                if (kind == SS_MUL) {
-                    e.mext = mtil;
-                    HIPC(launch_mul_batch(e, nb, s));
-                } else {
-                    e.mext = rExt;
-                    HIPC(kind == SS_EXT ? launch_ext_ss_batch(e, nb, s) : launch_tomo_adj_ss(e, nb, s));
-                }
+                if (kind == SS_MUL) HIPC(launch_mul_batch(e, nb, s));
+                else if (kind == SS_EXT) HIPC(launch_ext_ss_batch(e, nb, s));
+                else HIPC(launch_tomo_adj_ss(e, nb, s));
             };
+            auto capOff = [&](int k) { return (long)(k % capSlices) * nzx; };
             // Forward in time.  ssKind >= 0: inject ring samples made from
-            // `from`'s W; capture: 1 = into the launch slots' W at the sample
-            // index (nts slices), 2 = into the capture pair; imageR: after each
-            // coarse step, img += R[its] * captured sample (adjoint leg 1).
-            auto forward_pass = [&](unsigned mode, const ShotSlot* on, int ssKind, const ShotSlot* from, int capture,
-                                    bool imageR) {
+            // `from`'s W (scaled by `scale`); capture: 1 = into the launch
+            // slots' W at the sample index (nts slices), 2 = into the capture
+            // ring; done(k) runs once coarse sample k is complete.
+            auto forward_pass = [&](unsigned mode, const ShotSlot* on, int ssKind, const ShotSlot* from,
+                                    const float* scale, int capture, const std::function<void(int)>& done) {
                 HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
                 StepArgs x = a;
                 x.slots = on;
                 x.cur = 1;
                 x.win = 0;
                 int next = 0;
-                auto capOff = [&](int k) { return capture == 1 ? (long)k * nzx : (long)(k % 2) * nzx; };
+                auto off = [&](int k) { return capture == 1 ? (long)k * nzx : capOff(k); };
                 for (int its = 0; its < p.nts - 1; ++its) {
                     while (ssKind >= 0 && next <= its + 1) {
                         const int n = std::min(extK, p.nts - next);
-                        produce(ssKind, from, next, n);
+                        produce(ssKind, from, scale, next, n);
                         next += n;
                     }
                     for (int it2 = 1; it2 <= p.sub; ++it2) {
@@ -1049,8 +1099,8 @@ void run(aiso_ctx* c, const Request& q)
                             x.ssOffB = ssSlot(its + 1);
                         }
                         if (capture) {
-                            x.wA = capOff(its);
-                            x.wB = capOff(its + 1);
+                            x.wA = off(its);
+                            x.wB = off(its + 1);
                             x.skipA = x.fe == 0.f;
                             x.skipB = x.ff == 0.f;
                             x.assignA = its == 0 && it2 == 1;
@@ -1064,31 +1114,55 @@ void run(aiso_ctx* c, const Request& q)
                         ring(mode, x);
                         x.cur ^= 1;
                     }
-                    if (imageR) HIPC(launch_img_acc(dSlotsQ, (long)its * nzx, on, capOff(its), nb, p.nz, p.nx, s));
+                    if (done) done(its);
                 }
-                if (imageR)
-                    HIPC(launch_img_acc(dSlotsQ, (long)(p.nts - 1) * nzx, on, capOff(p.nts - 1), nb, p.nz, p.nx, s));
+                if (done) done(p.nts - 1);
             };
-            if (q.op == Op::TomoFwd) {
-                if (c->leg1) {   // source -> extended reflectivity -> P -> model -> data
-                    forward_pass(MODE_TOMO_CAP, dSlotsQ, SS_EXT, dSlots, 1, false);
-                    forward_pass(MODE_BORN_SCAT, dSlotsQ, -1, nullptr, 0, false);
+            // Backward in time on the adjoint stepper from ring samples made
+            // from `from`'s W, injected with the weights of fine index it2 + 1,
+            // captured into the ring; done(k) once coarse sample k is complete.
+            auto backward_pass = [&](int ssKind, const ShotSlot* from, const float* scale,
+                                     const std::function<void(int)>& done) {
+                HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
+                HIPC(launch_fill(cap, (long)nb * capSlices * nzx, s));
+                StepArgs y = a;
+                y.slots = dSlotsC;
+                y.cur = 1;
+                y.win = 0;
+                int next = p.nts - 1;   // highest sample not yet in the ring
+                for (int its = p.nts - 2; its >= 0; --its) {
+                    while (next >= its) {
+                        const int n = std::min(extK, next + 1);
+                        produce(ssKind, from, scale, next - n + 1, n);
+                        next -= n;
+                    }
+                    for (int it2 = p.sub - 1; it2 >= 0; --it2) {
+                        y.its = its;
+                        y.fa = f[it2 + 1];
+                        y.fb = f[h + it2 + 1];
+                        y.ssA = y.ssB = 0;
+                        y.ssOffA = ssSlot(its);
+                        y.ssOffB = ssSlot(its + 1);
+                        y.wA = capOff(its);
+                        y.wB = capOff(its + 1);
+                        y.fe = f[it2];
+                        y.ff = f[h + it2];
+                        y.skipA = y.fe == 0.f;
+                        y.skipB = y.ff == 0.f;
+                        y.assignA = it2 == p.sub - 1;
+                        y.assignB = 0;
+                        launch_timed(c, MODE_TOMO_ADJ_CAP, AISO_K_TOMO, y, nb, true);
+                        y.cur ^= 1;
+                    }
+                    done(its + 1);
                 }
-                if (c->leg2) {   // source -> model -> P -> extended reflectivity -> data
-                    forward_pass(MODE_TOMO_CAP, dSlotsQ, SS_MUL, dSlots, 1, false);
-                    forward_pass(MODE_EXT_SCAT, dSlotsQ, SS_EXT, dSlotsQ, 0, false);
-                }
-                for (int k = 0; k < nb; ++k) {
-                    const int shot = b0 + k;
-                    const aiso_device* rd = q.rec[q.nRec == 1 ? 0 : shot];
-                    DevGeom* rg = recG[q.nRec == 1 ? 0 : shot];
-                    HIPC(launch_dev_forward(dreg + (size_t)k * nRecRegMax * p.nts, dDataOut + (size_t)shot * nRecDev * p.nts,
-                                            rg->cornerReg.as<int>(), rg->weight.as<float>(), nRecDev, rd->g.nb, p.nts,
-                                            q.add, s));
-                }
-            } else {
-                HIPC(launch_fill(img, (long)nb * nzx, s));
-                // receiver wavefield R into P (computeTomoRecWfld, :1481-1515)
+                done(0);
+            };
+            // img += X[k] * captured sample k, one coarse sample at a time
+            auto image_with = [&](const ShotSlot* X) {
+                return [&, X](int k) { HIPC(launch_img_acc(X, (long)k * nzx, dSlotsC, capOff(k), nb, p.nz, p.nx, s)); };
+            };
+            if (dataInput) {   // receiver wavefield R into P (computeTomoRecWfld, :1481-1515)
                 HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
                 StepArgs x = a;
                 x.slots = dSlotsQ;
@@ -1112,46 +1186,70 @@ void run(aiso_ctx* c, const Request& q)
                         ring(MODE_TOMO_REC, x);
                         x.cur ^= 1;
                     }
-                if (c->leg1)   // forward from the extended scattering of W'', imaged against R
-                    forward_pass(MODE_TOMO_CAP, dSlotsC, SS_EXT, dSlots, 2, true);
-                if (c->leg2) {   // backward from the adjoint extended scattering of R, imaged against W''
-                    HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
-                    HIPC(launch_fill(cap, (long)nb * 2 * nzx, s));
-                    StepArgs y = a;
-                    y.slots = dSlotsC;
-                    y.cur = 1;
-                    y.win = 0;
-                    int next = p.nts - 1;   // highest sample not yet in the ring
-                    for (int its = p.nts - 2; its >= 0; --its) {
-                        while (next >= its) {
-                            const int n = std::min(extK, next + 1);
-                            produce(SS_ADJ, dSlotsQ, next - n + 1, n);
-                            next -= n;
-                        }
-                        for (int it2 = p.sub - 1; it2 >= 0; --it2) {
-                            y.its = its;
-                            y.fa = f[it2 + 1];
-                            y.fb = f[h + it2 + 1];
-                            y.ssA = y.ssB = 0;
-                            y.ssOffA = ssSlot(its);
-                            y.ssOffB = ssSlot(its + 1);
-                            y.wA = (long)(its % 2) * nzx;
-                            y.wB = (long)((its + 1) % 2) * nzx;
-                            y.fe = f[it2];
-                            y.ff = f[h + it2];
-                            y.skipA = y.fe == 0.f;
-                            y.skipB = y.ff == 0.f;
-                            y.assignA = it2 == p.sub - 1;
-                            y.assignB = 0;
-                            launch_timed(c, MODE_TOMO_ADJ_CAP, AISO_K_TOMO, y, nb, true);
-                            y.cur ^= 1;
-                        }
-                        HIPC(launch_img_acc(dSlots, (long)(its + 1) * nzx, dSlotsC, (long)((its + 1) % 2) * nzx, nb,
-                                            p.nz, p.nx, s));
-                    }
-                    HIPC(launch_img_acc(dSlots, 0, dSlotsC, 0, nb, p.nz, p.nx, s));
+            }
+            if (img) HIPC(launch_fill(img, (long)nb * nImg * nzx, s));
+            if (q.op == Op::TomoFwd) {
+                if (c->leg1) {   // source -> extended reflectivity -> P -> model -> data
+                    forward_pass(MODE_TOMO_CAP, dSlotsQ, SS_EXT, dSlots, rExt, 1, nullptr);
+                    forward_pass(MODE_BORN_SCAT, dSlotsQ, -1, nullptr, nullptr, 0, nullptr);
+                }
+                if (c->leg2) {   // source -> model -> P -> extended reflectivity -> data
+                    forward_pass(MODE_TOMO_CAP, dSlotsQ, SS_MUL, dSlots, mtil, 1, nullptr);
+                    forward_pass(MODE_EXT_SCAT, dSlotsQ, SS_EXT, dSlotsQ, rExt, 0, nullptr);
+                }
+                for (int k = 0; k < nb; ++k) {
+                    const int shot = b0 + k;
+                    const aiso_device* rd = q.rec[q.nRec == 1 ? 0 : shot];
+                    DevGeom* rg = recG[q.nRec == 1 ? 0 : shot];
+                    HIPC(launch_dev_forward(dreg + (size_t)k * nRecRegMax * p.nts, dDataOut + (size_t)shot * nRecDev * p.nts,
+                                            rg->cornerReg.as<int>(), rg->weight.as<float>(), nRecDev, rd->g.nb, p.nts,
+                                            q.add, s));
+                }
+            } else if (q.op == Op::TomoAdj || q.op == Op::WemvaAdj) {
+                // The same two passes: forward from the extended scattering of
+                // W'' imaged against R, and backward from the adjoint extended
+                // scattering of R imaged against W''; the tomographic adjoint
+                // runs them as legs 1, 2, WEMVA's as legs 2, 1.
+                auto fwdLeg = [&] { forward_pass(MODE_TOMO_CAP, dSlotsC, SS_EXT, dSlots, rExt, 2, image_with(dSlotsQ)); };
+                auto bwdLeg = [&] { backward_pass(SS_ADJ, dSlotsQ, rExt, image_with(dSlots)); };
+                if (q.op == Op::TomoAdj) {
+                    if (c->leg1) fwdLeg();
+                    if (c->leg2) bwdLeg();
+                } else {
+                    if (c->leg1) bwdLeg();
+                    if (c->leg2) fwdLeg();
                 }
                 HIPC(launch_born_adj_finish(dSlots, nb, -1, base.v2dt2, c->refl.as<float>(), p.nz, p.nx, s));
+            } else {   // WEMVA forward: velocity perturbation -> extended image
+                std::vector<int> pend;
+                auto flush = [&](bool wemvaFwd) {
+                    if (pend.empty()) return;
+                    ExtBatchArgs e{};
+                    e.slots = wemvaFwd ? dSlotsJ : dSlotsI;
+                    e.v2dt2 = base.v2dt2;
+                    e.nz = p.nz; e.nx = p.nx; e.hExt = p.hExt;
+                    e.nk = (int)pend.size();
+                    for (int j = 0; j < e.nk; ++j) {
+                        const int k = pend[j];
+                        e.ssOff[j] = wemvaFwd ? (long)k * nzx : capOff(k);   // receiver side: R[k] / sample k
+                        e.wOff[j] = wemvaFwd ? capOff(k) : (long)k * nzx;    // source side: sample k / W''[k]
+                    }
+                    HIPC(launch_ext_img_batch(e, nb, s, wemvaFwd));
+                    pend.clear();
+                };
+                if (c->leg1) {   // forward from m * W'', imaged against R (batches of extK samples)
+                    forward_pass(MODE_TOMO_CAP, dSlotsC, SS_MUL, dSlots, mtil, 2, [&](int k) {
+                        pend.push_back(k);
+                        if ((int)pend.size() == extK || k == p.nts - 1) flush(true);
+                    });
+                }
+                if (c->leg2) {   // backward from m * R, imaged against W''
+                    backward_pass(SS_MUL, dSlotsQ, mtil, [&](int k) {
+                        pend.push_back(k);
+                        if ((int)pend.size() == extK || k == 0) flush(false);
+                    });
+                }
+                HIPC(launch_ext_adj_finish(dSlots, nb, c->refl.as<float>(), p.nz, p.nx, p.nExt, s));
             }
         }
 
@@ -1313,7 +1411,7 @@ void run(aiso_ctx* c, const Request& q)
             }
         }
         if (needImg) {
-            for (int k = 0; k < nb; ++k) HIPC(launch_accumulate(acc, img + (size_t)k * nExt * nzx, nzx * nExt, s));
+            for (int k = 0; k < nb; ++k) HIPC(launch_accumulate(acc, img + (size_t)k * nImg * nzx, nzx * nImg, s));
         }
         if (q.op == Op::NlAdj) {
             std::vector<float> fineAll((size_t)nb * nSrcRegMax * p.ntw);
@@ -1345,7 +1443,7 @@ void run(aiso_ctx* c, const Request& q)
     }
 
     // ---- outputs
-    if (needImg) HIPC(launch_add_to(dModelOut, acc, nzx * nExt, q.add, s));
+    if (needImg) HIPC(launch_add_to(dModelOut, acc, nzx * nImg, q.add, s));
     if (q.op == Op::NlAdj) {
         if (q.nShotSig == 1)
             for (size_t i = 0; i < nlAcc.size(); ++i) hModel[i] += nlAcc[i];
@@ -1353,8 +1451,9 @@ void run(aiso_ctx* c, const Request& q)
         else std::memcpy(q.modelOut, hModel.data(), modelN * 4);
     }
     if (q.mem == AISO_MEM_HOST) {
-        if (is_adj(q.op) && is_born(q.op)) HIPC(hipMemcpyAsync(q.modelOut, dModelOut, modelN * 4, hipMemcpyDeviceToHost, s));
-        if (!is_adj(q.op)) HIPC(hipMemcpyAsync(q.dataOut, dDataOut, dataN * 4, hipMemcpyDeviceToHost, s));
+        if ((is_adj(q.op) && is_born(q.op)) || wemva)
+            HIPC(hipMemcpyAsync(q.modelOut, dModelOut, mOutN * 4, hipMemcpyDeviceToHost, s));
+        if (!is_adj(q.op) && !wemva) HIPC(hipMemcpyAsync(q.dataOut, dDataOut, dataN * 4, hipMemcpyDeviceToHost, s));
     }
     HIPC(hipStreamSynchronize(s));
     HIPC(hipGetLastError());
@@ -1992,6 +2091,20 @@ aiso_status aiso_tomo_adjoint(aiso_ctx* c, int add, float* model, const float* d
                               int nShot, const float* wav, int nWav, aiso_device* const* rec, int nRec, int mem)
 {
     return op_call(c, Op::TomoAdj, add, nullptr, model, data, nullptr, 1, src, nShot, wav, nWav, rec, nRec, mem);
+}
+
+aiso_status aiso_wemva_forward(aiso_ctx* c, int add, const float* model, float* image, const float* data,
+                               aiso_device* const* src, int nShot, const float* wav, int nWav,
+                               aiso_device* const* rec, int nRec, int mem)
+{
+    return op_call(c, Op::WemvaFwd, add, model, image, data, nullptr, 1, src, nShot, wav, nWav, rec, nRec, mem);
+}
+
+aiso_status aiso_wemva_adjoint(aiso_ctx* c, int add, float* model, const float* image, const float* data,
+                               aiso_device* const* src, int nShot, const float* wav, int nWav,
+                               aiso_device* const* rec, int nRec, int mem)
+{
+    return op_call(c, Op::WemvaAdj, add, image, model, data, nullptr, 1, src, nShot, wav, nWav, rec, nRec, mem);
 }
 
 int aiso_pad_plus(int n, int pad, int blockSize) { return pad_plus(n, pad, blockSize); }

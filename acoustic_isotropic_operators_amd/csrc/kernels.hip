@@ -1350,11 +1350,13 @@ __global__ __launch_bounds__(256) void k_ext_ss_tile(ExtBatchArgs a, int nshots,
 // scaleSecondarySourceFd + imagingOffsetAdjGpu (F/src/kernelsGpu.cu:270-275,
 // 398-411): for each shot and each sample k in order,
 //   img[e][x] += (ss_k[x+h] * v2dt2[x+h]) * W_k[x-h],  h = e - hExt,
-// for x in [FAT+hExt, nx-FAT-hExt).  The thread's image values of a shot
-// stay in registers across the batch, so the extended image is read and
-// written once per batch; ss_k * v2dt2 and W_k pass through LDS, the next
-// pair's tiles fetched into registers while the current one is computed.
-template <int HX>
+// for x in [FAT+hExt, nx-FAT-hExt).  With WF (imagingOffsetWemvaScaleFwdGpu,
+// :429-443, the WEMVA forward) the product is (W_k[x-h] * ss_k[x+h]) *
+// v2dt2[x+h] instead.  The thread's image values of a shot stay in
+// registers across the batch, so the extended image is read and written
+// once per batch; the two operands pass through LDS, the next pair's tiles
+// fetched into registers while the current one is computed.
+template <int HX, bool WF>
 __global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nshots, int spb)
 {
     constexpr int NE = 2 * HX + 1;
@@ -1379,7 +1381,7 @@ __global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nshots
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
         const int j = 4 * u + g, gx = x0 - hExt + j;
-        vv[u] = (zok && j < lw && gx < a.nx) ? a.v2dt2[(long)gx * nz + z] : 0.f;
+        vv[u] = (!WF && zok && j < lw && gx < a.nx) ? a.v2dt2[(long)gx * nz + z] : 0.f;
     }
     auto fetch = [&](int it) {
         const ShotSlot& S = a.slots[sh0 + it / a.nk];
@@ -1395,6 +1397,14 @@ __global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nshots
     };
     const int jr = xb - x0;                      // R window entry i: column xb + i - hExt
     const int jw = xb - x0 - 2 * HX + 2 * hExt;  // W window entry i: column xb + i - 2*HX + hExt
+    float vw[WF ? RW : 1];                       // WF: v2dt2 on the R window
+    if constexpr (WF) {
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+            const int gx = xb + i - hExt;
+            vw[i] = (zok && gx < a.nx) ? a.v2dt2[(long)gx * nz + z] : 0.f;
+        }
+    }
     float m[NE][kExtCX];
     if (nIt > 0) fetch(0);
     for (int it = 0; it < nIt; ++it) {
@@ -1409,7 +1419,7 @@ __global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nshots
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
             if (4 * u + g < LW) {
-                Rl[(4 * u + g) * kExtTZ + tz] = pr[u] * vv[u];
+                Rl[(4 * u + g) * kExtTZ + tz] = WF ? pr[u] : pr[u] * vv[u];
                 Wl[(4 * u + g) * kExtTZ + tz] = pw[u];
             }
         }
@@ -1428,7 +1438,10 @@ __global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nshots
 #pragma unroll
             for (int c = 0; c < kExtCX; ++c)
                 // R at xb+c+h is entry c + e; W at xb+c-h is entry c - e + 2*HX
-                if (e < ne) m[e][c] += rw[c + e] * ww[c - e + 2 * HX];
+                if (e < ne) {
+                    if constexpr (WF) m[e][c] += (ww[c - e + 2 * HX] * rw[c + e]) * vw[c + e];
+                    else m[e][c] += rw[c + e] * ww[c - e + 2 * HX];
+                }
         if (it % a.nk == a.nk - 1) {
 #pragma unroll
             for (int e = 0; e < NE; ++e)
@@ -1461,6 +1474,7 @@ __global__ void k_ext_ss_any(ExtBatchArgs a)
     }
 }
 
+template <bool WF>
 __global__ void k_ext_img_any(ExtBatchArgs a)
 {
     const ShotSlot& S = a.slots[blockIdx.y];
@@ -1474,8 +1488,10 @@ __global__ void k_ext_img_any(ExtBatchArgs a)
         const long ir = (long)(ix + h) * nz + iz, iw = (long)(ix - h) * nz + iz;
         float* mp = S.img + nzx * e + (long)ix * nz + iz;
         float m = *mp;
-        for (int k = 0; k < a.nk; ++k)
-            m += (S.ss[0][a.ssOff[k] + ir] * a.v2dt2[ir]) * S.W[a.wOff[k] + iw];
+        for (int k = 0; k < a.nk; ++k) {
+            if constexpr (WF) m += (S.W[a.wOff[k] + iw] * S.ss[0][a.ssOff[k] + ir]) * a.v2dt2[ir];
+            else m += (S.ss[0][a.ssOff[k] + ir] * a.v2dt2[ir]) * S.W[a.wOff[k] + iw];
+        }
         *mp = m;
     }
 }
@@ -1846,22 +1862,29 @@ hipError_t launch_ext_ss_batch(const ExtBatchArgs& a, int nshots, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_ext_img_batch(const ExtBatchArgs& a, int nshots, hipStream_t s)
+template <bool WF>
+static void ext_img_launch(const ExtBatchArgs& a, int nshots, hipStream_t s, int nzi, int nxr)
+{
+    if (a.hExt <= 20) {
+        const int spb = 1;
+        dim3 g((nxr + kExtTX - 1) / kExtTX, (nshots + spb - 1) / spb, (nzi + kExtTZ - 1) / kExtTZ), b(kExtTZ, 4);
+        if (a.hExt <= 2) hipLaunchKernelGGL((k_ext_img_tile<2, WF>), g, b, 0, s, a, nshots, spb);
+        else if (a.hExt <= 5) hipLaunchKernelGGL((k_ext_img_tile<5, WF>), g, b, 0, s, a, nshots, spb);
+        else if (a.hExt <= 10) hipLaunchKernelGGL((k_ext_img_tile<10, WF>), g, b, 0, s, a, nshots, spb);
+        else hipLaunchKernelGGL((k_ext_img_tile<20, WF>), g, b, 0, s, a, nshots, spb);
+    } else {
+        dim3 g((nzi + 63) / 64, nshots, nxr);
+        hipLaunchKernelGGL(k_ext_img_any<WF>, g, dim3(64), 0, s, a);
+    }
+}
+
+hipError_t launch_ext_img_batch(const ExtBatchArgs& a, int nshots, hipStream_t s, bool wemvaFwd)
 {
     if (a.nk < 1 || a.nk > kExtBatchMax) return hipErrorInvalidValue;
     const int nzi = a.nz - 2 * FAT, nxr = a.nx - 2 * FAT - 2 * a.hExt;
     if (nzi <= 0 || nxr <= 0) return hipSuccess;
-    if (a.hExt <= 20) {
-        const int spb = 1;
-        dim3 g((nxr + kExtTX - 1) / kExtTX, (nshots + spb - 1) / spb, (nzi + kExtTZ - 1) / kExtTZ), b(kExtTZ, 4);
-        if (a.hExt <= 2) hipLaunchKernelGGL(k_ext_img_tile<2>, g, b, 0, s, a, nshots, spb);
-        else if (a.hExt <= 5) hipLaunchKernelGGL(k_ext_img_tile<5>, g, b, 0, s, a, nshots, spb);
-        else if (a.hExt <= 10) hipLaunchKernelGGL(k_ext_img_tile<10>, g, b, 0, s, a, nshots, spb);
-        else hipLaunchKernelGGL(k_ext_img_tile<20>, g, b, 0, s, a, nshots, spb);
-    } else {
-        dim3 g((nzi + 63) / 64, nshots, nxr);
-        hipLaunchKernelGGL(k_ext_img_any, g, dim3(64), 0, s, a);
-    }
+    if (wemvaFwd) ext_img_launch<true>(a, nshots, s, nzi, nxr);
+    else ext_img_launch<false>(a, nshots, s, nzi, nxr);
     return hipGetLastError();
 }
 

@@ -191,7 +191,8 @@ hipError_t launch_scale_ext(const float* m, float* out, const float* refl, int n
 hipError_t launch_born_adj_finish(const ShotSlot* slots, int nshots, int ssR, const float* v2dt2,
                                   const float* refl, int nz, int nx, hipStream_t s);
 hipError_t launch_ext_ss_batch(const ExtBatchArgs& a, int nshots, hipStream_t s);
-hipError_t launch_ext_img_batch(const ExtBatchArgs& a, int nshots, hipStream_t s);
+// wemvaFwd: the WEMVA forward's imaging, (W_k[x-h] * ss_k[x+h]) * v2dt2[x+h]
+hipError_t launch_ext_img_batch(const ExtBatchArgs& a, int nshots, hipStream_t s, bool wemvaFwd = false);
 hipError_t launch_ext_adj_finish(const ShotSlot* slots, int nshots, const float* refl, int nz, int nx,
                                  int nExt, hipStream_t s);
 // Tomographic operator: W <- second time derivative of W (nts coarse

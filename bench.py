@@ -46,10 +46,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-shots", type=int, default=1)
     ap.add_argument("--opt", action="append", default=[], help="engine option key=value (aiso_ctx_set_option)")
-    ap.add_argument("--workload", default="born", choices=["born", "bornext21", "tomo21"],
+    ap.add_argument("--workload", default="born", choices=["born", "bornext21", "tomo21", "wemva21"],
                     help="born: the headline Born fwd+adj; bornext21: subsurface-offset extended Born "
                          "fwd+adj with 21 offsets on the same grid and shots; tomo21: the offset-extended "
-                         "tomographic operator fwd+adj about a 21-offset reflectivity")
+                         "tomographic operator fwd+adj about a 21-offset reflectivity; wemva21: the "
+                         "offset-extended WEMVA operator fwd+adj (21 offsets)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy 1 GiB device-to-device 3 times (HBM counter calibration)")
     return ap.parse_args()
@@ -102,8 +103,8 @@ def main():
 
     if a.workload == "bornext21":
         return bench_ext(a, torch, A, born_lsrtm)
-    if a.workload == "tomo21":
-        return bench_tomo(a, torch, A, born_lsrtm)
+    if a.workload in ("tomo21", "wemva21"):
+        return bench_tomo(a, torch, A, born_lsrtm, wemva=a.workload == "wemva21")
     w = born_lsrtm(nShot=a.shots)
     p = w.params
     ctx = A.Context(p)
@@ -246,11 +247,11 @@ def bench_ext(a, torch, A, born_lsrtm, nExt=21):
     print(json.dumps(out), flush=True)
 
 
-def bench_tomo(a, torch, A, born_lsrtm, nExt=21):
-    """Offset-extended tomographic operator (both legs) forward + adjoint on
-    the headline grid: one JSON line.  Point-updates count the propagations
-    per shot: source + 4 forward (two per leg); source + receiver + 2
-    adjoint."""
+def bench_tomo(a, torch, A, born_lsrtm, nExt=21, wemva=False):
+    """Offset-extended tomographic (or WEMVA) operator, both legs, forward +
+    adjoint on the headline grid: one JSON line.  Point-updates count the
+    propagations per shot: tomographic source + 4 forward (two per leg),
+    source + receiver + 2 adjoint; WEMVA source + receiver + 2 each way."""
     w = born_lsrtm(nShot=a.shots)
     p = dict(w.params, nExt=nExt, extension=A.EXT_OFFSET)
     ctx = A.Context(p)
@@ -267,11 +268,18 @@ def bench_tomo(a, torch, A, born_lsrtm, nExt=21):
     dv = torch.from_numpy((w.vel * 0.01).astype(np.float32)).to(dev)
     data = torch.empty((w.nShot, rec[0].nDev, p["nts"]), dtype=torch.float32, device=dev)
     grad = torch.empty_like(dv)
+    if wemva:   # observed data: the Born data of the reflectivity
+        ctx.born_forward(torch.from_numpy(w.model).to(dev), data, srcs, w.wavelet, rec, mem=A.MEM_DEVICE)
+        image = torch.empty((nExt,) + tuple(dv.shape), dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
 
     def step():
-        ctx.tomo_forward(dv, data, srcs, w.wavelet, rec, mem=A.MEM_DEVICE)
-        ctx.tomo_adjoint(grad, data, srcs, w.wavelet, rec, mem=A.MEM_DEVICE)
+        if wemva:
+            ctx.wemva_forward(dv, image, data, srcs, w.wavelet, rec, mem=A.MEM_DEVICE)
+            ctx.wemva_adjoint(grad, image, data, srcs, w.wavelet, rec, mem=A.MEM_DEVICE)
+        else:
+            ctx.tomo_forward(dv, data, srcs, w.wavelet, rec, mem=A.MEM_DEVICE)
+            ctx.tomo_adjoint(grad, data, srcs, w.wavelet, rec, mem=A.MEM_DEVICE)
 
     for _ in range(a.warmup):
         step()
@@ -283,11 +291,13 @@ def bench_tomo(a, torch, A, born_lsrtm, nExt=21):
     torch.cuda.synchronize()
     t = (time.perf_counter() - t0) / a.steps
     stats = {n: ctx.profile_read(getattr(A, n)) for n in ("K_SRC", "K_TOMO")}
-    pts = 9.0 * w.fine_steps * w.interior * w.nShot
-    out = {"metric": f"Gpts/s, tomographic fwd+adj nExt={nExt}, {w.nShot} shots 500x1200", "value": pts / t / 1e9,
+    pts = (8.0 if wemva else 9.0) * w.fine_steps * w.interior * w.nShot
+    name = "WEMVA" if wemva else "tomographic"
+    out = {"metric": f"Gpts/s, {name} fwd+adj nExt={nExt}, {w.nShot} shots 500x1200", "value": pts / t / 1e9,
            "unit": "Gpts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": t * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"tomo{nExt}_{w.nShot}x500x1200", "nExt": nExt, "shots": w.nShot,
+           "config": {"workload": f"{'wemva' if wemva else 'tomo'}{nExt}_{w.nShot}x500x1200", "nExt": nExt,
+                      "shots": w.nShot,
                       "shots_per_launch": ctx.info()["batch"]},
            "step_kernels_ms_per_step": {n: v["ms"] / a.steps for n, v in stats.items() if v["launches"]},
            "finite": bool(torch.isfinite(grad).all().item()) and bool(torch.isfinite(data).all().item())}

@@ -244,6 +244,24 @@ aiso_status aiso_tomo_adjoint(aiso_ctx* ctx, int add, float* model, const float*
                               const float* wavelets, int nWavelet,
                               aiso_device* const* receivers, int nRecGeom, int mem);
 
+/* ------------------------------------------------- WEMVA (extension=offset) */
+/* wemvaExtShotsGpu::forward / adjoint (F/include/wemvaExtShotsGpu.h:36-57,
+ * F/src/wemvaExtShotsGpu.cpp, F/src/wemvaExtShotsGpuFunctions.cu:458-808):
+ * velocity perturbation model [nx][nz] <-> extended image [nExt][nx][nz],
+ * linearized about the receiver wavefields of the observed data
+ * [nShot][nRec][nts] (the reference's receiversSignalsVector, passed with
+ * every call).  The forward writes (add: accumulates into) image, the
+ * adjoint model.  Legs by the options "leg1" / "leg2"; needs extension =
+ * AISO_EXT_OFFSET (the time-lag form is not supported). */
+aiso_status aiso_wemva_forward(aiso_ctx* ctx, int add, const float* model, float* image, const float* data,
+                               aiso_device* const* sources, int nShot,
+                               const float* wavelets, int nWavelet,
+                               aiso_device* const* receivers, int nRecGeom, int mem);
+aiso_status aiso_wemva_adjoint(aiso_ctx* ctx, int add, float* model, const float* image, const float* data,
+                               aiso_device* const* sources, int nShot,
+                               const float* wavelets, int nWavelet,
+                               aiso_device* const* receivers, int nRecGeom, int mem);
+
 /* ---------------------------------------------------------------- misc */
 /* padFileGpuMain padding rule (F/src/padFileGpuMain.cpp:49-66, no free
  * surface): returns padPlus for a side of n samples with pad on the minus side. */

@@ -71,3 +71,38 @@ def test_tomo_add_and_dot_product():
     b = float(np.vdot(m.astype(np.float64).ravel(), ma.astype(np.float64).ravel()))
     print(f"<Fm,d> {a:.8e} <m,F'd> {b:.8e} rel {abs(a - b) / abs(b):.2e}")
     assert abs(a - b) <= 1e-5 * abs(b)
+
+
+@pytest.mark.parametrize("sub,nExt,extBatch", [(2, 5, 16), (3, 3, 4), (2, 21, 5), (2, 43, 3)])
+def test_wemva_matches_oracle(sub, nExt, extBatch):
+    """WEMVA (aiso_wemva_forward / aiso_wemva_adjoint), both legs and each
+    alone, bitwise equal to the oracle; nExt = 43 takes the untiled
+    imaging kernel."""
+    c = make_case(seed=85, nShot=3, nts=23, sub=sub, dts=0.001 * sub, nExt=nExt)
+    ctx, srcs, recs = _engine(c)
+    ctx.set_option("extBatch", extBatch)
+    g, os_, or_ = c.grid, c.oracle_sources(), c.oracle_receivers()
+    wav = c.wavelet[None, :].astype(np.float32)
+    rng = np.random.default_rng(86)
+    dobs = rng.standard_normal((c.nShot, recs[0].nDev, c.nts)).astype(np.float32)
+    m = rng.standard_normal((c.nx, c.nz)).astype(np.float32)
+    im = rng.standard_normal((nExt, c.nx, c.nz)).astype(np.float32)
+    for legs in [(1, 1), (1, 0), (0, 1)]:
+        ctx.set_option("leg1", legs[0])
+        ctx.set_option("leg2", legs[1])
+        of = O.wemva_forward(g, c.vel, os_, [wav], or_, dobs, m, *legs)
+        af = O.wemva_adjoint(g, c.vel, os_, [wav], or_, dobs, im, *legs)
+        out = np.zeros_like(im)
+        ctx.wemva_forward(m, out, dobs, srcs, wav, recs)
+        ma = np.zeros_like(m)
+        ctx.wemva_adjoint(ma, im, dobs, srcs, wav, recs)
+        print(f"sub {sub} nExt {nExt} legs {legs}: fwd relL2 {rel_l2(out, of):.2e} adj relL2 {rel_l2(ma, af):.2e}",
+              flush=True)
+        assert np.abs(of).max() > 0 and np.abs(af).max() > 0
+        np.testing.assert_array_equal(out, of)
+        np.testing.assert_array_equal(ma, af)
+    ctx.set_option("leg1", 1)
+    ctx.set_option("leg2", 1)
+    out2 = np.ones_like(im)
+    ctx.wemva_forward(m, out2, dobs, srcs, wav, recs, add=True)
+    np.testing.assert_array_equal(out2, np.ones_like(im) + O.wemva_forward(g, c.vel, os_, [wav], or_, dobs, m))
