@@ -316,7 +316,7 @@ def tomoExtOpInitFloat(args, client=None):
         refl = sep.getSepVector(axes=[zAxis, xAxis, extAxis])
         refl.scale(0.0)
     else:
-        refl = sep.read(reflFile)
+        refl = sep.read(reflFile, ndims=3)
     model = sep.getSepVector(vel.getHyper())
     sources, srcAxis = buildSourceGeometry(par, vel)
     receivers, recAxis = buildReceiversGeometry(par, vel)
@@ -361,7 +361,7 @@ def wemvaExtOpInitFloat(args):
     dataFile = par.getString("seismicData", "noWemvaDataFile")
     if dataFile == "noWemvaDataFile":
         raise ValueError("**** ERROR: User did not provide wemva seismic data file ****")
-    allData = sep.read(dataFile).getNdArray()
+    allData = sep.read(dataFile, ndims=3).getNdArray()
     recSignals = []
     for s in range(srcAxis.n):
         sl = sep.getSepVector(axes=[time, recAxis])

@@ -238,13 +238,21 @@ def read(path, ndims=None):
             ns.append(1); os_.append(0.0); ds.append(1.0)
         while len(ns) > ndims and ns[-1] == 1:
             ns.pop(); os_.pop(); ds.pop()
+    else:   # trailing axes of length 1 (n3=1 ... n9=1 in history blocks) are not dimensions
+        while len(ns) > 1 and ns[-1] == 1:
+            ns.pop(); os_.pop(); ds.pop()
+    # The last in= of the history wins; a relative path is taken next to the header.
     binpath = kv.get("in", path + "@")
     if not os.path.isabs(binpath) and not os.path.exists(binpath):
         binpath = os.path.join(os.path.dirname(path), os.path.basename(binpath))
     fmt = kv.get("data_format", "xdr_float")
-    dt = ">f4" if "xdr" in fmt else "<f4"
+    if fmt not in ("xdr_float", "native_float") or int(kv.get("esize", 4)) != 4:
+        raise ValueError(f"{path}: only 4-byte xdr_float / native_float SEP data are supported (data_format={fmt})")
+    dt = ">f4" if fmt == "xdr_float" else "<f4"
     count = int(np.prod(ns))
     arr = np.fromfile(binpath, dtype=dt, count=count).astype(np.float32)
+    if arr.size != count:
+        raise ValueError(f"{binpath}: holds {arr.size} samples, the header describes {count}")
     return FloatVector(hypercube(ns=ns, os_=os_, ds=ds), arr)
 
 
