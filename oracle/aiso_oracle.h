@@ -30,6 +30,7 @@ typedef struct {
     int minPad;      /* min(zPadMinus, zPadPlus, xPadMinus, xPadPlus)      */
     int nExt;        /* extension length (odd), 1 = none                  */
     double dz, dx, dts, ots, alphaCos;
+    int freeSurface; /* 1: free surface at iz = FAT (no top sponge)        */
 } orc_grid;
 
 #define ORC_DECL(REAL, SFX)                                                          \
@@ -37,7 +38,7 @@ void orc_tables##SFX(const orc_grid* g, REAL* zc, REAL* xc, REAL* filt, REAL* cd
 void orc_vel_scales##SFX(const orc_grid* g, const REAL* vel, REAL* v2dt2,           \
                          REAL* refl);                                                \
 void orc_step_fwd##SFX(const orc_grid* g, const REAL* zc, const REAL* xc,           \
-                       REAL* o, const REAL* c, REAL* n, const REAL* v2dt2);          \
+                       REAL* o, REAL* c, REAL* n, const REAL* v2dt2);                \
 void orc_step_adj##SFX(const orc_grid* g, const REAL* zc, const REAL* xc,           \
                        REAL* o, const REAL* c, REAL* n, const REAL* v2dt2,           \
                        REAL* tmp);                                                   \

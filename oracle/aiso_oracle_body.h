@@ -62,11 +62,29 @@ void FN(orc_vel_scales)(const orc_grid* g, const REAL* vel, REAL* v2dt2, REAL* r
             long i = (long)ix * (g)->nz + iz;
 #define INTERIOR_END }}
 
-/* stepFwdGpu, F/src/kernelsGpu.cu:513-552.  In place when o == n. */
-void FN(orc_step_fwd)(const orc_grid* g, const REAL* zc, const REAL* xc,
-                      REAL* o, const REAL* c, REAL* n, const REAL* v2dt2)
+/* setFsConditionFwdGpu, F/src/kernelsGpu.cu:555-567: on the current slice,
+ * before each forward step with a free surface, the surface row (iz = FAT)
+ * is zeroed and the FAT rows above it take the opposite of their mirror
+ * images below it. */
+static void FN(set_fs)(const orc_grid* g, REAL* c)
 {
     const long nz = g->nz;
+    for (int ix = FAT; ix < g->nx - FAT; ++ix) {
+        REAL* col = c + (long)ix * nz;
+        col[FAT] = 0;
+        for (int k = 0; k < FAT; ++k) col[k] = -col[2 * FAT - k];
+    }
+}
+
+/* stepFwdGpu, F/src/kernelsGpu.cu:513-552.  In place when o == n.  With a
+ * free surface the current slice is first conditioned by set_fs, in place,
+ * as the reference's drivers do before every forward step
+ * (e.g. F/src/BornShotsGpuFunctions.cu:579-583). */
+void FN(orc_step_fwd)(const orc_grid* g, const REAL* zc, const REAL* xc,
+                      REAL* o, REAL* c, REAL* n, const REAL* v2dt2)
+{
+    const long nz = g->nz;
+    if (g->freeSurface) FN(set_fs)(g, c);
     INTERIOR_BEGIN(g)
         REAL lap = zc[0] * c[i]
                  + zc[1] * (c[i - 1] + c[i + 1])
@@ -85,21 +103,33 @@ void FN(orc_step_fwd)(const orc_grid* g, const REAL* zc, const REAL* xc,
 }
 
 /* stepAdjGpu, F/src/kernelsGpu.cu:574-610: the Laplacian acts on v2dt2*c
- * (formed on the halo too).  In place when o == n.  tmp holds nz*nx values. */
+ * (formed on the halo too).  In place when o == n.  tmp holds nz*nx values.
+ * With a free surface, stepAdjFsGpu (:613-700): the surface row is left as
+ * it is and the four rows below it drop the z terms that the forward's
+ * mirror folds onto them. */
 void FN(orc_step_adj)(const orc_grid* g, const REAL* zc, const REAL* xc,
                       REAL* o, const REAL* c, REAL* n, const REAL* v2dt2, REAL* tmp)
 {
     const long nz = g->nz, nzx = (long)g->nz * g->nx;
+    const int fs = g->freeSurface;
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < nzx; ++i) tmp[i] = c[i] * v2dt2[i];
     const REAL* s = tmp;
     INTERIOR_BEGIN(g)
+        if (fs && iz == FAT) continue;
         REAL lap = zc[0] * s[i]
                  + zc[1] * (s[i - 1] + s[i + 1])
                  + zc[2] * (s[i - 2] + s[i + 2])
                  + zc[3] * (s[i - 3] + s[i + 3])
                  + zc[4] * (s[i - 4] + s[i + 4])
-                 + zc[5] * (s[i - 5] + s[i + 5])
+                 + zc[5] * (s[i - 5] + s[i + 5]);
+        if (fs) {
+            if (iz == FAT + 1) lap = lap - zc[2] * s[i] - zc[3] * s[i + 1] - zc[4] * s[i + 2] - zc[5] * s[i + 3];
+            else if (iz == FAT + 2) lap = lap - zc[3] * s[i - 1] - zc[4] * s[i] - zc[5] * s[i + 1];
+            else if (iz == FAT + 3) lap = lap - zc[4] * s[i - 2] - zc[5] * s[i - 1];
+            else if (iz == FAT + 4) lap = lap - zc[5] * s[i - 3];
+        }
+        lap = lap
                  + xc[0] * s[i]
                  + xc[1] * (s[i + nz] + s[i - nz])
                  + xc[2] * (s[i + 2 * nz] + s[i - 2 * nz])
@@ -110,12 +140,14 @@ void FN(orc_step_adj)(const orc_grid* g, const REAL* zc, const REAL* xc,
     INTERIOR_END
 }
 
-/* dampCosineEdge, F/src/kernelsGpu.cu:202-219: both slices are damped. */
+/* dampCosineEdge, F/src/kernelsGpu.cu:202-219: both slices are damped.
+ * With a free surface (dampCosineEdgeFs, :221-239) there is no sponge at
+ * the top. */
 void FN(orc_damp)(const orc_grid* g, const REAL* cd, REAL* p1, REAL* p2)
 {
-    const int nz = g->nz, nx = g->nx, minPad = g->minPad;
+    const int nz = g->nz, nx = g->nx, minPad = g->minPad, fs = g->freeSurface;
     INTERIOR_BEGIN(g)
-        int a = iz - FAT, b = ix - FAT, c = nz - iz - 1 - FAT, d = nx - ix - 1 - FAT;
+        int a = fs ? ix - FAT : iz - FAT, b = ix - FAT, c = nz - iz - 1 - FAT, d = nx - ix - 1 - FAT;
         int m1 = a < b ? a : b, m2 = c < d ? c : d;
         int dist = m1 < m2 ? m1 : m2;
         if (dist < minPad) {

@@ -51,7 +51,7 @@ class _Grid(C.Structure):
     _fields_ = [("nz", C.c_int), ("nx", C.c_int), ("nts", C.c_int), ("sub", C.c_int),
                 ("minPad", C.c_int), ("nExt", C.c_int), ("dz", C.c_double),
                 ("dx", C.c_double), ("dts", C.c_double), ("ots", C.c_double),
-                ("alphaCos", C.c_double)]
+                ("alphaCos", C.c_double), ("freeSurface", C.c_int)]
 
 
 @dataclass
@@ -68,6 +68,7 @@ class Grid:
     ots: float = 0.0
     alphaCos: float = 0.99
     nExt: int = 1
+    freeSurface: int = 0
 
     @property
     def ntw(self):
@@ -75,7 +76,7 @@ class Grid:
 
     def c(self):
         return _Grid(self.nz, self.nx, self.nts, self.sub, self.minPad, self.nExt,
-                     self.dz, self.dx, self.dts, self.ots, self.alphaCos)
+                     self.dz, self.dx, self.dts, self.ots, self.alphaCos, self.freeSurface)
 
 
 def _dt(prec):
