@@ -67,6 +67,7 @@ _SIGS = {
     "aiso_wemva_forward": (_I, [_P, _I, _P, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_wemva_adjoint": (_I, [_P, _I, _P, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_pad_plus": (_I, [_I, _I, _I]),
+    "aiso_pad_plus_free_surface": (_I, [_I, _I, _I]),
 }
 
 _LIB = None
@@ -106,6 +107,10 @@ def _ptr(x):
 
 def pad_plus(n, pad, block=16):
     return lib().aiso_pad_plus(n, pad, block)
+
+
+def pad_plus_free_surface(nz, zPad, block=16):
+    return lib().aiso_pad_plus_free_surface(nz, zPad, block)
 
 
 class Device:

@@ -453,6 +453,17 @@ void run(aiso_ctx* c, const Request& q)
     for (int i = 0; i < q.nRec; ++i)
         for (int v : q.rec[i]->g.gridIdx)
             if (v >= p.nzx()) throw Error(AISO_ERR_GEOM, "receiver outside the velocity grid");
+    if (p.freeSurface) {   // the surface condition rewrites the rows above it every step
+        auto above = [&](const aiso_device* d) {
+            for (size_t k = 0; k < d->g.gridIdx.size(); ++k)
+                if (d->g.weight[k] != 0.f && d->g.gridIdx[k] % p.nz < kFat) return true;
+            return false;
+        };
+        for (int i = 0; i < q.nShot; ++i)
+            if (above(q.src[i])) throw Error(AISO_ERR_GEOM, "with a free surface, sources must lie at or below the surface row");
+        for (int i = 0; i < q.nRec; ++i)
+            if (above(q.rec[i])) throw Error(AISO_ERR_GEOM, "with a free surface, receivers must lie at or below the surface row");
+    }
 
     hipStream_t s = c->stream;
     if (q.mem == AISO_MEM_DEVICE) wait_default_stream(c);
@@ -528,7 +539,10 @@ void run(aiso_ctx* c, const Request& q)
     // ---- batch size from the HBM budget
     // The Born forward with sub == 2 runs its two propagations in lockstep:
     // the secondary source then lives in a 3-slice ring instead of nts slices.
-    const bool lockstep = q.op == Op::BornFwd && p.sub == 2 && c->lockstep;
+    // A free surface runs on the generic step kernel, without the window
+    // capture, the lockstep pair or two steps per launch.
+    const bool fsurf = p.freeSurface != 0;
+    const bool lockstep = q.op == Op::BornFwd && p.sub == 2 && c->lockstep && !fsurf;
     // Two fine steps per launch (k_fwd_pair2) when no source or receiver
     // point lies in the FAT ring, whose values that kernel takes as 0.
     // The Born adjoint with sub == 2 likewise runs both of its passes two
@@ -537,7 +551,7 @@ void run(aiso_ctx* c, const Request& q)
     for (int i = 0; noRing && i < q.nShot; ++i) noRing = srcG[i]->nRing == 0;
     for (int i = 0; noRing && i < q.nRec; ++i) noRing = recG[i]->nRing == 0;
     const bool pair2 = lockstep && c->pair2 && noRing;
-    const bool adj2 = q.op == Op::BornAdj && p.sub == 2 && c->pair2 && noRing;
+    const bool adj2 = q.op == Op::BornAdj && p.sub == 2 && c->pair2 && noRing && !fsurf;
     const int pairSlices = pair2 || adj2 ? 4 : 2;
     const long wSlices = lockstep ? 3 : p.nts;
     const bool needW = is_born(q.op), needSS = is_ext(q.op) || is_tomo(q.op) || wemva || q.op == Op::BornAdj;
@@ -653,6 +667,7 @@ void run(aiso_ctx* c, const Request& q)
     for (int k = 0; k < 6; ++k) { base.zc[k] = p.zc[k]; base.xc[k] = p.xc[k]; }
     base.wA = base.wB = base.wImg = 0;
     base.ntW = c->streamNT;
+    base.fs = p.freeSurface;
 
     std::vector<float> sigHost;
     for (int b0 = 0; b0 < q.nShot; b0 += B) {
@@ -767,12 +782,12 @@ void run(aiso_ctx* c, const Request& q)
         // Three consecutive fine slices hold everything a coarse sample needs
         // when sub == 2, so captures and imaging are written once per coarse
         // step instead of accumulated at every fine step.
-        const bool win = p.sub == 2;
+        const bool win = p.sub == 2 && !fsurf;
         // The extended-Born operators step on the unspecialized kernel: the
         // specialized one ended a GPU run in a memory fault on this path with
         // no cause found in its source (it is exact and in bounds when the
         // same source runs on the host).
-        const bool generic = is_ext(q.op) || tomo;
+        const bool generic = is_ext(q.op) || tomo || fsurf;
 
         if (pair2) {
             // The scattered field lags the source field by one launch (two
@@ -1498,6 +1513,7 @@ void run_nl_grad(aiso_ctx* c, const float* model, int nShotSig, const float* dat
     if (nShotSig != 1 && nShotSig != nShot) throw Error(AISO_ERR_PARAM, "model shot axis must be 1 or nShot");
     if (p.sub >= kSubMax) throw Error(AISO_ERR_PARAM, "**** ERROR: Subsampling parameter too high ****");
     if (every < 0) throw Error(AISO_ERR_PARAM, "checkpoint interval must be >= 0");
+    if (p.freeSurface) throw Error(AISO_ERR_PARAM, "the velocity gradient does not support a free surface");
     const int nRecDev = recs[0]->g.nDev, nSrcDev = srcs[0]->g.nDev;
     for (int i = 0; i < nRec; ++i)
         if (recs[i]->g.nDev != nRecDev) throw Error(AISO_ERR_GEOM, "receivers per shot must be constant");
@@ -2108,5 +2124,6 @@ aiso_status aiso_wemva_adjoint(aiso_ctx* c, int add, float* model, const float* 
 }
 
 int aiso_pad_plus(int n, int pad, int blockSize) { return pad_plus(n, pad, blockSize); }
+int aiso_pad_plus_free_surface(int nz, int zPad, int blockSize) { return pad_plus_free_surface(nz, zPad, blockSize); }
 
 }  // extern "C"

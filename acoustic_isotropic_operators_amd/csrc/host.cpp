@@ -26,10 +26,11 @@ FdParams make_params(const aiso_fd_params& a)
     if (a.sub < 1) fail(AISO_ERR_PARAM, "sub must be at least 1");
     if (a.dts <= 0.f || a.dz <= 0.f || a.dx <= 0.f) fail(AISO_ERR_PARAM, "dts, dz and dx must be positive");
     if (a.fat != kFat) fail(AISO_ERR_PARAM, "fat must be 5 for the 10th-order stencil");
-    // fdParam.cpp:41-45 (free surface flag vs top padding); only the
-    // absorbing top is implemented.
-    if (a.freeSurface == 1) fail(AISO_ERR_PARAM, "free surface is not supported by this engine");
-    if (a.zPadMinus == 0) fail(AISO_ERR_PARAM, "[fdParam]: zPadMinus and freeSurface flag are inconsistent");
+    // fdParam.cpp:29-45: a free surface sits on the first interior row and
+    // comes with no top padding; an absorbing top needs some.
+    p.freeSurface = a.freeSurface == 1;
+    if ((a.zPadMinus == 0 && !p.freeSurface) || (a.zPadMinus > 0 && p.freeSurface))
+        fail(AISO_ERR_PARAM, "[fdParam]: zPadMinus and freeSurface flag are inconsistent");
     if (a.zPadMinus < 0 || a.zPadPlus < 0 || a.xPadMinus < 0 || a.xPadPlus < 0)
         fail(AISO_ERR_PARAM, "padding must be non-negative");
     // fdParam.cpp:69-70
@@ -41,7 +42,8 @@ FdParams make_params(const aiso_fd_params& a)
     p.ntw = (p.nts - 1) * p.sub + 1;
     p.dtw = p.dts / float(p.sub);
     // fdParam.cpp:33-36,57,93
-    int zPad = std::min(p.zPadMinus, p.zPadPlus), xPad = std::min(p.xPadMinus, p.xPadPlus);
+    int zPad = p.freeSurface ? p.zPadPlus : std::min(p.zPadMinus, p.zPadPlus);
+    int xPad = std::min(p.xPadMinus, p.xPadPlus);
     p.minPad = std::min(zPad, xPad);
     // BornShotsGpuFunctions.cu:155,171,185
     if (p.sub > kSubMax) fail(AISO_ERR_PARAM, "Subsampling parameter is too high");
@@ -344,6 +346,14 @@ int pad_plus(int n, int pad, int blockSize)
     long total = (long)pad * 2 + n;
     float ratio = std::ceil(float(total) / float(blockSize));
     return int((long)ratio * blockSize - n - pad);
+}
+
+// F/src/padFileGpuMain.cpp:37-47 (free surface: no top padding, all of it below).
+int pad_plus_free_surface(int nz, int zPad, int blockSize)
+{
+    long total = (long)zPad + nz;
+    float ratio = std::ceil(float(total) / float(blockSize));
+    return int((long)ratio * blockSize - nz);
 }
 
 }  // namespace aiso

@@ -33,6 +33,7 @@ struct FdParams {
     int minPad = 0;
     float alphaCos = 0.99f, fMax = 1000.f;
     int nExt = 1, hExt = 0, extension = AISO_EXT_NONE;
+    int freeSurface = 0;       // 1: free surface on the first interior row, no top sponge
 
     // Derived tables, computed like initBornGpu (F/src/BornShotsGpuFunctions.cu:131-182).
     float zc[6] = {0}, xc[6] = {0};
@@ -82,5 +83,6 @@ void scale_source(const FdParams& p, const float* vel, const std::vector<int>& p
                   int nt, float* sig);
 
 int pad_plus(int n, int pad, int blockSize);
+int pad_plus_free_surface(int nz, int zPad, int blockSize);
 
 }  // namespace aiso

@@ -56,6 +56,14 @@ __device__ __forceinline__ float sponge(const float* __restrict__ cd, int gz, in
     return dist < minPad ? cd[dist] : 1.0f;
 }
 
+// dampCosineEdgeFs (F/src/kernelsGpu.cu:221-239): no sponge at a free surface.
+__device__ __forceinline__ float sponge_fs(const float* __restrict__ cd, int gz, int gx,
+                                           int nz, int nx, int minPad)
+{
+    int dist = imin(gx - FAT, imin(nz - gz - 1 - FAT, nx - gx - 1 - FAT));
+    return dist < minPad ? cd[dist] : 1.0f;
+}
+
 // Global-memory views: pointers read from the ShotSlot table are generic to
 // the compiler; casting them lets it emit global (not flat) loads and stores
 // with a uniform 64-bit base and a 32-bit per-lane offset.
@@ -998,6 +1006,26 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
         }
     }
     __syncthreads();
+    if constexpr (!ADJ) {
+        // Free surface (setFsConditionFwdGpu, F/src/kernelsGpu.cu:555-567), on
+        // the staged copy of a top tile's interior columns: the surface row
+        // is zero and the rows above it mirror the rows below with the
+        // opposite sign.  The stored slice keeps its surface value; the
+        // old-slice read below takes it as zero, as the reference's slice
+        // holds it after the condition.
+        if (a.fs && z0 == FAT) {
+            for (int lx = wy; lx < LX; lx += NXW) {
+                const int gx = x0 - FAT + lx;
+                if (gx >= FAT && gx < nx - FAT && lane == 0) {
+                    float* col = &sC[lx * LZ];
+                    col[FAT] = 0.f;
+#pragma unroll
+                    for (int k = 0; k < FAT; ++k) col[k] = -col[2 * FAT - k];
+                }
+            }
+            __syncthreads();
+        }
+    }
 
     const int gz = z0 + lane;
     const int lx0 = wy * PX;
@@ -1015,27 +1043,41 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
         ok[j] = zok && gx < nx - FAT;
         const float* s = &sC[(lx0 + j + FAT) * LZ + lane + FAT];
         const float cc = w[j + FAT];
-        const float lap = a.zc[0] * cc
-                        + a.zc[1] * (s[-1] + s[1])
-                        + a.zc[2] * (s[-2] + s[2])
-                        + a.zc[3] * (s[-3] + s[3])
-                        + a.zc[4] * (s[-4] + s[4])
-                        + a.zc[5] * (s[-5] + s[5])
-                        + a.xc[0] * cc
-                        + a.xc[1] * (w[j + 6] + w[j + 4])
-                        + a.xc[2] * (w[j + 7] + w[j + 3])
-                        + a.xc[3] * (w[j + 8] + w[j + 2])
-                        + a.xc[4] * (w[j + 9] + w[j + 1])
-                        + a.xc[5] * (w[j + 10] + w[j + 0]);
+        float lap = a.zc[0] * cc
+                  + a.zc[1] * (s[-1] + s[1])
+                  + a.zc[2] * (s[-2] + s[2])
+                  + a.zc[3] * (s[-3] + s[3])
+                  + a.zc[4] * (s[-4] + s[4])
+                  + a.zc[5] * (s[-5] + s[5]);
+        if constexpr (ADJ) {
+            // stepAdjFsGpu (F/src/kernelsGpu.cu:613-700): below a free
+            // surface the four rows drop the z terms the forward folds onto them.
+            if (a.fs && gz <= FAT + 4) {
+                if (gz == FAT + 1) lap = lap - a.zc[2] * s[0] - a.zc[3] * s[1] - a.zc[4] * s[2] - a.zc[5] * s[3];
+                else if (gz == FAT + 2) lap = lap - a.zc[3] * s[-1] - a.zc[4] * s[0] - a.zc[5] * s[1];
+                else if (gz == FAT + 3) lap = lap - a.zc[4] * s[-2] - a.zc[5] * s[-1];
+                else if (gz == FAT + 4) lap = lap - a.zc[5] * s[-3];
+            }
+        }
+        lap = lap
+            + a.xc[0] * cc
+            + a.xc[1] * (w[j + 6] + w[j + 4])
+            + a.xc[2] * (w[j + 7] + w[j + 3])
+            + a.xc[3] * (w[j + 8] + w[j + 2])
+            + a.xc[4] * (w[j + 9] + w[j + 1])
+            + a.xc[5] * (w[j + 10] + w[j + 0]);
         float n = 0.f, d = 1.f;
         if (ok[j]) {
             const long gi = (long)gx * nz + gz;
-            d = sponge(a.cd, gz, gx, nz, nx, a.minPad);
-            const float os = po[gi];
+            const bool surf = a.fs && gz == FAT;
+            d = a.fs ? sponge_fs(a.cd, gz, gx, nz, nx, a.minPad) : sponge(a.cd, gz, gx, nz, nx, a.minPad);
+            // forward: the reference zeroed the old slice's surface value when
+            // it was the current slice
+            const float os = (!ADJ && surf) ? 0.f : po[gi];
             const float o = os * d;                  // second damping of the old slice
             if constexpr (ADJ) {
                 const float c = pc[gi];
-                n = lap + c + c - o;
+                n = surf ? o : lap + c + c - o;      // the adjoint leaves the surface row as it is
                 if constexpr (MODE & M_WIN) cur[j] = c;
             } else {
                 n = v2dt2[gi] * lap + cc + cc - o;
@@ -1728,6 +1770,9 @@ hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a,
 {
     StepKernel k = nullptr;
     switch (mode) {
+    case MODE_NL_FWD: k = pick_generic_tile<MODE_NL_FWD>(tz, tx); break;
+    case MODE_NL_ADJ: k = pick_generic_tile<MODE_NL_ADJ>(tz, tx); break;
+    case MODE_BORN_REC: k = pick_generic_tile<MODE_BORN_REC>(tz, tx); break;
     case MODE_SRC: k = pick_generic_tile<MODE_SRC>(tz, tx); break;
     case MODE_SRC_WIN: k = pick_generic_tile<MODE_SRC_WIN>(tz, tx); break;
     case MODE_BORN_SCAT: k = pick_generic_tile<MODE_BORN_SCAT>(tz, tx); break;

@@ -74,6 +74,7 @@ struct StepArgs {
     int win, winOut;        // M_WIN: capture at this step, and its destination
     float wa, wb, wc;       // M_WIN weights of the old, current and new slices
     int ntW;                // non-temporal accesses to the W / SS stream
+    int fs;                 // free surface at iz = FAT (k_step_generic only)
     // lockstep Born forward (k_fwd_pair): scattered-field parity and the
     // secondary-source slices it reads
     int curQ;
@@ -177,9 +178,10 @@ constexpr unsigned MODE_TOMO_REC = M_ADJ | M_INJ_DATA | M_CAPTURE | M_CAP_W;
 // e0/e1 (optional): events timestamped by the dispatch itself.
 hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a, hipStream_t s,
                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-// The unspecialized step kernel (modes of the extended-Born and tomographic
-// operators: MODE_SRC, MODE_SRC_WIN, MODE_BORN_SCAT, MODE_EXT_SCAT,
-// MODE_EXT_REC, MODE_REC_WIN, MODE_TOMO_*).
+// The unspecialized step kernel: the extended-Born, tomographic and WEMVA
+// operators, and every operator with a free surface (MODE_NL_*, MODE_SRC,
+// MODE_SRC_WIN, MODE_BORN_SCAT, MODE_BORN_REC, MODE_EXT_*, MODE_REC_WIN,
+// MODE_TOMO_*).
 hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a, hipStream_t s,
                                hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 hipError_t launch_ring(unsigned mode, const StepArgs& a, int maxRing, hipStream_t s);

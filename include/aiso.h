@@ -266,6 +266,9 @@ aiso_status aiso_wemva_adjoint(aiso_ctx* ctx, int add, float* model, const float
 /* padFileGpuMain padding rule (F/src/padFileGpuMain.cpp:49-66, no free
  * surface): returns padPlus for a side of n samples with pad on the minus side. */
 int aiso_pad_plus(int n, int pad, int blockSize);
+/* The same with a free surface (F/src/padFileGpuMain.cpp:37-47): zPadMinus is
+ * 0 and zPadPlus pads the depth axis alone to a multiple of blockSize. */
+int aiso_pad_plus_free_surface(int nz, int zPad, int blockSize);
 
 #ifdef __cplusplus
 }
