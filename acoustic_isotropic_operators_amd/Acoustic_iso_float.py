@@ -244,19 +244,21 @@ class BornShotsGpu(_Wrapped):
 
 
 # --------------------------------------------------------- extended Born --
-def _ext_axis(par):
-    """Offset extension axis centred on zero, d = dx (PY/Acoustic_iso_float.py:1006-1030)."""
+def _ext_axis(par, time_lags=True):
+    """Extension axis centred on zero: d = dts for time lags, d = dx for
+    subsurface offsets (PY/Acoustic_iso_float.py:996-1020)."""
     extension = par.getString("extension", "noExtensionType")
     if extension == "noExtensionType":
         raise ValueError("**** ERROR: User did not provide extension type ****")
-    if extension != "offset":
-        raise NotImplementedError(f"extension={extension}: only the subsurface-offset extension is provided")
+    if extension not in ("offset", "time") or (extension == "time" and not time_lags):
+        raise NotImplementedError(f"extension={extension}: this operator provides "
+                                  + ("the offset and time-lag extensions" if time_lags else "the subsurface-offset extension"))
     nExt = par.getInt("nExt", -1)
     if nExt == -1:
         raise ValueError("**** ERROR: User did not provide size of extension axis ****")
     if nExt % 2 == 0:
         raise ValueError("Length of extension axis must be an uneven number")
-    dExt = par.getFloat("dx", -1.0)
+    dExt = par.getFloat("dts" if extension == "time" else "dx", -1.0)
     hExt = (nExt - 1) // 2
     return sep.axis(n=nExt, o=-dExt * hExt, d=dExt)
 
@@ -279,7 +281,7 @@ def BornExtOpInitFloat(args, client=None):
 
 
 class BornExtShotsGpu(_Wrapped):
-    """Subsurface-offset extended Born operator (PY/Acoustic_iso_float.py:1080-1174)."""
+    """Extended Born operator, subsurface offsets or time lags (PY/Acoustic_iso_float.py:1080-1174)."""
 
     _mod = pyAcoustic_iso_float_born_ext
     _name = " BornExtOp "
@@ -309,7 +311,7 @@ def tomoExtOpInitFloat(args, client=None):
     vel = _read_vel(par)
     zAxis, xAxis = vel.getHyper().axes[0], vel.getHyper().axes[1]
     time = _time_axis(par)
-    extAxis = _ext_axis(par)
+    extAxis = _ext_axis(par, time_lags=False)
     signals = _read_sources(par)
     reflFile = par.getString("reflectivity", "None")
     if reflFile == "None":
@@ -356,7 +358,7 @@ def wemvaExtOpInitFloat(args):
     receivers, recAxis = buildReceiversGeometry(par, vel)
     zAxis, xAxis = vel.getHyper().axes[0], vel.getHyper().axes[1]
     time = _time_axis(par)
-    extAxis = _ext_axis(par)
+    extAxis = _ext_axis(par, time_lags=False)
     signals = _read_sources(par)
     dataFile = par.getString("seismicData", "noWemvaDataFile")
     if dataFile == "noWemvaDataFile":

@@ -14,7 +14,7 @@ import numpy as np
 from . import lib_path
 
 MEM_HOST, MEM_DEVICE = 0, 1
-EXT_NONE, EXT_OFFSET = 0, 1
+EXT_NONE, EXT_OFFSET, EXT_TIME = 0, 1, 2
 K_ALL, K_NL_FWD, K_NL_ADJ, K_SRC, K_BORN_SCAT, K_BORN_REC, K_EXT_SCAT, K_EXT_REC, K_BORN_PAIR, K_TOMO = range(-1, 9)
 
 

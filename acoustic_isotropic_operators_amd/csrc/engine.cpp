@@ -112,6 +112,10 @@ struct aiso_ctx {
     DBuf extRefl, rExt, P, cap, slotsC, slotsI, slotsJ, imgDev;
     bool haveExtRefl = false;
     int leg1 = 1, leg2 = 1;
+    // time-lag extension: the per-point model scale of the call and the
+    // sloth key (scale by -v^2 dtw^2 instead of 2/v^3 v^2 dtw^2)
+    DBuf tscale;
+    int sloth = 0;
     hipEvent_t evIn = nullptr;          // orders device inputs after the default stream's work
     // Lanes: the shots of a batch split over streams whose kernels run
     // concurrently, so one lane's launch gaps and tails are filled by the
@@ -428,8 +432,8 @@ void run(aiso_ctx* c, const Request& q)
     if (q.nRec != 1 && q.nRec != q.nShot) throw Error(AISO_ERR_PARAM, "receiver geometry count must be 1 or nShot");
     if ((!is_born(q.op) || is_ext(q.op)) && p.sub >= kSubMax) throw Error(AISO_ERR_PARAM, "**** ERROR: Subsampling parameter too high ****");
     if ((!is_born(q.op) || is_ext(q.op)) && p.minPad >= kPadMax) throw Error(AISO_ERR_PARAM, "**** ERROR: Padding value is too high ****");
-    if (is_ext(q.op) && p.extension != AISO_EXT_OFFSET && p.nExt > 1)
-        throw Error(AISO_ERR_PARAM, "extended Born needs extension=offset");
+    if (is_ext(q.op) && p.extension == AISO_EXT_NONE && p.nExt > 1)
+        throw Error(AISO_ERR_PARAM, "extended Born needs extension=offset or extension=time");
     if (is_tomo(q.op) || is_wemva(q.op)) {
         if (p.extension != AISO_EXT_OFFSET)
             throw Error(AISO_ERR_PARAM, is_tomo(q.op) ? "the tomographic operator needs extension=offset (time-lag is not supported)"
@@ -561,6 +565,22 @@ void run(aiso_ctx* c, const Request& q)
     const int extK = std::min(c->extBatch, p.nts);
     const int ssSlices = is_ext(q.op) || is_tomo(q.op) || wemva ? extK + 1 : 2;
     auto ssSlot = [&](int k) { return (long)(k % ssSlices) * nzx; };
+    // Time lags of a batch: sample k uses slices e in [lo, hi), those whose
+    // source sample k - 2 (e - hExt) lies in [0, nts)
+    // (F/src/BornExtShotsGpuFunctions.cu:362-366, 374-378).
+    auto time_lags = [&](ExtBatchArgs& e) {
+        e.nts = p.nts;
+        e.eLo = 2 * p.hExt + 1;
+        e.eHi = 0;
+        for (int j = 0; j < e.nk; ++j) {
+            const int k = (int)(e.wOff[j] / nzx);
+            e.lo[j] = std::max((k + 1 - p.nts) / 2, -p.hExt) + p.hExt;
+            e.hi[j] = std::min(k / 2, p.hExt) + p.hExt + 1;
+            e.eLo = std::min(e.eLo, e.lo[j]);
+            e.eHi = std::max(e.eHi, e.hi[j]);
+        }
+        if (e.eLo > e.eHi) e.eLo = e.eHi;
+    };
     const bool needImg = q.op == Op::BornAdj || q.op == Op::ExtAdj || q.op == Op::TomoAdj || wemva;
     const bool tomo = is_tomo(q.op) || wemva;   // the tomographic and WEMVA passes share the machinery below
     const int capSlices = wemva ? extK + 1 : 2;     // captured coarse samples kept per shot
@@ -625,9 +645,20 @@ void run(aiso_ctx* c, const Request& q)
         float* m = (float*)c->mtil.need(nzx * 4);
         HIPC(launch_scale_model(dModelIn, m, c->v2dt2.as<float>(), c->refl.as<float>(), p.nz, p.nx, s));
         mtil = m;
-    } else if (q.op == Op::ExtFwd) {
+    }
+    // Per-point scale of the extended model: 2/v^3 for offsets; for time
+    // lags also v^2 dtw^2, or -v^2 dtw^2 with sloth (which the reference's
+    // free-surface routines do not take).
+    const bool tlag = is_ext(q.op) && p.extension == AISO_EXT_TIME;
+    const float* extScale = c->refl.as<float>();
+    if (tlag) {
+        float* t = (float*)c->tscale.need(nzx * 4);
+        HIPC(launch_time_scale(c->v2dt2.as<float>(), c->refl.as<float>(), c->sloth && !fsurf, t, p.nz, p.nx, s));
+        extScale = t;
+    }
+    if (q.op == Op::ExtFwd) {
         float* m = (float*)c->mtil.need(nzx * nExt * 4);
-        HIPC(launch_scale_ext(dModelIn, m, c->refl.as<float>(), p.nz, p.nx, nExt, s));
+        HIPC(launch_scale_ext(dModelIn, m, extScale, p.nz, p.nx, nExt, s));
         mtil = m;
     }
     const float* rExt = nullptr;   // extended reflectivity (tomographic) or image (WEMVA adjoint) scaled by 2/v^3
@@ -1294,7 +1325,12 @@ void run(aiso_ctx* c, const Request& q)
                             e.ssOff[j] = ssSlot(nextSS + j);
                             e.wOff[j] = (long)(nextSS + j) * nzx;
                         }
-                        HIPC(launch_ext_ss_batch(e, nb, s));
+                        if (tlag) {
+                            time_lags(e);
+                            HIPC(launch_time_ss_batch(e, nb, s));
+                        } else {
+                            HIPC(launch_ext_ss_batch(e, nb, s));
+                        }
                         nextSS += e.nk;
                     }
                 };
@@ -1370,7 +1406,12 @@ void run(aiso_ctx* c, const Request& q)
                             e.ssOff[j] = ssSlot(pend[j]);
                             e.wOff[j] = (long)pend[j] * nzx;
                         }
-                        HIPC(launch_ext_img_batch(e, nbr, s));
+                        if (tlag) {
+                            time_lags(e);
+                            HIPC(launch_time_img_batch(e, nbr, s));
+                        } else {
+                            HIPC(launch_ext_img_batch(e, nbr, s));
+                        }
                         pend.clear();
                     };
                     fork_lanes(c);
@@ -1420,7 +1461,7 @@ void run(aiso_ctx* c, const Request& q)
                                                  p.nx, s));
                         pend.push_back(0);
                         ext_img_flush();
-                        HIPC(launch_ext_adj_finish(sl, nbr, c->refl.as<float>(), p.nz, p.nx, nExt, s));
+                        HIPC(launch_ext_adj_finish(sl, nbr, extScale, p.nz, p.nx, nExt, s));
                     }
                 }
             }
@@ -1852,6 +1893,7 @@ aiso_status aiso_ctx_set_option(aiso_ctx* c, const char* key, int value)
         else if (k == "pair2") c->pair2 = value != 0;
         else if (k == "leg1") c->leg1 = value != 0;
         else if (k == "leg2") c->leg2 = value != 0;
+        else if (k == "sloth") c->sloth = value != 0;
         else if (k == "extBatch") {
             if (value < 1 || value > kExtBatchMax) throw Error(AISO_ERR_PARAM, "extBatch must be 1..16");
             c->extBatch = value;

@@ -36,8 +36,8 @@ FdParams make_params(const aiso_fd_params& a)
     // fdParam.cpp:69-70
     if (a.nExt < 1 || a.nExt % 2 == 0) fail(AISO_ERR_PARAM, "Length of extended axis must be an uneven number");
     p.hExt = (a.nExt - 1) / 2;
-    if (a.extension != AISO_EXT_NONE && a.extension != AISO_EXT_OFFSET)
-        fail(AISO_ERR_PARAM, "only the subsurface-offset extension is supported");
+    if (a.extension != AISO_EXT_NONE && a.extension != AISO_EXT_OFFSET && a.extension != AISO_EXT_TIME)
+        fail(AISO_ERR_PARAM, "extension must be none, offset or time");
     // fdParam.cpp:18-25
     p.ntw = (p.nts - 1) * p.sub + 1;
     p.dtw = p.dts / float(p.sub);
@@ -52,7 +52,8 @@ FdParams make_params(const aiso_fd_params& a)
     // fdParam.cpp:258-268
     if ((p.nz - 2 * kFat) <= 0 || (p.nz - 2 * kFat) % a.blockSize != 0) fail(AISO_ERR_PARAM, "nz not a multiple of block size");
     if ((p.nx - 2 * kFat) <= 0 || (p.nx - 2 * kFat) % a.blockSize != 0) fail(AISO_ERR_PARAM, "nx not a multiple of block size");
-    if (p.nExt > 1 && 2 * p.hExt >= p.nx - 2 * kFat) fail(AISO_ERR_PARAM, "extension longer than the model");
+    if (p.nExt > 1 && p.extension == AISO_EXT_OFFSET && 2 * p.hExt >= p.nx - 2 * kFat)
+        fail(AISO_ERR_PARAM, "extension longer than the model");
 
     // Laplacian coefficients: double literal / float(d*d), stored as float.
     static const double C[6] = {-2.927222222, 1.666666667, -0.238095238,

@@ -1553,6 +1553,72 @@ __global__ void k_ext_adj_finish(const ShotSlot* slots, const float* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Time-lag extended Born (F/src/BornExtShotsGpuFunctions.cu, extension=time).
+// Lag slice e scatters the source wavefield sample k - 2 (e - hExt) into
+// sample k.  Both kernels take one thread per interior point of one shot and
+// walk the lag slices in ascending order; the nk samples of the batch live in
+// registers, so each model (or image) value is touched once per batch.
+
+// imagingTimeFwdGpu (F/src/kernelsGpu.cu:337-350) for nk samples:
+// ss_j = sum over e in [lo_j, hi_j), ascending, of mext[e] * W[k_j - 2 (e - hExt)].
+__global__ void k_time_ss(ExtBatchArgs a)
+{
+    const ShotSlot& S = a.slots[blockIdx.y];
+    const long nz = a.nz, nzx = nz * a.nx;
+    const int iz = FAT + blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = FAT + blockIdx.z;
+    if (iz >= a.nz - FAT) return;
+    const long i = (long)ix * nz + iz;
+    float acc[kExtBatchMax];
+#pragma unroll
+    for (int j = 0; j < kExtBatchMax; ++j) acc[j] = 0.f;
+    for (int e = a.eLo; e < a.eHi; ++e) {
+        const float mv = a.mext[nzx * e + i];
+        const long sh = 2L * (e - a.hExt) * nzx;
+#pragma unroll
+        for (int j = 0; j < kExtBatchMax; ++j)
+            if (j < a.nk && e >= a.lo[j] && e < a.hi[j]) acc[j] += mv * S.W[a.wOff[j] - sh + i];
+    }
+#pragma unroll
+    for (int j = 0; j < kExtBatchMax; ++j)
+        if (j < a.nk) S.ss[0][a.ssOff[j] + i] = acc[j];
+}
+
+// imagingTimeAdjGpu (F/src/kernelsGpu.cu:352-364) for nk receiver samples,
+// in batch order: img[e] += ss_j * W[k_j - 2 (e - hExt)] for e in [lo_j, hi_j).
+__global__ void k_time_img(ExtBatchArgs a)
+{
+    const ShotSlot& S = a.slots[blockIdx.y];
+    const long nz = a.nz, nzx = nz * a.nx;
+    const int iz = FAT + blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = FAT + blockIdx.z;
+    if (iz >= a.nz - FAT) return;
+    const long i = (long)ix * nz + iz;
+    float sv[kExtBatchMax];
+#pragma unroll
+    for (int j = 0; j < kExtBatchMax; ++j) sv[j] = j < a.nk ? S.ss[0][a.ssOff[j] + i] : 0.f;
+    for (int e = a.eLo; e < a.eHi; ++e) {
+        const long sh = 2L * (e - a.hExt) * nzx;
+        float* mp = S.img + nzx * e + i;
+        float m = *mp;
+#pragma unroll
+        for (int j = 0; j < kExtBatchMax; ++j)
+            if (j < a.nk && e >= a.lo[j] && e < a.hi[j]) m += sv[j] * S.W[a.wOff[j] - sh + i];
+        *mp = m;
+    }
+}
+
+// Per-point scale of the time-lag model: scaleReflectivityExt
+// (F/src/kernelsGpu.cu:277-288), 2/v^3 * v^2 dtw^2, or with sloth
+// scaleReflectivitySlothExt (:290-302), -v^2 dtw^2.
+__global__ void k_time_scale(const float* __restrict__ v2dt2, const float* __restrict__ refl, int sloth,
+                             float* __restrict__ out, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = sloth ? -v2dt2[i] : refl[i] * v2dt2[i];
+}
+
+// ---------------------------------------------------------------------------
 // Offset-extended tomographic operator (F/src/tomoExtShotsGpuFunctions.cu).
 
 // computeTomoSrcWfldDt2 + srcWfldSecondTimeDerivative (:1251-1360,
@@ -1960,6 +2026,46 @@ hipError_t launch_img_acc(const ShotSlot* X, long xOff, const ShotSlot* Y, long 
                           hipStream_t s)
 {
     hipLaunchKernelGGL(k_img_acc, column_grid(nz, nx, nshots), dim3(64), 0, s, X, xOff, Y, yOff, nz, nx);
+    return hipGetLastError();
+}
+
+// The lag ranges must lie inside [0, 2 hExt] and reach W samples inside the
+// stored [0, nts) only (W is read at k - 2 (e - hExt), largest at e = lo).
+static bool time_batch_ok(const ExtBatchArgs& a)
+{
+    if (a.nk < 1 || a.nk > kExtBatchMax || a.hExt < 0) return false;
+    if (a.eLo < 0 || a.eHi > 2 * a.hExt + 1 || a.eLo > a.eHi) return false;
+    const long nzx = (long)a.nz * a.nx;
+    for (int j = 0; j < a.nk; ++j) {
+        if (a.lo[j] < a.eLo || a.hi[j] > a.eHi || a.lo[j] > a.hi[j]) return false;
+        if (a.wOff[j] % nzx != 0) return false;
+        const long k = a.wOff[j] / nzx;
+        if (k < 0 || k >= a.nts) return false;
+        if (a.lo[j] < a.hi[j] && (k - 2L * (a.lo[j] - a.hExt) >= a.nts || k - 2L * (a.hi[j] - 1 - a.hExt) < 0))
+            return false;
+    }
+    return true;
+}
+
+hipError_t launch_time_ss_batch(const ExtBatchArgs& a, int nshots, hipStream_t s)
+{
+    if (!time_batch_ok(a)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_time_ss, column_grid(a.nz, a.nx, nshots), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_time_img_batch(const ExtBatchArgs& a, int nshots, hipStream_t s)
+{
+    if (!time_batch_ok(a)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_time_img, column_grid(a.nz, a.nx, nshots), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_time_scale(const float* v2dt2, const float* refl, int sloth, float* out, int nz, int nx,
+                             hipStream_t s)
+{
+    const long n = (long)nz * nx;
+    hipLaunchKernelGGL(k_time_scale, flat_grid(n), dim3(256), 0, s, v2dt2, refl, sloth, out, n);
     return hipGetLastError();
 }
 

@@ -93,6 +93,11 @@ struct ExtBatchArgs {
     int nk;                 // samples in the batch, 1..kExtBatchMax
     long ssOff[kExtBatchMax];
     long wOff[kExtBatchMax];
+    // time-lag extension: sample j uses lag slices [lo[j], hi[j]); eLo..eHi
+    // spans them all
+    int lo[kExtBatchMax], hi[kExtBatchMax];
+    int eLo, eHi;
+    int nts;                // coarse samples of W (bounds check of the time-lag batches)
 };
 
 // Two fine steps of both fields of the lockstep Born forward (sub == 2) per
@@ -195,6 +200,10 @@ hipError_t launch_born_adj_finish(const ShotSlot* slots, int nshots, int ssR, co
 hipError_t launch_ext_ss_batch(const ExtBatchArgs& a, int nshots, hipStream_t s);
 // wemvaFwd: the WEMVA forward's imaging, (W_k[x-h] * ss_k[x+h]) * v2dt2[x+h]
 hipError_t launch_ext_img_batch(const ExtBatchArgs& a, int nshots, hipStream_t s, bool wemvaFwd = false);
+hipError_t launch_time_ss_batch(const ExtBatchArgs& a, int nshots, hipStream_t s);
+hipError_t launch_time_img_batch(const ExtBatchArgs& a, int nshots, hipStream_t s);
+hipError_t launch_time_scale(const float* v2dt2, const float* refl, int sloth, float* out, int nz, int nx,
+                             hipStream_t s);
 hipError_t launch_ext_adj_finish(const ShotSlot* slots, int nshots, const float* refl, int nz, int nx,
                                  int nExt, hipStream_t s);
 // Tomographic operator: W <- second time derivative of W (nts coarse

@@ -267,8 +267,7 @@ struct ShotsBase {
         q.freeSurface = par.i("freeSurface", 0);
         q.nExt = par.i("nExt", 1);
         std::string ext = par.s("extension", "none");
-        q.extension = ext == "offset" ? AISO_EXT_OFFSET : AISO_EXT_NONE;
-        if (ext == "time" && q.nExt > 1) throw std::runtime_error("**** ERROR: time-lag extension is not supported ****");
+        q.extension = ext == "offset" ? AISO_EXT_OFFSET : ext == "time" ? AISO_EXT_TIME : AISO_EXT_NONE;
         if (ctx) {
             aiso_ctx_destroy(ctx);
             ctx = nullptr;
@@ -284,6 +283,8 @@ struct ShotsBase {
         if (tz > 0 || tx > 0) check(aiso_ctx_set_tile(ctx, tz > 0 ? tz : 32, tx > 0 ? tx : 32));
         check(aiso_ctx_set_option(ctx, "lockstep", par.i("lockstep", 1)));
         check(aiso_ctx_set_option(ctx, "pair2", par.i("twoStep", 0)));
+        // BornExtGpu's sloth key (F/src/BornExtGpu.cpp:14)
+        check(aiso_ctx_set_option(ctx, "sloth", par.i("sloth", 0)));
         fp = q;
         dirty = false;
     }

@@ -1,4 +1,4 @@
-// pyAcoustic_iso_float_born_ext -- BornExtShotsGpu (subsurface-offset
+// pyAcoustic_iso_float_born_ext -- BornExtShotsGpu (subsurface-offset or time-lag
 // extension) with the names and methods of
 // PY/pyAcoustic_iso_float_born_ext.cpp:14-35, backed by libaiso.so.
 #include "born_impl.hpp"

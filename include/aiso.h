@@ -49,7 +49,7 @@ typedef enum {
 } aiso_status;
 
 enum { AISO_MEM_HOST = 0, AISO_MEM_DEVICE = 1 };
-enum { AISO_EXT_NONE = 0, AISO_EXT_OFFSET = 1 };
+enum { AISO_EXT_NONE = 0, AISO_EXT_OFFSET = 1, AISO_EXT_TIME = 2 };
 
 /* Parfile keys read by fdParam (F/src/fdParam.cpp:14-96).  nz/nx are the
  * padded sizes of the velocity model, oz/ox its origins. */
@@ -64,9 +64,9 @@ typedef struct {
     int blockSize;      /* must be 16 (model-size rule of fdParam.cpp:258-268) */
     float alphaCos;     /* default 0.99                                      */
     float fMax;         /* default 1000                                      */
-    int freeSurface;    /* only 0 is supported                               */
+    int freeSurface;    /* 0 or 1 (1 needs zPadMinus == 0)                   */
     int nExt;           /* odd, default 1                                    */
-    int extension;      /* AISO_EXT_NONE or AISO_EXT_OFFSET                  */
+    int extension;      /* AISO_EXT_NONE, AISO_EXT_OFFSET or AISO_EXT_TIME   */
 } aiso_fd_params;
 
 typedef struct aiso_ctx aiso_ctx;
@@ -109,7 +109,11 @@ aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
  *       launch of the extended Born operators (default 16);
  *   "lanes" 1..4: streams the shots of a launch are split over (default 1);
  *   "leg1", "leg2" 0/1: legs of the tomographic operator (parfile keys
- *       leg1/leg2 of F/src/tomoExtGpu.cpp:10-11, default 1). */
+ *       leg1/leg2 of F/src/tomoExtGpu.cpp:10-11, default 1);
+ *   "sloth" 0/1: time-lag extended Born model is a slowness-squared
+ *       perturbation, scaled by -v^2 dtw^2 (parfile key sloth of
+ *       F/src/BornExtGpu.cpp:14, default 0; ignored with a free surface, as
+ *       by the reference's free-surface routines). */
 aiso_status aiso_ctx_set_option(aiso_ctx* ctx, const char* key, int value);
 /* Derived sizes: ntw, sub, nShotsPerLaunch actually used by the last call. */
 aiso_status aiso_ctx_info(const aiso_ctx* ctx, int* ntw, int* batchUsed, size_t* bytesPerShot);
@@ -212,9 +216,10 @@ aiso_status aiso_born_adjoint(aiso_ctx* ctx, int add, float* model, const float*
                               const float* wavelets, int nWavelet,
                               aiso_device* const* receivers, int nRecGeom, int mem);
 
-/* BornExtShotsGpu::forward/adjoint with extension=offset
- * (F/src/BornExtGpu.cpp:33-134 -> F/src/BornExtShotsGpuFunctions.cu:582-724,
- * 1848-1989).  model [nExt][nx][nz]. */
+/* BornExtShotsGpu::forward/adjoint (F/src/BornExtGpu.cpp:33-134) with
+ * extension=offset (-> F/src/BornExtShotsGpuFunctions.cu:582-724, 1848-1989)
+ * or extension=time (-> :269-418, 1525-1678; free surface :885-1036,
+ * 2152-2304).  model [nExt][nx][nz]. */
 aiso_status aiso_bornext_forward(aiso_ctx* ctx, int add, const float* model, float* data,
                                  aiso_device* const* sources, int nShot,
                                  const float* wavelets, int nWavelet,

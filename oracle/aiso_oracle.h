@@ -68,6 +68,14 @@ void orc_bornext_adj##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* refl
                           const REAL* sig, const int* srcPos, int nSrc,              \
                           const int* recPos, int nRec, REAL* model, const REAL* dreg,\
                           REAL* W);                                                  \
+void orc_borntime_fwd##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* refl,   \
+                           int sloth, const REAL* sig, const int* srcPos, int nSrc,  \
+                           const int* recPos, int nRec, const REAL* model,           \
+                           REAL* dreg, REAL* W);                                     \
+void orc_borntime_adj##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* refl,   \
+                           int sloth, const REAL* sig, const int* srcPos, int nSrc,  \
+                           const int* recPos, int nRec, REAL* model,                 \
+                           const REAL* dreg, REAL* W);                               \
 void orc_tomo_fwd##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* refl,      \
                        const REAL* sig, const int* srcPos, int nSrc,                 \
                        const int* recPos, int nRec, const REAL* model,               \

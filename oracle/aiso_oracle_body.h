@@ -589,6 +589,114 @@ void FN(orc_bornext_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
 }
 
 /* ---------------------------------------------------------------------------
+ * Time-lag extended Born (BornExtShotsGpu, extension=time).  Model slice e
+ * is the image at time lag e - hExt (coarse samples); it scatters the source
+ * wavefield 2 (e - hExt) coarse samples earlier.
+ */
+
+/* Slices e in [lo, hi) whose source sample k - 2 (e - hExt) lies in
+ * [0, nts) (F/src/BornExtShotsGpuFunctions.cu:362-366, 374-378).  The
+ * reference's (k + 1 - nts) / 2 truncates toward zero, which for the
+ * non-positive numerator is the rounding up the bound needs. */
+static void FN(time_lags)(const orc_grid* g, int k, int* lo, int* hi)
+{
+    const int hExt = (g->nExt - 1) / 2;
+    int a = (k + 1 - g->nts) / 2;
+    if (a < -hExt) a = -hExt;
+    int b = k / 2;
+    if (b > hExt) b = hExt;
+    *lo = a + hExt;
+    *hi = b + hExt + 1;
+}
+
+/* imagingTimeFwdGpu, F/src/kernelsGpu.cu:337-350: ss += m[e] * W[k - 2 (e - hExt)]. */
+static void FN(time_ss)(const orc_grid* g, const void* vctx, int k, REAL* ss)
+{
+    const FN(born_ctx)* c = (const FN(born_ctx)*)vctx;
+    const int hExt = (g->nExt - 1) / 2;
+    const long nzx = (long)g->nz * g->nx;
+    int lo, hi;
+    FN(time_lags)(g, k, &lo, &hi);
+    INTERIOR_BEGIN(g)
+        for (int e = lo; e < hi; ++e)
+            ss[i] += c->m[nzx * e + i] * c->W[nzx * (k - 2 * (e - hExt)) + i];
+    INTERIOR_END
+}
+
+/* imagingTimeAdjGpu, F/src/kernelsGpu.cu:352-364: m[e] += ss * W[k - 2 (e - hExt)]. */
+static void FN(time_img)(const orc_grid* g, void* vctx, int k, REAL* ss)
+{
+    FN(born_ctx)* c = (FN(born_ctx)*)vctx;
+    const int hExt = (g->nExt - 1) / 2;
+    const long nzx = (long)g->nz * g->nx;
+    REAL* m = (REAL*)c->m;
+    int lo, hi;
+    FN(time_lags)(g, k, &lo, &hi);
+    INTERIOR_BEGIN(g)
+        for (int e = lo; e < hi; ++e)
+            m[nzx * e + i] += ss[i] * c->W[nzx * (k - 2 * (e - hExt)) + i];
+    INTERIOR_END
+}
+
+/* scaleReflectivityExt (F/src/kernelsGpu.cu:277-288): m *= 2/v^3 * v^2 dtw^2;
+ * sloth != 0: scaleReflectivitySlothExt (:290-302), m *= -v^2 dtw^2.  The
+ * free-surface routines always take the first (BornTimeShotsFwdFsGpu and
+ * BornTimeShotsAdjFsGpu have no sloth argument,
+ * F/src/BornExtShotsGpuFunctions.cu:885, 2152). */
+static void FN(scale_refl_time)(const orc_grid* g, REAL* m, const REAL* refl, const REAL* v2dt2, int sloth)
+{
+    const long nzx = (long)g->nz * g->nx;
+    if (g->freeSurface) sloth = 0;
+    for (int e = 0; e < g->nExt; ++e) {
+        REAL* me = m + nzx * e;
+        if (sloth) {
+            INTERIOR_BEGIN(g)
+                me[i] *= -v2dt2[i];
+            INTERIOR_END
+        } else {
+            INTERIOR_BEGIN(g)
+                me[i] *= refl[i] * v2dt2[i];
+            INTERIOR_END
+        }
+    }
+}
+
+/* BornTimeShotsFwdGpu, F/src/BornExtShotsGpuFunctions.cu:269-418 (free
+ * surface: BornTimeShotsFwdFsGpu, :885-1036).  model: nExt*nz*nx (read only). */
+void FN(orc_borntime_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl, int sloth,
+                          const REAL* sig, const int* srcPos, int nSrc,
+                          const int* recPos, int nRec, const REAL* model, REAL* dreg, REAL* W)
+{
+    REAL zc[6], xc[6], f[2 * 101], cd[512];
+    FN(orc_tables)(g, zc, xc, f, cd);
+    const long nzx = (long)g->nz * g->nx, nm = nzx * g->nExt;
+    FN(src_pass)(g, zc, xc, f, cd, v2dt2, sig, srcPos, nSrc, W);
+    REAL* m = FN(zalloc)(nm);
+    memcpy(m, model, sizeof(REAL) * (size_t)nm);
+    FN(scale_refl_time)(g, m, refl, v2dt2, sloth);
+    FN(born_ctx) ctx = {m, W, v2dt2, refl};
+    FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(time_ss), &ctx, 1);
+    free(m);
+}
+
+/* BornTimeShotsAdjGpu, F/src/BornExtShotsGpuFunctions.cu:1525-1678 (free
+ * surface: BornTimeShotsAdjFsGpu, :2152-2304).  model (output, nExt*nz*nx)
+ * is overwritten with this shot's image. */
+void FN(orc_borntime_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl, int sloth,
+                          const REAL* sig, const int* srcPos, int nSrc,
+                          const int* recPos, int nRec, REAL* model, const REAL* dreg, REAL* W)
+{
+    REAL zc[6], xc[6], f[2 * 101], cd[512];
+    FN(orc_tables)(g, zc, xc, f, cd);
+    const long nzx = (long)g->nz * g->nx;
+    FN(src_pass)(g, zc, xc, f, cd, v2dt2, sig, srcPos, nSrc, W);
+    memset(model, 0, sizeof(REAL) * (size_t)nzx * g->nExt);
+    FN(born_ctx) ctx = {model, W, v2dt2, refl};
+    FN(rec_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(time_img), &ctx);
+    FN(scale_refl_time)(g, model, refl, v2dt2, sloth);
+}
+
+/* ---------------------------------------------------------------------------
  * Offset-extended tomographic operator (tomoExtShotsGpu, extension=offset).
  */
 

@@ -17,6 +17,8 @@ reference's host classes do:
   F/src/BornShotsGpu.cpp:60-143,245-339;
 * ``bornext_forward`` / ``bornext_adjoint``: F/src/BornExtGpu.cpp:33-134 inside
   F/src/BornExtShotsGpu.cpp (same shot loop);
+* ``borntime_forward`` / ``borntime_adjoint``: the same with extension=time
+  (F/src/BornExtShotsGpuFunctions.cu:269-418, 1525-1678);
 * ``tomo_forward`` / ``tomo_adjoint``: F/src/tomoExtGpu.cpp:34-100 inside
   F/src/tomoExtShotsGpu.cpp:62-145 (same shot loop), extension=offset;
 * ``wemva_forward`` / ``wemva_adjoint``: F/src/wemvaExtGpu.cpp inside
@@ -316,6 +318,26 @@ def born_adj_reg(g, v2dt2, refl, sig, srcPos, recPos, dreg, prec="f", ext=False)
     return model
 
 
+def borntime_fwd_reg(g, v2dt2, refl, sig, srcPos, recPos, model, sloth=0, prec="f"):
+    dt = _dt(prec)
+    dreg = np.zeros((len(recPos), g.nts), dt)
+    W = np.zeros((g.nts, g.nx, g.nz), dt)
+    gc = g.c()
+    _fn("orc_borntime_fwd", prec)(C.byref(gc), _p(v2dt2), _p(refl), int(sloth), _p(sig), _p(srcPos), len(srcPos),
+                                  _p(recPos), len(recPos), _p(np.ascontiguousarray(model, dt)), _p(dreg), _p(W))
+    return dreg
+
+
+def borntime_adj_reg(g, v2dt2, refl, sig, srcPos, recPos, dreg, sloth=0, prec="f"):
+    dt = _dt(prec)
+    model = np.zeros((g.nExt, g.nx, g.nz), dt)
+    W = np.zeros((g.nts, g.nx, g.nz), dt)
+    gc = g.c()
+    _fn("orc_borntime_adj", prec)(C.byref(gc), _p(v2dt2), _p(refl), int(sloth), _p(sig), _p(srcPos), len(srcPos),
+                                  _p(recPos), len(recPos), _p(model), _p(np.ascontiguousarray(dreg, dt)), _p(W))
+    return model
+
+
 def _tomo_fns(name, prec):
     f = _fn(name, prec)
     f.restype = None
@@ -507,6 +529,35 @@ def bornext_forward(g, vel, sources, wavelets, receivers, model, prec="f"):
 
 def bornext_adjoint(g, vel, sources, wavelets, receivers, data, prec="f"):
     return born_adjoint(g, vel, sources, wavelets, receivers, data, prec, ext=True)
+
+
+def borntime_forward(g, vel, sources, wavelets, receivers, model, sloth=0, prec="f"):
+    """Time-lag extended Born (extension=time): model [nExt][nx][nz] -> data."""
+    dt = _dt(prec)
+    vel = np.ascontiguousarray(vel, dt)
+    v2, rf = vel_scales(g, vel, prec)
+    nShot = len(sources)
+    nRec = _shot_geom(receivers, 0).nDev
+    data = np.zeros((nShot, nRec, g.nts), dt)
+    for s in range(nShot):
+        src, rec = sources[s], _shot_geom(receivers, s)
+        fine = born_source_fine(g, vel, src, _shot_geom(wavelets, s), prec)
+        dreg = borntime_fwd_reg(g, v2, rf, fine, src.regPos, rec.regPos, model, sloth, prec)
+        data[s] += rec.forward(dreg, prec=prec)
+    return data
+
+
+def borntime_adjoint(g, vel, sources, wavelets, receivers, data, sloth=0, prec="f"):
+    dt = _dt(prec)
+    vel = np.ascontiguousarray(vel, dt)
+    v2, rf = vel_scales(g, vel, prec)
+    image = np.zeros((g.nExt, g.nx, g.nz), dt)
+    for s in range(len(sources)):
+        src, rec = sources[s], _shot_geom(receivers, s)
+        fine = born_source_fine(g, vel, src, _shot_geom(wavelets, s), prec)
+        dreg = rec.adjoint(data[s], prec=prec)
+        image += borntime_adj_reg(g, v2, rf, fine, src.regPos, rec.regPos, dreg, sloth, prec)
+    return image
 
 
 def tomo_forward(g, vel, sources, wavelets, receivers, model, ext_refl, leg1=1, leg2=1, prec="f"):
