@@ -244,15 +244,14 @@ class BornShotsGpu(_Wrapped):
 
 
 # --------------------------------------------------------- extended Born --
-def _ext_axis(par, time_lags=True):
+def _ext_axis(par):
     """Extension axis centred on zero: d = dts for time lags, d = dx for
     subsurface offsets (PY/Acoustic_iso_float.py:996-1020)."""
     extension = par.getString("extension", "noExtensionType")
     if extension == "noExtensionType":
         raise ValueError("**** ERROR: User did not provide extension type ****")
-    if extension not in ("offset", "time") or (extension == "time" and not time_lags):
-        raise NotImplementedError(f"extension={extension}: this operator provides "
-                                  + ("the offset and time-lag extensions" if time_lags else "the subsurface-offset extension"))
+    if extension not in ("offset", "time"):
+        raise ValueError(f"extension={extension}: the extension is offset or time")
     nExt = par.getInt("nExt", -1)
     if nExt == -1:
         raise ValueError("**** ERROR: User did not provide size of extension axis ****")
@@ -311,7 +310,7 @@ def tomoExtOpInitFloat(args, client=None):
     vel = _read_vel(par)
     zAxis, xAxis = vel.getHyper().axes[0], vel.getHyper().axes[1]
     time = _time_axis(par)
-    extAxis = _ext_axis(par, time_lags=False)
+    extAxis = _ext_axis(par)
     signals = _read_sources(par)
     reflFile = par.getString("reflectivity", "None")
     if reflFile == "None":
@@ -358,7 +357,7 @@ def wemvaExtOpInitFloat(args):
     receivers, recAxis = buildReceiversGeometry(par, vel)
     zAxis, xAxis = vel.getHyper().axes[0], vel.getHyper().axes[1]
     time = _time_axis(par)
-    extAxis = _ext_axis(par, time_lags=False)
+    extAxis = _ext_axis(par)
     signals = _read_sources(par)
     dataFile = par.getString("seismicData", "noWemvaDataFile")
     if dataFile == "noWemvaDataFile":

@@ -435,9 +435,9 @@ void run(aiso_ctx* c, const Request& q)
     if (is_ext(q.op) && p.extension == AISO_EXT_NONE && p.nExt > 1)
         throw Error(AISO_ERR_PARAM, "extended Born needs extension=offset or extension=time");
     if (is_tomo(q.op) || is_wemva(q.op)) {
-        if (p.extension != AISO_EXT_OFFSET)
-            throw Error(AISO_ERR_PARAM, is_tomo(q.op) ? "the tomographic operator needs extension=offset (time-lag is not supported)"
-                                                      : "the WEMVA operator needs extension=offset (time-lag is not supported)");
+        if (p.extension == AISO_EXT_NONE)
+            throw Error(AISO_ERR_PARAM, is_tomo(q.op) ? "the tomographic operator needs extension=offset or extension=time"
+                                                      : "the WEMVA operator needs extension=offset or extension=time");
         if (is_tomo(q.op) && !c->haveExtRefl) throw Error(AISO_ERR_STATE, "extended reflectivity not set");
         if (p.sub >= kSubMax) throw Error(AISO_ERR_PARAM, "**** ERROR: Subsampling parameter too high ****");
         if (p.minPad >= kPadMax) throw Error(AISO_ERR_PARAM, "**** ERROR: Padding value is too high ****");
@@ -457,7 +457,9 @@ void run(aiso_ctx* c, const Request& q)
     for (int i = 0; i < q.nRec; ++i)
         for (int v : q.rec[i]->g.gridIdx)
             if (v >= p.nzx()) throw Error(AISO_ERR_GEOM, "receiver outside the velocity grid");
-    if (p.freeSurface) {   // the surface condition rewrites the rows above it every step
+    // The reference's WEMVA has no free-surface routines: it propagates with
+    // the absorbing top whatever the key (F/src/wemvaExtGpu.cpp:131-135).
+    if (p.freeSurface && !is_wemva(q.op)) {   // the surface condition rewrites the rows above it every step
         auto above = [&](const aiso_device* d) {
             for (size_t k = 0; k < d->g.gridIdx.size(); ++k)
                 if (d->g.weight[k] != 0.f && d->g.gridIdx[k] % p.nz < kFat) return true;
@@ -545,7 +547,7 @@ void run(aiso_ctx* c, const Request& q)
     // the secondary source then lives in a 3-slice ring instead of nts slices.
     // A free surface runs on the generic step kernel, without the window
     // capture, the lockstep pair or two steps per launch.
-    const bool fsurf = p.freeSurface != 0;
+    const bool fsurf = p.freeSurface != 0 && !wemva;
     const bool lockstep = q.op == Op::BornFwd && p.sub == 2 && c->lockstep && !fsurf;
     // Two fine steps per launch (k_fwd_pair2) when no source or receiver
     // point lies in the FAT ring, whose values that kernel takes as 0.
@@ -566,16 +568,23 @@ void run(aiso_ctx* c, const Request& q)
     const int ssSlices = is_ext(q.op) || is_tomo(q.op) || wemva ? extK + 1 : 2;
     auto ssSlot = [&](int k) { return (long)(k % ssSlices) * nzx; };
     // Time lags of a batch: sample k uses slices e in [lo, hi), those whose
-    // source sample k - 2 (e - hExt) lies in [0, nts)
-    // (F/src/BornExtShotsGpuFunctions.cu:362-366, 374-378).
-    auto time_lags = [&](ExtBatchArgs& e) {
+    // W sample k - dir * 2 (e - hExt) lies in [0, nts)
+    // (F/src/BornExtShotsGpuFunctions.cu:362-366, 374-378 for dir = +1;
+    // F/src/tomoExtShotsGpuFunctions.cu:2630-2633 for dir = -1).
+    auto time_lags = [&](ExtBatchArgs& e, int dir) {
         e.nts = p.nts;
+        e.lagDir = dir;
         e.eLo = 2 * p.hExt + 1;
         e.eHi = 0;
         for (int j = 0; j < e.nk; ++j) {
             const int k = (int)(e.wOff[j] / nzx);
-            e.lo[j] = std::max((k + 1 - p.nts) / 2, -p.hExt) + p.hExt;
-            e.hi[j] = std::min(k / 2, p.hExt) + p.hExt + 1;
+            if (dir > 0) {
+                e.lo[j] = std::max((k + 1 - p.nts) / 2, -p.hExt) + p.hExt;
+                e.hi[j] = std::min(k / 2, p.hExt) + p.hExt + 1;
+            } else {
+                e.lo[j] = std::max((-k) / 2, -p.hExt) + p.hExt;
+                e.hi[j] = std::min((p.nts - 1 - k) / 2, p.hExt) + p.hExt + 1;
+            }
             e.eLo = std::min(e.eLo, e.lo[j]);
             e.eHi = std::max(e.eHi, e.hi[j]);
         }
@@ -647,13 +656,14 @@ void run(aiso_ctx* c, const Request& q)
         mtil = m;
     }
     // Per-point scale of the extended model: 2/v^3 for offsets; for time
-    // lags also v^2 dtw^2, or -v^2 dtw^2 with sloth (which the reference's
-    // free-surface routines do not take).
-    const bool tlag = is_ext(q.op) && p.extension == AISO_EXT_TIME;
+    // lags also v^2 dtw^2, or -v^2 dtw^2 with sloth (which neither the
+    // reference's free-surface routines nor its WEMVA take).
+    const bool tlag = (is_ext(q.op) || tomo) && p.extension == AISO_EXT_TIME;
     const float* extScale = c->refl.as<float>();
     if (tlag) {
         float* t = (float*)c->tscale.need(nzx * 4);
-        HIPC(launch_time_scale(c->v2dt2.as<float>(), c->refl.as<float>(), c->sloth && !fsurf, t, p.nz, p.nx, s));
+        HIPC(launch_time_scale(c->v2dt2.as<float>(), c->refl.as<float>(), c->sloth && !fsurf && !wemva, t, p.nz,
+                               p.nx, s));
         extScale = t;
     }
     if (q.op == Op::ExtFwd) {
@@ -669,15 +679,15 @@ void run(aiso_ctx* c, const Request& q)
             mtil = m;
         }
         float* r = (float*)c->rExt.need(nzx * p.nExt * 4);
-        HIPC(launch_scale_ext(c->extRefl.as<float>(), r, c->refl.as<float>(), p.nz, p.nx, p.nExt, s));
+        HIPC(launch_scale_ext(c->extRefl.as<float>(), r, extScale, p.nz, p.nx, p.nExt, s));
         rExt = r;
     } else if (q.op == Op::WemvaFwd) {
         float* m = (float*)c->mtil.need(nzx * 4);
         HIPC(launch_scale_model(dModelIn, m, c->v2dt2.as<float>(), c->refl.as<float>(), p.nz, p.nx, s));
         mtil = m;
-    } else if (q.op == Op::WemvaAdj) {   // the extended image in, scaled by 2/v^3
+    } else if (q.op == Op::WemvaAdj) {   // the extended image in, scaled like a model
         float* r = (float*)c->rExt.need(nzx * p.nExt * 4);
-        HIPC(launch_scale_ext(dModelIn, r, c->refl.as<float>(), p.nz, p.nx, p.nExt, s));
+        HIPC(launch_scale_ext(dModelIn, r, extScale, p.nz, p.nx, p.nExt, s));
         rExt = r;
     }
     if (needImg) {
@@ -698,7 +708,7 @@ void run(aiso_ctx* c, const Request& q)
     for (int k = 0; k < 6; ++k) { base.zc[k] = p.zc[k]; base.xc[k] = p.xc[k]; }
     base.wA = base.wB = base.wImg = 0;
     base.ntW = c->streamNT;
-    base.fs = p.freeSurface;
+    base.fs = fsurf;
 
     std::vector<float> sigHost;
     for (int b0 = 0; b0 < q.nShot; b0 += B) {
@@ -760,11 +770,13 @@ void run(aiso_ctx* c, const Request& q)
             HIPC(hipMemcpyAsync(dSlotsC, slotsCv.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
         }
         if (q.op == Op::WemvaFwd) {
+            // offsets: I = (captures, W''), J = (R, captures); time lags:
+            // J = (captures, R), the lag-shifted field being the W side
             slotsIv = slots;
-            slotsJv = slotsCv;
+            slotsJv = tlag ? slotsP : slotsCv;
             for (int k = 0; k < nb; ++k) {
                 slotsIv[k].ss[0] = cap + (size_t)k * capSlices * nzx;
-                slotsJv[k].ss[0] = Pw + (size_t)k * p.nts * nzx;
+                slotsJv[k].ss[0] = tlag ? cap + (size_t)k * capSlices * nzx : Pw + (size_t)k * p.nts * nzx;
             }
             HIPC(hipMemcpyAsync(dSlotsI, slotsIv.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
             HIPC(hipMemcpyAsync(dSlotsJ, slotsJv.data(), sizeof(ShotSlot) * nb, hipMemcpyHostToDevice, s));
@@ -1109,9 +1121,16 @@ void run(aiso_ctx* c, const Request& q)
                     e.ssOff[j] = ssSlot(k0 + j);
                     e.wOff[j] = (long)(k0 + j) * nzx;
                 }
-                if (kind == SS_MUL) HIPC(launch_mul_batch(e, nb, s));
-                else if (kind == SS_EXT) HIPC(launch_ext_ss_batch(e, nb, s));
-                else HIPC(launch_tomo_adj_ss(e, nb, s));
+                if (kind == SS_MUL) {
+                    HIPC(launch_mul_batch(e, nb, s));
+                } else if (tlag) {   // imagingTimeFwdGpu / imagingTimeTomoAdjGpu
+                    time_lags(e, kind == SS_EXT ? 1 : -1);
+                    HIPC(launch_time_ss_batch(e, nb, s));
+                } else if (kind == SS_EXT) {
+                    HIPC(launch_ext_ss_batch(e, nb, s));
+                } else {
+                    HIPC(launch_tomo_adj_ss(e, nb, s));
+                }
             };
             auto capOff = [&](int k) { return (long)(k % capSlices) * nzx; };
             // Forward in time.  ssKind >= 0: inject ring samples made from
@@ -1277,10 +1296,16 @@ void run(aiso_ctx* c, const Request& q)
                     e.nk = (int)pend.size();
                     for (int j = 0; j < e.nk; ++j) {
                         const int k = pend[j];
-                        e.ssOff[j] = wemvaFwd ? (long)k * nzx : capOff(k);   // receiver side: R[k] / sample k
-                        e.wOff[j] = wemvaFwd ? capOff(k) : (long)k * nzx;    // source side: sample k / W''[k]
+                        const bool rSide = wemvaFwd && !tlag;
+                        e.ssOff[j] = rSide ? (long)k * nzx : capOff(k);   // receiver side: R[k] / sample k
+                        e.wOff[j] = rSide ? capOff(k) : (long)k * nzx;    // source side: sample k / W''[k] (R[k])
                     }
-                    HIPC(launch_ext_img_batch(e, nb, s, wemvaFwd));
+                    if (tlag) {   // imagingWemvaTimeAdjGpu / imagingTimeAdjGpu
+                        time_lags(e, wemvaFwd ? -1 : 1);
+                        HIPC(launch_time_img_batch(e, nb, s));
+                    } else {
+                        HIPC(launch_ext_img_batch(e, nb, s, wemvaFwd));
+                    }
                     pend.clear();
                 };
                 if (c->leg1) {   // forward from m * W'', imaged against R (batches of extK samples)
@@ -1295,7 +1320,7 @@ void run(aiso_ctx* c, const Request& q)
                         if ((int)pend.size() == extK || k == 0) flush(false);
                     });
                 }
-                HIPC(launch_ext_adj_finish(dSlots, nb, c->refl.as<float>(), p.nz, p.nx, p.nExt, s));
+                HIPC(launch_ext_adj_finish(dSlots, nb, extScale, p.nz, p.nx, p.nExt, s));
             }
         }
 
@@ -1326,7 +1351,7 @@ void run(aiso_ctx* c, const Request& q)
                             e.wOff[j] = (long)(nextSS + j) * nzx;
                         }
                         if (tlag) {
-                            time_lags(e);
+                            time_lags(e, 1);
                             HIPC(launch_time_ss_batch(e, nb, s));
                         } else {
                             HIPC(launch_ext_ss_batch(e, nb, s));
@@ -1407,7 +1432,7 @@ void run(aiso_ctx* c, const Request& q)
                             e.wOff[j] = (long)pend[j] * nzx;
                         }
                         if (tlag) {
-                            time_lags(e);
+                            time_lags(e, 1);
                             HIPC(launch_time_img_batch(e, nbr, s));
                         } else {
                             HIPC(launch_ext_img_batch(e, nbr, s));

@@ -17,6 +17,7 @@
 // interior points; 256 threads = TZ lanes along z x (256/TZ) lanes along x,
 // each thread computing PX consecutive x-points with an x-window in
 // registers.  The tile and its 5-point halo are staged in LDS.
+#include <algorithm>
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -1553,14 +1554,16 @@ __global__ void k_ext_adj_finish(const ShotSlot* slots, const float* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Time-lag extended Born (F/src/BornExtShotsGpuFunctions.cu, extension=time).
-// Lag slice e scatters the source wavefield sample k - 2 (e - hExt) into
-// sample k.  Both kernels take one thread per interior point of one shot and
-// walk the lag slices in ascending order; the nk samples of the batch live in
-// registers, so each model (or image) value is touched once per batch.
+// Time-lag extension (extension=time of F/src/BornExtShotsGpuFunctions.cu,
+// tomoExtShotsGpuFunctions.cu, wemvaExtShotsGpuFunctions.cu).  Lag slice e
+// pairs sample k with W sample k - lagDir * 2 (e - hExt).  Both kernels take
+// one thread per interior point of one shot and walk the lag slices in
+// ascending order; the nk samples of the batch live in registers, so each
+// model (or image) value is touched once per batch.
 
-// imagingTimeFwdGpu (F/src/kernelsGpu.cu:337-350) for nk samples:
-// ss_j = sum over e in [lo_j, hi_j), ascending, of mext[e] * W[k_j - 2 (e - hExt)].
+// imagingTimeFwdGpu (F/src/kernelsGpu.cu:337-350, lagDir = +1) and
+// imagingTimeTomoAdjGpu (:445-458, lagDir = -1) for nk samples:
+// ss_j = sum over e in [lo_j, hi_j), ascending, of mext[e] * W[k_j - lagDir * 2 (e - hExt)].
 __global__ void k_time_ss(ExtBatchArgs a)
 {
     const ShotSlot& S = a.slots[blockIdx.y];
@@ -1574,7 +1577,7 @@ __global__ void k_time_ss(ExtBatchArgs a)
     for (int j = 0; j < kExtBatchMax; ++j) acc[j] = 0.f;
     for (int e = a.eLo; e < a.eHi; ++e) {
         const float mv = a.mext[nzx * e + i];
-        const long sh = 2L * (e - a.hExt) * nzx;
+        const long sh = 2L * a.lagDir * (e - a.hExt) * nzx;
 #pragma unroll
         for (int j = 0; j < kExtBatchMax; ++j)
             if (j < a.nk && e >= a.lo[j] && e < a.hi[j]) acc[j] += mv * S.W[a.wOff[j] - sh + i];
@@ -1584,8 +1587,9 @@ __global__ void k_time_ss(ExtBatchArgs a)
         if (j < a.nk) S.ss[0][a.ssOff[j] + i] = acc[j];
 }
 
-// imagingTimeAdjGpu (F/src/kernelsGpu.cu:352-364) for nk receiver samples,
-// in batch order: img[e] += ss_j * W[k_j - 2 (e - hExt)] for e in [lo_j, hi_j).
+// imagingTimeAdjGpu (F/src/kernelsGpu.cu:352-364, lagDir = +1) and
+// imagingWemvaTimeAdjGpu (:366-378, lagDir = -1) for nk samples, in batch
+// order: img[e] += ss_j * W[k_j - lagDir * 2 (e - hExt)] for e in [lo_j, hi_j).
 __global__ void k_time_img(ExtBatchArgs a)
 {
     const ShotSlot& S = a.slots[blockIdx.y];
@@ -1598,7 +1602,7 @@ __global__ void k_time_img(ExtBatchArgs a)
 #pragma unroll
     for (int j = 0; j < kExtBatchMax; ++j) sv[j] = j < a.nk ? S.ss[0][a.ssOff[j] + i] : 0.f;
     for (int e = a.eLo; e < a.eHi; ++e) {
-        const long sh = 2L * (e - a.hExt) * nzx;
+        const long sh = 2L * a.lagDir * (e - a.hExt) * nzx;
         float* mp = S.img + nzx * e + i;
         float m = *mp;
 #pragma unroll
@@ -2030,10 +2034,11 @@ hipError_t launch_img_acc(const ShotSlot* X, long xOff, const ShotSlot* Y, long 
 }
 
 // The lag ranges must lie inside [0, 2 hExt] and reach W samples inside the
-// stored [0, nts) only (W is read at k - 2 (e - hExt), largest at e = lo).
+// stored [0, nts) only (W is read at k - lagDir * 2 (e - hExt), extreme at
+// e = lo and e = hi - 1).
 static bool time_batch_ok(const ExtBatchArgs& a)
 {
-    if (a.nk < 1 || a.nk > kExtBatchMax || a.hExt < 0) return false;
+    if (a.nk < 1 || a.nk > kExtBatchMax || a.hExt < 0 || (a.lagDir != 1 && a.lagDir != -1)) return false;
     if (a.eLo < 0 || a.eHi > 2 * a.hExt + 1 || a.eLo > a.eHi) return false;
     const long nzx = (long)a.nz * a.nx;
     for (int j = 0; j < a.nk; ++j) {
@@ -2041,8 +2046,10 @@ static bool time_batch_ok(const ExtBatchArgs& a)
         if (a.wOff[j] % nzx != 0) return false;
         const long k = a.wOff[j] / nzx;
         if (k < 0 || k >= a.nts) return false;
-        if (a.lo[j] < a.hi[j] && (k - 2L * (a.lo[j] - a.hExt) >= a.nts || k - 2L * (a.hi[j] - 1 - a.hExt) < 0))
-            return false;
+        if (a.lo[j] < a.hi[j]) {
+            const long s0 = k - 2L * a.lagDir * (a.lo[j] - a.hExt), s1 = k - 2L * a.lagDir * (a.hi[j] - 1 - a.hExt);
+            if (std::min(s0, s1) < 0 || std::max(s0, s1) >= a.nts) return false;
+        }
     }
     return true;
 }

@@ -94,9 +94,10 @@ struct ExtBatchArgs {
     long ssOff[kExtBatchMax];
     long wOff[kExtBatchMax];
     // time-lag extension: sample j uses lag slices [lo[j], hi[j]); eLo..eHi
-    // spans them all
+    // spans them all; slice e reads W at k_j - lagDir * 2 (e - hExt)
     int lo[kExtBatchMax], hi[kExtBatchMax];
     int eLo, eHi;
+    int lagDir;             // +1 or -1
     int nts;                // coarse samples of W (bounds check of the time-lag batches)
 };
 

@@ -79,22 +79,24 @@ void orc_borntime_adj##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* ref
 void orc_tomo_fwd##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* refl,      \
                        const REAL* sig, const int* srcPos, int nSrc,                 \
                        const int* recPos, int nRec, const REAL* model,               \
-                       const REAL* extRefl, int leg1, int leg2, REAL* dreg,          \
-                       REAL* W, REAL* P);                                            \
+                       const REAL* extRefl, int leg1, int leg2, int timeLag,         \
+                       int sloth, REAL* dreg, REAL* W, REAL* P);                     \
 void orc_tomo_adj##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* refl,      \
                        const REAL* sig, const int* srcPos, int nSrc,                 \
                        const int* recPos, int nRec, REAL* model, const REAL* dreg,   \
-                       const REAL* extRefl, int leg1, int leg2, REAL* W, REAL* P);   \
+                       const REAL* extRefl, int leg1, int leg2, int timeLag,         \
+                       int sloth, REAL* W, REAL* P);                                 \
 void orc_dt2_coarse##SFX(const orc_grid* g, REAL* W);                                \
 void orc_wemva_fwd##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* refl,     \
                         const REAL* sig, const int* srcPos, int nSrc,                \
                         const int* recPos, int nRec, const REAL* model,              \
-                        const REAL* dobs, int leg1, int leg2, REAL* image,           \
-                        REAL* W, REAL* P);                                           \
+                        const REAL* dobs, int leg1, int leg2, int timeLag,           \
+                        REAL* image, REAL* W, REAL* P);                              \
 void orc_wemva_adj##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* refl,     \
                         const REAL* sig, const int* srcPos, int nSrc,                \
                         const int* recPos, int nRec, REAL* model, const REAL* image, \
-                        const REAL* dobs, int leg1, int leg2, REAL* W, REAL* P);     \
+                        const REAL* dobs, int leg1, int leg2, int timeLag,           \
+                        REAL* W, REAL* P);                                           \
 void orc_interp_fwd##SFX(int nts, double dts, double ots, int sub, int nDev,        \
                          int add, const REAL* m, REAL* d);                           \
 void orc_interp_adj##SFX(int nts, double dts, double ots, int sub, int nDev,        \

@@ -638,6 +638,51 @@ static void FN(time_img)(const orc_grid* g, void* vctx, int k, REAL* ss)
     INTERIOR_END
 }
 
+/* Slices e in [lo, hi) whose sample k + 2 (e - hExt) lies in [0, nts): the
+ * bounds of the adjoint-side time-lag kernels
+ * (F/src/tomoExtShotsGpuFunctions.cu:2630-2633, 2638-2641). */
+static void FN(time_lags_rev)(const orc_grid* g, int k, int* lo, int* hi)
+{
+    const int hExt = (g->nExt - 1) / 2;
+    int a = (-k) / 2;
+    if (a < -hExt) a = -hExt;
+    int b = (g->nts - 1 - k) / 2;
+    if (b > hExt) b = hExt;
+    *lo = a + hExt;
+    *hi = b + hExt + 1;
+}
+
+/* imagingTimeTomoAdjGpu (F/src/kernelsGpu.cu:445-458): secondary source of
+ * the backward leg from the receiver wavefield R at coarse sample k,
+ * ss += r[e] * R[k + 2 (e - hExt)] into a zeroed slice. */
+static void FN(time_adj_ss)(const orc_grid* g, const REAL* R, const REAL* r,
+                            const REAL* v2dt2, int k, REAL* ss)
+{
+    (void)v2dt2;
+    const int hExt = (g->nExt - 1) / 2;
+    const long nzx = (long)g->nz * g->nx;
+    int lo, hi;
+    FN(time_lags_rev)(g, k, &lo, &hi);
+    INTERIOR_BEGIN(g)
+        for (int e = lo; e < hi; ++e)
+            ss[i] += r[nzx * e + i] * R[nzx * (k + 2 * (e - hExt)) + i];
+    INTERIOR_END
+}
+
+/* imagingWemvaTimeAdjGpu (F/src/kernelsGpu.cu:366-378):
+ * img[e] += scat * R[k + 2 (e - hExt)]. */
+static void FN(wemva_time_img)(const orc_grid* g, REAL* img, const REAL* scat, const REAL* R, int k)
+{
+    const int hExt = (g->nExt - 1) / 2;
+    const long nzx = (long)g->nz * g->nx;
+    int lo, hi;
+    FN(time_lags_rev)(g, k, &lo, &hi);
+    INTERIOR_BEGIN(g)
+        for (int e = lo; e < hi; ++e)
+            img[nzx * e + i] += scat[i] * R[nzx * (k + 2 * (e - hExt)) + i];
+    INTERIOR_END
+}
+
 /* scaleReflectivityExt (F/src/kernelsGpu.cu:277-288): m *= 2/v^3 * v^2 dtw^2;
  * sloth != 0: scaleReflectivitySlothExt (:290-302), m *= -v^2 dtw^2.  The
  * free-surface routines always take the first (BornTimeShotsFwdFsGpu and
@@ -781,6 +826,8 @@ static void FN(rec_wfld)(const orc_grid* g, const REAL* zc, const REAL* xc,
     free(p0); free(p1); free(tmp);
 }
 
+typedef void (*FN(adj_ss_fn))(const orc_grid*, const REAL*, const REAL*, const REAL*, int, REAL*);
+
 /* imagingOffsetTomoAdjGpu (F/src/kernelsGpu.cu:460-475): secondary source of
  * leg 2 of the adjoint from the receiver wavefield R at coarse sample k,
  * ss[x] += r[e][x+e] * R[k][x+2e] * v2dt2[x+2e] into a zeroed slice. */
@@ -812,16 +859,30 @@ static void FN(img_acc)(const orc_grid* g, REAL* m, const REAL* X, int k, const 
     INTERIOR_END
 }
 
+/* The scaled extended reflectivity r of the tomographic and WEMVA
+ * operators: scaleReflectivityLinExt for offsets, scaleReflectivityExt (or
+ * with sloth scaleReflectivitySlothExt) for time lags. */
+static void FN(scale_ext_any)(const orc_grid* g, REAL* r, const REAL* refl, const REAL* v2dt2,
+                              int timeLag, int sloth)
+{
+    if (timeLag) FN(scale_refl_time)(g, r, refl, v2dt2, sloth);
+    else FN(scale_refl_ext)(g, r, refl);
+}
+
 /* tomoOffsetShotsFwdGpu (F/src/tomoExtShotsGpuFunctions.cu:467-572) with
  * computeTomoLeg1OffsetFwd (:2005-2113) and computeTomoLeg2OffsetFwd
- * (:2227-2317).  model: nz*nx velocity perturbation; extRefl: nExt*nz*nx
- * extended reflectivity (both read only); W, P: nts*nz*nx scratch.
+ * (:2227-2317); timeLag != 0: tomoTimeShotsFwdGpu (:355-466) with
+ * computeTomoLeg1TimeFwd (:1554-1657) and computeTomoLeg2TimeFwd
+ * (:1782-1882), the same legs with the time-lag scattering (free surface:
+ * tomoTimeShotsFwdFsGpu, :576-683, no sloth).  model: nz*nx velocity
+ * perturbation; extRefl: nExt*nz*nx extended reflectivity (both read only);
+ * W, P: nts*nz*nx scratch.
  *   leg 1: s -> extended reflectivity -> P -> model -> data
  *   leg 2: s -> model -> P -> extended reflectivity -> data */
 void FN(orc_tomo_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
                       const REAL* sig, const int* srcPos, int nSrc,
                       const int* recPos, int nRec, const REAL* model, const REAL* extRefl,
-                      int leg1, int leg2, REAL* dreg, REAL* W, REAL* P)
+                      int leg1, int leg2, int timeLag, int sloth, REAL* dreg, REAL* W, REAL* P)
 {
     REAL zc[6], xc[6], f[2 * 101], cd[512];
     FN(orc_tables)(g, zc, xc, f, cd);
@@ -833,17 +894,18 @@ void FN(orc_tomo_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
     FN(scale_refl)(g, m, refl, v2dt2);
     REAL* r = FN(zalloc)(nr);
     memcpy(r, extRefl, sizeof(REAL) * (size_t)nr);
-    FN(scale_refl_ext)(g, r, refl);
+    FN(scale_ext_any)(g, r, refl, v2dt2, timeLag, sloth);
     memset(dreg, 0, sizeof(REAL) * (size_t)nRec * g->nts);
+    FN(ss_fn) extSS = timeLag ? FN(time_ss) : FN(ext_ss);
     FN(born_ctx) rW = {r, W, v2dt2, refl}, mW = {m, W, v2dt2, refl};
     FN(born_ctx) rP = {r, P, v2dt2, refl}, mP = {m, P, v2dt2, refl};
     if (leg1) {
-        FN(cap_pass)(g, zc, xc, f, cd, v2dt2, FN(ext_ss), &rW, P);
+        FN(cap_pass)(g, zc, xc, f, cd, v2dt2, extSS, &rW, P);
         FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(born_ss), &mP, 0);
     }
     if (leg2) {
         FN(cap_pass)(g, zc, xc, f, cd, v2dt2, FN(born_ss), &mW, P);
-        FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(ext_ss), &rP, 0);
+        FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, extSS, &rP, 0);
     }
     free(m); free(r);
 }
@@ -856,7 +918,7 @@ void FN(orc_tomo_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
  * F/src/wemvaExtShotsGpuFunctions.cu:1502-1571). */
 static void FN(adj_leg_fwd)(const orc_grid* g, const REAL* zc, const REAL* xc, const REAL* f,
                             const REAL* cd, const REAL* v2dt2, const REAL* r, const REAL* W,
-                            const REAL* X, REAL* model)
+                            const REAL* X, REAL* model, FN(ss_fn) make_ss)
 {
     const int nts = g->nts, sub = g->sub, h = sub + 1;
     const long nzx = (long)g->nz * g->nx;
@@ -867,9 +929,9 @@ static void FN(adj_leg_fwd)(const orc_grid* g, const REAL* zc, const REAL* xc, c
     REAL* scL = FN(zalloc)(nzx);
     REAL* scR = FN(zalloc)(nzx);
     FN(born_ctx) rW = {r, W, v2dt2, NULL};
-    FN(ext_ss)(g, &rW, 0, ssL);
+    make_ss(g, &rW, 0, ssL);
     for (int its = 0; its < nts - 1; ++its) {
-        FN(ext_ss)(g, &rW, its + 1, ssR);
+        make_ss(g, &rW, its + 1, ssR);
         for (int it2 = 1; it2 < sub + 1; ++it2) {
             FN(orc_step_fwd)(g, zc, xc, p0, p1, p0, v2dt2);
             FN(inject_secondary)(g, f, h, ssL, ssR, p0, it2 - 1);
@@ -895,7 +957,7 @@ static void FN(adj_leg_fwd)(const orc_grid* g, const REAL* zc, const REAL* xc, c
  * F/src/wemvaExtShotsGpuFunctions.cu:1428-1500). */
 static void FN(adj_leg_bwd)(const orc_grid* g, const REAL* zc, const REAL* xc, const REAL* f,
                             const REAL* cd, const REAL* v2dt2, const REAL* r, const REAL* W,
-                            const REAL* R, REAL* model)
+                            const REAL* R, REAL* model, FN(adj_ss_fn) make_ss)
 {
     const int nts = g->nts, sub = g->sub, h = sub + 1;
     const long nzx = (long)g->nz * g->nx;
@@ -906,9 +968,9 @@ static void FN(adj_leg_bwd)(const orc_grid* g, const REAL* zc, const REAL* xc, c
     REAL* scL = FN(zalloc)(nzx);
     REAL* scR = FN(zalloc)(nzx);
     REAL* tmp = FN(zalloc)(nzx);
-    FN(tomo_adj_ss)(g, R, r, v2dt2, nts - 1, ssR);
+    make_ss(g, R, r, v2dt2, nts - 1, ssR);
     for (int its = nts - 2; its > -1; --its) {
-        FN(tomo_adj_ss)(g, R, r, v2dt2, its, ssL);
+        make_ss(g, R, r, v2dt2, its, ssL);
         for (int it2 = sub - 1; it2 > -1; --it2) {
             FN(orc_step_adj)(g, zc, xc, p0, p1, p0, v2dt2, tmp);
             FN(inject_secondary)(g, f, h, ssL, ssR, p0, it2 + 1);
@@ -928,13 +990,15 @@ static void FN(adj_leg_bwd)(const orc_grid* g, const REAL* zc, const REAL* xc, c
 
 /* tomoOffsetShotsAdjGpu (F/src/tomoExtShotsGpuFunctions.cu:912-1020) with
  * computeTomoLeg1OffsetAdj (:2788-2860) and computeTomoLeg2OffsetAdj
- * (:2949-3020).  model (output, nz*nx) is overwritten with this shot's
+ * (:2949-3020); timeLag != 0: tomoTimeShotsAdjGpu (:797-911) with
+ * computeTomoLeg1TimeAdj (:2442-2528) and computeTomoLeg2TimeAdj
+ * (:2619-2702).  model (output, nz*nx) is overwritten with this shot's
  * contribution; W holds the source wavefield's second time derivative and P
  * the receiver wavefield. */
 void FN(orc_tomo_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
                       const REAL* sig, const int* srcPos, int nSrc,
                       const int* recPos, int nRec, REAL* model, const REAL* dreg,
-                      const REAL* extRefl, int leg1, int leg2, REAL* W, REAL* P)
+                      const REAL* extRefl, int leg1, int leg2, int timeLag, int sloth, REAL* W, REAL* P)
 {
     REAL zc[6], xc[6], f[2 * 101], cd[512];
     FN(orc_tables)(g, zc, xc, f, cd);
@@ -944,10 +1008,10 @@ void FN(orc_tomo_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
     FN(rec_wfld)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, P);
     REAL* r = FN(zalloc)(nr);
     memcpy(r, extRefl, sizeof(REAL) * (size_t)nr);
-    FN(scale_refl_ext)(g, r, refl);
+    FN(scale_ext_any)(g, r, refl, v2dt2, timeLag, sloth);
     memset(model, 0, sizeof(REAL) * (size_t)nzx);
-    if (leg1) FN(adj_leg_fwd)(g, zc, xc, f, cd, v2dt2, r, W, P, model);
-    if (leg2) FN(adj_leg_bwd)(g, zc, xc, f, cd, v2dt2, r, W, P, model);
+    if (leg1) FN(adj_leg_fwd)(g, zc, xc, f, cd, v2dt2, r, W, P, model, timeLag ? FN(time_ss) : FN(ext_ss));
+    if (leg2) FN(adj_leg_bwd)(g, zc, xc, f, cd, v2dt2, r, W, P, model, timeLag ? FN(time_adj_ss) : FN(tomo_adj_ss));
     FN(scale_refl)(g, model, refl, v2dt2);
     free(r);
 }
@@ -977,14 +1041,21 @@ static void FN(wemva_img_fwd)(const orc_grid* g, REAL* img, const REAL* scat, co
 
 /* wemvaOffsetShotsFwdGpu (F/src/wemvaExtShotsGpuFunctions.cu:458-574) with
  * computeWemvaLeg1OffsetFwd (:1123-1188) and computeWemvaLeg2OffsetFwd
- * (:1190-1264).  model: nz*nx velocity perturbation (read only); dobs: the
- * shot's observed data on the regular receiver grid; image (output,
- * nExt*nz*nx) is overwritten with this shot's extended image. */
-void FN(orc_wemva_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
+ * (:1190-1264); timeLag != 0: wemvaTimeShotsFwdGpu (:344-457) with
+ * computeWemvaLeg1TimeFwd (:961-1036) and computeWemvaLeg2TimeFwd
+ * (:1038-1121).  The reference's WEMVA has no free-surface routines and
+ * propagates with the absorbing top whatever the freeSurface key
+ * (F/src/wemvaExtGpu.cpp:131-135).  model: nz*nx velocity perturbation
+ * (read only); dobs: the shot's observed data on the regular receiver grid;
+ * image (output, nExt*nz*nx) is overwritten with this shot's extended image. */
+void FN(orc_wemva_fwd)(const orc_grid* gIn, const REAL* v2dt2, const REAL* refl,
                        const REAL* sig, const int* srcPos, int nSrc,
                        const int* recPos, int nRec, const REAL* model, const REAL* dobs,
-                       int leg1, int leg2, REAL* image, REAL* W, REAL* P)
+                       int leg1, int leg2, int timeLag, REAL* image, REAL* W, REAL* P)
 {
+    orc_grid g0 = *gIn;
+    g0.freeSurface = 0;
+    const orc_grid* g = &g0;
     REAL zc[6], xc[6], f[2 * 101], cd[512];
     FN(orc_tables)(g, zc, xc, f, cd);
     const int nts = g->nts, sub = g->sub, h = sub + 1;
@@ -1015,12 +1086,14 @@ void FN(orc_wemva_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
                 FN(capture2)(g, f, h, p0, scL, scR, it2);
                 FN(swap)(&p0, &p1);
             }
-            FN(wemva_img_fwd)(g, image, scL, P, its, v2dt2);
+            if (timeLag) FN(wemva_time_img)(g, image, scL, P, its);
+            else FN(wemva_img_fwd)(g, image, scL, P, its, v2dt2);
             FN(swap)(&ssL, &ssR);
             FN(swap)(&scL, &scR);
             memset(scR, 0, sizeof(REAL) * (size_t)nzx);
         }
-        FN(wemva_img_fwd)(g, image, scL, P, nts - 1, v2dt2);
+        if (timeLag) FN(wemva_time_img)(g, image, scL, P, nts - 1);
+        else FN(wemva_img_fwd)(g, image, scL, P, nts - 1, v2dt2);
     }
     if (leg2) {   /* backward from m * R, imaged against W'' */
         memset(p0, 0, sizeof(REAL) * (size_t)nzx);
@@ -1041,26 +1114,33 @@ void FN(orc_wemva_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
                 FN(capture2)(g, f, h, p0, scL, scR, it2);
                 FN(swap)(&p0, &p1);
             }
-            FN(ext_img)(g, &iW, its + 1, scR);   /* scR *= v2dt2, then the offset imaging */
+            /* offsets: scR *= v2dt2, then the offset imaging */
+            (timeLag ? FN(time_img) : FN(ext_img))(g, &iW, its + 1, scR);
             FN(swap)(&ssR, &ssL);
             FN(swap)(&scR, &scL);
             memset(scL, 0, sizeof(REAL) * (size_t)nzx);
         }
-        FN(ext_img)(g, &iW, 0, scR);
+        (timeLag ? FN(time_img) : FN(ext_img))(g, &iW, 0, scR);
     }
-    FN(scale_refl_ext)(g, image, refl);
+    FN(scale_ext_any)(g, image, refl, v2dt2, timeLag, 0);
     free(m); free(p0); free(p1); free(ssL); free(ssR); free(scL); free(scR); free(tmp);
 }
 
-/* wemvaOffsetShotsAdjGpu (F/src/wemvaExtShotsGpuFunctions.cu:689-808):
- * image (read only, nExt*nz*nx) -> model (output, nz*nx), overwritten with
- * this shot's contribution.  Leg 1 is the backward pass, leg 2 the forward
- * one, in that order. */
-void FN(orc_wemva_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
+/* wemvaOffsetShotsAdjGpu (F/src/wemvaExtShotsGpuFunctions.cu:689-808);
+ * timeLag != 0: wemvaTimeShotsAdjGpu (:575-688) with computeWemvaLeg1TimeAdj
+ * (:1270-1350) and computeWemvaLeg2TimeAdj (:1352-1426); absorbing top
+ * whatever the freeSurface key, as in the forward.  image (read only,
+ * nExt*nz*nx) -> model (output, nz*nx), overwritten with this shot's
+ * contribution.  Leg 1 is the backward pass, leg 2 the forward one, in that
+ * order. */
+void FN(orc_wemva_adj)(const orc_grid* gIn, const REAL* v2dt2, const REAL* refl,
                        const REAL* sig, const int* srcPos, int nSrc,
                        const int* recPos, int nRec, REAL* model, const REAL* image, const REAL* dobs,
-                       int leg1, int leg2, REAL* W, REAL* P)
+                       int leg1, int leg2, int timeLag, REAL* W, REAL* P)
 {
+    orc_grid g0 = *gIn;
+    g0.freeSurface = 0;
+    const orc_grid* g = &g0;
     REAL zc[6], xc[6], f[2 * 101], cd[512];
     FN(orc_tables)(g, zc, xc, f, cd);
     const long nzx = (long)g->nz * g->nx, nr = nzx * g->nExt;
@@ -1069,10 +1149,10 @@ void FN(orc_wemva_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
     FN(rec_wfld)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dobs, P);
     REAL* r = FN(zalloc)(nr);
     memcpy(r, image, sizeof(REAL) * (size_t)nr);
-    FN(scale_refl_ext)(g, r, refl);
+    FN(scale_ext_any)(g, r, refl, v2dt2, timeLag, 0);
     memset(model, 0, sizeof(REAL) * (size_t)nzx);
-    if (leg1) FN(adj_leg_bwd)(g, zc, xc, f, cd, v2dt2, r, W, P, model);
-    if (leg2) FN(adj_leg_fwd)(g, zc, xc, f, cd, v2dt2, r, W, P, model);
+    if (leg1) FN(adj_leg_bwd)(g, zc, xc, f, cd, v2dt2, r, W, P, model, timeLag ? FN(time_adj_ss) : FN(tomo_adj_ss));
+    if (leg2) FN(adj_leg_fwd)(g, zc, xc, f, cd, v2dt2, r, W, P, model, timeLag ? FN(time_ss) : FN(ext_ss));
     FN(scale_refl)(g, model, refl, v2dt2);
     free(r);
 }

@@ -20,10 +20,11 @@ reference's host classes do:
 * ``borntime_forward`` / ``borntime_adjoint``: the same with extension=time
   (F/src/BornExtShotsGpuFunctions.cu:269-418, 1525-1678);
 * ``tomo_forward`` / ``tomo_adjoint``: F/src/tomoExtGpu.cpp:34-100 inside
-  F/src/tomoExtShotsGpu.cpp:62-145 (same shot loop), extension=offset;
+  F/src/tomoExtShotsGpu.cpp:62-145 (same shot loop), extension=offset or
+  time;
 * ``wemva_forward`` / ``wemva_adjoint``: F/src/wemvaExtGpu.cpp inside
   F/src/wemvaExtShotsGpu.cpp (same shot loop, observed data per shot),
-  extension=offset.
+  extension=offset or time.
 """
 from __future__ import annotations
 
@@ -344,7 +345,8 @@ def _tomo_fns(name, prec):
     return f
 
 
-def tomo_fwd_reg(g, v2dt2, refl, sig, srcPos, recPos, model, ext_refl, leg1=1, leg2=1, prec="f"):
+def tomo_fwd_reg(g, v2dt2, refl, sig, srcPos, recPos, model, ext_refl, leg1=1, leg2=1, prec="f", time=False,
+                 sloth=0):
     dt = _dt(prec)
     dreg = np.zeros((len(recPos), g.nts), dt)
     W = np.zeros((g.nts, g.nx, g.nz), dt)
@@ -352,12 +354,13 @@ def tomo_fwd_reg(g, v2dt2, refl, sig, srcPos, recPos, model, ext_refl, leg1=1, l
     gc = g.c()
     _tomo_fns("orc_tomo_fwd", prec)(C.byref(gc), _p(v2dt2), _p(refl), _p(sig), _p(srcPos), len(srcPos),
                                     _p(recPos), len(recPos), _p(np.ascontiguousarray(model, dt)),
-                                    _p(np.ascontiguousarray(ext_refl, dt)), int(leg1), int(leg2), _p(dreg),
-                                    _p(W), _p(P))
+                                    _p(np.ascontiguousarray(ext_refl, dt)), int(leg1), int(leg2), int(time),
+                                    int(sloth), _p(dreg), _p(W), _p(P))
     return dreg
 
 
-def tomo_adj_reg(g, v2dt2, refl, sig, srcPos, recPos, dreg, ext_refl, leg1=1, leg2=1, prec="f"):
+def tomo_adj_reg(g, v2dt2, refl, sig, srcPos, recPos, dreg, ext_refl, leg1=1, leg2=1, prec="f", time=False,
+                 sloth=0):
     dt = _dt(prec)
     model = np.zeros((g.nx, g.nz), dt)
     W = np.zeros((g.nts, g.nx, g.nz), dt)
@@ -365,11 +368,12 @@ def tomo_adj_reg(g, v2dt2, refl, sig, srcPos, recPos, dreg, ext_refl, leg1=1, le
     gc = g.c()
     _tomo_fns("orc_tomo_adj", prec)(C.byref(gc), _p(v2dt2), _p(refl), _p(sig), _p(srcPos), len(srcPos),
                                     _p(recPos), len(recPos), _p(model), _p(np.ascontiguousarray(dreg, dt)),
-                                    _p(np.ascontiguousarray(ext_refl, dt)), int(leg1), int(leg2), _p(W), _p(P))
+                                    _p(np.ascontiguousarray(ext_refl, dt)), int(leg1), int(leg2), int(time),
+                                    int(sloth), _p(W), _p(P))
     return model
 
 
-def wemva_fwd_reg(g, v2dt2, refl, sig, srcPos, recPos, model, dobs, leg1=1, leg2=1, prec="f"):
+def wemva_fwd_reg(g, v2dt2, refl, sig, srcPos, recPos, model, dobs, leg1=1, leg2=1, prec="f", time=False):
     dt = _dt(prec)
     image = np.zeros((g.nExt, g.nx, g.nz), dt)
     W = np.zeros((g.nts, g.nx, g.nz), dt)
@@ -377,12 +381,12 @@ def wemva_fwd_reg(g, v2dt2, refl, sig, srcPos, recPos, model, dobs, leg1=1, leg2
     gc = g.c()
     _tomo_fns("orc_wemva_fwd", prec)(C.byref(gc), _p(v2dt2), _p(refl), _p(sig), _p(srcPos), len(srcPos),
                                      _p(recPos), len(recPos), _p(np.ascontiguousarray(model, dt)),
-                                     _p(np.ascontiguousarray(dobs, dt)), int(leg1), int(leg2), _p(image), _p(W),
-                                     _p(P))
+                                     _p(np.ascontiguousarray(dobs, dt)), int(leg1), int(leg2), int(time),
+                                     _p(image), _p(W), _p(P))
     return image
 
 
-def wemva_adj_reg(g, v2dt2, refl, sig, srcPos, recPos, image, dobs, leg1=1, leg2=1, prec="f"):
+def wemva_adj_reg(g, v2dt2, refl, sig, srcPos, recPos, image, dobs, leg1=1, leg2=1, prec="f", time=False):
     dt = _dt(prec)
     model = np.zeros((g.nx, g.nz), dt)
     W = np.zeros((g.nts, g.nx, g.nz), dt)
@@ -390,7 +394,8 @@ def wemva_adj_reg(g, v2dt2, refl, sig, srcPos, recPos, image, dobs, leg1=1, leg2
     gc = g.c()
     _tomo_fns("orc_wemva_adj", prec)(C.byref(gc), _p(v2dt2), _p(refl), _p(sig), _p(srcPos), len(srcPos),
                                      _p(recPos), len(recPos), _p(model), _p(np.ascontiguousarray(image, dt)),
-                                     _p(np.ascontiguousarray(dobs, dt)), int(leg1), int(leg2), _p(W), _p(P))
+                                     _p(np.ascontiguousarray(dobs, dt)), int(leg1), int(leg2), int(time),
+                                     _p(W), _p(P))
     return model
 
 
@@ -560,10 +565,12 @@ def borntime_adjoint(g, vel, sources, wavelets, receivers, data, sloth=0, prec="
     return image
 
 
-def tomo_forward(g, vel, sources, wavelets, receivers, model, ext_refl, leg1=1, leg2=1, prec="f"):
-    """Offset-extended tomographic operator (tomoExtShotsGpu, extension=offset):
-    velocity perturbation [nx][nz] -> data [nShot][nRec][nts], about the
-    extended reflectivity ext_refl [nExt][nx][nz]."""
+def tomo_forward(g, vel, sources, wavelets, receivers, model, ext_refl, leg1=1, leg2=1, prec="f", time=False,
+                 sloth=0):
+    """Extended tomographic operator (tomoExtShotsGpu, extension=offset, or
+    extension=time with ``time=True``): velocity perturbation [nx][nz] ->
+    data [nShot][nRec][nts], about the extended reflectivity ext_refl
+    [nExt][nx][nz]."""
     dt = _dt(prec)
     vel = np.ascontiguousarray(vel, dt)
     v2, rf = vel_scales(g, vel, prec)
@@ -573,15 +580,17 @@ def tomo_forward(g, vel, sources, wavelets, receivers, model, ext_refl, leg1=1, 
     for s in range(nShot):
         src, rec = sources[s], _shot_geom(receivers, s)
         fine = born_source_fine(g, vel, src, _shot_geom(wavelets, s), prec)
-        dreg = tomo_fwd_reg(g, v2, rf, fine, src.regPos, rec.regPos, model, ext_refl, leg1, leg2, prec)
+        dreg = tomo_fwd_reg(g, v2, rf, fine, src.regPos, rec.regPos, model, ext_refl, leg1, leg2, prec, time, sloth)
         data[s] += rec.forward(dreg, prec=prec)
     return data
 
 
-def wemva_forward(g, vel, sources, wavelets, receivers, dobs, model, leg1=1, leg2=1, prec="f"):
-    """Offset-extended WEMVA operator (wemvaExtShotsGpu, extension=offset):
-    velocity perturbation [nx][nz] -> extended image [nExt][nx][nz], about
-    the receiver wavefields of the observed data dobs [nShot][nRec][nts]."""
+def wemva_forward(g, vel, sources, wavelets, receivers, dobs, model, leg1=1, leg2=1, prec="f", time=False):
+    """Extended WEMVA operator (wemvaExtShotsGpu, extension=offset, or
+    extension=time with ``time=True``): velocity perturbation [nx][nz] ->
+    extended image [nExt][nx][nz], about the receiver wavefields of the
+    observed data dobs [nShot][nRec][nts].  Like the reference, it ignores a
+    free surface."""
     dt = _dt(prec)
     vel = np.ascontiguousarray(vel, dt)
     v2, rf = vel_scales(g, vel, prec)
@@ -590,11 +599,11 @@ def wemva_forward(g, vel, sources, wavelets, receivers, dobs, model, leg1=1, leg
         src, rec = sources[s], _shot_geom(receivers, s)
         fine = born_source_fine(g, vel, src, _shot_geom(wavelets, s), prec)
         dreg = rec.adjoint(dobs[s], prec=prec)
-        image += wemva_fwd_reg(g, v2, rf, fine, src.regPos, rec.regPos, model, dreg, leg1, leg2, prec)
+        image += wemva_fwd_reg(g, v2, rf, fine, src.regPos, rec.regPos, model, dreg, leg1, leg2, prec, time)
     return image
 
 
-def wemva_adjoint(g, vel, sources, wavelets, receivers, dobs, image, leg1=1, leg2=1, prec="f"):
+def wemva_adjoint(g, vel, sources, wavelets, receivers, dobs, image, leg1=1, leg2=1, prec="f", time=False):
     dt = _dt(prec)
     vel = np.ascontiguousarray(vel, dt)
     v2, rf = vel_scales(g, vel, prec)
@@ -603,11 +612,12 @@ def wemva_adjoint(g, vel, sources, wavelets, receivers, dobs, image, leg1=1, leg
         src, rec = sources[s], _shot_geom(receivers, s)
         fine = born_source_fine(g, vel, src, _shot_geom(wavelets, s), prec)
         dreg = rec.adjoint(dobs[s], prec=prec)
-        model += wemva_adj_reg(g, v2, rf, fine, src.regPos, rec.regPos, image, dreg, leg1, leg2, prec)
+        model += wemva_adj_reg(g, v2, rf, fine, src.regPos, rec.regPos, image, dreg, leg1, leg2, prec, time)
     return model
 
 
-def tomo_adjoint(g, vel, sources, wavelets, receivers, data, ext_refl, leg1=1, leg2=1, prec="f"):
+def tomo_adjoint(g, vel, sources, wavelets, receivers, data, ext_refl, leg1=1, leg2=1, prec="f", time=False,
+                 sloth=0):
     dt = _dt(prec)
     vel = np.ascontiguousarray(vel, dt)
     v2, rf = vel_scales(g, vel, prec)
@@ -616,5 +626,5 @@ def tomo_adjoint(g, vel, sources, wavelets, receivers, data, ext_refl, leg1=1, l
         src, rec = sources[s], _shot_geom(receivers, s)
         fine = born_source_fine(g, vel, src, _shot_geom(wavelets, s), prec)
         dreg = rec.adjoint(data[s], prec=prec)
-        model += tomo_adj_reg(g, v2, rf, fine, src.regPos, rec.regPos, dreg, ext_refl, leg1, leg2, prec)
+        model += tomo_adj_reg(g, v2, rf, fine, src.regPos, rec.regPos, dreg, ext_refl, leg1, leg2, prec, time, sloth)
     return model

@@ -2,7 +2,8 @@
 C-ABI: nonlinear, Born, extended-Born, tomographic and WEMVA operators,
 bitwise equal to the float oracle, on sub = 2 (whose window capture is
 switched off with a free surface) and sub = 3, with receivers below and on
-the surface row."""
+the surface row.  The WEMVA operator, like the reference's, propagates with
+the absorbing top whatever the key."""
 import numpy as np
 import pytest
 
