@@ -46,11 +46,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-shots", type=int, default=1)
     ap.add_argument("--opt", action="append", default=[], help="engine option key=value (aiso_ctx_set_option)")
-    ap.add_argument("--workload", default="born", choices=["born", "bornext21", "tomo21", "wemva21"],
+    ap.add_argument("--workload", default="born", choices=["born", "bornext21", "borntime21", "tomo21", "wemva21"],
                     help="born: the headline Born fwd+adj; bornext21: subsurface-offset extended Born "
-                         "fwd+adj with 21 offsets on the same grid and shots; tomo21: the offset-extended "
-                         "tomographic operator fwd+adj about a 21-offset reflectivity; wemva21: the "
-                         "offset-extended WEMVA operator fwd+adj (21 offsets)")
+                         "fwd+adj with 21 offsets on the same grid and shots; borntime21: the same with 21 "
+                         "time lags; tomo21: the offset-extended tomographic operator fwd+adj about a "
+                         "21-offset reflectivity; wemva21: the offset-extended WEMVA operator fwd+adj "
+                         "(21 offsets)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy 1 GiB device-to-device 3 times (HBM counter calibration)")
     return ap.parse_args()
@@ -101,6 +102,8 @@ def main():
     from acoustic_isotropic_operators_amd import _capi as A
     from acoustic_isotropic_operators_amd.workloads import born_lsrtm
 
+    if a.workload == "borntime21":
+        return bench_ext(a, torch, A, born_lsrtm, time_lags=True)
     if a.workload == "bornext21":
         return bench_ext(a, torch, A, born_lsrtm)
     if a.workload in ("tomo21", "wemva21"):
@@ -201,12 +204,13 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def bench_ext(a, torch, A, born_lsrtm, nExt=21):
-    """Extended Born (subsurface offset, nExt offsets) forward + adjoint on the
-    headline grid and shots: one JSON line, same fields as the headline where
-    they apply.  Point-updates count the four propagations per shot."""
+def bench_ext(a, torch, A, born_lsrtm, nExt=21, time_lags=False):
+    """Extended Born (nExt subsurface offsets, or time lags) forward + adjoint
+    on the headline grid and shots: one JSON line, same fields as the
+    headline where they apply.  Point-updates count the four propagations per
+    shot."""
     w = born_lsrtm(nShot=a.shots)
-    p = dict(w.params, nExt=nExt, extension=A.EXT_OFFSET)
+    p = dict(w.params, nExt=nExt, extension=A.EXT_TIME if time_lags else A.EXT_OFFSET)
     ctx = A.Context(p)
     ctx.set_velocity(w.vel)
     for kv in a.opt:
@@ -237,10 +241,12 @@ def bench_ext(a, torch, A, born_lsrtm, nExt=21):
     t = (time.perf_counter() - t0) / a.steps
     stats = {n: ctx.profile_read(getattr(A, n)) for n in ("K_SRC", "K_EXT_SCAT", "K_EXT_REC")}
     pts = 4.0 * w.fine_steps * w.interior * w.nShot
-    out = {"metric": f"Gpts/s, extended Born fwd+adj nExt={nExt}, {w.nShot} shots 500x1200", "value": pts / t / 1e9,
+    kind = "time-lag" if time_lags else "extended"
+    out = {"metric": f"Gpts/s, {kind} Born fwd+adj nExt={nExt}, {w.nShot} shots 500x1200", "value": pts / t / 1e9,
            "unit": "Gpts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": t * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"bornext{nExt}_{w.nShot}x500x1200", "nExt": nExt, "shots": w.nShot,
+           "config": {"workload": f"born{'time' if time_lags else 'ext'}{nExt}_{w.nShot}x500x1200", "nExt": nExt,
+                      "shots": w.nShot,
                       "shots_per_launch_adj": ctx.info()["batch"]},
            "step_kernels_ms_per_step": {n: v["ms"] / a.steps for n, v in stats.items() if v["launches"]},
            "finite": bool(torch.isfinite(image).all().item())}
