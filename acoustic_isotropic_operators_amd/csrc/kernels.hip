@@ -1559,7 +1559,9 @@ __global__ void k_ext_adj_finish(const ShotSlot* slots, const float* __restrict_
 // pairs sample k with W sample k - lagDir * 2 (e - hExt).  Both kernels take
 // one thread per interior point of one shot and walk the lag slices in
 // ascending order; the nk samples of the batch live in registers, so each
-// model (or image) value is touched once per batch.
+// model (or image) value is touched once per batch.  The grid is (z blocks,
+// shots, x columns): workgroups dispatched together share the model slices
+// of one column across shots (a shot-major grid measured slower).
 
 // imagingTimeFwdGpu (F/src/kernelsGpu.cu:337-350, lagDir = +1) and
 // imagingTimeTomoAdjGpu (:445-458, lagDir = -1) for nk samples:
@@ -1572,19 +1574,22 @@ __global__ void k_time_ss(ExtBatchArgs a)
     const int ix = FAT + blockIdx.z;
     if (iz >= a.nz - FAT) return;
     const long i = (long)ix * nz + iz;
+    cgf* W = G(S.W) + i;
+    cgf* M = G(a.mext) + i;
+    gf* SS = G(S.ss[0]) + i;
     float acc[kExtBatchMax];
 #pragma unroll
     for (int j = 0; j < kExtBatchMax; ++j) acc[j] = 0.f;
     for (int e = a.eLo; e < a.eHi; ++e) {
-        const float mv = a.mext[nzx * e + i];
+        const float mv = M[nzx * e];
         const long sh = 2L * a.lagDir * (e - a.hExt) * nzx;
 #pragma unroll
         for (int j = 0; j < kExtBatchMax; ++j)
-            if (j < a.nk && e >= a.lo[j] && e < a.hi[j]) acc[j] += mv * S.W[a.wOff[j] - sh + i];
+            if (j < a.nk && e >= a.lo[j] && e < a.hi[j]) acc[j] += mv * W[a.wOff[j] - sh];
     }
 #pragma unroll
     for (int j = 0; j < kExtBatchMax; ++j)
-        if (j < a.nk) S.ss[0][a.ssOff[j] + i] = acc[j];
+        if (j < a.nk) SS[a.ssOff[j]] = acc[j];
 }
 
 // imagingTimeAdjGpu (F/src/kernelsGpu.cu:352-364, lagDir = +1) and
@@ -1598,17 +1603,100 @@ __global__ void k_time_img(ExtBatchArgs a)
     const int ix = FAT + blockIdx.z;
     if (iz >= a.nz - FAT) return;
     const long i = (long)ix * nz + iz;
+    cgf* W = G(S.W) + i;
+    cgf* SS = G(S.ss[0]) + i;
+    gf* IMG = G(S.img) + i;
     float sv[kExtBatchMax];
 #pragma unroll
-    for (int j = 0; j < kExtBatchMax; ++j) sv[j] = j < a.nk ? S.ss[0][a.ssOff[j] + i] : 0.f;
+    for (int j = 0; j < kExtBatchMax; ++j) sv[j] = j < a.nk ? SS[a.ssOff[j]] : 0.f;
     for (int e = a.eLo; e < a.eHi; ++e) {
         const long sh = 2L * a.lagDir * (e - a.hExt) * nzx;
-        float* mp = S.img + nzx * e + i;
-        float m = *mp;
+        float m = IMG[nzx * e];
 #pragma unroll
         for (int j = 0; j < kExtBatchMax; ++j)
-            if (j < a.nk && e >= a.lo[j] && e < a.hi[j]) m += sv[j] * S.W[a.wOff[j] - sh + i];
-        *mp = m;
+            if (j < a.nk && e >= a.lo[j] && e < a.hi[j]) m += sv[j] * W[a.wOff[j] - sh];
+        IMG[nzx * e] = m;
+    }
+}
+
+// The same two operations for a full interior batch: NK consecutive samples
+// (k_j = k0 + DIRK * j) with every lag |l| <= hExt <= HX usable.  The W
+// samples the batch touches, k0 - 2 hExt ... k0 + NK - 1 + 2 hExt, are read
+// once into a register window; slot r of the window holds W sample
+// kmin - 2 HX + r, so that every (sample, lag) pair has a compile-time
+// register index.  Each output receives the general kernels' operations in
+// their order (lags ascending; samples in batch order).
+template <int HX, int NK, int DIRK, int LAGDIR>
+__device__ __forceinline__ void time_window(const ExtBatchArgs& a, cgf* W, long nzx, float (&w)[NK + 4 * HX])
+{
+    const long kmin = DIRK > 0 ? a.wOff[0] / nzx : a.wOff[0] / nzx - (NK - 1);
+    const int h = a.hExt;
+#pragma unroll
+    for (int r = 0; r < NK + 4 * HX; ++r) {
+        const bool need = r >= 2 * (HX - h) && r < NK + 2 * HX + 2 * h;
+        w[r] = need ? W[(kmin - 2 * HX + r) * nzx] : 0.f;
+    }
+}
+
+template <int HX, int NK, int LAGDIR>
+__global__ void __launch_bounds__(64) k_time_ss_win(ExtBatchArgs a)
+{
+    const ShotSlot& S = a.slots[blockIdx.y];
+    const long nz = a.nz, nzx = nz * a.nx;
+    const int iz = FAT + blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = FAT + blockIdx.z;
+    if (iz >= a.nz - FAT) return;
+    const long i = (long)ix * nz + iz;
+    float w[NK + 4 * HX];
+    time_window<HX, NK, 1, LAGDIR>(a, G(S.W) + i, nzx, w);
+    const int h = a.hExt;
+    cgf* M = G(a.mext) + i;
+    float mv[2 * HX + 1];   // every lag's model value in flight before the arithmetic
+#pragma unroll
+    for (int l = -HX; l <= HX; ++l) mv[l + HX] = (l >= -h && l <= h) ? M[nzx * (l + h)] : 0.f;
+    float acc[NK];
+#pragma unroll
+    for (int j = 0; j < NK; ++j) acc[j] = 0.f;
+#pragma unroll
+    for (int l = -HX; l <= HX; ++l) {
+        if (l < -h || l > h) continue;
+#pragma unroll
+        for (int j = 0; j < NK; ++j) acc[j] += mv[l + HX] * w[j - 2 * LAGDIR * l + 2 * HX];
+    }
+    gf* SS = G(S.ss[0]) + i;
+#pragma unroll
+    for (int j = 0; j < NK; ++j) SS[a.ssOff[j]] = acc[j];
+}
+
+template <int HX, int NK, int DIRK, int LAGDIR>
+__global__ void __launch_bounds__(64) k_time_img_win(ExtBatchArgs a)
+{
+    const ShotSlot& S = a.slots[blockIdx.y];
+    const long nz = a.nz, nzx = nz * a.nx;
+    const int iz = FAT + blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = FAT + blockIdx.z;
+    if (iz >= a.nz - FAT) return;
+    const long i = (long)ix * nz + iz;
+    float w[NK + 4 * HX];
+    time_window<HX, NK, DIRK, LAGDIR>(a, G(S.W) + i, nzx, w);
+    cgf* SS = G(S.ss[0]) + i;
+    gf* IMG = G(S.img) + i;
+    float sv[NK];
+#pragma unroll
+    for (int j = 0; j < NK; ++j) sv[j] = SS[a.ssOff[j]];
+    const int h = a.hExt;
+    float m[2 * HX + 1];    // every lag's image value in flight before the arithmetic
+#pragma unroll
+    for (int l = -HX; l <= HX; ++l) m[l + HX] = (l >= -h && l <= h) ? IMG[nzx * (l + h)] : 0.f;
+#pragma unroll
+    for (int l = -HX; l <= HX; ++l) {
+        if (l < -h || l > h) continue;
+#pragma unroll
+        for (int j = 0; j < NK; ++j) {
+            const int jj = DIRK > 0 ? j : NK - 1 - j;   // k_j - kmin
+            m[l + HX] += sv[j] * w[jj - 2 * LAGDIR * l + 2 * HX];
+        }
+        IMG[nzx * (l + h)] = m[l + HX];
     }
 }
 
@@ -2054,17 +2142,51 @@ static bool time_batch_ok(const ExtBatchArgs& a)
     return true;
 }
 
+// A batch the register-window kernels take: kExtBatchMax samples k0 + dirK * j
+// with every lag usable (checked by time_batch_ok to stay inside [0, nts)).
+static bool time_batch_full(const ExtBatchArgs& a, int dirK)
+{
+    if (a.nk != kExtBatchMax || a.hExt > 20 || a.eLo != 0 || a.eHi != 2 * a.hExt + 1) return false;
+    const long nzx = (long)a.nz * a.nx;
+    for (int j = 0; j < a.nk; ++j)
+        if (a.lo[j] != 0 || a.hi[j] != 2 * a.hExt + 1 || a.wOff[j] != a.wOff[0] + dirK * j * nzx) return false;
+    return true;
+}
+
+template <int HX>
+static void time_ss_win(const ExtBatchArgs& a, dim3 g, hipStream_t s)
+{
+    if (a.lagDir > 0) hipLaunchKernelGGL((k_time_ss_win<HX, kExtBatchMax, 1>), g, dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((k_time_ss_win<HX, kExtBatchMax, -1>), g, dim3(64), 0, s, a);
+}
+
+template <int HX, int DIRK>
+static void time_img_win(const ExtBatchArgs& a, dim3 g, hipStream_t s)
+{
+    if (a.lagDir > 0) hipLaunchKernelGGL((k_time_img_win<HX, kExtBatchMax, DIRK, 1>), g, dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((k_time_img_win<HX, kExtBatchMax, DIRK, -1>), g, dim3(64), 0, s, a);
+}
+
 hipError_t launch_time_ss_batch(const ExtBatchArgs& a, int nshots, hipStream_t s)
 {
     if (!time_batch_ok(a)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_time_ss, column_grid(a.nz, a.nx, nshots), dim3(64), 0, s, a);
+    const dim3 g = column_grid(a.nz, a.nx, nshots);
+    if (!time_batch_full(a, 1)) hipLaunchKernelGGL(k_time_ss, g, dim3(64), 0, s, a);
+    else if (a.hExt <= 5) time_ss_win<5>(a, g, s);
+    else if (a.hExt <= 10) time_ss_win<10>(a, g, s);
+    else time_ss_win<20>(a, g, s);
     return hipGetLastError();
 }
 
 hipError_t launch_time_img_batch(const ExtBatchArgs& a, int nshots, hipStream_t s)
 {
     if (!time_batch_ok(a)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_time_img, column_grid(a.nz, a.nx, nshots), dim3(64), 0, s, a);
+    const dim3 g = column_grid(a.nz, a.nx, nshots);
+    const int dirK = time_batch_full(a, 1) ? 1 : time_batch_full(a, -1) ? -1 : 0;
+    if (dirK == 0) hipLaunchKernelGGL(k_time_img, g, dim3(64), 0, s, a);
+    else if (a.hExt <= 5) dirK > 0 ? time_img_win<5, 1>(a, g, s) : time_img_win<5, -1>(a, g, s);
+    else if (a.hExt <= 10) dirK > 0 ? time_img_win<10, 1>(a, g, s) : time_img_win<10, -1>(a, g, s);
+    else dirK > 0 ? time_img_win<20, 1>(a, g, s) : time_img_win<20, -1>(a, g, s);
     return hipGetLastError();
 }
 

@@ -54,6 +54,15 @@ def test_time_lag_matches_oracle(sub, nExt, sloth):
     _check(c, c.grid, ctx, srcs, recs, sloth, (1, 5, 8, 16), 72)
 
 
+@pytest.mark.parametrize("nExt,nts", [(5, 80), (13, 90), (41, 120)])
+def test_full_interior_batches_match_oracle(nExt, nts):
+    # long enough for batches of 16 samples with every lag usable: the
+    # register-window kernels (hExt up to 5, 10, 20) next to the general ones
+    c = make_case(seed=77, nShot=2, nts=nts, sub=1, dts=0.001, nExt=nExt)
+    ctx, srcs, recs = _engine(c)
+    _check(c, c.grid, ctx, srcs, recs, 0, (16, 7), 78)
+
+
 def test_time_lag_free_surface_matches_oracle():
     c = make_case(seed=73, nShot=2, nts=30, sub=2, dts=0.002, nExt=9)
     ctx, srcs, recs = _engine(c, fs=1)

@@ -67,6 +67,15 @@ def test_time_lag_tomo_and_wemva_match_oracle(sub, nExt, sloth, extBatch):
     _run(c, c.grid, ctx, srcs, recs, sloth, extBatch, 92)
 
 
+@pytest.mark.parametrize("nExt,sloth", [(9, 0), (21, 1)])
+def test_full_interior_batches_match_oracle(nExt, sloth):
+    # batches of 16 samples with every lag usable in both lag directions and
+    # both sample orders (the register-window kernels)
+    c = make_case(seed=95, nShot=2, nts=100, sub=1, dts=0.001, nExt=nExt)
+    ctx, srcs, recs = _engine(c)
+    _run(c, c.grid, ctx, srcs, recs, sloth, 16, 96)
+
+
 def test_time_lag_tomo_and_wemva_with_free_surface():
     c = make_case(seed=93, nShot=2, nts=30, sub=2, dts=0.002, nExt=7)
     ctx, srcs, recs = _engine(c, fs=1)
