@@ -992,6 +992,50 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
     float* __restrict__ po = S.p[a.cur ^ 1];
     const float* __restrict__ v2dt2 = a.v2dt2;
 
+    // Per-point operands.  The forward modes load them before staging the
+    // tile, so that their latency overlaps it; the adjoint modes, whose
+    // staging reads the same current slice, load them after it (measured
+    // faster each way on MI355X).  The values are the same either way.
+    const int gz = z0 + lane;
+    const int lx0 = wy * PX;
+    const bool zok = gz < nz - FAT;
+    constexpr bool SSB = MODE & M_SS_BORN, SSP = MODE & M_SS_PRE, SSS = MODE & M_SS_SLICE;
+    constexpr bool HAS_SS = SSB || SSP || SSS;
+    const bool useB = a.fb != 0.f;
+    bool ok[PX];
+    float os[PX], pv[PX];
+    float sA[HAS_SS ? PX : 1], sB[HAS_SS ? PX : 1], mm[SSB ? PX : 1];
+    auto load_operands = [&] {
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+            const int gx = x0 + lx0 + j;
+            ok[j] = zok && gx < nx - FAT;
+            os[j] = 0.f;
+            pv[j] = 0.f;
+            if constexpr (HAS_SS) { sA[j] = 0.f; sB[j] = 0.f; }
+            if constexpr (SSB) mm[j] = 0.f;
+            if (ok[j]) {
+                const long gi = (long)gx * nz + gz;
+                os[j] = po[gi];
+                pv[j] = ADJ ? pc[gi] : v2dt2[gi];
+                if constexpr (SSB) {
+                    mm[j] = a.mtil[gi];
+                    sA[j] = S.W[a.wA + gi];
+                    if (useB) sB[j] = S.W[a.wB + gi];
+                }
+                if constexpr (SSP) {
+                    sA[j] = S.W[a.wA + gi];
+                    if (useB) sB[j] = S.W[a.wB + gi];
+                }
+                if constexpr (SSS) {
+                    sA[j] = S.ss[a.ssA][a.ssOffA + gi];
+                    if (useB) sB[j] = S.ss[a.ssB][a.ssOffB + gi];
+                }
+            }
+        }
+    };
+    if constexpr (!ADJ) load_operands();
+
     // Stage the current slice (times v2dt2 for the adjoint) with its halo.
     for (int lx = wy; lx < LX; lx += NXW) {
         const int gx = x0 - FAT + lx;
@@ -1028,20 +1072,16 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
         }
     }
 
-    const int gz = z0 + lane;
-    const int lx0 = wy * PX;
-    const bool zok = gz < nz - FAT;
+    if constexpr (ADJ) load_operands();
     float w[PX + 2 * FAT];
 #pragma unroll
     for (int j = 0; j < PX + 2 * FAT; ++j) w[j] = sC[(lx0 + j) * LZ + lane + FAT];
 
     float nv[PX], dmp[PX];
     float ost[(MODE & M_WIN) ? PX : 1], cur[(MODE & M_WIN) ? PX : 1];
-    bool ok[PX];
 #pragma unroll
     for (int j = 0; j < PX; ++j) {
         const int gx = x0 + lx0 + j;
-        ok[j] = zok && gx < nx - FAT;
         const float* s = &sC[(lx0 + j + FAT) * LZ + lane + FAT];
         const float cc = w[j + FAT];
         float lap = a.zc[0] * cc
@@ -1069,41 +1109,33 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
             + a.xc[5] * (w[j + 10] + w[j + 0]);
         float n = 0.f, d = 1.f;
         if (ok[j]) {
-            const long gi = (long)gx * nz + gz;
             const bool surf = a.fs && gz == FAT;
             d = a.fs ? sponge_fs(a.cd, gz, gx, nz, nx, a.minPad) : sponge(a.cd, gz, gx, nz, nx, a.minPad);
             // forward: the reference zeroed the old slice's surface value when
             // it was the current slice
-            const float os = (!ADJ && surf) ? 0.f : po[gi];
-            const float o = os * d;                  // second damping of the old slice
+            const float osj = (!ADJ && surf) ? 0.f : os[j];
+            const float o = osj * d;                 // second damping of the old slice
             if constexpr (ADJ) {
-                const float c = pc[gi];
+                const float c = pv[j];
                 n = surf ? o : lap + c + c - o;      // the adjoint leaves the surface row as it is
                 if constexpr (MODE & M_WIN) cur[j] = c;
             } else {
-                n = v2dt2[gi] * lap + cc + cc - o;
+                n = pv[j] * lap + cc + cc - o;
                 if constexpr (MODE & M_WIN) cur[j] = cc;
             }
-            if constexpr (MODE & M_WIN) ost[j] = os;
-            if constexpr (MODE & M_SS_BORN) {
-                const float m = a.mtil[gi];
-                const float sa = m * S.W[a.wA + gi];
-                if (a.fb != 0.f) {
-                    const float sb = m * S.W[a.wB + gi];
+            if constexpr (MODE & M_WIN) ost[j] = osj;
+            if constexpr (SSB) {
+                const float sa = mm[j] * sA[j];
+                if (useB) {
+                    const float sb = mm[j] * sB[j];
                     n += sa * a.fa + sb * a.fb;
                 } else {
                     n += sa * a.fa;
                 }
             }
-            if constexpr (MODE & M_SS_PRE) {
-                const float sa = S.W[a.wA + gi];
-                if (a.fb != 0.f) n += sa * a.fa + S.W[a.wB + gi] * a.fb;
-                else n += sa * a.fa;
-            }
-            if constexpr (MODE & M_SS_SLICE) {
-                const float sa = S.ss[a.ssA][a.ssOffA + gi];
-                if (a.fb != 0.f) n += sa * a.fa + S.ss[a.ssB][a.ssOffB + gi] * a.fb;
-                else n += sa * a.fa;
+            if constexpr (SSP || SSS) {
+                if (useB) n += sA[j] * a.fa + sB[j] * a.fb;
+                else n += sA[j] * a.fa;
             }
         }
         nv[j] = n;
