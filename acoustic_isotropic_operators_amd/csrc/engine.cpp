@@ -142,13 +142,17 @@ struct aiso_ctx {
     bool profile = false;
     // With lanes > 1 the dispatches of different lanes overlap in time, so
     // their durations are those of concurrent kernels.
-    struct Timed { hipEvent_t e0, e1; int kind; double pts; };
+    struct Timed { hipEvent_t e0, e1; int kind; double pts; int n = 1; };   // n launches between e0 and e1
     std::vector<Timed> ev;
     struct Batch { hipEvent_t e0, e1; int shots; };
     std::vector<Batch> batchEv;
     std::vector<std::pair<float, int>> batchDone;
     std::vector<hipEvent_t> evPool;
     long long launches[AISO_K_COUNT] = {0};
+    // Events go on every profEvery-th dispatch of each kind (sampling keeps
+    // the timestamps' cost off the other dispatches).
+    int profEvery = 1;
+    long long profSeq[AISO_K_COUNT] = {0};
     double ms[AISO_K_COUNT] = {0}, pts[AISO_K_COUNT] = {0};   // pts: point-updates (both fields of a pair)
 };
 
@@ -353,6 +357,16 @@ void join_lanes(aiso_ctx* c)
     }
 }
 
+// Whether this dispatch of `kind` carries profiling events.  A sampled
+// dispatch (profEvery > 1) that follows other kernels back to back reads
+// several microseconds longer than the kernel runs (MI355X, against
+// rocprofv3), so sampled figures other than the pair kernel's windows are
+// upper bounds.
+bool sample_event(aiso_ctx* c, int kind)
+{
+    return c->profile && c->profSeq[kind]++ % c->profEvery == 0;
+}
+
 void launch_timed(aiso_ctx* c, unsigned mode, int kind, const StepArgs& a, int nshots, bool generic = false)
 {
     (void)nshots;
@@ -362,7 +376,7 @@ void launch_timed(aiso_ctx* c, unsigned mode, int kind, const StepArgs& a, int n
         if (cnt == 0) continue;
         const StepArgs y = c->curLanes > 1 ? lane_args(a, k0, cnt) : a;
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (c->profile) {
+        if (sample_event(c, kind)) {
             e0 = pooled_event(c);
             e1 = pooled_event(c);
             c->ev.push_back({e0, e1, kind, double(c->p.interior()) * cnt});
@@ -381,7 +395,7 @@ void drain_profile(aiso_ctx* c)
         HIPC(hipEventElapsedTime(&ms, t.e0, t.e1));
         c->ms[t.kind] += ms;
         c->pts[t.kind] += t.pts;
-        c->launches[t.kind]++;
+        c->launches[t.kind] += t.n;
         c->evPool.push_back(t.e0);
         c->evPool.push_back(t.e1);
     }
@@ -808,16 +822,17 @@ void run(aiso_ctx* c, const Request& q)
             HIPC(launch_fill(dreg, (long)nb * nRecRegMax * p.nts, s));
         }
 
-        auto ring = [&](unsigned mode, const StepArgs& sa) {
+        auto ring = [&](unsigned mode, const StepArgs& sa) {   // true if it launched anything
             int mr = 0;
             if (mode & (M_INJ_SIG | M_REC_SIG)) mr = std::max(mr, maxSrcRing);
             if (mode & (M_INJ_DATA | M_REC_DATA)) mr = std::max(mr, maxRecRing);
-            if (mr <= 0) return;
+            if (mr <= 0) return false;
             for (int l = 0; l < c->curLanes; ++l) {
                 int k0, cnt;
                 lane_range(sa.nshots, c->curLanes, l, k0, cnt);
                 if (cnt > 0) HIPC(launch_ring(mode, c->curLanes > 1 ? lane_args(sa, k0, cnt) : sa, mr, lane_stream(c, l)));
             }
+            return true;
         };
         c->curLanes = is_ext(q.op) || tomo || pair2 || adj2 ? 1 : std::max(1, std::min(c->lanes, nb));
         if (c->curLanes > 1) ensure_lanes(c);
@@ -879,7 +894,7 @@ void run(aiso_ctx* c, const Request& q)
                 a2.ssPrev = m >= 1 ? ring3(m - 1) : 0;
                 a2.its = m - 1;
                 hipEvent_t e0 = nullptr, e1 = nullptr;
-                if (c->profile) {
+                if (sample_event(c, AISO_K_BORN_PAIR)) {
                     e0 = pooled_event(c);
                     e1 = pooled_event(c);
                     c->ev.push_back({e0, e1, AISO_K_BORN_PAIR, 2.0 * (a2.doS + a2.doQ) * double(p.interior()) * nb});
@@ -907,6 +922,26 @@ void run(aiso_ctx* c, const Request& q)
             a.wb = f[h + 2];
             a.wc = f[1];
             fork_lanes(c);
+            // With sampled profiling on one lane, the pair kernel is timed
+            // over windows of profEvery consecutive launches between two
+            // plain events (the average time per launch, back to back); a
+            // window that another kernel entered is dropped.
+            struct { hipEvent_t e0 = nullptr; int n = 0; bool spoiled = false; } pw;
+            const bool windows = c->profile && c->profEvery > 1 && c->curLanes == 1;
+            auto close_window = [&] {
+                if (!pw.e0) return;
+                hipEvent_t e1 = pooled_event(c);
+                HIPC(hipEventRecord(e1, s));
+                if (!pw.spoiled && pw.n > 0) {
+                    c->ev.push_back({pw.e0, e1, AISO_K_BORN_PAIR, 2.0 * double(p.interior()) * nb * pw.n, pw.n});
+                } else {
+                    c->evPool.push_back(pw.e0);
+                    c->evPool.push_back(e1);
+                }
+                pw.e0 = nullptr;
+                pw.n = 0;
+                pw.spoiled = false;
+            };
             for (int t = 1; t <= T + 2; ++t) {
                 const bool doS = t <= T, doQ = t - 2 >= 1;
                 const int itsS = (t - 1) / p.sub, it2S = t - itsS * p.sub;
@@ -925,28 +960,43 @@ void run(aiso_ctx* c, const Request& q)
                 x.fe = f[it2Q];
                 x.ff = f[h + it2Q];
                 if (doS && doQ) {
-                    for (int l = 0; l < c->curLanes; ++l) {
-                        int k0, cnt;
-                        lane_range(nb, c->curLanes, l, k0, cnt);
-                        if (cnt == 0) continue;
-                        hipEvent_t e0 = nullptr, e1 = nullptr;
-                        if (c->profile) {
-                            e0 = pooled_event(c);
-                            e1 = pooled_event(c);
-                            c->ev.push_back({e0, e1, AISO_K_BORN_PAIR, 2.0 * double(p.interior()) * cnt});
+                    if (windows) {
+                        if (!pw.e0) {
+                            pw.e0 = pooled_event(c);
+                            HIPC(hipEventRecord(pw.e0, s));
                         }
-                        HIPC(launch_fwd_pair(c->tz, c->tx, c->curLanes > 1 ? lane_args(x, k0, cnt) : x, lane_stream(c, l),
-                                             e0, e1));
+                        HIPC(launch_fwd_pair(c->tz, c->tx, x, s, nullptr, nullptr));
+                        ++pw.n;
+                    } else {
+                        for (int l = 0; l < c->curLanes; ++l) {
+                            int k0, cnt;
+                            lane_range(nb, c->curLanes, l, k0, cnt);
+                            if (cnt == 0) continue;
+                            hipEvent_t e0 = nullptr, e1 = nullptr;
+                            if (sample_event(c, AISO_K_BORN_PAIR)) {
+                                e0 = pooled_event(c);
+                                e1 = pooled_event(c);
+                                c->ev.push_back({e0, e1, AISO_K_BORN_PAIR, 2.0 * double(p.interior()) * cnt});
+                            }
+                            HIPC(launch_fwd_pair(c->tz, c->tx, c->curLanes > 1 ? lane_args(x, k0, cnt) : x,
+                                                 lane_stream(c, l), e0, e1));
+                        }
                     }
-                    ring(MODE_SRC_WIN, x);
+                    bool other = ring(MODE_SRC_WIN, x);
                     StepArgs xq = x;
                     xq.slots = dSlotsQ;
                     xq.cur = curQ;
-                    ring(MODE_BORN_SCAT_PRE, xq);
+                    other = ring(MODE_BORN_SCAT_PRE, xq) || other;
+                    if (windows) {
+                        pw.spoiled = pw.spoiled || other;
+                        if (pw.n == c->profEvery) close_window();
+                    }
                 } else if (doS) {
+                    if (windows) close_window();
                     launch_timed(c, MODE_SRC_WIN, AISO_K_SRC, x, nb);
                     ring(MODE_SRC_WIN, x);
                 } else {
+                    if (windows) close_window();
                     StepArgs xq = x;
                     xq.slots = dSlotsQ;
                     xq.cur = curQ;
@@ -957,7 +1007,8 @@ void run(aiso_ctx* c, const Request& q)
                 }
                 if (doS) curS ^= 1;
                 if (doQ) curQ ^= 1;
-                if (t == T)   // last coarse sample of the source wavefield
+                if (t == T) {   // last coarse sample of the source wavefield
+                    if (windows) close_window();
                     for (int l = 0; l < c->curLanes; ++l) {
                         int k0, cnt;
                         lane_range(nb, c->curLanes, l, k0, cnt);
@@ -965,7 +1016,9 @@ void run(aiso_ctx* c, const Request& q)
                             HIPC(launch_win_tail(dSlots + k0, cnt, curS, WIN_SS, f[h + 1], f[h + 2], ring3(p.nts - 1),
                                                  0, 0, mtil, p.nz, p.nx, lane_stream(c, l)));
                     }
+                }
             }
+            if (windows) close_window();
             join_lanes(c);
             for (int k = 0; k < nb; ++k) {
                 const int shot = b0 + k;
@@ -1016,7 +1069,7 @@ void run(aiso_ctx* c, const Request& q)
                 a2.injB = 2 * m + 1;
                 a2.ssNew = (long)m * nzx;
                 hipEvent_t e0 = nullptr, e1 = nullptr;
-                if (c->profile) {
+                if (sample_event(c, AISO_K_SRC)) {
                     e0 = pooled_event(c);
                     e1 = pooled_event(c);
                     c->ev.push_back({e0, e1, AISO_K_SRC, 2.0 * double(p.interior()) * nb});
@@ -1048,7 +1101,7 @@ void run(aiso_ctx* c, const Request& q)
                 r2.wImg = (long)(its + 1) * nzx;
                 r2.tail = its == 0;
                 hipEvent_t e0 = nullptr, e1 = nullptr;
-                if (c->profile) {
+                if (sample_event(c, AISO_K_BORN_REC)) {
                     e0 = pooled_event(c);
                     e1 = pooled_event(c);
                     c->ev.push_back({e0, e1, AISO_K_BORN_REC, 2.0 * double(p.interior()) * nb});
@@ -1942,6 +1995,10 @@ aiso_status aiso_ctx_set_option(aiso_ctx* c, const char* key, int value)
             c->spb = value;
         }
         else if (k == "cacheBudgetMB") c->cacheBudget = (long)std::max(1, value) << 20;
+        else if (k == "profileEvery") {
+            if (value < 1) throw Error(AISO_ERR_PARAM, "profileEvery must be >= 1");
+            c->profEvery = value;
+        }
         else throw Error(AISO_ERR_PARAM, "unknown option " + k);
     });
 }
@@ -1974,6 +2031,7 @@ aiso_status aiso_ctx_profile(aiso_ctx* c, int enable)
             c->launches[k] = 0;
             c->ms[k] = 0;
             c->pts[k] = 0;
+            c->profSeq[k] = 0;
         }
         c->batchDone.clear();
     });

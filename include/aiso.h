@@ -105,6 +105,8 @@ aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
  *   "batchRec": group size of the adjoint receiver pass within a batch;
  *   "shotsPerBlock" 1/2: shots advanced by one workgroup of the step kernel;
  *   "cacheBudgetMB": on-die cache budget the automatic batch size fills;
+ *   "profileEvery" N >= 1: with profiling on, events on every N-th dispatch
+ *       of each kernel kind (default 1: every dispatch);
  *   "extBatch" 1..16: coarse samples per offset-imaging / secondary-source
  *       launch of the extended Born operators (default 16);
  *   "lanes" 1..4: streams the shots of a launch are split over (default 1);
