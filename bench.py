@@ -51,12 +51,13 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-shots", type=int, default=1)
     ap.add_argument("--opt", action="append", default=[], help="engine option key=value (aiso_ctx_set_option)")
-    ap.add_argument("--workload", default="born", choices=["born", "bornext21", "borntime21", "tomo21", "wemva21"],
+    ap.add_argument("--workload", default="born",
+                    choices=["born", "bornext21", "borntime21", "tomo21", "wemva21", "tomotime21", "wemvatime21"],
                     help="born: the headline Born fwd+adj; bornext21: subsurface-offset extended Born "
                          "fwd+adj with 21 offsets on the same grid and shots; borntime21: the same with 21 "
                          "time lags; tomo21: the offset-extended tomographic operator fwd+adj about a "
                          "21-offset reflectivity; wemva21: the offset-extended WEMVA operator fwd+adj "
-                         "(21 offsets)")
+                         "(21 offsets); tomotime21, wemvatime21: the same with 21 time lags")
     ap.add_argument("--profile-every", type=int, default=16,
                     help="profiling in the timed region: the pair kernel timed over windows of N consecutive "
                          "launches between two events, other step kernels on every N-th launch (1: events on "
@@ -115,8 +116,9 @@ def main():
         return bench_ext(a, torch, A, born_lsrtm, time_lags=True)
     if a.workload == "bornext21":
         return bench_ext(a, torch, A, born_lsrtm)
-    if a.workload in ("tomo21", "wemva21"):
-        return bench_tomo(a, torch, A, born_lsrtm, wemva=a.workload == "wemva21")
+    if a.workload in ("tomo21", "wemva21", "tomotime21", "wemvatime21"):
+        return bench_tomo(a, torch, A, born_lsrtm, wemva=a.workload.startswith("wemva"),
+                          time_lags="time" in a.workload)
     w = born_lsrtm(nShot=a.shots)
     p = w.params
     ctx = A.Context(p)
@@ -226,6 +228,7 @@ def bench_ext(a, torch, A, born_lsrtm, nExt=21, time_lags=False):
     for kv in a.opt:
         k, v = kv.split("=")
         ctx.set_option(k, int(v))
+    ctx.set_option("profileEvery", a.profile_every)
     srcs = [A.Device.grid(1, iz, 1, 1, ix, 1, p["nz"], p["nx"], p["nts"]) for iz, ix in w.src]
     rec = [A.Device.grid(*w.rec, p["nz"], p["nx"], p["nts"])]
     dev = torch.device("cuda:0")
@@ -258,23 +261,24 @@ def bench_ext(a, torch, A, born_lsrtm, nExt=21, time_lags=False):
            "config": {"workload": f"born{'time' if time_lags else 'ext'}{nExt}_{w.nShot}x500x1200", "nExt": nExt,
                       "shots": w.nShot,
                       "shots_per_launch_adj": ctx.info()["batch"]},
-           "step_kernels_ms_per_step": {n: v["ms"] / a.steps for n, v in stats.items() if v["launches"]},
+           "step_kernels_avg_us_sampled": {n: 1e3 * v["ms"] / v["launches"] for n, v in stats.items() if v["launches"]},
            "finite": bool(torch.isfinite(image).all().item())}
     print(json.dumps(out), flush=True)
 
 
-def bench_tomo(a, torch, A, born_lsrtm, nExt=21, wemva=False):
+def bench_tomo(a, torch, A, born_lsrtm, nExt=21, wemva=False, time_lags=False):
     """Offset-extended tomographic (or WEMVA) operator, both legs, forward +
     adjoint on the headline grid: one JSON line.  Point-updates count the
     propagations per shot: tomographic source + 4 forward (two per leg),
     source + receiver + 2 adjoint; WEMVA source + receiver + 2 each way."""
     w = born_lsrtm(nShot=a.shots)
-    p = dict(w.params, nExt=nExt, extension=A.EXT_OFFSET)
+    p = dict(w.params, nExt=nExt, extension=A.EXT_TIME if time_lags else A.EXT_OFFSET)
     ctx = A.Context(p)
     ctx.set_velocity(w.vel)
     for kv in a.opt:
         k, v = kv.split("=")
         ctx.set_option(k, int(v))
+    ctx.set_option("profileEvery", a.profile_every)
     srcs = [A.Device.grid(1, iz, 1, 1, ix, 1, p["nz"], p["nx"], p["nts"]) for iz, ix in w.src]
     rec = [A.Device.grid(*w.rec, p["nz"], p["nx"], p["nts"])]
     dev = torch.device("cuda:0")
@@ -308,14 +312,15 @@ def bench_tomo(a, torch, A, born_lsrtm, nExt=21, wemva=False):
     t = (time.perf_counter() - t0) / a.steps
     stats = {n: ctx.profile_read(getattr(A, n)) for n in ("K_SRC", "K_TOMO")}
     pts = (8.0 if wemva else 9.0) * w.fine_steps * w.interior * w.nShot
-    name = "WEMVA" if wemva else "tomographic"
+    name = ("WEMVA" if wemva else "tomographic") + (" time-lag" if time_lags else "")
     out = {"metric": f"Gpts/s, {name} fwd+adj nExt={nExt}, {w.nShot} shots 500x1200", "value": pts / t / 1e9,
            "unit": "Gpts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": t * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": f"{'wemva' if wemva else 'tomo'}{nExt}_{w.nShot}x500x1200", "nExt": nExt,
+           "config": {"workload": f"{'wemva' if wemva else 'tomo'}{'time' if time_lags else ''}{nExt}_{w.nShot}x500x1200",
+                      "nExt": nExt,
                       "shots": w.nShot,
                       "shots_per_launch": ctx.info()["batch"]},
-           "step_kernels_ms_per_step": {n: v["ms"] / a.steps for n, v in stats.items() if v["launches"]},
+           "step_kernels_avg_us_sampled": {n: 1e3 * v["ms"] / v["launches"] for n, v in stats.items() if v["launches"]},
            "finite": bool(torch.isfinite(grad).all().item()) and bool(torch.isfinite(data).all().item())}
     print(json.dumps(out), flush=True)
 
