@@ -120,6 +120,39 @@ def born_lsrtm(nShot=80, seed=2024, nts=2000, dts=0.002, sub=2):
     return Workload("born_lsrtm_80x500x1200", params, vp, wav, src, rec, mp)
 
 
+def fwi_layered(nShot=32, seed=31, nts=4000, dts=0.001, sub=2):
+    """400 x 1000 grid at 10 m, 50-point sponge (alpha 0.95): a seeded model of
+    8 gently dipping layers (1500-4500 m/s) with a salt-like 4500 m/s blob
+    (the true model), and that model smoothed by a 500 m Gaussian (the
+    starting model, returned as `vel`; the true one as `model`).  nShot
+    sources and 1000 receivers at 20 m depth, nts=4000 at 1 ms (sub 2,
+    Courant 0.225), 15 Hz Ricker."""
+    rng = np.random.default_rng(seed)
+    nz, nx, d, pad = 400, 1000, 10.0, 50
+    zz, xx = np.meshgrid(np.arange(nz, dtype=np.float64), np.arange(nx, dtype=np.float64))
+    tops = np.sort(rng.uniform(0.08, 0.92, 7)) * nz
+    slopes = rng.uniform(-0.08, 0.08, 7)
+    v = np.full((nx, nz), 1500.0)
+    for k in range(7):
+        below = zz >= tops[k] + slopes[k] * (xx - nx / 2)
+        v[below] = 1500.0 + 3000.0 * (k + 1) / 7.0
+    cz, cx = rng.uniform(0.35, 0.6) * nz, rng.uniform(0.3, 0.7) * nx
+    blob = ((zz - cz) / (0.12 * nz)) ** 2 + ((xx - cx) / (0.15 * nx)) ** 2 <= 1.0
+    v[blob] = 4500.0
+    v0 = _smooth(v, 50.0)                    # 500 m Gaussian
+    vt, zPlus, xPlus = pad_model(v.astype(np.float32), pad, pad)
+    vs, _, _ = pad_model(v0.astype(np.float32), pad, pad)
+    NX, NZ = vs.shape
+    params = dict(nz=NZ, nx=NX, dz=d, dx=d, nts=nts, dts=dts, sub=sub, zPadMinus=pad, zPadPlus=zPlus,
+                  xPadMinus=pad, xPadPlus=xPlus, alphaCos=0.95, fMax=37.5)
+    izs = 3 - 1 + pad + FAT                  # 20 m depth
+    spacing = nx // nShot
+    src = [(izs, FAT + pad + spacing // 2 + spacing * s) for s in range(nShot)]
+    rec = (1, izs, 1, nx, FAT + pad, 1)
+    wav = ricker(nts, dts, 15.0, 0.1)[None, :]
+    return Workload(f"fwi_layered_{nShot}x400x1000", params, vs, wav, src, rec, vt)
+
+
 def nonlinear_small(seed=7, nts=1000, dts=0.004, sub=4):
     """200 x 400 grid at 10 m, 50-point sponge (alpha 0.95), 1500-3500 m/s
     gradient plus one Gaussian anomaly (+/-300 m/s, sigma 200 m), one source

@@ -52,12 +52,15 @@ def parse():
     ap.add_argument("--cpu-shots", type=int, default=1)
     ap.add_argument("--opt", action="append", default=[], help="engine option key=value (aiso_ctx_set_option)")
     ap.add_argument("--workload", default="born",
-                    choices=["born", "bornext21", "borntime21", "tomo21", "wemva21", "tomotime21", "wemvatime21"],
+                    choices=["born", "bornext21", "borntime21", "tomo21", "wemva21", "tomotime21", "wemvatime21",
+                             "fwi32"],
                     help="born: the headline Born fwd+adj; bornext21: subsurface-offset extended Born "
                          "fwd+adj with 21 offsets on the same grid and shots; borntime21: the same with 21 "
                          "time lags; tomo21: the offset-extended tomographic operator fwd+adj about a "
                          "21-offset reflectivity; wemva21: the offset-extended WEMVA operator fwd+adj "
-                         "(21 offsets); tomotime21, wemvatime21: the same with 21 time lags")
+                         "(21 offsets); tomotime21, wemvatime21: the same with 21 time lags; fwi32: L2 misfit and "
+                         "its adjoint-state velocity gradient through torch.autograd, 32 shots on 400x1000 "
+                         "(use --shots 32)")
     ap.add_argument("--profile-every", type=int, default=16,
                     help="profiling in the timed region: the pair kernel timed over windows of N consecutive "
                          "launches between two events, other step kernels on every N-th launch (1: events on "
@@ -116,6 +119,8 @@ def main():
         return bench_ext(a, torch, A, born_lsrtm, time_lags=True)
     if a.workload == "bornext21":
         return bench_ext(a, torch, A, born_lsrtm)
+    if a.workload == "fwi32":
+        return bench_fwi(a, torch, A)
     if a.workload in ("tomo21", "wemva21", "tomotime21", "wemvatime21"):
         return bench_tomo(a, torch, A, born_lsrtm, wemva=a.workload.startswith("wemva"),
                           time_lags="time" in a.workload)
@@ -322,6 +327,54 @@ def bench_tomo(a, torch, A, born_lsrtm, nExt=21, wemva=False, time_lags=False):
                       "shots_per_launch": ctx.info()["batch"]},
            "step_kernels_avg_us_sampled": {n: 1e3 * v["ms"] / v["launches"] for n, v in stats.items() if v["launches"]},
            "finite": bool(torch.isfinite(grad).all().item()) and bool(torch.isfinite(data).all().item())}
+    print(json.dumps(out), flush=True)
+
+
+def bench_fwi(a, torch, A):
+    """FWI step on the layered model with a salt blob: nonlinear data of the
+    starting model, L2 misfit against the true model's data, and the
+    adjoint-state velocity gradient (checkpointed recomputation every
+    sqrt(fine steps)), through torch.autograd with every tensor on the GPU.
+    Point-updates count 4 propagations per shot: the forward, and the
+    gradient's checkpointing forward, recomputation and adjoint."""
+    from acoustic_isotropic_operators_amd.fwi import NonlinearVelocityOperator
+    from acoustic_isotropic_operators_amd.workloads import fwi_layered
+    w = fwi_layered(nShot=a.shots)
+    p = w.params
+    op = NonlinearVelocityOperator(p, [("grid", (1, iz, 1, 1, ix, 1)) for iz, ix in w.src], [("grid", w.rec)],
+                                   w.wavelet[None])
+    for kv in a.opt:
+        k, v = kv.split("=")
+        op.context.set_option(k, int(v))
+    dev = torch.device("cuda:0")
+    with torch.no_grad():
+        observed = op(torch.from_numpy(w.model).to(dev))
+    v0 = torch.from_numpy(w.vel).to(dev)
+    torch.cuda.synchronize()
+
+    def step():
+        v = v0.clone().requires_grad_(True)
+        r = op(v) - observed
+        loss = 0.5 * (r * r).sum()
+        loss.backward()
+        return loss, v.grad
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        loss, grad = step()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / a.steps
+    pts = 4.0 * w.fine_steps * w.interior * w.nShot
+    out = {"metric": f"Gpts/s, FWI misfit + velocity gradient, {w.nShot} shots 400x1000", "value": pts / t / 1e9,
+           "unit": "Gpts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": t * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": w.name, "shots": w.nShot, "nts": p["nts"], "sub": p["sub"],
+                      "grid": "400x1000 (padded %dx%d)" % (p["nz"], p["nx"])},
+           "gradients_per_s": 1.0 / t, "misfit": float(loss.item()),
+           "finite": bool(torch.isfinite(grad).all().item())}
     print(json.dumps(out), flush=True)
 
 
