@@ -23,10 +23,19 @@ def pad_plus(n, pad, block=16):
     return int(np.ceil(np.float32(total) / np.float32(block))) * block - n - pad
 
 
-def pad_model(m, zPad, xPad, block=16):
-    """Edge-replicate a physical [nx][nz] model into the padded grid; FAT ring = 0."""
+def pad_plus_free_surface(nz, zPad, block=16):
+    """zPadPlus with a free surface, F/src/padFileGpuMain.cpp:37-47: no top
+    padding, all of it below."""
+    return int(np.ceil(np.float32(zPad + nz) / np.float32(block))) * block - nz
+
+
+def pad_model(m, zPad, xPad, block=16, free_surface=False):
+    """Edge-replicate a physical [nx][nz] model into the padded grid; FAT ring = 0.
+    With a free surface the model starts on the first interior row."""
     nx, nz = m.shape
     zPlus, xPlus = pad_plus(nz, zPad, block), pad_plus(nx, xPad, block)
+    if free_surface:
+        zPlus, zPad = pad_plus_free_surface(nz, zPad, block), 0
     NZ, NX = nz + zPad + zPlus + 2 * FAT, nx + xPad + xPlus + 2 * FAT
     out = np.zeros((NX, NZ), np.float32)
     z0, x0 = FAT + zPad, FAT + xPad
@@ -82,12 +91,13 @@ def _smooth(field, sigma):
     return gaussian_filter(field, sigma, mode="nearest")
 
 
-def born_lsrtm(nShot=80, seed=2024, nts=2000, dts=0.002, sub=2):
+def born_lsrtm(nShot=80, seed=2024, nts=2000, dts=0.002, sub=2, free_surface=False):
     """500 x 1200 grid at 10 m, 50-point sponge (alpha 0.95), 1500-4000 m/s
     gradient with +/-5 % smoothed random perturbations (sigma 300 m),
     nShot sources and 1200 receivers at 20 m depth, nts=2000 at 2 ms (sub 2,
     Courant ~0.42), 15 Hz Ricker, reflectivity = sparse N(0,1) spikes on 10
-    dipping interfaces."""
+    dipping interfaces.  free_surface: the same with a free surface on the
+    model's top row instead of the top sponge."""
     rng = np.random.default_rng(seed)
     nz, nx, d, pad = 500, 1200, 10.0, 50
     z = np.arange(nz) / (nz - 1)
@@ -103,21 +113,23 @@ def born_lsrtm(nShot=80, seed=2024, nts=2000, dts=0.002, sub=2):
         zi = np.clip(np.round(z0 + slope * (xs - nx / 2)).astype(int), 0, nz - 1)
         keep = rng.random(nx) < 0.5          # ~1 % of all model points carry a spike
         refl[xs[keep], zi[keep]] = rng.standard_normal(int(keep.sum())).astype(np.float32)
-    vp, zPlus, xPlus = pad_model(v, pad, pad)
-    mp, _, _ = pad_model(refl, pad, pad)
+    vp, zPlus, xPlus = pad_model(v, pad, pad, free_surface=free_surface)
+    mp, _, _ = pad_model(refl, pad, pad, free_surface=free_surface)
+    zTop = 0 if free_surface else pad
     # reflectivity lives only in the physical domain
     mask = np.zeros_like(mp)
-    mask[FAT + pad:FAT + pad + nx, FAT + pad:FAT + pad + nz] = 1.0
+    mask[FAT + pad:FAT + pad + nx, FAT + zTop:FAT + zTop + nz] = 1.0
     mp *= mask
     NX, NZ = vp.shape
-    params = dict(nz=NZ, nx=NX, dz=d, dx=d, nts=nts, dts=dts, sub=sub, zPadMinus=pad, zPadPlus=zPlus,
-                  xPadMinus=pad, xPadPlus=xPlus, alphaCos=0.95, fMax=37.5)
-    izs = 3 - 1 + pad + FAT                  # zSource=3 -> 20 m depth
+    params = dict(nz=NZ, nx=NX, dz=d, dx=d, nts=nts, dts=dts, sub=sub, zPadMinus=zTop, zPadPlus=zPlus,
+                  xPadMinus=pad, xPadPlus=xPlus, alphaCos=0.95, fMax=37.5, freeSurface=int(free_surface))
+    izs = 3 - 1 + zTop + FAT                 # zSource=3 -> 20 m depth
     spacing = nx // nShot
     src = [(izs, FAT + pad + spacing // 2 + spacing * s) for s in range(nShot)]
     rec = (1, izs, 1, nx, FAT + pad, 1)      # every column at 20 m depth
     wav = ricker(nts, dts, 15.0, 0.1)[None, :]
-    return Workload("born_lsrtm_80x500x1200", params, vp, wav, src, rec, mp)
+    name = "born_lsrtm_fs_80x500x1200" if free_surface else "born_lsrtm_80x500x1200"
+    return Workload(name, params, vp, wav, src, rec, mp)
 
 
 def fwi_layered(nShot=32, seed=31, nts=4000, dts=0.001, sub=2):

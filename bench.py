@@ -52,9 +52,9 @@ def parse():
     ap.add_argument("--cpu-shots", type=int, default=1)
     ap.add_argument("--opt", action="append", default=[], help="engine option key=value (aiso_ctx_set_option)")
     ap.add_argument("--workload", default="born",
-                    choices=["born", "bornext21", "borntime21", "tomo21", "wemva21", "tomotime21", "wemvatime21",
-                             "fwi32"],
-                    help="born: the headline Born fwd+adj; bornext21: subsurface-offset extended Born "
+                    choices=["born", "bornfs", "bornext21", "borntime21", "tomo21", "wemva21", "tomotime21",
+                             "wemvatime21", "fwi32"],
+                    help="born: the headline Born fwd+adj; bornfs: the same with a free surface; bornext21: subsurface-offset extended Born "
                          "fwd+adj with 21 offsets on the same grid and shots; borntime21: the same with 21 "
                          "time lags; tomo21: the offset-extended tomographic operator fwd+adj about a "
                          "21-offset reflectivity; wemva21: the offset-extended WEMVA operator fwd+adj "
@@ -124,7 +124,8 @@ def main():
     if a.workload in ("tomo21", "wemva21", "tomotime21", "wemvatime21"):
         return bench_tomo(a, torch, A, born_lsrtm, wemva=a.workload.startswith("wemva"),
                           time_lags="time" in a.workload)
-    w = born_lsrtm(nShot=a.shots)
+    fs = a.workload == "bornfs"
+    w = born_lsrtm(nShot=a.shots, free_surface=fs)
     p = w.params
     ctx = A.Context(p)
     ctx.set_velocity(w.vel)
@@ -190,7 +191,7 @@ def main():
     pts = 4.0 * w.fine_steps * w.interior * w.nShot
     job_bytes = w.nShot * w.interior * (4 * w.fine_steps * 16.0 + p["nts"] * 28.0)
     out = {
-        "metric": "Gpts/s, Born fwd+adj 80 shots 500x1200",
+        "metric": "Gpts/s, Born fwd+adj 80 shots 500x1200" + (", free surface" if fs else ""),
         "value": pts / t / 1e9,
         "unit": "Gpts/s",
         "n_gpus": 1,
@@ -216,7 +217,7 @@ def main():
         "kernels": kern,
         "finite": ok,
     }
-    if not a.no_cpu:
+    if not a.no_cpu and not fs:   # the CPU baseline is the headline workload's
         out["cpu_baseline"] = cpu_baseline(w, a.cpu_shots)
     print(json.dumps(out), flush=True)
 
