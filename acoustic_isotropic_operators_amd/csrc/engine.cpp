@@ -559,8 +559,8 @@ void run(aiso_ctx* c, const Request& q)
     // ---- batch size from the HBM budget
     // The Born forward with sub == 2 runs its two propagations in lockstep:
     // the secondary source then lives in a 3-slice ring instead of nts slices.
-    // A free surface runs on the generic step kernel, without the window
-    // capture, the lockstep pair or two steps per launch.
+    // A free surface runs on the generic step kernel, without the lockstep
+    // pair or two steps per launch.
     const bool fsurf = p.freeSurface != 0 && !wemva;
     const bool lockstep = q.op == Op::BornFwd && p.sub == 2 && c->lockstep && !fsurf;
     // Two fine steps per launch (k_fwd_pair2) when no source or receiver
@@ -840,7 +840,7 @@ void run(aiso_ctx* c, const Request& q)
         // Three consecutive fine slices hold everything a coarse sample needs
         // when sub == 2, so captures and imaging are written once per coarse
         // step instead of accumulated at every fine step.
-        const bool win = p.sub == 2 && !fsurf;
+        const bool win = p.sub == 2;
         // The extended-Born operators step on the unspecialized kernel: the
         // specialized one ended a GPU run in a memory fault on this path with
         // no cause found in its source (it is exact and in bounds when the

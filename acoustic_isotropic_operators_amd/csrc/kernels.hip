@@ -1121,9 +1121,12 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
                 if constexpr (MODE & M_WIN) cur[j] = c;
             } else {
                 n = pv[j] * lap + cc + cc - o;
-                if constexpr (MODE & M_WIN) cur[j] = cc;
+                // The window takes the slices as stored (their values when
+                // they were new); at a free surface the staged current value
+                // is the mirrored one, so the stored one is read there.
+                if constexpr (MODE & M_WIN) cur[j] = surf ? pc[(long)gx * nz + gz] : cc;
             }
-            if constexpr (MODE & M_WIN) ost[j] = osj;
+            if constexpr (MODE & M_WIN) ost[j] = os[j];
             if constexpr (SSB) {
                 const float sa = mm[j] * sA[j];
                 if (useB) {
@@ -1966,6 +1969,7 @@ hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a,
     case MODE_SRC: k = pick_generic_tile<MODE_SRC>(tz, tx); break;
     case MODE_SRC_WIN: k = pick_generic_tile<MODE_SRC_WIN>(tz, tx); break;
     case MODE_BORN_SCAT: k = pick_generic_tile<MODE_BORN_SCAT>(tz, tx); break;
+    case MODE_BORN_SCAT_PRE: k = pick_generic_tile<MODE_BORN_SCAT_PRE>(tz, tx); break;
     case MODE_EXT_SCAT: k = pick_generic_tile<MODE_EXT_SCAT>(tz, tx); break;
     case MODE_TOMO_CAP: k = pick_generic_tile<MODE_TOMO_CAP>(tz, tx); break;
     case MODE_TOMO_ADJ_CAP: k = pick_generic_tile<MODE_TOMO_ADJ_CAP>(tz, tx); break;
