@@ -1,8 +1,8 @@
 """Free surface (freeSurface = 1, zPadMinus = 0) on the GPU through the
 C-ABI: nonlinear, Born, extended-Born, tomographic and WEMVA operators,
-bitwise equal to the float oracle, on sub = 2 (whose window capture is
-switched off with a free surface) and sub = 3, with receivers below and on
-the surface row.  The WEMVA operator, like the reference's, propagates with
+bitwise equal to the float oracle, on sub = 2 (window capture, which reads
+the stored surface values) and sub = 3, with receivers below and on the
+surface row.  The WEMVA operator, like the reference's, propagates with
 the absorbing top whatever the key."""
 import numpy as np
 import pytest
