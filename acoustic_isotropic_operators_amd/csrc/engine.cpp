@@ -116,6 +116,9 @@ struct aiso_ctx {
     // sloth key (scale by -v^2 dtw^2 instead of 2/v^3 v^2 dtw^2)
     DBuf tscale;
     int sloth = 0;
+    // Born and extended-Born adjoints: one image per shot ([nShot][nExt][nx][nz])
+    // instead of their sum, for callers that batch independent requests.
+    int imagePerShot = 0;
     hipEvent_t evIn = nullptr;          // orders device inputs after the default stream's work
     // Lanes: the shots of a batch split over streams whose kernels run
     // concurrently, so one lane's launch gaps and tails are filled by the
@@ -493,9 +496,10 @@ void run(aiso_ctx* c, const Request& q)
     // WEMVA: the velocity perturbation and the extended image are its two
     // sides (model in / out differ in size); both directions read data.
     const bool wemva = is_wemva(q.op);
-    const long mInN = q.op == Op::WemvaAdj ? nzx * p.nExt : modelN;
-    const long mOutN = q.op == Op::WemvaFwd ? nzx * p.nExt : modelN;
     const int nImg = is_ext(q.op) || q.op == Op::WemvaFwd ? p.nExt : 1;   // slices of the per-shot image
+    const bool perShotImg = c->imagePerShot && (q.op == Op::BornAdj || q.op == Op::ExtAdj);
+    const long mInN = q.op == Op::WemvaAdj ? nzx * p.nExt : modelN;
+    const long mOutN = q.op == Op::WemvaFwd ? nzx * p.nExt : perShotImg ? (long)q.nShot * nImg * nzx : modelN;
     const bool dataInput = is_adj(q.op) || wemva;
     const long dataN = (long)q.nShot * nRecDev * p.nts;
     const int h = p.h();
@@ -529,8 +533,10 @@ void run(aiso_ctx* c, const Request& q)
         dModelIn = mi;
         dModelOut = mo;
     } else {
-        float* m = c->modelDev.need(modelN * 4) ? c->modelDev.as<float>() : nullptr;
-        if (!is_adj(q.op) || q.add) HIPC(hipMemcpyAsync(m, is_adj(q.op) ? q.modelOut : q.modelIn, modelN * 4, hipMemcpyHostToDevice, s));
+        float* m = c->modelDev.need(std::max(modelN, mOutN) * 4) ? c->modelDev.as<float>() : nullptr;
+        if (!is_adj(q.op) || q.add)
+            HIPC(hipMemcpyAsync(m, is_adj(q.op) ? q.modelOut : q.modelIn, (is_adj(q.op) ? mOutN : modelN) * 4,
+                                hipMemcpyHostToDevice, s));
         dModelIn = m;
         dModelOut = m;
     }
@@ -704,7 +710,7 @@ void run(aiso_ctx* c, const Request& q)
         HIPC(launch_scale_ext(dModelIn, r, extScale, p.nz, p.nx, p.nExt, s));
         rExt = r;
     }
-    if (needImg) {
+    if (needImg && !perShotImg) {
         acc = (float*)c->acc.need(nzx * nImg * 4);
         HIPC(launch_fill(acc, nzx * nImg, s));
     }
@@ -1544,7 +1550,11 @@ void run(aiso_ctx* c, const Request& q)
                 }
             }
         }
-        if (needImg) {
+        if (needImg && perShotImg) {
+            for (int k = 0; k < nb; ++k)
+                HIPC(launch_add_to(dModelOut + (size_t)(b0 + k) * nImg * nzx, img + (size_t)k * nImg * nzx, nzx * nImg,
+                                   q.add, s));
+        } else if (needImg) {
             for (int k = 0; k < nb; ++k) HIPC(launch_accumulate(acc, img + (size_t)k * nImg * nzx, nzx * nImg, s));
         }
         if (q.op == Op::NlAdj) {
@@ -1577,7 +1587,7 @@ void run(aiso_ctx* c, const Request& q)
     }
 
     // ---- outputs
-    if (needImg) HIPC(launch_add_to(dModelOut, acc, nzx * nImg, q.add, s));
+    if (needImg && !perShotImg) HIPC(launch_add_to(dModelOut, acc, nzx * nImg, q.add, s));
     if (q.op == Op::NlAdj) {
         if (q.nShotSig == 1)
             for (size_t i = 0; i < nlAcc.size(); ++i) hModel[i] += nlAcc[i];
@@ -1972,6 +1982,7 @@ aiso_status aiso_ctx_set_option(aiso_ctx* c, const char* key, int value)
         else if (k == "leg1") c->leg1 = value != 0;
         else if (k == "leg2") c->leg2 = value != 0;
         else if (k == "sloth") c->sloth = value != 0;
+        else if (k == "imagePerShot") c->imagePerShot = value != 0;
         else if (k == "extBatch") {
             if (value < 1 || value > kExtBatchMax) throw Error(AISO_ERR_PARAM, "extBatch must be 1..16");
             c->extBatch = value;

@@ -165,6 +165,32 @@ def fwi_layered(nShot=32, seed=31, nts=4000, dts=0.001, sub=2):
     return Workload(f"fwi_layered_{nShot}x400x1000", params, vs, wav, src, rec, vt)
 
 
+def serving_models(seed=50):
+    """Four resident velocity models for the serving mix, grids from 200 x 400
+    to 1000 x 3000 at 10 m with nts from 4000 down to 1000: each a
+    1500-4000 m/s gradient with +/-5 % smoothed random perturbations, a
+    50-point sponge, receivers every column at 20 m depth, 15 Hz Ricker.
+    Returns [(id, params, vel, receivers, wavelet, source x range)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, (nz, nx, nts, dts, sub) in enumerate([(200, 400, 4000, 0.002, 2), (500, 1200, 2000, 0.002, 2),
+                                                 (800, 2000, 2000, 0.002, 2), (1000, 3000, 1000, 0.002, 2)]):
+        d, pad = 10.0, 50
+        z = np.arange(nz) / (nz - 1)
+        v = np.broadcast_to(1500.0 + 2500.0 * z, (nx, nz)).astype(np.float64)
+        pert = _smooth(rng.standard_normal((nx, nz)), 20.0)
+        pert /= np.abs(pert).max()
+        v = (v * (1.0 + 0.05 * pert)).astype(np.float32)
+        vp, zPlus, xPlus = pad_model(v, pad, pad)
+        NX, NZ = vp.shape
+        params = dict(nz=NZ, nx=NX, dz=d, dx=d, nts=nts, dts=dts, sub=sub, zPadMinus=pad, zPadPlus=zPlus,
+                      xPadMinus=pad, xPadPlus=xPlus, alphaCos=0.95, fMax=37.5)
+        izs = 3 - 1 + pad + FAT
+        rec = (1, izs, 1, nx, FAT + pad, 1)
+        out.append((f"m{k}_{nz}x{nx}", params, vp, rec, ricker(nts, dts, 15.0, 0.1), (izs, FAT + pad, FAT + pad + nx)))
+    return out
+
+
 def nonlinear_small(seed=7, nts=1000, dts=0.004, sub=4):
     """200 x 400 grid at 10 m, 50-point sponge (alpha 0.95), 1500-3500 m/s
     gradient plus one Gaussian anomaly (+/-300 m/s, sigma 200 m), one source

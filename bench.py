@@ -53,14 +53,17 @@ def parse():
     ap.add_argument("--opt", action="append", default=[], help="engine option key=value (aiso_ctx_set_option)")
     ap.add_argument("--workload", default="born",
                     choices=["born", "bornfs", "bornext21", "borntime21", "tomo21", "wemva21", "tomotime21",
-                             "wemvatime21", "fwi32"],
+                             "wemvatime21", "fwi32", "serve"],
                     help="born: the headline Born fwd+adj; bornfs: the same with a free surface; bornext21: subsurface-offset extended Born "
                          "fwd+adj with 21 offsets on the same grid and shots; borntime21: the same with 21 "
                          "time lags; tomo21: the offset-extended tomographic operator fwd+adj about a "
                          "21-offset reflectivity; wemva21: the offset-extended WEMVA operator fwd+adj "
                          "(21 offsets); tomotime21, wemvatime21: the same with 21 time lags; fwi32: L2 misfit and "
                          "its adjoint-state velocity gradient through torch.autograd, 32 shots on 400x1000 "
-                         "(use --shots 32)")
+                         "(use --shots 32); serve: a Poisson stream of one-shot requests (nonlinear, Born "
+                         "forward/adjoint, 21-offset extended Born adjoint) on 4 resident models")
+    ap.add_argument("--requests", type=int, default=300, help="serve: requests in the stream")
+    ap.add_argument("--rate", type=float, default=50.0, help="serve: mean arrivals per second")
     ap.add_argument("--profile-every", type=int, default=16,
                     help="profiling in the timed region: the pair kernel timed over windows of N consecutive "
                          "launches between two events, other step kernels on every N-th launch (1: events on "
@@ -121,6 +124,8 @@ def main():
         return bench_ext(a, torch, A, born_lsrtm)
     if a.workload == "fwi32":
         return bench_fwi(a, torch, A)
+    if a.workload == "serve":
+        return bench_serve(a)
     if a.workload in ("tomo21", "wemva21", "tomotime21", "wemvatime21"):
         return bench_tomo(a, torch, A, born_lsrtm, wemva=a.workload.startswith("wemva"),
                           time_lags="time" in a.workload)
@@ -376,6 +381,65 @@ def bench_fwi(a, torch, A):
                       "grid": "400x1000 (padded %dx%d)" % (p["nz"], p["nx"])},
            "gradients_per_s": 1.0 / t, "misfit": float(loss.item()),
            "finite": bool(torch.isfinite(grad).all().item())}
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(a):
+    """Serving: a seeded Poisson stream (mean --rate per second) of --requests
+    one-shot requests, mixed nonlinear modeling (30 %), Born forward (30 %),
+    Born adjoint (25 %) and 21-offset extended-Born adjoint (15 %), on 4
+    resident models from 200 x 400 (nts 4000) to 1000 x 3000 (nts 1000), the
+    smaller grids drawn more often.  Host buffers in and out (each request
+    brings its data and takes its result).  Reports requests (shots) per
+    second over the stream and the latency from arrival to completion."""
+    from acoustic_isotropic_operators_amd.serve import Request, Server, run_stream
+    from acoustic_isotropic_operators_amd.workloads import serving_models
+    rng = np.random.default_rng(2026)
+    models = serving_models()
+    srv = Server(max_models=4)
+    info = {}
+    for mid, params, vel, rec, wav, (iz, x0, x1) in models:
+        srv.add_model(mid, params, vel, rec, wav)
+        refl = np.zeros((params["nx"], params["nz"]), np.float32)
+        refl[x0:x1, iz + 40:iz + 43] = 0.1
+        info[mid] = (params, rec[3], iz, x0, x1, refl)
+    kinds = ["nl_forward", "born_forward", "born_adjoint", "bornext_adjoint"]
+
+    def make(k, arrival=0.0):
+        mid = models[rng.choice(4, p=[0.4, 0.3, 0.2, 0.1])][0]
+        params, nrec, iz, x0, x1, refl = info[mid]
+        kind = kinds[rng.choice(4, p=[0.3, 0.3, 0.25, 0.15])]
+        r = Request(kind, mid, (iz, int(rng.integers(x0, x1))), arrival=arrival)
+        if kind == "born_forward":
+            r.payload, r.reflectivity_id = refl, mid
+        elif kind != "nl_forward":
+            r.payload = rng.standard_normal((nrec, params["nts"])).astype(np.float32)
+        return r
+
+    # untimed: one request of every (model, kind), so buffers and geometries exist
+    for mid in info:
+        for kind in kinds:
+            params, nrec, iz, x0, x1, refl = info[mid]
+            r = Request(kind, mid, (iz, x0))
+            if kind == "born_forward":
+                r.payload, r.reflectivity_id = refl, mid
+            elif kind != "nl_forward":
+                r.payload = np.zeros((nrec, params["nts"]), np.float32)
+            srv.submit(r)
+            srv.step()
+    arrivals = np.cumsum(rng.exponential(1.0 / a.rate, a.requests))
+    reqs = [make(k, float(t)) for k, t in enumerate(arrivals)]
+    calls0 = srv.calls
+    t = run_stream(srv, reqs)
+    lat = np.array([r.done - r.arrival for r in reqs]) * 1e3
+    by_kind = {k: sum(r.kind == k for r in reqs) for k in kinds}
+    out = {"metric": "shots/s, serving mix (NL, Born fwd/adj, 21-offset extended Born adj), 4 models 200x400-1000x3000",
+           "value": len(reqs) / t, "unit": "shots/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": t * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "serve_mix_4models", "requests": len(reqs), "arrival_rate_per_s": a.rate,
+                      "requests_by_kind": by_kind, "stream_s": float(arrivals[-1])},
+           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+           "engine_calls": srv.calls - calls0, "finite": all(np.isfinite(r.result).all() for r in reqs)}
     print(json.dumps(out), flush=True)
 
 

@@ -107,6 +107,9 @@ aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
  *   "cacheBudgetMB": on-die cache budget the automatic batch size fills;
  *   "profileEvery" N >= 1: with profiling on, events on every N-th dispatch
  *       of each kernel kind (default 1: every dispatch);
+ *   "imagePerShot" 0/1: aiso_born_adjoint / aiso_bornext_adjoint write one
+ *       image per shot, model [nShot][nExt][nx][nz], instead of their sum
+ *       (for callers that batch independent requests; default 0);
  *   "extBatch" 1..16: coarse samples per offset-imaging / secondary-source
  *       launch of the extended Born operators (default 16);
  *   "lanes" 1..4: streams the shots of a launch are split over (default 1);
