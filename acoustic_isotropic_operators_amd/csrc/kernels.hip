@@ -38,6 +38,23 @@ __device__ __forceinline__ int xcd_logical(int bid, int gridPadded)
     return (bid & 7) * per + (bid >> 3);
 }
 
+// Block (x, y, z) of an nbx x nby x nbz grid launched as one 1-D grid padded
+// to a multiple of 8, x fastest: x-neighbour tiles run on one XCD, so the
+// column halos they share are served from its L2.  False for padding blocks.
+struct Blk3 {
+    int x, y, z;
+};
+__device__ __forceinline__ bool xcd_block3(int nbx, int nby, int nbz, Blk3& b)
+{
+    const int l = xcd_logical(blockIdx.x, gridDim.x);
+    if (l >= nbx * nby * nbz) return false;
+    b.x = l % nbx;
+    const int r = l / nbx;
+    b.z = r % nbz;
+    b.y = r / nbz;
+    return true;
+}
+
 // Streams touched once per coarse step (W, SS, image) can bypass the caches
 // so they do not evict the wavefield slices reused at every fine step.
 __device__ __forceinline__ float ld_stream(const float* p, int nt)
@@ -1337,6 +1354,7 @@ __global__ void k_born_adj_finish(const ShotSlot* slots, int ssR, const float* _
 // time so that the per-thread offset loops index registers only.
 
 constexpr int kExtTZ = 64, kExtTX = 16, kExtCX = 4;
+constexpr int kExtSsSpb = 2;   // shots per block of k_ext_ss_tile: the model registers serve 2 * nk tiles
 static_assert(kExtTX == 4 * kExtCX, "four column groups of kExtCX columns");
 
 // imagingOffsetFwdGpu + scaleSecondarySourceFd (F/src/kernelsGpu.cu:381-396,
@@ -1348,7 +1366,7 @@ static_assert(kExtTX == 4 * kExtCX, "four column groups of kExtCX columns");
 // registers for all of them, and the next pair's W tile is fetched into
 // registers while the current one is computed.
 template <int HX>
-__global__ __launch_bounds__(256) void k_ext_ss_tile(ExtBatchArgs a, int nshots, int spb)
+__global__ __launch_bounds__(256) void k_ext_ss_tile(ExtBatchArgs a, int nshots, int nbx, int nby, int nbz)
 {
     constexpr int NE = 2 * HX + 1;
     constexpr int LW = kExtTX + 4 * HX;        // LDS columns at hExt == HX
@@ -1356,12 +1374,14 @@ __global__ __launch_bounds__(256) void k_ext_ss_tile(ExtBatchArgs a, int nshots,
     constexpr int PF = LW / 4;                 // tile columns fetched per thread (its own row)
     static_assert(LW % 4 == 0, "four column groups fetch the tile");
     __shared__ float Wl[LW * kExtTZ];
+    Blk3 blk;
+    if (!xcd_block3(nbx, nby, nbz, blk)) return;     // padding blocks (block-uniform)
     const int hExt = a.hExt, ne = 2 * hExt + 1, lw = kExtTX + 4 * hExt;
     const long nz = a.nz, nzx = nz * a.nx;
     const int tz = threadIdx.x, g = threadIdx.y;
-    const int z = FAT + blockIdx.z * kExtTZ + tz;
+    const int z = FAT + blk.z * kExtTZ + tz;
     const bool zok = z < a.nz - FAT;
-    const int x0 = FAT + blockIdx.x * kExtTX;   // first output column of the tile
+    const int x0 = FAT + blk.x * kExtTX;        // first output column of the tile
     const int xb = x0 + kExtCX * g;
     int lo[kExtCX], hi[kExtCX];
     float mv[NE][kExtCX];
@@ -1374,31 +1394,53 @@ __global__ __launch_bounds__(256) void k_ext_ss_tile(ExtBatchArgs a, int nshots,
 #pragma unroll
         for (int e = 0; e < NE; ++e) {
             const int h = e - hExt;
-            mv[e][c] = (e < ne && h >= lo[c] && h <= hi[c]) ? a.mext[nzx * e + (long)(x - h) * nz + z] : 0.f;
+            mv[e][c] = (e < ne && h >= lo[c] && h <= hi[c]) ? G(a.mext)[nzx * e + (long)(x - h) * nz + z] : 0.f;
         }
     }
-    const int sh0 = blockIdx.y * spb;
-    const int nIt = max(0, min(spb, nshots - sh0)) * a.nk;
+    const int sh0 = blk.y * kExtSsSpb;
+    const int nsh = max(0, min(kExtSsSpb, nshots - sh0));
+    const int nIt = nsh * a.nk;
+    // The shots' W and ss bases and the thread's v2dt2 values are loaded
+    // once: in the sample loop the only global loads are the next pair's W
+    // tile, so waiting for the current tile never waits for the next one.
+    cgf* W0 = G(nsh > 0 ? a.slots[sh0].W : nullptr);
+    cgf* W1 = G(nsh > 1 ? a.slots[sh0 + 1].W : nullptr);
+    gf* S0 = G(nsh > 0 ? a.slots[sh0].ss[0] : nullptr);
+    gf* S1 = G(nsh > 1 ? a.slots[sh0 + 1].ss[0] : nullptr);
+    float vs[kExtCX];
+#pragma unroll
+    for (int c = 0; c < kExtCX; ++c) {
+        const int x = xb + c;
+        vs[c] = (zok && x < a.nx - FAT) ? G(a.v2dt2)[(long)x * nz + z] : 0.f;
+    }
     // Wl column j holds global column x0 - 2*hExt + j; thread (g, tz)
-    // fetches columns j = 4u + g of its own row.
+    // fetches columns j = 4u + g of its own row.  Columns outside the
+    // interior are taken as 0: the in-range offsets never reach them, and
+    // the out-of-range ones carry a zero model value.
     float pre[PF];
-    auto fetch = [&](int it) {
-        const float* W = a.slots[sh0 + it / a.nk].W + a.wOff[it % a.nk];
+    auto fetch = [&](int q, int k) {
+        cgf* W = (q ? W1 : W0) + a.wOff[k];
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
             const int j = 4 * u + g, gx = x0 - 2 * hExt + j;
-            pre[u] = (zok && j < lw && gx >= 0 && gx < a.nx) ? W[(long)gx * nz + z] : 0.f;
+            pre[u] = (zok && j < lw && gx >= FAT && gx < a.nx - FAT) ? W[(long)gx * nz + z] : 0.f;
         }
     };
     // Window entry i holds column xb + i - 4*HX + 2*hExt, so W_k[x - 2h] of
     // column c and offset index e is entry c - 2e + 4*HX.
     const int jb = xb - x0 - 4 * HX + 4 * hExt;
-    if (nIt > 0) fetch(0);
+    int q = 0, k = 0;
+    if (nIt > 0) fetch(0, 0);
     for (int it = 0; it < nIt; ++it) {
+        int qn = q, kn = k + 1;
+        if (kn == a.nk) {
+            kn = 0;
+            ++qn;
+        }
 #pragma unroll
         for (int u = 0; u < PF; ++u) Wl[(4 * u + g) * kExtTZ + tz] = pre[u];
         __syncthreads();
-        if (it + 1 < nIt) fetch(it + 1);
+        if (it + 1 < nIt) fetch(qn, kn);
         float ww[WW];
 #pragma unroll
         for (int i = 0; i < WW; ++i) {
@@ -1407,21 +1449,22 @@ __global__ __launch_bounds__(256) void k_ext_ss_tile(ExtBatchArgs a, int nshots,
         }
         __syncthreads();
         if (zok) {
-            float* ss = a.slots[sh0 + it / a.nk].ss[0] + a.ssOff[it % a.nk];
+            gf* ss = (q ? S1 : S0) + a.ssOff[k];
 #pragma unroll
             for (int c = 0; c < kExtCX; ++c) {
                 const int x = xb + c;
                 if (x >= a.nx - FAT) continue;
+                // Offsets outside [lo, hi] have mv == 0 and a finite W, so
+                // their products add a zero: the sum is that over [lo, hi]
+                // in the same order, without a mask per product.
                 float acc = 0.f;
 #pragma unroll
-                for (int e = 0; e < NE; ++e) {
-                    const int h = e - hExt;
-                    if (e < ne && h >= lo[c] && h <= hi[c]) acc += mv[e][c] * ww[c - 2 * e + 4 * HX];
-                }
-                const long i = (long)x * nz + z;
-                ss[i] = acc * a.v2dt2[i];
+                for (int e = 0; e < NE; ++e) acc += mv[e][c] * ww[c - 2 * e + 4 * HX];
+                ss[(long)x * nz + z] = acc * vs[c];
             }
         }
+        q = qn;
+        k = kn;
     }
 }
 
@@ -1435,7 +1478,7 @@ __global__ __launch_bounds__(256) void k_ext_ss_tile(ExtBatchArgs a, int nshots,
 // once per batch; the two operands pass through LDS, the next pair's tiles
 // fetched into registers while the current one is computed.
 template <int HX, bool WF>
-__global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nshots, int spb)
+__global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nbx, int nby, int nbz)
 {
     constexpr int NE = 2 * HX + 1;
     constexpr int LW = kExtTX + 2 * HX;
@@ -1443,28 +1486,34 @@ __global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nshots
     constexpr int PF = (LW + 3) / 4;
     __shared__ float Rl[LW * kExtTZ];
     __shared__ float Wl[LW * kExtTZ];
+    Blk3 blk;                                   // blk.y: the shot
+    if (!xcd_block3(nbx, nby, nbz, blk)) return;     // padding blocks (block-uniform)
     const int hExt = a.hExt, ne = 2 * hExt + 1, lw = kExtTX + 2 * hExt;
     const long nz = a.nz, nzx = nz * a.nx;
     const int tz = threadIdx.x, g = threadIdx.y;
-    const int z = FAT + blockIdx.z * kExtTZ + tz;
-    const int x0 = FAT + hExt + blockIdx.x * kExtTX;
+    const int z = FAT + blk.z * kExtTZ + tz;
+    const int x0 = FAT + hExt + blk.x * kExtTX;
     const int xEnd = a.nx - FAT - hExt;
     const bool zok = z < a.nz - FAT;
     const int xb = x0 + kExtCX * g;
-    const int sh0 = blockIdx.y * spb;
-    const int nIt = max(0, min(spb, nshots - sh0)) * a.nk;
+    // The shot's bases are loaded once: in the sample loop the only global
+    // loads are the next sample's tiles, so waiting for the current tiles
+    // never waits for the next ones.
+    const ShotSlot& S = a.slots[blk.y];
+    cgf* ssB = G(S.ss[0]);
+    cgf* WB = G(S.W);
+    gf* img = G(S.img);
     // Rl/Wl column j holds global column x0 - hExt + j; thread (g, tz)
     // fetches columns j = 4u + g of its own row.
     float vv[PF], pr[PF], pw[PF];
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
         const int j = 4 * u + g, gx = x0 - hExt + j;
-        vv[u] = (!WF && zok && j < lw && gx < a.nx) ? a.v2dt2[(long)gx * nz + z] : 0.f;
+        vv[u] = (!WF && zok && j < lw && gx < a.nx) ? G(a.v2dt2)[(long)gx * nz + z] : 0.f;
     }
-    auto fetch = [&](int it) {
-        const ShotSlot& S = a.slots[sh0 + it / a.nk];
-        const float* ss = S.ss[0] + a.ssOff[it % a.nk];
-        const float* W = S.W + a.wOff[it % a.nk];
+    auto fetch = [&](int k) {
+        cgf* ss = ssB + a.ssOff[k];
+        cgf* W = WB + a.wOff[k];
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
             const int j = 4 * u + g, gx = x0 - hExt + j;
@@ -1480,20 +1529,17 @@ __global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nshots
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
             const int gx = xb + i - hExt;
-            vw[i] = (zok && gx < a.nx) ? a.v2dt2[(long)gx * nz + z] : 0.f;
+            vw[i] = (zok && gx < a.nx) ? G(a.v2dt2)[(long)gx * nz + z] : 0.f;
         }
     }
     float m[NE][kExtCX];
-    if (nIt > 0) fetch(0);
-    for (int it = 0; it < nIt; ++it) {
-        float* img = a.slots[sh0 + it / a.nk].img;
-        if (it % a.nk == 0) {
 #pragma unroll
-            for (int e = 0; e < NE; ++e)
+    for (int e = 0; e < NE; ++e)
 #pragma unroll
-                for (int c = 0; c < kExtCX; ++c)
-                    m[e][c] = (e < ne && zok && xb + c < xEnd) ? img[nzx * e + (long)(xb + c) * nz + z] : 0.f;
-        }
+        for (int c = 0; c < kExtCX; ++c)
+            m[e][c] = (e < ne && zok && xb + c < xEnd) ? img[nzx * e + (long)(xb + c) * nz + z] : 0.f;
+    fetch(0);
+    for (int k = 0; k < a.nk; ++k) {
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
             if (4 * u + g < LW) {
@@ -1502,7 +1548,7 @@ __global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nshots
             }
         }
         __syncthreads();
-        if (it + 1 < nIt) fetch(it + 1);
+        if (k + 1 < a.nk) fetch(k + 1);
         float rw[RW], ww[RW];
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
@@ -1520,14 +1566,16 @@ __global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nshots
                     if constexpr (WF) m[e][c] += (ww[c - e + 2 * HX] * rw[c + e]) * vw[c + e];
                     else m[e][c] += rw[c + e] * ww[c - e + 2 * HX];
                 }
-        if (it % a.nk == a.nk - 1) {
-#pragma unroll
-            for (int e = 0; e < NE; ++e)
-#pragma unroll
-                for (int c = 0; c < kExtCX; ++c)
-                    if (e < ne && zok && xb + c < xEnd) img[nzx * e + (long)(xb + c) * nz + z] = m[e][c];
-        }
     }
+    // The store addresses are formed again here rather than kept live
+    // across the sample loop from the loads above.
+    float* imgOut = S.img;
+    asm volatile("" : "+v"(imgOut));
+#pragma unroll
+    for (int e = 0; e < NE; ++e)
+#pragma unroll
+        for (int c = 0; c < kExtCX; ++c)
+            if (e < ne && zok && xb + c < xEnd) G(imgOut)[nzx * e + (long)(xb + c) * nz + z] = m[e][c];
 }
 
 // The same two operations for offsets beyond the register-tiled kernels:
@@ -2082,18 +2130,22 @@ hipError_t launch_born_adj_finish(const ShotSlot* slots, int nshots, int ssR, co
 
 // Register-tiled variants by the bound HX >= hExt; larger offsets take the
 // untiled kernels.
+// Blocks of a grid mapped by xcd_block3: the count rounded up to a multiple of 8.
+static unsigned xcd_padded(int blocks) { return (unsigned)((blocks + 7) / 8 * 8); }
+
 hipError_t launch_ext_ss_batch(const ExtBatchArgs& a, int nshots, hipStream_t s)
 {
     if (a.nk < 1 || a.nk > kExtBatchMax) return hipErrorInvalidValue;
     const int nzi = a.nz - 2 * FAT, nxi = a.nx - 2 * FAT;
     if (nzi <= 0 || nxi <= 0) return hipSuccess;
     if (a.hExt <= 20) {
-        const int spb = 2;   // shots per block: the model registers serve 2 * nk tiles
-        dim3 g((nxi + kExtTX - 1) / kExtTX, (nshots + spb - 1) / spb, (nzi + kExtTZ - 1) / kExtTZ), b(kExtTZ, 4);
-        if (a.hExt <= 2) hipLaunchKernelGGL(k_ext_ss_tile<2>, g, b, 0, s, a, nshots, spb);
-        else if (a.hExt <= 5) hipLaunchKernelGGL(k_ext_ss_tile<5>, g, b, 0, s, a, nshots, spb);
-        else if (a.hExt <= 10) hipLaunchKernelGGL(k_ext_ss_tile<10>, g, b, 0, s, a, nshots, spb);
-        else hipLaunchKernelGGL(k_ext_ss_tile<20>, g, b, 0, s, a, nshots, spb);
+        const int bx = (nxi + kExtTX - 1) / kExtTX, by = (nshots + kExtSsSpb - 1) / kExtSsSpb;
+        const int bz = (nzi + kExtTZ - 1) / kExtTZ;
+        const dim3 g(xcd_padded(bx * by * bz)), b(kExtTZ, 4);
+        if (a.hExt <= 2) hipLaunchKernelGGL(k_ext_ss_tile<2>, g, b, 0, s, a, nshots, bx, by, bz);
+        else if (a.hExt <= 5) hipLaunchKernelGGL(k_ext_ss_tile<5>, g, b, 0, s, a, nshots, bx, by, bz);
+        else if (a.hExt <= 10) hipLaunchKernelGGL(k_ext_ss_tile<10>, g, b, 0, s, a, nshots, bx, by, bz);
+        else hipLaunchKernelGGL(k_ext_ss_tile<20>, g, b, 0, s, a, nshots, bx, by, bz);
     } else {
         dim3 g((nzi + 63) / 64, nshots, nxi);
         hipLaunchKernelGGL(k_ext_ss_any, g, dim3(64), 0, s, a);
@@ -2105,12 +2157,12 @@ template <bool WF>
 static void ext_img_launch(const ExtBatchArgs& a, int nshots, hipStream_t s, int nzi, int nxr)
 {
     if (a.hExt <= 20) {
-        const int spb = 1;
-        dim3 g((nxr + kExtTX - 1) / kExtTX, (nshots + spb - 1) / spb, (nzi + kExtTZ - 1) / kExtTZ), b(kExtTZ, 4);
-        if (a.hExt <= 2) hipLaunchKernelGGL((k_ext_img_tile<2, WF>), g, b, 0, s, a, nshots, spb);
-        else if (a.hExt <= 5) hipLaunchKernelGGL((k_ext_img_tile<5, WF>), g, b, 0, s, a, nshots, spb);
-        else if (a.hExt <= 10) hipLaunchKernelGGL((k_ext_img_tile<10, WF>), g, b, 0, s, a, nshots, spb);
-        else hipLaunchKernelGGL((k_ext_img_tile<20, WF>), g, b, 0, s, a, nshots, spb);
+        const int bx = (nxr + kExtTX - 1) / kExtTX, by = nshots, bz = (nzi + kExtTZ - 1) / kExtTZ;
+        const dim3 g(xcd_padded(bx * by * bz)), b(kExtTZ, 4);
+        if (a.hExt <= 2) hipLaunchKernelGGL((k_ext_img_tile<2, WF>), g, b, 0, s, a, bx, by, bz);
+        else if (a.hExt <= 5) hipLaunchKernelGGL((k_ext_img_tile<5, WF>), g, b, 0, s, a, bx, by, bz);
+        else if (a.hExt <= 10) hipLaunchKernelGGL((k_ext_img_tile<10, WF>), g, b, 0, s, a, bx, by, bz);
+        else hipLaunchKernelGGL((k_ext_img_tile<20, WF>), g, b, 0, s, a, bx, by, bz);
     } else {
         dim3 g((nzi + 63) / 64, nshots, nxr);
         hipLaunchKernelGGL(k_ext_img_any<WF>, g, dim3(64), 0, s, a);
