@@ -1853,6 +1853,107 @@ __global__ void k_tomo_adj_ss(ExtBatchArgs a)
     }
 }
 
+// k_tomo_adj_ss on full tiles: the same sums, in the same order, with the
+// thread's model values r[e][x+h] in registers for all the samples of its
+// shots, v2dt2 staged in LDS once, and each sample's R tile staged through
+// LDS (the next one fetched into registers while the current one is
+// computed).
+// Window entry i of column c is column xb + i - 2*hExt, so the terms of
+// offset index e (h = e - hExt) use entry c + 2e.
+template <int HX>
+__global__ __launch_bounds__(256) void k_tomo_adj_ss_tile(ExtBatchArgs a, int nshots, int nbx, int nby, int nbz)
+{
+    constexpr int NE = 2 * HX + 1;
+    constexpr int LW = kExtTX + 4 * HX;        // LDS columns at hExt == HX
+    constexpr int WN = kExtCX + 4 * HX;        // register windows of R and v2dt2 columns
+    constexpr int PF = LW / 4;
+    static_assert(LW % 4 == 0, "four column groups fetch the tile");
+    __shared__ float Rl[LW * kExtTZ];
+    __shared__ float Vl[LW * kExtTZ];            // v2dt2 on the same columns, staged once
+    Blk3 blk;
+    if (!xcd_block3(nbx, nby, nbz, blk)) return;     // padding blocks (block-uniform)
+    const int hExt = a.hExt, ne = 2 * hExt + 1, lw = kExtTX + 4 * hExt;
+    const long nz = a.nz, nzx = nz * a.nx;
+    const int tz = threadIdx.x, g = threadIdx.y;
+    const int z = FAT + blk.z * kExtTZ + tz;
+    const bool zok = z < a.nz - FAT;
+    const int x0 = FAT + blk.x * kExtTX;
+    const int xb = x0 + kExtCX * g;
+    float mv[NE][kExtCX];
+#pragma unroll
+    for (int c = 0; c < kExtCX; ++c) {
+        const int x = xb + c;
+        int lo = max(-hExt, FAT + hExt - x);
+        const int hi = min(a.nx - 1 - FAT - x - hExt, hExt);
+        if (!zok || x >= a.nx - FAT) lo = hi + 1;
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+            const int h = e - hExt;
+            mv[e][c] = (e < ne && h >= lo && h <= hi) ? G(a.mext)[nzx * e + (long)(x + h) * nz + z] : 0.f;
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        const int j = 4 * u + g, gx = x0 - 2 * hExt + j;
+        Vl[j * kExtTZ + tz] = (zok && gx >= 0 && gx < a.nx) ? G(a.v2dt2)[(long)gx * nz + z] : 0.f;
+    }
+    const int sh0 = blk.y * kExtSsSpb;
+    const int nsh = max(0, min(kExtSsSpb, nshots - sh0));
+    const int nIt = nsh * a.nk;
+    cgf* R0 = G(nsh > 0 ? a.slots[sh0].W : nullptr);
+    cgf* R1 = G(nsh > 1 ? a.slots[sh0 + 1].W : nullptr);
+    gf* S0 = G(nsh > 0 ? a.slots[sh0].ss[0] : nullptr);
+    gf* S1 = G(nsh > 1 ? a.slots[sh0 + 1].ss[0] : nullptr);
+    // Rl column j holds global column x0 - 2*hExt + j; columns outside the
+    // interior are taken as 0 (the in-range offsets never reach them).
+    float pre[PF];
+    auto fetch = [&](int q, int k) {
+        cgf* R = (q ? R1 : R0) + a.wOff[k];
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int j = 4 * u + g, gx = x0 - 2 * hExt + j;
+            pre[u] = (zok && j < lw && gx >= FAT && gx < a.nx - FAT) ? R[(long)gx * nz + z] : 0.f;
+        }
+    };
+    int q = 0, k = 0;
+    if (nIt > 0) fetch(0, 0);
+    for (int it = 0; it < nIt; ++it) {
+        int qn = q, kn = k + 1;
+        if (kn == a.nk) {
+            kn = 0;
+            ++qn;
+        }
+#pragma unroll
+        for (int u = 0; u < PF; ++u) Rl[(4 * u + g) * kExtTZ + tz] = pre[u];
+        __syncthreads();
+        if (it + 1 < nIt) fetch(qn, kn);
+        float rw[WN];
+#pragma unroll
+        for (int i = 0; i < WN; ++i) {
+            const int j = kExtCX * g + i;
+            rw[i] = j < lw ? Rl[j * kExtTZ + tz] : 0.f;
+        }
+        __syncthreads();
+        if (zok) {
+            gf* ss = (q ? S1 : S0) + a.ssOff[k];
+            const float* vl = Vl + kExtCX * g * kExtTZ + tz;   // v2dt2 window entry i at vl[i * kExtTZ]
+#pragma unroll
+            for (int c = 0; c < kExtCX; ++c) {
+                const int x = xb + c;
+                if (x >= a.nx - FAT) continue;
+                // Offsets outside [lo, hi] have mv == 0 and finite R and
+                // v2dt2: they add zeros to the same ordered sum.
+                float acc = 0.f;
+#pragma unroll
+                for (int e = 0; e < NE; ++e) acc += (mv[e][c] * rw[c + 2 * e]) * vl[(c + 2 * e) * kExtTZ];
+                ss[(long)x * nz + z] = acc;
+            }
+        }
+        q = qn;
+        k = kn;
+    }
+}
+
 // imagingAdjGpu (F/src/kernelsGpu.cu:318-325): img += X.W[xOff] * Y.W[yOff]
 // at interior points, img from X's slot.
 __global__ void k_img_acc(const ShotSlot* X, long xOff, const ShotSlot* Y, long yOff, int nz, int nx)
@@ -2198,7 +2299,18 @@ hipError_t launch_mul_batch(const ExtBatchArgs& a, int nshots, hipStream_t s)
 hipError_t launch_tomo_adj_ss(const ExtBatchArgs& a, int nshots, hipStream_t s)
 {
     if (a.nk < 1 || a.nk > kExtBatchMax) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_tomo_adj_ss, column_grid(a.nz, a.nx, nshots), dim3(64), 0, s, a);
+    const int nzi = a.nz - 2 * FAT, nxi = a.nx - 2 * FAT;
+    if (nzi <= 0 || nxi <= 0) return hipSuccess;
+    if (a.hExt <= 10) {
+        const int bx = (nxi + kExtTX - 1) / kExtTX, by = (nshots + kExtSsSpb - 1) / kExtSsSpb;
+        const int bz = (nzi + kExtTZ - 1) / kExtTZ;
+        const dim3 g(xcd_padded(bx * by * bz)), b(kExtTZ, 4);
+        if (a.hExt <= 2) hipLaunchKernelGGL(k_tomo_adj_ss_tile<2>, g, b, 0, s, a, nshots, bx, by, bz);
+        else if (a.hExt <= 5) hipLaunchKernelGGL(k_tomo_adj_ss_tile<5>, g, b, 0, s, a, nshots, bx, by, bz);
+        else hipLaunchKernelGGL(k_tomo_adj_ss_tile<10>, g, b, 0, s, a, nshots, bx, by, bz);
+    } else {
+        hipLaunchKernelGGL(k_tomo_adj_ss, column_grid(a.nz, a.nx, nshots), dim3(64), 0, s, a);
+    }
     return hipGetLastError();
 }
 

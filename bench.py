@@ -134,15 +134,7 @@ def main():
     p = w.params
     ctx = A.Context(p)
     ctx.set_velocity(w.vel)
-    if a.tile:
-        tz, tx = (int(v) for v in a.tile.split(","))
-        ctx.set_tile(tz, tx)
-    if a.batch:
-        ctx.set_batch(a.batch)
-    for kv in a.opt:
-        k, v = kv.split("=")
-        ctx.set_option(k, int(v))
-    ctx.set_option("profileEvery", a.profile_every)
+    configure(ctx, a)
     srcs = [A.Device.grid(1, iz, 1, 1, ix, 1, p["nz"], p["nx"], p["nts"]) for iz, ix in w.src]
     rec = [A.Device.grid(*w.rec, p["nz"], p["nx"], p["nts"])]
     nrec = rec[0].nDev
@@ -227,6 +219,19 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def configure(ctx, a):
+    """Tile, batch and engine options from the command line."""
+    if a.tile:
+        tz, tx = (int(v) for v in a.tile.split(","))
+        ctx.set_tile(tz, tx)
+    if a.batch:
+        ctx.set_batch(a.batch)
+    for kv in a.opt:
+        k, v = kv.split("=")
+        ctx.set_option(k, int(v))
+    ctx.set_option("profileEvery", a.profile_every)
+
+
 def bench_ext(a, torch, A, born_lsrtm, nExt=21, time_lags=False):
     """Extended Born (nExt subsurface offsets, or time lags) forward + adjoint
     on the headline grid and shots: one JSON line, same fields as the
@@ -236,10 +241,7 @@ def bench_ext(a, torch, A, born_lsrtm, nExt=21, time_lags=False):
     p = dict(w.params, nExt=nExt, extension=A.EXT_TIME if time_lags else A.EXT_OFFSET)
     ctx = A.Context(p)
     ctx.set_velocity(w.vel)
-    for kv in a.opt:
-        k, v = kv.split("=")
-        ctx.set_option(k, int(v))
-    ctx.set_option("profileEvery", a.profile_every)
+    configure(ctx, a)
     srcs = [A.Device.grid(1, iz, 1, 1, ix, 1, p["nz"], p["nx"], p["nts"]) for iz, ix in w.src]
     rec = [A.Device.grid(*w.rec, p["nz"], p["nx"], p["nts"])]
     dev = torch.device("cuda:0")
@@ -286,10 +288,7 @@ def bench_tomo(a, torch, A, born_lsrtm, nExt=21, wemva=False, time_lags=False):
     p = dict(w.params, nExt=nExt, extension=A.EXT_TIME if time_lags else A.EXT_OFFSET)
     ctx = A.Context(p)
     ctx.set_velocity(w.vel)
-    for kv in a.opt:
-        k, v = kv.split("=")
-        ctx.set_option(k, int(v))
-    ctx.set_option("profileEvery", a.profile_every)
+    configure(ctx, a)
     srcs = [A.Device.grid(1, iz, 1, 1, ix, 1, p["nz"], p["nx"], p["nts"]) for iz, ix in w.src]
     rec = [A.Device.grid(*w.rec, p["nz"], p["nx"], p["nts"])]
     dev = torch.device("cuda:0")
