@@ -188,7 +188,7 @@ def main():
     pts = 4.0 * w.fine_steps * w.interior * w.nShot
     job_bytes = w.nShot * w.interior * (4 * w.fine_steps * 16.0 + p["nts"] * 28.0)
     out = {
-        "metric": "Gpts/s, Born fwd+adj 80 shots 500x1200" + (", free surface" if fs else ""),
+        "metric": f"Gpts/s, Born fwd+adj {w.nShot} shots 500x1200" + (", free surface" if fs else ""),
         "value": pts / t / 1e9,
         "unit": "Gpts/s",
         "n_gpus": 1,
