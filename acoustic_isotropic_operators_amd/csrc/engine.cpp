@@ -445,10 +445,10 @@ void run(aiso_ctx* c, const Request& q)
     const FdParams& p = c->p;
     c->tz = is_adj(q.op) ? c->tzAdj : c->tzFwd;
     c->tx = is_adj(q.op) ? c->txAdj : c->txFwd;
-    // The forward passes of the extended and tomographic operators run on
-    // the generic step kernel, which is faster on 32 x 32 tiles than on the
-    // specialized forward kernels' 32 x 16 (2-3 % on MI355X, 80 shots).
-    if (!is_adj(q.op) && !c->tileFwdSet && (is_ext(q.op) || is_tomo(q.op))) {
+    // The forward passes of the extended, tomographic and WEMVA operators
+    // run on the generic step kernel, which is faster on 32 x 32 tiles than
+    // on the specialized forward kernels' 32 x 16 (2-4 % on MI355X).
+    if (!is_adj(q.op) && !c->tileFwdSet && (is_ext(q.op) || is_tomo(q.op) || is_wemva(q.op))) {
         c->tz = 32;
         c->tx = 32;
     }

@@ -1799,21 +1799,30 @@ __global__ void k_time_scale(const float* __restrict__ v2dt2, const float* __res
 // computeTomoSrcWfldDt2 + srcWfldSecondTimeDerivative (:1251-1360,
 // F/src/kernelsGpu.cu:188-197): W[k] <- cSide*(W[k-1] + W[k+1]) +
 // cCenter*W[k], zero beyond both ends, in place; one thread walks the nts
-// samples of one interior point.
+// samples of one interior point.  The samples are read in chunks of 16, all
+// loads of a chunk (samples k0+1 .. k0+16, still unwritten) before its
+// stores (k0 .. k0+15), so a chunk waits for its loads once.
 __global__ void k_dt2_coarse(const ShotSlot* slots, int nts, float cSide, float cCenter, int nz, int nx)
 {
+    constexpr int D = 16;
     const ShotSlot& S = slots[blockIdx.y];
     const long nzx = (long)nz * nx;
     const int iz = FAT + blockIdx.x * blockDim.x + threadIdx.x;
     const int ix = FAT + blockIdx.z;
     if (iz >= nz - FAT) return;
-    float* w = S.W + (long)ix * nz + iz;
+    gf* w = G(S.W) + (long)ix * nz + iz;
     float prev = 0.f, cur = w[0];
-    for (int k = 0; k < nts; ++k) {
-        const float next = k + 1 < nts ? w[(long)(k + 1) * nzx] : 0.f;
-        w[(long)k * nzx] = cSide * (prev + next) + cCenter * cur;
-        prev = cur;
-        cur = next;
+    for (int k0 = 0; k0 < nts; k0 += D) {
+        float nxt[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) nxt[j] = k0 + 1 + j < nts ? w[(long)(k0 + 1 + j) * nzx] : 0.f;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            if (k0 + j >= nts) break;
+            w[(long)(k0 + j) * nzx] = cSide * (prev + nxt[j]) + cCenter * cur;
+            prev = cur;
+            cur = nxt[j];
+        }
     }
 }
 
