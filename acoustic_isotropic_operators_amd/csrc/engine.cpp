@@ -445,10 +445,12 @@ void run(aiso_ctx* c, const Request& q)
     const FdParams& p = c->p;
     c->tz = is_adj(q.op) ? c->tzAdj : c->tzFwd;
     c->tx = is_adj(q.op) ? c->txAdj : c->txFwd;
-    // The forward passes of the extended, tomographic and WEMVA operators
-    // run on the generic step kernel, which is faster on 32 x 32 tiles than
-    // on the specialized forward kernels' 32 x 16 (2-4 % on MI355X).
-    if (!is_adj(q.op) && !c->tileFwdSet && (is_ext(q.op) || is_tomo(q.op) || is_wemva(q.op))) {
+    // The forward passes of the extended, tomographic and WEMVA operators,
+    // and every pass with a free surface, run on the generic step kernel,
+    // which is faster on 32 x 32 tiles than on the specialized forward
+    // kernels' 32 x 16 (2-6 % on MI355X).
+    const bool onGeneric = is_ext(q.op) || is_tomo(q.op) || is_wemva(q.op) || c->p.freeSurface != 0;
+    if (!is_adj(q.op) && !c->tileFwdSet && onGeneric) {
         c->tz = 32;
         c->tx = 32;
     }

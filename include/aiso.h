@@ -93,8 +93,8 @@ aiso_status aiso_ctx_set_velocity(aiso_ctx* ctx, const float* vel, int mem);
 aiso_status aiso_ctx_set_batch(aiso_ctx* ctx, int shotsPerLaunch);
 /* Tile shape of the time-step kernel for forward and adjoint calls: tz in
  * {32,64}, tx in {16,32} (default 32 x 16 forward, 32 x 32 adjoint; the
- * extended, tomographic and WEMVA forwards default to 32 x 32 until a tile
- * is set). */
+ * extended, tomographic and WEMVA forwards, and forwards with a free
+ * surface, default to 32 x 32 until a tile is set). */
 aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
 /* Engine tuning knobs by name (no counterpart in the reference):
  *   "streamNT" 0/1: non-temporal access to the coarse wavefield stream;
