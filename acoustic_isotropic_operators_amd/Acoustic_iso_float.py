@@ -72,9 +72,12 @@ def _dipole(par):
 
 def buildSourceGeometry(parObject, vel, client=None):
     """One deviceGpu per shot (PY/Acoustic_iso_float.py:237-284): from
-    ``sourceGeomFile`` (coordinates, axis 1 = x,y,z, axis 2 = shot; bilinear
-    constructor) or from zSource/xSource/spacingShots on the padded grid
-    (iz = zSource-1+zPadMinus+fat)."""
+    ``sourceGeomFile`` (bilinear constructor on physical coordinates) or from
+    zSource/xSource/spacingShots on the padded grid (iz = zSource-1+zPadMinus+fat).
+
+    The geometry file is read as the reference's code reads it
+    (PY/Acoustic_iso_float.py:250-259): its array is [3][nShot] (SEP axis 1 =
+    shot, axis 2 = coordinate x, y, z), x at [0, shot] and z at [2, shot]."""
     _no_client(client)
     nShot = parObject.getInt("nShot")
     nts = parObject.getInt("nts")
@@ -82,12 +85,12 @@ def buildSourceGeometry(parObject, vel, client=None):
     geomFile = parObject.getString("sourceGeomFile", "None")
     sources = []
     if geomFile != "None":
-        g = sep.read(geomFile, ndims=2).getNdArray()          # [nShot][3]
-        if g.shape[0] != nShot:
-            raise ValueError(f"Number of shots (#shot={nShot}) not consistent with geometry file (#shots={g.shape[0]})!")
+        g = sep.read(geomFile, ndims=2).getNdArray()          # [3][nShot]
+        if g.shape[1] != nShot:
+            raise ValueError(f"Number of shots (#shot={nShot}) not consistent with geometry file (#shots={g.shape[1]})!")
         for s in range(nShot):
-            z = np.array([g[s, 2]], np.float32)
-            x = np.array([g[s, 0]], np.float32)
+            z = np.array([g[2, s]], np.float32)
+            x = np.array([g[0, s]], np.float32)
             sources.append(deviceGpu(z, x, _cpp(vel), nts, dip, float(zs), float(xs)))
         return sources, sep.axis(n=nShot, o=1.0, d=1.0)
     xax = vel.getHyper().axes[1]
@@ -104,28 +107,32 @@ def buildSourceGeometry(parObject, vel, client=None):
 
 def buildReceiversGeometry(parObject, vel, client=None):
     """Receivers (PY/Acoustic_iso_float.py:386-456): one deviceGpu per shot
-    from ``receiverGeomFile`` (axis 1 = x,y,z, axis 2 = receiver, axis 3 =
-    shot), or one constant regular line from depthReceiver/nReceiver/
-    oReceiver/dReceiver."""
+    from ``receiverGeomFile``, or one constant regular line from
+    depthReceiver/nReceiver/oReceiver/dReceiver.
+
+    The geometry file is read as the reference's code reads it
+    (PY/Acoustic_iso_float.py:414-431): its array is [3][nRec][nShot] (SEP
+    axis 1 = shot, axis 2 = receiver, axis 3 = coordinate x, y, z), x at
+    [0, :, shot] and z at [2, :, shot]."""
     _no_client(client)
     nts = parObject.getInt("nts")
     dip, zs, xs = _dipole(parObject)
     geomFile = parObject.getString("receiverGeomFile", "None")
     receivers = []
     if geomFile != "None":
-        g = sep.read(geomFile, ndims=3).getNdArray()          # [nShot][nRec][3]
+        g = sep.read(geomFile, ndims=3).getNdArray()          # [3][nRec][nShot]
         nShot = parObject.getInt("nShot", -1)
-        if nShot != g.shape[0]:
+        if nShot != g.shape[2]:
             raise ValueError(f"**** ERROR [buildReceiversGeometry]: Number of shots from parfile (#shot={nShot}) "
-                             f"not consistent with receivers' geometry file (#shots={g.shape[0]}) ****")
+                             f"not consistent with receivers' geometry file (#shots={g.shape[2]}) ****")
         nRec = parObject.getInt("nReceiverPerShot", -1)
         if nRec != g.shape[1]:
             raise ValueError(f"**** ERROR [buildReceiversGeometry]: Number of receivers from parfile "
                              f"(#nReceiverPerShot={nRec}) not consistent with receivers' geometry file "
                              f"(#recs={g.shape[1]}) ****")
         for s in range(nShot):
-            z = np.ascontiguousarray(g[s, :, 2], np.float32)
-            x = np.ascontiguousarray(g[s, :, 0], np.float32)
+            z = np.ascontiguousarray(g[2, :, s], np.float32)
+            x = np.ascontiguousarray(g[0, :, s], np.float32)
             receivers.append(deviceGpu(z, x, _cpp(vel), nts, dip, float(zs), float(xs)))
         return receivers, sep.axis(n=nRec, o=1.0, d=1.0)
     xax = vel.getHyper().axes[1]

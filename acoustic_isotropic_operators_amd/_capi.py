@@ -105,6 +105,41 @@ def _ptr(x):
     return C.c_void_p(int(x))
 
 
+def _nelem(x):
+    return int(x.numel()) if hasattr(x, "numel") else int(x.size)
+
+
+def _check_arg(x, n, mem, name, writable=False):
+    """Refuse an operand the engine would misread or overrun: it must be a
+    float32, C-contiguous array (numpy for MEM_HOST, a GPU tensor for
+    MEM_DEVICE) of exactly ``n`` elements, writable when it is an output."""
+    if x is None:
+        raise ValueError(f"{name}: missing operand")
+    if mem == MEM_DEVICE:
+        if not hasattr(x, "data_ptr") or not hasattr(x, "is_cuda"):
+            raise ValueError(f"{name}: MEM_DEVICE operands must be GPU tensors")
+        if not x.is_cuda:
+            raise ValueError(f"{name}: tensor is not on the GPU")
+        import torch
+        if x.dtype != torch.float32:
+            raise ValueError(f"{name}: dtype {x.dtype}, expected float32")
+        if not x.is_contiguous():
+            raise ValueError(f"{name}: tensor is not contiguous")
+    elif mem == MEM_HOST:
+        if not isinstance(x, np.ndarray):
+            raise ValueError(f"{name}: MEM_HOST operands must be numpy arrays")
+        if x.dtype != np.float32:
+            raise ValueError(f"{name}: dtype {x.dtype}, expected float32")
+        if not x.flags.c_contiguous:
+            raise ValueError(f"{name}: array is not C-contiguous")
+        if writable and not x.flags.writeable:
+            raise ValueError(f"{name}: output array is read-only")
+    else:
+        raise ValueError(f"mem must be MEM_HOST or MEM_DEVICE, got {mem}")
+    if _nelem(x) != n:
+        raise ValueError(f"{name}: {_nelem(x)} elements, expected {n}")
+
+
 def pad_plus(n, pad, block=16):
     return lib().aiso_pad_plus(n, pad, block)
 
@@ -184,6 +219,7 @@ class Context:
         for k, _ in FdParams._fields_:
             setattr(fp, k, merged[k])
         self.params = merged
+        self.options = {}
         self.h = C.c_void_p()
         _check(lib().aiso_ctx_create(C.byref(fp), device, C.byref(self.h)))
 
@@ -199,6 +235,7 @@ class Context:
     def set_velocity(self, vel, mem=MEM_HOST):
         if mem == MEM_HOST:
             vel = np.ascontiguousarray(vel, np.float32)
+        _check_arg(vel, self._nzx(), mem, "velocity")
         _check(lib().aiso_ctx_set_velocity(self.h, _ptr(vel), mem))
 
     def set_batch(self, n):
@@ -209,6 +246,21 @@ class Context:
 
     def set_option(self, key, value):
         _check(lib().aiso_ctx_set_option(self.h, key.encode(), int(value)))
+        self.options[key] = int(value)
+
+    # operand sizes ---------------------------------------------------------
+    def _nzx(self):
+        return int(self.params["nz"]) * int(self.params["nx"])
+
+    def _nts(self):
+        return int(self.params["nts"])
+
+    def _data_n(self, sources, receivers):
+        return len(sources) * receivers[0].nDev * self._nts()
+
+    def _image_n(self, nShot, ext):
+        n = self._nzx() * (int(self.params.get("nExt", 1)) if ext else 1)
+        return n * nShot if self.options.get("imagePerShot", 0) else n
 
     def info(self):
         a, b, c = _I(), _I(), C.c_size_t()
@@ -238,12 +290,22 @@ class Context:
 
     # operators ------------------------------------------------------------
     def nl_forward(self, model, data, sources, receivers, add=False, mem=MEM_HOST):
+        nSig = model.shape[0]
+        if nSig not in (1, len(sources)):
+            raise ValueError(f"model: shot axis {nSig}, expected 1 or {len(sources)}")
+        _check_arg(model, nSig * sources[0].nDev * self._nts(), mem, "model")
+        _check_arg(data, self._data_n(sources, receivers), mem, "data", writable=True)
         s, _ks = _devarray(sources)
         r, _kr = _devarray(receivers)
         _check(lib().aiso_nl_forward(self.h, int(add), _ptr(model), model.shape[0], _ptr(data), s, len(sources),
                                      r, len(receivers), mem))
 
     def nl_adjoint(self, model, data, sources, receivers, add=False, mem=MEM_HOST):
+        nSig = model.shape[0]
+        if nSig not in (1, len(sources)):
+            raise ValueError(f"model: shot axis {nSig}, expected 1 or {len(sources)}")
+        _check_arg(model, nSig * sources[0].nDev * self._nts(), mem, "model", writable=True)
+        _check_arg(data, self._data_n(sources, receivers), mem, "data")
         s, _ks = _devarray(sources)
         r, _kr = _devarray(receivers)
         _check(lib().aiso_nl_adjoint(self.h, int(add), _ptr(model), model.shape[0], _ptr(data), s, len(sources),
@@ -256,45 +318,73 @@ class Context:
         r, _kr = _devarray(receivers)
         if mem == MEM_HOST:
             model = np.ascontiguousarray(model, np.float32)
+        nSig = model.shape[0]
+        if nSig not in (1, len(sources)):
+            raise ValueError(f"model: shot axis {nSig}, expected 1 or {len(sources)}")
+        _check_arg(model, nSig * sources[0].nDev * self._nts(), mem, "model")
+        _check_arg(data_grad, self._data_n(sources, receivers), mem, "data_grad")
+        _check_arg(vel_grad, self._nzx(), mem, "vel_grad", writable=True)
         _check(lib().aiso_nl_velocity_gradient(self.h, _ptr(model), model.shape[0], _ptr(data_grad), _ptr(vel_grad),
                                                s, len(sources), r, len(receivers), int(checkpoint_every), mem))
 
-    def _born(self, fn, model, data, sources, wavelets, receivers, add, mem):
+    def _born(self, fn, model, data, sources, wavelets, receivers, add, mem, adjoint, ext):
+        if adjoint:
+            _check_arg(model, self._image_n(len(sources), ext), mem, "model", writable=True)
+            _check_arg(data, self._data_n(sources, receivers), mem, "data")
+        else:
+            _check_arg(model, self._nzx() * (int(self.params.get("nExt", 1)) if ext else 1), mem, "model")
+            _check_arg(data, self._data_n(sources, receivers), mem, "data", writable=True)
         s, _ks = _devarray(sources)
         r, _kr = _devarray(receivers)
-        w = np.ascontiguousarray(wavelets, np.float32)
+        w = self._wavelets(wavelets, sources)
         nwav = w.shape[0] if w.ndim == 3 else 1
         _check(fn(self.h, int(add), _ptr(model), _ptr(data), s, len(sources), _ptr(w), nwav, r, len(receivers), mem))
 
+    def _wavelets(self, wavelets, sources):
+        """Host wavelets [nSrcDev][nts] or [nWavelet][nSrcDev][nts], nWavelet in {1, nShot}."""
+        w = np.ascontiguousarray(wavelets, np.float32)
+        nwav = w.shape[0] if w.ndim == 3 else 1
+        if nwav not in (1, len(sources)):
+            raise ValueError(f"wavelets: {nwav} blocks, expected 1 or {len(sources)}")
+        n = nwav * sources[0].nDev * self._nts()
+        if w.size != n:
+            raise ValueError(f"wavelets: {w.size} elements, expected {n}")
+        return w
+
     def born_forward(self, model, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
-        self._born(lib().aiso_born_forward, model, data, sources, wavelets, receivers, add, mem)
+        self._born(lib().aiso_born_forward, model, data, sources, wavelets, receivers, add, mem, False, False)
 
     def born_adjoint(self, model, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
-        self._born(lib().aiso_born_adjoint, model, data, sources, wavelets, receivers, add, mem)
+        self._born(lib().aiso_born_adjoint, model, data, sources, wavelets, receivers, add, mem, True, False)
 
     def bornext_forward(self, model, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
-        self._born(lib().aiso_bornext_forward, model, data, sources, wavelets, receivers, add, mem)
+        self._born(lib().aiso_bornext_forward, model, data, sources, wavelets, receivers, add, mem, False, True)
 
     def bornext_adjoint(self, model, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
-        self._born(lib().aiso_bornext_adjoint, model, data, sources, wavelets, receivers, add, mem)
+        self._born(lib().aiso_bornext_adjoint, model, data, sources, wavelets, receivers, add, mem, True, True)
 
     def set_ext_reflectivity(self, refl, mem=MEM_HOST):
         """Extended reflectivity [nExt][nx][nz] of the tomographic operator."""
         if mem == MEM_HOST:
             refl = np.ascontiguousarray(refl, np.float32)
+        _check_arg(refl, self._nzx() * int(self.params.get("nExt", 1)), mem, "reflectivity")
         self._ext_refl = refl
         _check(lib().aiso_ctx_set_ext_reflectivity(self.h, _ptr(refl), mem))
 
     def tomo_forward(self, model, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
-        self._born(lib().aiso_tomo_forward, model, data, sources, wavelets, receivers, add, mem)
+        self._born(lib().aiso_tomo_forward, model, data, sources, wavelets, receivers, add, mem, False, False)
 
     def tomo_adjoint(self, model, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
-        self._born(lib().aiso_tomo_adjoint, model, data, sources, wavelets, receivers, add, mem)
+        self._born(lib().aiso_tomo_adjoint, model, data, sources, wavelets, receivers, add, mem, True, False)
 
-    def _wemva(self, fn, model, image, data, sources, wavelets, receivers, add, mem):
+    def _wemva(self, fn, model, image, data, sources, wavelets, receivers, add, mem, adjoint):
+        nExt = int(self.params.get("nExt", 1))
+        _check_arg(model, self._nzx(), mem, "model", writable=adjoint)
+        _check_arg(image, self._nzx() * nExt, mem, "image", writable=not adjoint)
+        _check_arg(data, self._data_n(sources, receivers), mem, "data")
         s, _ks = _devarray(sources)
         r, _kr = _devarray(receivers)
-        w = np.ascontiguousarray(wavelets, np.float32)
+        w = self._wavelets(wavelets, sources)
         nwav = w.shape[0] if w.ndim == 3 else 1
         _check(fn(self.h, int(add), _ptr(model), _ptr(image), _ptr(data), s, len(sources), _ptr(w), nwav, r,
                   len(receivers), mem))
@@ -302,7 +392,7 @@ class Context:
     def wemva_forward(self, model, image, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
         """Velocity perturbation [nx][nz] -> extended image [nExt][nx][nz] about
         the observed data [nShot][nRec][nts]."""
-        self._wemva(lib().aiso_wemva_forward, model, image, data, sources, wavelets, receivers, add, mem)
+        self._wemva(lib().aiso_wemva_forward, model, image, data, sources, wavelets, receivers, add, mem, False)
 
     def wemva_adjoint(self, model, image, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
-        self._wemva(lib().aiso_wemva_adjoint, model, image, data, sources, wavelets, receivers, add, mem)
+        self._wemva(lib().aiso_wemva_adjoint, model, image, data, sources, wavelets, receivers, add, mem, True)

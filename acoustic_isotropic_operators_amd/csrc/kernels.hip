@@ -2069,6 +2069,8 @@ static StepKernel pick_tile(int tz, int tx, int spb)
     return pick_tile_spb<MODE, 1>(tz, tx);
 }
 
+// The extended-Born modes (MODE_EXT_SCAT, MODE_EXT_REC) are not instantiated
+// here: they run on k_step_generic only, and launch_step refuses them.
 static StepKernel pick_step(unsigned mode, int tz, int tx, int spb)
 {
     switch (mode) {
@@ -2077,8 +2079,6 @@ static StepKernel pick_step(unsigned mode, int tz, int tx, int spb)
     case MODE_SRC: return pick_tile<MODE_SRC>(tz, tx, spb);
     case MODE_BORN_SCAT: return pick_tile<MODE_BORN_SCAT>(tz, tx, spb);
     case MODE_BORN_REC: return pick_tile<MODE_BORN_REC>(tz, tx, spb);
-    case MODE_EXT_SCAT: return pick_tile<MODE_EXT_SCAT>(tz, tx, spb);
-    case MODE_EXT_REC: return pick_tile<MODE_EXT_REC>(tz, tx, spb);
     case MODE_SRC_WIN: return pick_tile<MODE_SRC_WIN>(tz, tx, spb);
     case MODE_BORN_SCAT_PRE: return pick_tile<MODE_BORN_SCAT_PRE>(tz, tx, spb);
     case MODE_REC_WIN: return pick_tile<MODE_REC_WIN>(tz, tx, spb);
@@ -2100,6 +2100,7 @@ hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a
 {
     StepKernel k = pick_step(mode, tz, tx, spb);
     if (!k) return hipErrorInvalidValue;
+    if ((mode & M_WIN) && a.winOut == WIN_SLICE) return hipErrorInvalidValue;   // extended-Born capture: generic only
     // One workgroup per (tile, group of spb shots).
     StepArgs b = a;
     b.gridLogical = a.ntiles * ((a.nshots + spb - 1) / spb);

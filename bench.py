@@ -49,7 +49,8 @@ def parse():
     ap.add_argument("--batch", type=int, default=0, help="shots per launch (0 = by HBM budget)")
     ap.add_argument("--tile", type=str, default="", help="tz,tx for every call (default: engine's per-direction tiles)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-shots", type=int, default=1)
+    ap.add_argument("--cpu-shots", type=int, default=8, help="CPU baseline: at most this many shots")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline: stop after the shot that passes this")
     ap.add_argument("--opt", action="append", default=[], help="engine option key=value (aiso_ctx_set_option)")
     ap.add_argument("--workload", default="born",
                     choices=["born", "bornfs", "bornext21", "borntime21", "tomo21", "wemva21", "tomotime21",
@@ -74,38 +75,100 @@ def parse():
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fwd_pair_born_lsrtm_80x500x1200_hbm_traffic.md")
+KERNEL_SRC = os.path.join(ROOT, "acoustic_isotropic_operators_amd", "csrc", "kernels.hip")
+
+
+def _sha16(path):
+    import hashlib
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
 
 
 def measured_traffic():
-    """HBM bytes per launch of the lockstep Born forward kernel, from the committed
-    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    """Fabric bytes per launch of the lockstep Born forward kernel from the
+    committed FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), with the
+    summary's file name; the figure is used only when the summary was taken
+    on the kernels.hip in this tree (its KERNELS_SHA16 line), else None."""
+    vals = {}
     try:
         for line in open(TRAFFIC_FILE):
-            if line.startswith("TRAFFIC_BYTES_PER_LAUNCH="):
-                return float(line.split("=", 1)[1])
+            if "=" in line and line.split("=", 1)[0].isupper():
+                k, v = line.strip().split("=", 1)
+                vals[k] = v
+    except OSError:
+        return None, "no counter summary"
+    src = os.path.relpath(TRAFFIC_FILE, ROOT)
+    if vals.get("KERNELS_SHA16") != _sha16(KERNEL_SRC):
+        return None, f"{src} was taken on another kernels.hip"
+    try:
+        return float(vals["TRAFFIC_BYTES_PER_LAUNCH"]), src
+    except (KeyError, ValueError):
+        return None, f"{src} holds no TRAFFIC_BYTES_PER_LAUNCH"
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return None
+    return "unknown"
 
 
-def cpu_baseline(w, nshots):
-    """Oracle (float, OpenMP over ix) on the host: Born fwd + adj of the first
-    `nshots` shots of the same workload, as Gpts/s."""
-    threads = min(16, len(os.sched_getaffinity(0)))
+def cpu_baseline(w, seconds=20.0, max_shots=8):
+    """Oracle (float, OpenMP over ix, the vectorized timing build of
+    oracle/Makefile picked for this CPU) on the host: Born fwd + adj shot by
+    shot over the same workload until `seconds` have passed or `max_shots`
+    ran (at least one); Gpts/s over the shots run, per-shot p50/p99, and the
+    80-shot time extrapolated linearly from them.  Threads: OMP_NUM_THREADS
+    (the host-core share of one GPU), else every core of the affinity mask."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
     os.environ["OMP_NUM_THREADS"] = str(threads)
     from oracle import oracle as O
+    build = O.use_timing_build()
     p = w.params
     g = O.Grid(nz=p["nz"], nx=p["nx"], nts=p["nts"], sub=p["sub"], minPad=min(p["zPadMinus"], p["zPadPlus"], p["xPadMinus"], p["xPadPlus"]),
                dz=p["dz"], dx=p["dx"], dts=p["dts"], alphaCos=p["alphaCos"])
-    srcs = [O.Device.grid(1, iz, 1, 1, ix, 1, p["nz"], p["nx"], p["nts"]) for iz, ix in w.src[:nshots]]
     rec = [O.Device.grid(*w.rec, p["nz"], p["nx"], p["nts"])]
-    t0 = time.perf_counter()
-    d = O.born_forward(g, w.vel, srcs, [w.wavelet], rec, w.model, "f")
-    O.born_adjoint(g, w.vel, srcs, [w.wavelet], rec, d, "f")
-    t = time.perf_counter() - t0
-    pts = 4.0 * w.fine_steps * w.interior * nshots
+    per = []
+    t_all = time.perf_counter()
+    for iz, ix in w.src[:max_shots]:
+        src = [O.Device.grid(1, iz, 1, 1, ix, 1, p["nz"], p["nx"], p["nts"])]
+        t0 = time.perf_counter()
+        d = O.born_forward(g, w.vel, src, [w.wavelet], rec, w.model, "f")
+        O.born_adjoint(g, w.vel, src, [w.wavelet], rec, d, "f")
+        per.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_all >= seconds:
+            break
+    t = sum(per)
+    pts = 4.0 * w.fine_steps * w.interior * len(per)
     return {"value": pts / t / 1e9, "unit": "Gpts/s", "cores": threads, "kind": "port",
-            "sample": f"{nshots} of {w.nShot} shots, full grid and nts, Born fwd+adj, {t:.1f} s"}
+            "sample": f"{len(per)} of {w.nShot} shots run (full grid and nts, Born fwd+adj, {t:.1f} s); "
+                      f"80-shot time extrapolated linearly: {t / len(per) * w.nShot:.0f} s",
+            "shot_latency_ms": {"p50": 1e3 * float(np.percentile(per, 50)), "p99": 1e3 * float(np.percentile(per, 99))},
+            "cpu": _cpu_model(), "build": build}
+
+
+def shot_latencies(bms, bsh, nShot, calls):
+    """Per-shot latency of a job of `calls` operator calls per step, each
+    call carrying all nShot shots in consecutive batches (engine batch
+    events in issue order, shots in order within a call): a shot's latency
+    is the sum over the calls of the duration of the batch that carried it."""
+    lat, i = [], 0
+    while i < len(bms):
+        per_shot = np.zeros(nShot)
+        for _ in range(calls):
+            done = 0
+            while done < nShot:
+                if i >= len(bms):
+                    raise RuntimeError("batch events do not cover whole calls")
+                n = int(bsh[i])
+                per_shot[done:done + n] += float(bms[i])
+                done += n
+                i += 1
+        lat.extend(per_shot.tolist())
+    return lat
 
 
 def main():
@@ -178,13 +241,8 @@ def main():
     achieved = per_launch_bytes / (avg_ms / 1e3) / 1e9
     bms, bsh = ctx.batch_read()
     info = ctx.info()
-    nb = len(bms) // (2 * a.steps)
-    lat = []
-    for s in range(a.steps):
-        fw = bms[2 * nb * s: 2 * nb * s + nb]
-        ad = bms[2 * nb * s + nb: 2 * nb * (s + 1)]
-        for b in range(nb):
-            lat += [float(fw[b] + ad[b])] * int(bsh[2 * nb * s + b])
+    lat = shot_latencies(bms, bsh, w.nShot, 2)           # the forward call, then the adjoint call
+    traffic, traffic_src = measured_traffic() if dom == "K_BORN_PAIR" else (None, "not the pair kernel")
     pts = 4.0 * w.fine_steps * w.interior * w.nShot
     job_bytes = w.nShot * w.interior * (4 * w.fine_steps * 16.0 + p["nts"] * 28.0)
     out = {
@@ -205,17 +263,21 @@ def main():
                    "inputs": "model/data/image resident in HBM"},
         "job_roofline_frac": job_bytes / t / 1e9 / HBM_PEAK_GBS,
         "shot_latency_ms": {"p50": float(np.percentile(lat, 50)) if lat else None,
-                            "p99": float(np.percentile(lat, 99)) if lat else None},
+                            "p99": float(np.percentile(lat, 99)) if lat else None,
+                            "shots": len(lat),
+                            "definition": "forward batch + adjoint batch carrying the shot"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic() if dom == "K_BORN_PAIR" else None, "avg_launch_ms": avg_ms,
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "traffic_counts": "L2-to-fabric bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated), "
+                                       "HBM plus Infinity Cache hits", "avg_launch_ms": avg_ms,
                      "launches": st["launches"], "bytes_per_point": bpp,
                      "point_updates_per_launch": st["point_updates"] / max(st["launches"], 1)},
         "kernels": kern,
         "finite": ok,
     }
     if not a.no_cpu and not fs:   # the CPU baseline is the headline workload's
-        out["cpu_baseline"] = cpu_baseline(w, a.cpu_shots)
+        out["cpu_baseline"] = cpu_baseline(w, a.cpu_seconds, a.cpu_shots)
     print(json.dumps(out), flush=True)
 
 

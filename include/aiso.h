@@ -236,7 +236,7 @@ aiso_status aiso_bornext_adjoint(aiso_ctx* ctx, int add, float* model, const flo
                                  const float* wavelets, int nWavelet,
                                  aiso_device* const* receivers, int nRecGeom, int mem);
 
-/* ------------------------------------------- tomographic (extension=offset) */
+/* ------------------------------- tomographic (extension=offset or time) */
 /* tomoExtShotsGpu (F/include/tomoExtShotsGpu.h:20-58): the extended
  * reflectivity [nExt][nx][nz] the operator linearizes about, the constructor
  * argument and setReflectivityExt of the reference (:37, :55).  Copied; the
@@ -245,8 +245,9 @@ aiso_status aiso_ctx_set_ext_reflectivity(aiso_ctx* ctx, const float* reflectivi
 /* tomoExtShotsGpu::forward / adjoint (F/src/tomoExtShotsGpu.cpp:62-145 and
  * the adjoint below it; F/src/tomoExtGpu.cpp:34-100): velocity perturbation
  * model [nx][nz] <-> data [nShot][nRec][nts], legs selected by the options
- * "leg1" / "leg2".  Needs extension = AISO_EXT_OFFSET (the time-lag form
- * is not supported) and aiso_ctx_set_ext_reflectivity first. */
+ * "leg1" / "leg2".  Needs extension = AISO_EXT_OFFSET or AISO_EXT_TIME (the
+ * time-lag routines of F/src/tomoExtShotsGpuFunctions.cu) and
+ * aiso_ctx_set_ext_reflectivity first. */
 aiso_status aiso_tomo_forward(aiso_ctx* ctx, int add, const float* model, float* data,
                               aiso_device* const* sources, int nShot,
                               const float* wavelets, int nWavelet,
@@ -256,7 +257,7 @@ aiso_status aiso_tomo_adjoint(aiso_ctx* ctx, int add, float* model, const float*
                               const float* wavelets, int nWavelet,
                               aiso_device* const* receivers, int nRecGeom, int mem);
 
-/* ------------------------------------------------- WEMVA (extension=offset) */
+/* ------------------------------------- WEMVA (extension=offset or time) */
 /* wemvaExtShotsGpu::forward / adjoint (F/include/wemvaExtShotsGpu.h:36-57,
  * F/src/wemvaExtShotsGpu.cpp, F/src/wemvaExtShotsGpuFunctions.cu:458-808):
  * velocity perturbation model [nx][nz] <-> extended image [nExt][nx][nz],
@@ -264,7 +265,8 @@ aiso_status aiso_tomo_adjoint(aiso_ctx* ctx, int add, float* model, const float*
  * [nShot][nRec][nts] (the reference's receiversSignalsVector, passed with
  * every call).  The forward writes (add: accumulates into) image, the
  * adjoint model.  Legs by the options "leg1" / "leg2"; needs extension =
- * AISO_EXT_OFFSET (the time-lag form is not supported). */
+ * AISO_EXT_OFFSET or AISO_EXT_TIME (the time-lag routines of
+ * F/src/wemvaExtShotsGpuFunctions.cu). */
 aiso_status aiso_wemva_forward(aiso_ctx* ctx, int add, const float* model, float* image, const float* data,
                                aiso_device* const* sources, int nShot,
                                const float* wavelets, int nWavelet,

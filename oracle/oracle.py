@@ -40,14 +40,46 @@ _LIB = None
 FAT = 5
 
 
+_LIBNAME = "liborc.so"
+
+
 def _lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liborc.so")
+        path = os.path.join(_HERE, _LIBNAME)
         if not os.path.exists(path):
             subprocess.run(["make", "-s", "-C", _HERE], check=True)
         _LIB = C.CDLL(path)
     return _LIB
+
+
+def _cpu_flags():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("flags"):
+                return set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+def use_timing_build():
+    """Switch to the vectorized build this CPU supports (liborc_v4.so with
+    AVX-512, liborc_v3.so with AVX2/FMA, else the portable liborc.so), for
+    the CPU baseline's timing; same bits as the portable build.  Call before
+    the first oracle call.  Returns the build's file name."""
+    global _LIB, _LIBNAME
+    flags = _cpu_flags()
+    v4 = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+    v3 = {"avx", "avx2", "fma", "bmi1", "bmi2", "movbe", "f16c"}
+    name = "liborc.so"
+    if v4 <= flags and os.path.exists(os.path.join(_HERE, "liborc_v4.so")):
+        name = "liborc_v4.so"
+    elif v3 <= flags and os.path.exists(os.path.join(_HERE, "liborc_v3.so")):
+        name = "liborc_v3.so"
+    if name != _LIBNAME:
+        _LIBNAME, _LIB = name, None
+    return name
 
 
 class _Grid(C.Structure):

@@ -1,0 +1,182 @@
+"""The workloads of the configuration list at their full sizes on the GPU:
+
+* the nonlinear forward of one shot on a 200 x 400 grid (nts 1000 at 4 ms,
+  sub 4, 400 receivers), bitwise against the float oracle, with the error
+  against the FP64 oracle printed, and its dot test;
+* the FWI gradient on the 400 x 1000 layered model (32 shots, nts 4000 at
+  1 ms, sub 2): one shot bitwise against the float oracle's gradient, the
+  32-shot gradient against the sum of per-shot gradients, and central finite
+  differences of the misfit along 20 seeded smooth perturbations.
+
+Each test prints what it runs before it runs it.
+"""
+import numpy as np
+import pytest
+
+from cases import rel_l2
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(p):
+    return O.Grid(nz=p["nz"], nx=p["nx"], nts=p["nts"], sub=p["sub"],
+                  minPad=min(p["zPadMinus"], p["zPadPlus"], p["xPadMinus"], p["xPadPlus"]), dz=p["dz"], dx=p["dx"],
+                  dts=p["dts"], alphaCos=p["alphaCos"])
+
+
+def test_nonlinear_forward_200x400_matches_the_oracle():
+    from acoustic_isotropic_operators_amd import _capi as A
+    from acoustic_isotropic_operators_amd.workloads import nonlinear_small
+    w = nonlinear_small()
+    p = w.params
+    print(f"NL forward {w.name}: padded {p['nz']}x{p['nx']}, nts {p['nts']}, sub {p['sub']}", flush=True)
+    ctx = A.Context(p)
+    ctx.set_velocity(w.vel)
+    src = [A.Device.grid(1, iz, 1, 1, ix, 1, p["nz"], p["nx"], p["nts"]) for iz, ix in w.src]
+    rec = [A.Device.grid(*w.rec, p["nz"], p["nx"], p["nts"])]
+    assert rec[0].nDev == 400
+    m = w.wavelet[None].astype(np.float32)                      # [1][1][nts]
+    d = np.zeros((1, 400, p["nts"]), np.float32)
+    ctx.nl_forward(m, d, src, rec)
+    g = _grid(p)
+    osrc = [O.Device.grid(1, iz, 1, 1, ix, 1, p["nz"], p["nx"], p["nts"]) for iz, ix in w.src]
+    orec = [O.Device.grid(*w.rec, p["nz"], p["nx"], p["nts"])]
+    of = O.nl_forward(g, w.vel, osrc, orec, m, "f")
+    od = O.nl_forward(g, w.vel, osrc, orec, m.astype(np.float64), "d")
+    print(f"relL2 vs float oracle {rel_l2(d, of):.3e} (bitwise {np.array_equal(d, of)}), vs FP64 oracle "
+          f"{rel_l2(d, od):.3e}", flush=True)
+    assert np.abs(of).max() > 0
+    np.testing.assert_array_equal(d, of)
+    # dot test of the pair at the reference's tolerance (F/include/operator.h:50-88)
+    rng = np.random.default_rng(5)
+    dr = rng.standard_normal(d.shape).astype(np.float32)
+    ma = np.zeros_like(m)
+    ctx.nl_adjoint(ma, dr, src, rec)
+    np.testing.assert_array_equal(ma, O.nl_adjoint(g, w.vel, osrc, orec, dr, 1, "f"))
+    a = float(np.vdot(d.astype(np.float64).ravel(), dr.astype(np.float64).ravel()))
+    b = float(np.vdot(m.astype(np.float64).ravel(), ma.astype(np.float64).ravel()))
+    print(f"<Fm,d> {a:.9e} <m,F'd> {b:.9e} rel {abs(a - b) / abs(b):.2e}", flush=True)
+    assert abs(a - b) <= 1e-5 * abs(b)
+
+
+@pytest.fixture(scope="module")
+def fwi():
+    import torch
+    from acoustic_isotropic_operators_amd import _capi as A
+    from acoustic_isotropic_operators_amd.workloads import fwi_layered
+    w = fwi_layered()
+    p = w.params
+    print(f"FWI {w.name}: padded {p['nz']}x{p['nx']}, nts {p['nts']}, sub {p['sub']}, {w.nShot} shots", flush=True)
+    ctx = A.Context(p)
+    srcs = [A.Device.grid(1, iz, 1, 1, ix, 1, p["nz"], p["nx"], p["nts"]) for iz, ix in w.src]
+    rec = [A.Device.grid(*w.rec, p["nz"], p["nx"], p["nts"])]
+    dev = torch.device("cuda:0")
+    m = w.wavelet[None].astype(np.float32)
+    md = torch.from_numpy(m).to(dev)
+    dobs = torch.zeros((w.nShot, rec[0].nDev, p["nts"]), dtype=torch.float32, device=dev)
+    ctx.set_velocity(w.model)                       # the true model
+    ctx.nl_forward(md, dobs, srcs, rec, mem=A.MEM_DEVICE)
+    ctx.set_velocity(w.vel)                         # the starting model
+    d0 = torch.zeros_like(dobs)
+    ctx.nl_forward(md, d0, srcs, rec, mem=A.MEM_DEVICE)
+    r = (d0 - dobs).contiguous()
+    grad = torch.zeros(w.vel.shape, dtype=torch.float32, device=dev)
+    ctx.nl_velocity_gradient(md, r, grad, srcs, rec, mem=A.MEM_DEVICE)
+    torch.cuda.synchronize()
+    return dict(torch=torch, A=A, w=w, ctx=ctx, srcs=srcs, rec=rec, m=m, md=md, dobs=dobs, r=r, grad=grad)
+
+
+def test_fwi_gradient_one_shot_matches_the_oracle(fwi):
+    torch, A, w, ctx = fwi["torch"], fwi["A"], fwi["w"], fwi["ctx"]
+    p = w.params
+    s = 13
+    print(f"one-shot gradient, shot {s}: engine and float oracle", flush=True)
+    r1 = fwi["r"][s:s + 1].contiguous()
+    g1 = torch.zeros_like(fwi["grad"])
+    ctx.nl_velocity_gradient(fwi["md"], r1, g1, fwi["srcs"][s:s + 1], fwi["rec"], mem=A.MEM_DEVICE)
+    g1 = g1.cpu().numpy()
+    iz, ix = w.src[s]
+    osrc = [O.Device.grid(1, iz, 1, 1, ix, 1, p["nz"], p["nx"], p["nts"])]
+    orec = [O.Device.grid(*w.rec, p["nz"], p["nx"], p["nts"])]
+    ref = O.nl_velocity_gradient(_grid(p), w.vel, osrc, orec, fwi["m"], r1.cpu().numpy(), "f")
+    print(f"relL2 vs float oracle {rel_l2(g1, ref):.3e} (bitwise {np.array_equal(g1, ref)})", flush=True)
+    assert np.abs(ref).max() > 0
+    np.testing.assert_array_equal(g1, ref)
+
+
+def test_fwi_gradient_is_the_sum_of_shot_gradients(fwi):
+    torch, A, ctx = fwi["torch"], fwi["A"], fwi["ctx"]
+    n = len(fwi["srcs"])
+    acc = torch.zeros(fwi["grad"].shape, dtype=torch.float64, device=fwi["grad"].device)
+    for s0 in range(0, n, 8):
+        print(f"gradient of shots {s0}..{s0 + 7}", flush=True)
+        g = torch.zeros_like(fwi["grad"])
+        ctx.nl_velocity_gradient(fwi["md"], fwi["r"][s0:s0 + 8].contiguous(), g, fwi["srcs"][s0:s0 + 8], fwi["rec"],
+                                 mem=A.MEM_DEVICE)
+        acc += g.double()
+    e = rel_l2(fwi["grad"].cpu().numpy(), acc.cpu().numpy())
+    print(f"32-shot gradient vs sum of 8-shot gradients: relL2 {e:.2e}", flush=True)
+    assert e < 1e-6
+
+
+def test_fwi_gradient_central_differences_20_directions(fwi):
+    """J(v) = |d(v) - d_obs|^2 / 2 summed in FP64 on the device; central
+    differences J(v + e dv) - J(v - e dv) over 2e along 20 seeded smooth
+    directions (Gaussian-smoothed noise, sigma 200 m, max |dv| = 1 m/s) with
+    e = 10, against <grad, dv>.  Float data over 8000 fine steps carry
+    ~1e-5 relative rounding, which bounds the agreement; a missing term of
+    the gradient (the source-scaling term or the sponge) shows at 1e-2 or more."""
+    from scipy.ndimage import gaussian_filter
+    torch, A, w, ctx = fwi["torch"], fwi["A"], fwi["w"], fwi["ctx"]
+    dev = fwi["grad"].device
+    grad = fwi["grad"].double().cpu().numpy()
+    v0 = w.vel.astype(np.float64)
+    d = torch.zeros_like(fwi["dobs"])
+
+    def misfit(v):
+        ctx.set_velocity(v.astype(np.float32))
+        ctx.nl_forward(fwi["md"], d, fwi["srcs"], fwi["rec"], mem=A.MEM_DEVICE)
+        r = (d - fwi["dobs"]).double()
+        return 0.5 * float((r * r).sum())
+
+    rng = np.random.default_rng(20)
+    fw = 5                                                  # FAT
+    interior = np.zeros_like(v0)
+    interior[fw:-fw, fw:-fw] = 1.0
+    eps = 10.0
+    errs = []
+    try:
+        for k in range(20):
+            dv = gaussian_filter(rng.standard_normal(v0.shape), 20.0) * interior
+            dv /= np.abs(dv).max()
+            jp, jm = misfit(v0 + eps * dv), misfit(v0 - eps * dv)
+            fd = (jp - jm) / (2 * eps)
+            an = float(np.sum(grad * dv))
+            errs.append(abs(fd - an) / abs(fd))
+            print(f"direction {k:2d}: central difference {fd:.8e}, gradient {an:.8e}, rel {errs[-1]:.2e}", flush=True)
+    finally:
+        ctx.set_velocity(w.vel)
+    print(f"median rel {np.median(errs):.2e}, max {max(errs):.2e}", flush=True)
+    assert np.median(errs) < 2e-3
+    assert max(errs) < 1e-2
+
+
+def test_fwi_gradient_against_the_born_adjoint_gradient(fwi):
+    """The reference's FWI takes the Born adjoint of the residual as its
+    gradient direction (PY/fwiMain.py:123-142); the exact gradient differs
+    from it by the terms the Born operator does not differentiate (the source
+    scaling, the fine-time second derivative), so the two agree in direction."""
+    torch, A, w, ctx = fwi["torch"], fwi["A"], fwi["w"], fwi["ctx"]
+    img = torch.zeros_like(fwi["grad"])
+    ctx.born_adjoint(img, fwi["r"], fwi["srcs"], w.wavelet, fwi["rec"], mem=A.MEM_DEVICE)
+    a = fwi["grad"].double().cpu().numpy().ravel()
+    b = img.double().cpu().numpy().ravel()
+    mask = np.zeros(w.vel.shape, bool)
+    pz = w.params["zPadMinus"] + 5
+    px = w.params["xPadMinus"] + 5
+    mask[px:-px, pz + 10:-pz] = True                       # the physical model below the source rows
+    mask = mask.ravel()
+    cos = float(a[mask] @ b[mask] / np.linalg.norm(a[mask]) / np.linalg.norm(b[mask]))
+    print(f"cosine(exact gradient, Born adjoint of the residual) = {cos:.4f}", flush=True)
+    assert cos > 0.9

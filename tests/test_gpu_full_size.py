@@ -90,10 +90,12 @@ def test_linearity_and_dot_product_at_full_size(setup):
     a = float(np.vdot(d1.astype(np.float64).ravel(), dr.astype(np.float64).ravel()))
     b = float(np.vdot(m1.astype(np.float64).ravel(), _adj(torch, A, ctx, w, srcs, rec, dr).astype(np.float64).ravel()))
     print(f"linearity relL2 {lin:.2e}; <Fm,d> {a:.9e} <m,F'd> {b:.9e} rel {abs(a - b) / abs(b):.2e}")
-    # float rounding over 2 x 3998 fine steps: 1.2e-5 and 8.5e-6 measured on
-    # MI355X; a model-dependent term would show at order 1
+    # float rounding over 2 x 3998 fine steps: 1.2e-5 (linearity) and 8.5e-6
+    # (dot test) measured on MI355X; a model-dependent term would show at
+    # order 1.  The dot test holds to the 1e-5 bar of the operator
+    # (F/include/operator.h:50-88).
     assert lin < 1e-4
-    assert abs(a - b) <= 1e-4 * abs(b)
+    assert abs(a - b) <= 1e-5 * abs(b)
 
 
 def test_largest_grid_one_shot_dot_product():
@@ -124,4 +126,4 @@ def test_largest_grid_one_shot_dot_product():
     b = float(np.vdot(m.astype(np.float64).ravel(), img.cpu().numpy().astype(np.float64).ravel()))
     print(f"{NZ}x{NX}: <Fm,d> {a:.9e} <m,F'd> {b:.9e} rel {abs(a - b) / abs(b):.2e}")
     assert np.isfinite(a) and abs(a) > 0
-    assert abs(a - b) <= 1e-4 * abs(b)
+    assert abs(a - b) <= 1e-5 * abs(b)

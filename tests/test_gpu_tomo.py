@@ -24,7 +24,9 @@ def _engine(c):
     return ctx, srcs, recs
 
 
-@pytest.mark.parametrize("sub,nExt,extBatch", [(2, 5, 16), (3, 3, 4), (2, 21, 5)])
+# nExt 3/5/11/21 reach the tiled adjoint offset scattering for hExt <= 2, 5
+# and 10; nExt 23 (hExt 11) the untiled kernel.
+@pytest.mark.parametrize("sub,nExt,extBatch", [(2, 5, 16), (3, 3, 4), (2, 21, 5), (2, 11, 16), (2, 23, 8)])
 def test_tomo_matches_oracle(sub, nExt, extBatch):
     c = make_case(seed=81, nShot=3, nts=23, sub=sub, dts=0.001 * sub, nExt=nExt)
     ctx, srcs, recs = _engine(c)

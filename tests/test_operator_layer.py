@@ -29,16 +29,18 @@ def _write_case_files(tmp_path, c, nExt=1, geom_files=False):
     if nExt > 1:
         args += ["extension=offset", f"nExt={nExt}"]
     if geom_files:
-        src = np.zeros((c.nShot, 3), np.float32)
+        # The reference's layout (PY/Acoustic_iso_float.py:250-259, 414-431):
+        # arrays [3][nShot] and [3][nRec][nShot], i.e. SEP axis 1 = shot.
+        src = np.zeros((3, c.nShot), np.float32)
         for s in range(c.nShot):
-            src[s] = [(FAT + c.pad + 3 + 10 * s) * c.dx + 2.5, 0.0, (FAT + c.pad + 1) * c.dz + 1.5]
-        sep.write(str(tmp_path / "srcgeom.H"), sep.FloatVector(sep.hypercube(ns=[3, c.nShot]), src))
+            src[:, s] = [(FAT + c.pad + 3 + 10 * s) * c.dx + 2.5, 0.0, (FAT + c.pad + 1) * c.dz + 1.5]
+        sep.write(str(tmp_path / "srcgeom.H"), sep.FloatVector(sep.hypercube(ns=[c.nShot, 3]), src))
         nrec = 7
-        rec = np.zeros((c.nShot, nrec, 3), np.float32)
+        rec = np.zeros((3, nrec, c.nShot), np.float32)
         for s in range(c.nShot):
             for r in range(nrec):
-                rec[s, r] = [(FAT + c.pad + 2 + 4 * r + s) * c.dx, 0.0, (FAT + c.pad + 2) * c.dz + 4.0]
-        sep.write(str(tmp_path / "recgeom.H"), sep.FloatVector(sep.hypercube(ns=[3, nrec, c.nShot]), rec))
+                rec[:, r, s] = [(FAT + c.pad + 2 + 4 * r + s) * c.dx, 0.0, (FAT + c.pad + 2) * c.dz + 4.0]
+        sep.write(str(tmp_path / "recgeom.H"), sep.FloatVector(sep.hypercube(ns=[c.nShot, nrec, 3]), rec))
         args += [f"sourceGeomFile={tmp_path / 'srcgeom.H'}", f"receiverGeomFile={tmp_path / 'recgeom.H'}",
                  f"nReceiverPerShot={nrec}"]
     return args
@@ -66,18 +68,45 @@ def test_regular_geometry_builders(tmp_path):
     assert sax.n == 3 and sax.d == pytest.approx(12 * c.dx)
 
 
-def test_geometry_file_builders(tmp_path):
-    c = make_case(nShot=2)
+@pytest.mark.parametrize("nShot", [2, 3])
+def test_geometry_file_builders(tmp_path, nShot):
+    """Geometry files in the reference's layout, shot on SEP axis 1.  With
+    nShot == 3 a reader that took the coordinate axis for the shot axis would
+    accept the file and misread it, so the positions are checked per shot."""
+    c = make_case(nShot=nShot)
     par = AI.create_parObj(_write_case_files(tmp_path, c, geom_files=True))
     vel = sep.read(str(tmp_path / "vel.H"), ndims=2)
     srcs, _ = AI.buildSourceGeometry(par, vel)
     recs, rax = AI.buildReceiversGeometry(par, vel)
-    assert len(srcs) == 2 and len(recs) == 2 and rax.n == 7
+    assert len(srcs) == nShot and len(recs) == nShot and rax.n == 7
+    gs = sep.read(str(tmp_path / "srcgeom.H"), ndims=2).getNdArray()
+    gr = sep.read(str(tmp_path / "recgeom.H"), ndims=3).getNdArray()
+    assert gs.shape == (3, nShot) and gr.shape == (3, 7, nShot)
     # bilinear constructor on physical coordinates: the oracle restates it
-    for s in range(2):
-        g = sep.read(str(tmp_path / "recgeom.H"), ndims=3).getNdArray()[s]
-        o = O.Device.coords(g[:, 2], g[:, 0], c.nz, c.nx, 0.0, c.dz, 0.0, c.dx, c.nts)
+    for s in range(nShot):
+        o = O.Device.coords(gs[2, s:s + 1], gs[0, s:s + 1], c.nz, c.nx, 0.0, c.dz, 0.0, c.dx, c.nts)
+        np.testing.assert_array_equal(np.array(srcs[s].getRegPosUnique()), o.regPos)
+        o = O.Device.coords(gr[2, :, s], gr[0, :, s], c.nz, c.nx, 0.0, c.dz, 0.0, c.dx, c.nts)
         np.testing.assert_array_equal(np.array(recs[s].getRegPosUnique()), o.regPos)
+    # the sources move along x by 10 grid points per shot (src file layout pinned)
+    ix = [srcs[s].getRegPosUnique()[0] // c.nz for s in range(nShot)]
+    assert np.all(np.diff(ix) == 10)
+
+
+def test_geometry_file_transposed_is_rejected(tmp_path):
+    """A file written with the coordinate axis fastest ([nShot][3]) does not
+    match the reference's shape checks and is refused, as by the reference."""
+    c = make_case(nShot=2)
+    args = _write_case_files(tmp_path, c, geom_files=True)
+    src = np.zeros((c.nShot, 3), np.float32)
+    src[:, 0] = 300.0
+    src[:, 2] = 200.0
+    sep.write(str(tmp_path / "srcT.H"), sep.FloatVector(sep.hypercube(ns=[3, c.nShot]), src))
+    args = [a for a in args if not a.startswith("sourceGeomFile=")] + [f"sourceGeomFile={tmp_path / 'srcT.H'}"]
+    par = AI.create_parObj(args)
+    vel = sep.read(str(tmp_path / "vel.H"), ndims=2)
+    with pytest.raises(ValueError, match="not consistent with geometry file"):
+        AI.buildSourceGeometry(par, vel)
 
 
 def test_geometry_file_shot_count_is_checked(tmp_path):
