@@ -9,9 +9,14 @@ Differences from the reference, by design:
   parameter object is ``sep.ParamObj``; files are SEP ``.H`` headers read with
   ``sep.read`` -- genericIO/SepVector are external to the reference and absent;
 * the Dask multi-node path (``client=``) is not provided: one process drives
-  one GPU, and shots are batched on it by the engine;
-* wavefield-saving methods (``forwardWavefield``, ``getSrcWfld``...) raise, as
-  the engine never materializes full-rate wavefields on the host.
+  one GPU, and shots are batched on it by the engine.
+
+Wavefield saving follows the reference: ``forwardWavefield`` /
+``adjointWavefield`` run the operator and keep the coarse wavefields
+[nts][nx][nz] of shot ``wavefieldShotNumber`` (parfile key, default 0);
+``getWfld`` / ``getSrcWfld`` / ``getSecWfld`` return them as vectors on the
+velocity's z and x axes and the time axis (PY/Acoustic_iso_float.py:663-699,
+927-969).
 """
 from __future__ import annotations
 
@@ -184,10 +189,24 @@ class _Wrapped(Operator):
             self.pyOp.adjoint(add, _cpp(model), _cpp(data))
 
     def forwardWavefield(self, add, model, data):
-        self.pyOp.forwardWavefield(add, _cpp(model), _cpp(data))
+        with self._mod.ostream_redirect():
+            self.pyOp.forwardWavefield(add, _cpp(model), _cpp(data))
 
     def adjointWavefield(self, add, model, data):
-        self.pyOp.adjointWavefield(add, _cpp(model), _cpp(data))
+        with self._mod.ostream_redirect():
+            self.pyOp.adjointWavefield(add, _cpp(model), _cpp(data))
+
+    def _keep_axes(self, velocity, paramP):
+        """Axes of the saved wavefields: the velocity's z and x, and time."""
+        vax = velocity.getHyper().axes if hasattr(velocity, "getHyper") else None
+        self._wfld_axes = None
+        if vax is not None:
+            self._wfld_axes = [vax[0], vax[1], _time_axis(paramP)]
+
+    def _wfld_vector(self, arr):
+        hyper = sep.hypercube(axes=self._wfld_axes) if getattr(self, "_wfld_axes", None) else \
+            sep.hypercube(ns=[arr.shape[2], arr.shape[1], arr.shape[0]])
+        return sep.FloatVector(hyper, arr)
 
     def setVel(self, vel):
         with self._mod.ostream_redirect():
@@ -209,9 +228,12 @@ class nonlinearPropShotsGpu(_Wrapped):
         self.setDomainRange(domain, range)
         self.pyOp = pyAcoustic_iso_float_nl.nonlinearPropShotsGpu(_cpp(velocity), _cpp(paramP).param, sourceVector,
                                                                   receiversVector)
+        self._keep_axes(velocity, paramP)
 
     def getWfld(self):
-        return self.pyOp.getWfld()
+        """Wavefield saved by forwardWavefield / adjointWavefield."""
+        with self._mod.ostream_redirect():
+            return self._wfld_vector(self.pyOp.getWfld())
 
 
 # ------------------------------------------------------------------ Born --
@@ -242,12 +264,17 @@ class BornShotsGpu(_Wrapped):
         sig = [_cpp(s) for s in sourcesSignalsVector]
         self.pyOp = pyAcoustic_iso_float_born.BornShotsGpu(_cpp(velocity), _cpp(paramP).param, sourceVector, sig,
                                                            receiversVector)
+        self._keep_axes(velocity, paramP)
 
     def getSrcWfld(self):
-        return self.pyOp.getSrcWfld()
+        """Source wavefield saved by forwardWavefield / adjointWavefield."""
+        with self._mod.ostream_redirect():
+            return self._wfld_vector(self.pyOp.getSrcWfld())
 
     def getSecWfld(self):
-        return self.pyOp.getSecWfld()
+        """Scattered (forward) or scaled receiver (adjoint) wavefield."""
+        with self._mod.ostream_redirect():
+            return self._wfld_vector(self.pyOp.getSecWfld())
 
 
 # --------------------------------------------------------- extended Born --
@@ -297,12 +324,13 @@ class BornExtShotsGpu(_Wrapped):
         sig = [_cpp(s) for s in sourcesSignalsVector]
         self.pyOp = pyAcoustic_iso_float_born_ext.BornExtShotsGpu(_cpp(velocity), _cpp(paramP).param, sourceVector,
                                                                   sig, receiversVector)
+        self._keep_axes(velocity, paramP)
 
     def getSrcWfld(self):
-        return self.pyOp.getSrcWfld()
-
-    def getSecWfld(self):
-        return self.pyOp.getSecWfld()
+        """Source wavefield saved by forwardWavefield / adjointWavefield
+        (the reference's extended class keeps it without a Python getter)."""
+        with self._mod.ostream_redirect():
+            return self._wfld_vector(self.pyOp.getSrcWfld())
 
 
 # ------------------------------------------------------- tomographic op --

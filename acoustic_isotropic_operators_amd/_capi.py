@@ -14,6 +14,7 @@ import numpy as np
 from . import lib_path
 
 MEM_HOST, MEM_DEVICE = 0, 1
+WAVEFIELD_SOURCE, WAVEFIELD_SECONDARY = 0, 1
 EXT_NONE, EXT_OFFSET, EXT_TIME = 0, 1, 2
 K_ALL, K_NL_FWD, K_NL_ADJ, K_SRC, K_BORN_SCAT, K_BORN_REC, K_EXT_SCAT, K_EXT_REC, K_BORN_PAIR, K_TOMO = range(-1, 9)
 
@@ -62,6 +63,7 @@ _SIGS = {
     "aiso_bornext_forward": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_bornext_adjoint": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_ctx_set_ext_reflectivity": (_I, [_P, _P, _I]),
+    "aiso_ctx_wavefield": (_I, [_P, _I, _P, _I]),
     "aiso_tomo_forward": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_tomo_adjoint": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_wemva_forward": (_I, [_P, _I, _P, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
@@ -362,6 +364,14 @@ class Context:
 
     def bornext_adjoint(self, model, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
         self._born(lib().aiso_bornext_adjoint, model, data, sources, wavelets, receivers, add, mem, True, True)
+
+    def wavefield(self, which=WAVEFIELD_SOURCE):
+        """A saved wavefield [nts][nx][nz] (host copy) of the last call made
+        with the option saveWavefield set."""
+        p = self.params
+        out = np.zeros((int(p["nts"]), int(p["nx"]), int(p["nz"])), np.float32)
+        _check(lib().aiso_ctx_wavefield(self.h, int(which), _ptr(out), MEM_HOST))
+        return out
 
     def set_ext_reflectivity(self, refl, mem=MEM_HOST):
         """Extended reflectivity [nExt][nx][nz] of the tomographic operator."""

@@ -697,6 +697,11 @@ aiso_status aiso_ctx_set_option(aiso_ctx* c, const char* key, int value)
         else if (k == "leg2") c->leg2 = value != 0;
         else if (k == "sloth") c->sloth = value != 0;
         else if (k == "imagePerShot") c->imagePerShot = value != 0;
+        else if (k == "saveWavefield") c->saveWavefield = value != 0;
+        else if (k == "wavefieldShot") {
+            if (value < 0) throw Error(AISO_ERR_PARAM, "wavefieldShot must be >= 0");
+            c->wavefieldShot = value;
+        }
         else if (k == "extBatch") {
             if (value < 1 || value > kExtBatchMax) throw Error(AISO_ERR_PARAM, "extBatch must be 1..16");
             c->extBatch = value;
@@ -927,6 +932,25 @@ aiso_status aiso_bornext_adjoint(aiso_ctx* c, int add, float* model, const float
                                  int nShot, const float* wav, int nWav, aiso_device* const* rec, int nRec, int mem)
 {
     return op_call(c, Op::ExtAdj, add, nullptr, model, data, nullptr, 1, src, nShot, wav, nWav, rec, nRec, mem);
+}
+
+aiso_status aiso_ctx_wavefield(aiso_ctx* c, int which, float* out, int mem)
+{
+    return guard([&] {
+        if (!c || !out) throw Error(AISO_ERR_PARAM, "null argument");
+        if (which != AISO_WAVEFIELD_SOURCE && which != AISO_WAVEFIELD_SECONDARY)
+            throw Error(AISO_ERR_PARAM, "which must be AISO_WAVEFIELD_SOURCE or AISO_WAVEFIELD_SECONDARY");
+        const bool src = which == AISO_WAVEFIELD_SOURCE;
+        if (!(src ? c->wfSrcValid : c->wfSecValid))
+            throw Error(AISO_ERR_STATE, src ? "no wavefield saved: call an operator with saveWavefield set first"
+                                            : "no secondary wavefield saved: only the Born forward / adjoint save one");
+        HIPC(hipSetDevice(c->device));
+        const size_t bytes = (size_t)c->p.nts * c->p.nzx() * 4;
+        const aiso::DBuf& b = src ? c->wfSrc : c->wfSec;
+        HIPC(hipMemcpyAsync(out, b.p, bytes, mem == AISO_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                            c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+    });
 }
 
 aiso_status aiso_ctx_set_ext_reflectivity(aiso_ctx* c, const float* refl, int mem)

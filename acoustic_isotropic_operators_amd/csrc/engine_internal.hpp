@@ -133,7 +133,7 @@ struct aiso_ctx {
     // call, and its scattered (forward) or receiver (adjoint) wavefield; the
     // nonlinear operators save their propagated wavefield.
     int saveWavefield = 0, wavefieldShot = 0;
-    aiso::DBuf wfSrc, wfSec;
+    aiso::DBuf wfSrc, wfSec, wfScratch;
     int wfSrcValid = 0, wfSecValid = 0;
     hipEvent_t evIn = nullptr;          // orders device inputs after the default stream's work
     // Host staging of the per-batch tables and source traces: pinned, reused

@@ -1200,8 +1200,9 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
             // Forward passes capture the coarse source wavefield W; the
             // adjoint receiver pass accumulates into the two ss slices.
             constexpr bool CAP_SS = ADJ && !(MODE & M_CAP_W);
-            float* A = CAP_SS ? S.ss[a.ssA] + a.ssOffA : S.W + a.wA;
-            float* B = CAP_SS ? S.ss[a.ssB] + a.ssOffB : S.W + a.wB;
+            constexpr bool CAP_ALT = !ADJ && (MODE & M_CAP_ALT);
+            float* A = CAP_SS ? S.ss[a.ssA] + a.ssOffA : CAP_ALT ? S.img + a.wA : S.W + a.wA;
+            float* B = CAP_SS ? S.ss[a.ssB] + a.ssOffB : CAP_ALT ? S.img + a.wB : S.W + a.wB;
             // A zero weight leaves the slice as it is (x + n*0 == x); a
             // first touch writes instead of accumulating into a zeroed slice.
             if (!a.skipA) A[gi] = a.assignA ? n * a.fe : A[gi] + n * a.fe;
@@ -1977,6 +1978,18 @@ __global__ void k_img_acc(const ShotSlot* X, long xOff, const ShotSlot* Y, long 
 }
 
 // acc += img (one shot, in shot order).
+__global__ void k_record_scale(const ShotSlot* slot, int ssIdx, float* __restrict__ out,
+                               const float* __restrict__ refl, const float* __restrict__ v2dt2, int nz, int nx)
+{
+    const float* ss = slot->ss[ssIdx];
+    const long nzx = (long)nz * nx;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < nzx; i += (long)gridDim.x * blockDim.x) {
+        const int iz = int(i % nz), ix = int(i / nz);
+        if (iz < FAT || iz >= nz - FAT || ix < FAT || ix >= nx - FAT) continue;
+        out[i] = (ss[i] * refl[i]) * v2dt2[i];
+    }
+}
+
 __global__ void k_accumulate(float* __restrict__ acc, const float* __restrict__ img, long n)
 {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -2135,6 +2148,7 @@ hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a,
     case MODE_TOMO_REC: k = pick_generic_tile<MODE_TOMO_REC>(tz, tx); break;
     case MODE_EXT_REC: k = pick_generic_tile<MODE_EXT_REC>(tz, tx); break;
     case MODE_REC_WIN: k = pick_generic_tile<MODE_REC_WIN>(tz, tx); break;
+    case MODE_BORN_SCAT_CAP: k = pick_generic_tile<MODE_BORN_SCAT_CAP>(tz, tx); break;
     default: break;
     }
     if (!k) return hipErrorInvalidValue;
@@ -2414,6 +2428,13 @@ hipError_t launch_ext_adj_finish(const ShotSlot* slots, int nshots, const float*
     dim3 g = flat_grid((long)nz * nx * nExt);
     g.y = nshots;
     hipLaunchKernelGGL(k_ext_adj_finish, g, dim3(256), 0, s, slots, refl, nz, nx, nExt);
+    return hipGetLastError();
+}
+
+hipError_t launch_record_scale(const ShotSlot* slot, int ssIdx, float* out, const float* refl, const float* v2dt2,
+                               int nz, int nx, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_record_scale, flat_grid((long)nz * nx), dim3(256), 0, s, slot, ssIdx, out, refl, v2dt2, nz, nx);
     return hipGetLastError();
 }
 

@@ -41,6 +41,7 @@ enum : unsigned {
     M_WIN = 1u << 9,        // window capture V = (o*wa + c*wb) + n*wc when args.win != 0
     M_SS_PRE = 1u << 10,    // + SS[a]*fa + SS[b]*fb, SS = m~*W stored in the W array
     M_CAP_W = 1u << 11,     // adjoint stepper: capture into W[wA], W[wB] instead of ss
+    M_CAP_ALT = 1u << 12,   // forward stepper: capture into img[wA], img[wB] instead of W
 };
 
 // Where a window capture V goes (StepArgs::winOut).
@@ -179,6 +180,9 @@ constexpr unsigned MODE_REC_WIN = M_ADJ | M_INJ_DATA | M_WIN;
 constexpr unsigned MODE_TOMO_CAP = M_SS_SLICE | M_CAPTURE;
 constexpr unsigned MODE_TOMO_ADJ_CAP = M_ADJ | M_SS_SLICE | M_CAPTURE | M_CAP_W;
 constexpr unsigned MODE_TOMO_REC = M_ADJ | M_INJ_DATA | M_CAPTURE | M_CAP_W;
+// Wavefield saving: the Born scattered field captured at the coarse rate
+// into ShotSlot::img (the source wavefield W stays readable).
+constexpr unsigned MODE_BORN_SCAT_CAP = M_SS_BORN | M_CAPTURE | M_CAP_ALT;
 
 // spb: shots advanced per workgroup (1 or 2), sharing the tile's v2dt2 and sponge.
 // e0/e1 (optional): events timestamped by the dispatch itself.
@@ -218,6 +222,10 @@ hipError_t launch_tomo_adj_ss(const ExtBatchArgs& a, int nshots, hipStream_t s);
 hipError_t launch_img_acc(const ShotSlot* X, long xOff, const ShotSlot* Y, long yOff, int nshots, int nz, int nx,
                           hipStream_t s);
 hipError_t launch_accumulate(float* acc, const float* img, long n, hipStream_t s);
+// out = (ss[ssIdx] * refl) * v2dt2 on the interior of one shot slot
+// (recordScaleWavefield, F/src/kernelsGpu.cu:145-153, into a zeroed slice).
+hipError_t launch_record_scale(const ShotSlot* slot, int ssIdx, float* out, const float* refl, const float* v2dt2,
+                               int nz, int nx, hipStream_t s);
 hipError_t launch_add_to(float* out, const float* acc, long n, int add, hipStream_t s);
 hipError_t launch_dev_forward(const float* reg, float* out, const int* cornerReg, const float* weight,
                               int nDev, int nb, int nt, int add, hipStream_t s);

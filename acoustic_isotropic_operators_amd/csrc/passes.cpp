@@ -1309,7 +1309,157 @@ void Call::finish_nl_adjoint()
     }
 }
 
-void Call::save_wavefields() {}
+// Wavefield saving (the reference's forwardWavefield / adjointWavefield for
+// shot wavefieldShotNumber, F/src/BornShotsGpu.cpp:145-240, 341-438,
+// F/src/nonlinearPropShotsGpu.cpp:153-250, 360-458): shot `wavefieldShot`
+// of the call runs once more, alone, on the generic step kernel, capturing
+// the coarse wavefields [nts][nx][nz] the reference keeps:
+//   nonlinear forward / adjoint    the propagated field (getWfld)
+//   Born-type operators            the source wavefield (getSrcWfld)
+//   Born forward                   + the scattered wavefield (getSecWfld)
+//   Born adjoint                   + the receiver wavefield scaled like
+//                                    the image (recordScaleWavefield)
+// The call's outputs come from its regular passes; this pass only fills
+// the context's wavefield buffers, in the reference's arithmetic.
+void Call::save_wavefields()
+{
+    const int ws = c->wavefieldShot;
+    if (ws < 0 || ws >= q.nShot) throw Error(AISO_ERR_PARAM, "wavefieldShotNumber is not a shot of the call");
+    c->wfSrcValid = c->wfSecValid = 0;
+    const bool secFwd = q.op == Op::BornFwd, secAdj = q.op == Op::BornAdj;
+    if (tomo) return;   // the reference's tomographic and WEMVA operators save no wavefields
+    float* wsrc = (float*)c->wfSrc.need((size_t)p.nts * nzx * 4);
+    float* wsec = secFwd || secAdj ? (float*)c->wfSec.need((size_t)p.nts * nzx * 4) : nullptr;
+    float* scr = (float*)c->wfScratch.need((size_t)(secAdj ? 5 : 2) * nzx * 4);   // pair (+ ss pair + image)
+    // the captures write the interior; the FAT ring of a saved slice reads 0,
+    // as the reference's zeroed wavefield buffers do
+    HIPC(launch_fill(wsrc, (long)p.nts * nzx, s));
+    if (wsec) HIPC(launch_fill(wsec, (long)p.nts * nzx, s));
+    b0 = ws;
+    nb = 1;
+    const aiso_device* sd = q.src[ws];
+    DevGeom* sg = srcG[ws];
+    DevGeom* rg = rec_geom(ws);
+    ShotSlot S;
+    std::memset(&S, 0, sizeof(S));
+    S.p[0] = scr;
+    S.p[1] = scr + nzx;
+    S.W = wsrc;
+    if (secAdj) {
+        S.ss[0] = scr + 2 * nzx;
+        S.ss[1] = scr + 3 * nzx;
+        S.img = scr + 4 * nzx;
+    }
+    if (secFwd) S.img = wsec;
+    S.dreg = dreg;
+    S.sig = sig;
+    S.srcTile = sg->tilePtr.as<int>();
+    S.srcEnt = sg->ent.as<int2>();
+    S.srcRing = sg->ring.as<int2>();
+    S.nSrcRing = sg->nRing;
+    S.recTile = rg->tilePtr.as<int>();
+    S.recEnt = rg->ent.as<int2>();
+    S.recRing = rg->ring.as<int2>();
+    S.nRecRing = rg->nRing;
+    maxSrcRing = sg->nRing;
+    maxRecRing = rg->nRing;
+    std::vector<float> fine;
+    if (is_born(q.op)) {
+        fine = born_source(c, sd->g, q.wavelets + (size_t)(q.nWav == 1 ? 0 : ws) * nSrcDev * p.nts);
+    } else if (q.op == Op::NlFwd) {
+        fine = nl_source(c, sd->g, hModel.data() + (size_t)(q.nShotSig == 1 ? 0 : ws) * nSrcDev * p.nts);
+    }
+    stage_begin(c, stage_size(sizeof(ShotSlot)) + stage_size(fine.size() * 4));
+    stage_copy(c, dSlots, &S, sizeof(ShotSlot));
+    stage_copy(c, sig, fine.data(), fine.size() * 4);
+    stage_end(c);
+    HIPC(launch_fill(scr, (long)(secAdj ? 5 : 2) * nzx, s));
+    if (dataInput) {
+        HIPC(launch_dev_adjoint(dreg, dDataIn + (size_t)ws * nRecDev * p.nts, rg->adjPtr.as<int>(), rg->adjEnt.as<int>(),
+                                rg->weight.as<float>(), rg->nReg, rec_dev(ws)->g.nb, p.nts, s));
+    }
+    c->curLanes = 1;
+    StepArgs x = base;
+    x.slots = dSlots;
+    x.nshots = 1;
+    x.gridLogical = x.ntiles;
+    x.win = 0;
+    // a forward pass capturing into W (mode MODE_SRC) or img (MODE_BORN_SCAT_CAP)
+    auto forward = [&](unsigned mode) {
+        HIPC(launch_fill(scr, 2 * nzx, s));
+        x.cur = 1;
+        for (int its = 0; its < p.nts - 1; ++its)
+            for (int it2 = 1; it2 <= p.sub; ++it2) {
+                x.its = its;
+                x.itwInj = its * p.sub + it2 - 1;
+                x.fa = f[it2 - 1];
+                x.fb = f[h + it2 - 1];
+                x.wA = (long)its * nzx;
+                x.wB = (long)(its + 1) * nzx;
+                x.fe = f[it2];
+                x.ff = f[h + it2];
+                x.skipA = x.fe == 0.f;
+                x.skipB = x.ff == 0.f;
+                x.assignA = its == 0 && it2 == 1;
+                x.assignB = it2 == 1;
+                HIPC(launch_step_generic(mode, c->tz, c->tx, x, s));
+                ring(mode, x);
+                x.cur ^= 1;
+            }
+    };
+    if (q.op != Op::NlAdj) forward(MODE_SRC);          // NL wavefield, or the source wavefield
+    if (secFwd) {                                        // scattered wavefield
+        x.mtil = mtil;
+        forward(MODE_BORN_SCAT_CAP);
+    }
+    if (q.op == Op::NlAdj || secAdj) {
+        // backward in time from the data: the nonlinear adjoint field into W,
+        // or the Born receiver field into the ss pair, imaged against W and
+        // recorded scaled after each coarse step
+        HIPC(launch_fill(scr, 2 * nzx, s));
+        const unsigned mode = secAdj ? MODE_BORN_REC : MODE_TOMO_REC;
+        x.cur = 1;
+        int L = 0, R = 1;
+        for (int its = p.nts - 2; its >= 0; --its) {
+            for (int it2 = p.sub - 1; it2 >= 0; --it2) {
+                x.its = its;
+                x.itwRec = its * p.sub + it2;
+                x.fc = f[it2 + 1];
+                x.fd = f[h + it2 + 1];
+                x.fe = f[it2];
+                x.ff = f[h + it2];
+                x.skipA = x.fe == 0.f;
+                x.skipB = x.ff == 0.f;
+                if (secAdj) {
+                    x.ssA = L;
+                    x.ssB = R;
+                    x.ssOffA = x.ssOffB = 0;
+                    x.assignA = it2 == p.sub - 1;
+                    x.assignB = 0;
+                    x.image = it2 == 0;
+                    x.wImg = (long)(its + 1) * nzx;
+                } else {
+                    x.wA = (long)its * nzx;
+                    x.wB = (long)(its + 1) * nzx;
+                    x.assignA = it2 == p.sub - 1;
+                    x.assignB = its == p.nts - 2 && it2 == p.sub - 1;
+                }
+                HIPC(launch_step_generic(mode, c->tz, c->tx, x, s));
+                ring(mode, x);
+                x.cur ^= 1;
+            }
+            if (secAdj) {
+                HIPC(launch_record_scale(dSlots, R, wsec + (size_t)(its + 1) * nzx, c->refl.as<float>(), base.v2dt2,
+                                         p.nz, p.nx, s));
+                std::swap(L, R);
+            }
+        }
+        if (secAdj)
+            HIPC(launch_record_scale(dSlots, R, wsec, c->refl.as<float>(), base.v2dt2, p.nz, p.nx, s));
+    }
+    c->wfSrcValid = 1;
+    c->wfSecValid = wsec != nullptr;
+}
 
 void Call::end_batch()
 {
@@ -1367,6 +1517,7 @@ void Call::run()
         if (q.op == Op::NlAdj) finish_nl_adjoint();
         end_batch();
     }
+    if (c->saveWavefield) save_wavefields();
     outputs();
 }
 

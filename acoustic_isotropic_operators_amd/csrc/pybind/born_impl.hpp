@@ -70,15 +70,16 @@ template <bool EXT>
 void bind_born(py::module_& m, const char* name)
 {
     using T = BornShots<EXT>;
-    auto unsupported = [](T&, bool, py::object, py::object) {
-        throw std::runtime_error("**** ERROR: wavefield saving is not supported by this engine ****");
-    };
     py::class_<T, std::shared_ptr<T>>(m, name)
         .def(py::init<py::object, py::object, std::vector<DevPtr>, std::vector<py::object>, std::vector<DevPtr>>())
         .def("forward", [](T& s, bool add, py::object mo, py::object da) { s.run(false, add, mo, da); })
         .def("adjoint", [](T& s, bool add, py::object mo, py::object da) { s.run(true, add, mo, da); })
-        .def("forwardWavefield", unsupported)
-        .def("adjointWavefield", unsupported)
+        .def("forwardWavefield", [](T& s, bool add, py::object mo, py::object da) {
+            s.with_wavefield([&] { s.run(false, add, mo, da); });
+        })
+        .def("adjointWavefield", [](T& s, bool add, py::object mo, py::object da) {
+            s.with_wavefield([&] { s.run(true, add, mo, da); });
+        })
         .def("setVel", [](T& s, py::object v) { s.take_vel(v); })
         .def("getVel", [](T& s) { return s.vel; })
         .def("dotTest", [](T& s, bool verbose, float maxError) {
@@ -90,12 +91,8 @@ void bind_born(py::module_& m, const char* name)
                             [&](bool add, F32& a, F32& b) { s.run(false, add, a, b); },
                             [&](bool add, F32& a, F32& b) { s.run(true, add, a, b); }, verbose, maxError);
         }, py::arg("verbose") = false, py::arg("maxError") = 1e-5f)
-        .def("getSrcWfld", [](T&) -> py::object {
-            throw std::runtime_error("**** ERROR: wavefield saving is not supported by this engine ****");
-        })
-        .def("getSecWfld", [](T&) -> py::object {
-            throw std::runtime_error("**** ERROR: wavefield saving is not supported by this engine ****");
-        });
+        .def("getSrcWfld", [](T& s) { return s.wavefield(AISO_WAVEFIELD_SOURCE); })
+        .def("getSecWfld", [](T& s) { return s.wavefield(AISO_WAVEFIELD_SECONDARY); });
 }
 
 }  // namespace aisopy

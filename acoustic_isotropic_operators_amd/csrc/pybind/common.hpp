@@ -295,6 +295,57 @@ struct ShotsBase {
         for (auto& d : v) r.push_back(d->d);
         return r;
     }
+
+    // forwardWavefield / adjointWavefield (F/src/BornShotsGpu.cpp:145-240,
+    // 341-438): the operator call, saving the wavefields of shot
+    // wavefieldShotNumber (parfile key, default 0) on the way; they are
+    // copied to the host, as the reference keeps them in host vectors.
+    py::object savedSrc = py::none(), savedSec = py::none();
+
+    template <class F>
+    void with_wavefield(F&& call)
+    {
+        ensure();
+        check(aiso_ctx_set_option(ctx, "wavefieldShot", par.i("wavefieldShotNumber", 0)));
+        check(aiso_ctx_set_option(ctx, "saveWavefield", 1));
+        try {
+            call();
+        } catch (...) {
+            aiso_ctx_set_option(ctx, "saveWavefield", 0);
+            throw;
+        }
+        check(aiso_ctx_set_option(ctx, "saveWavefield", 0));
+        savedSrc = fetch(AISO_WAVEFIELD_SOURCE);
+        savedSec = fetch(AISO_WAVEFIELD_SECONDARY);
+    }
+
+    // A saved wavefield [nts][nx][nz] from the context, or None if the last
+    // call saved none of that kind.
+    py::object fetch(int which)
+    {
+        F32 out({(ssize_t)fp.nts, (ssize_t)fp.nx, (ssize_t)fp.nz});
+        float* o = out.mutable_data();
+        aiso_status st;
+        {
+            py::gil_scoped_release nogil;
+            st = aiso_ctx_wavefield(ctx, which, o, AISO_MEM_HOST);
+        }
+        if (st == AISO_ERR_STATE) return py::none();
+        check(st);
+        return std::move(out);
+    }
+
+    // getSrcWavefield / getSecWavefield / getWavefield of the reference
+    // classes (F/include/BornShotsGpu.h:50-51, nonlinearPropShotsGpu.h:47).
+    py::object wavefield(int which) const
+    {
+        const py::object& w = which == AISO_WAVEFIELD_SOURCE ? savedSrc : savedSec;
+        if (w.is_none())
+            throw std::runtime_error(which == AISO_WAVEFIELD_SOURCE
+                                         ? "**** ERROR: no wavefield saved: call forwardWavefield or adjointWavefield first ****"
+                                         : "**** ERROR: no secondary wavefield saved (Born forwardWavefield / adjointWavefield) ****");
+        return w;
+    }
 };
 
 // Operator::dotTest (F/include/operator.h:50-88) with uniform random vectors.

@@ -45,11 +45,11 @@ PYBIND11_MODULE(pyAcoustic_iso_float_nl, m)
              "Initialize a nonlinearPropShotsGpu")
         .def("forward", [](NlShots& s, bool add, py::object mo, py::object da) { s.run(false, add, mo, da); })
         .def("adjoint", [](NlShots& s, bool add, py::object mo, py::object da) { s.run(true, add, mo, da); })
-        .def("forwardWavefield", [](NlShots&, bool, py::object, py::object) {
-            throw std::runtime_error("**** ERROR: wavefield saving is not supported by this engine ****");
+        .def("forwardWavefield", [](NlShots& s, bool add, py::object mo, py::object da) {
+            s.with_wavefield([&] { s.run(false, add, mo, da); });
         })
-        .def("adjointWavefield", [](NlShots&, bool, py::object, py::object) {
-            throw std::runtime_error("**** ERROR: wavefield saving is not supported by this engine ****");
+        .def("adjointWavefield", [](NlShots& s, bool add, py::object mo, py::object da) {
+            s.with_wavefield([&] { s.run(true, add, mo, da); });
         })
         .def("setVel", [](NlShots& s, py::object v) { s.take_vel(v); })
         .def("getVel", [](NlShots& s) { return s.vel; })
@@ -60,8 +60,6 @@ PYBIND11_MODULE(pyAcoustic_iso_float_nl, m)
                             [&](bool add, F32& a, F32& b) { s.run(false, add, a, b); },
                             [&](bool add, F32& a, F32& b) { s.run(true, add, a, b); }, verbose, maxError);
         }, py::arg("verbose") = false, py::arg("maxError") = 1e-5f)
-        .def("getWfld", [](NlShots&) -> py::object {
-            throw std::runtime_error("**** ERROR: wavefield saving is not supported by this engine ****");
-        });
+        .def("getWfld", [](NlShots& s) { return s.wavefield(AISO_WAVEFIELD_SOURCE); });
     alias_module(m, "pyAcoustic_iso_float_nl");
 }

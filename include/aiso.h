@@ -120,7 +120,11 @@ aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
  *   "sloth" 0/1: time-lag extended Born model is a slowness-squared
  *       perturbation, scaled by -v^2 dtw^2 (parfile key sloth of
  *       F/src/BornExtGpu.cpp:14, default 0; ignored with a free surface, as
- *       by the reference's free-surface routines). */
+ *       by the reference's free-surface routines);
+ *   "saveWavefield" 0/1, "wavefieldShot" N: the next nonlinear and
+ *       Born-type calls also save the coarse wavefields of their shot N (see
+ *       aiso_ctx_wavefield; parfile keys saveWavefield / wavefieldShotNumber
+ *       of F/src/BornShotsGpu.cpp:18-19). */
 aiso_status aiso_ctx_set_option(aiso_ctx* ctx, const char* key, int value);
 /* Derived sizes: ntw, sub, nShotsPerLaunch actually used by the last call. */
 aiso_status aiso_ctx_info(const aiso_ctx* ctx, int* ntw, int* batchUsed, size_t* bytesPerShot);
@@ -153,6 +157,21 @@ aiso_status aiso_ctx_profile_read(aiso_ctx* ctx, int which, long long* launches,
  * the batch that carried it.  Writes at most maxBatches entries; *count gets
  * the number available. */
 aiso_status aiso_ctx_batch_read(aiso_ctx* ctx, float* ms, int* shots, int maxBatches, int* count);
+
+/* Saved wavefields [nts][nx][nz] of the last call made with "saveWavefield"
+ * (the reference's forwardWavefield / adjointWavefield and getters:
+ * F/src/BornShotsGpu.cpp:145-240, 341-438 with getSrcWavefield /
+ * getSecWavefield of F/include/BornShotsGpu.h:50-51;
+ * F/src/nonlinearPropShotsGpu.cpp:153-250, 360-458 with getWavefield of
+ * F/include/nonlinearPropShotsGpu.h:47; PY/pyAcoustic_iso_float_born.cpp:25-33,
+ * PY/pyAcoustic_iso_float_nl.cpp:34-42).  AISO_WAVEFIELD_SOURCE: the
+ * nonlinear operator's propagated wavefield, or the source wavefield of a
+ * Born-type operator; AISO_WAVEFIELD_SECONDARY: the Born forward's scattered
+ * wavefield or the Born adjoint's receiver wavefield scaled by
+ * 2/v^3 v^2 dtw^2 (F/src/kernelsGpu.cu:145-153).  out: nts*nx*nz floats,
+ * host or device memory by mem. */
+enum { AISO_WAVEFIELD_SOURCE = 0, AISO_WAVEFIELD_SECONDARY = 1 };
+aiso_status aiso_ctx_wavefield(aiso_ctx* ctx, int which, float* out, int mem);
 
 /* ------------------------------------------------------------ geometry --
  * deviceGpu.  nzVel/nxVel are the padded velocity sizes; nt the trace length. */

@@ -46,6 +46,20 @@ void orc_damp##SFX(const orc_grid* g, const REAL* cd, REAL* p1, REAL* p2);      
 void orc_nl_fwd##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* sig,         \
                      const int* srcPos, int nSrc, const int* recPos, int nRec,       \
                      REAL* dreg);                                                    \
+void orc_nl_fwd_wfld##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* sig,    \
+                          const int* srcPos, int nSrc, const int* recPos, int nRec,  \
+                          REAL* dreg, REAL* wfld);                                   \
+void orc_nl_adj_wfld##SFX(const orc_grid* g, const REAL* v2dt2, REAL* sig,          \
+                          const int* srcPos, int nSrc, const int* recPos, int nRec,  \
+                          const REAL* dreg, REAL* wfld);                             \
+void orc_born_fwd_wfld##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* refl, \
+                            const REAL* sig, const int* srcPos, int nSrc,            \
+                            const int* recPos, int nRec, const REAL* model,          \
+                            REAL* dreg, REAL* W, REAL* Wsec);                        \
+void orc_born_adj_wfld##SFX(const orc_grid* g, const REAL* v2dt2, const REAL* refl, \
+                            const REAL* sig, const int* srcPos, int nSrc,            \
+                            const int* recPos, int nRec, REAL* model,                \
+                            const REAL* dreg, REAL* W, REAL* Wsec);                  \
 void orc_nl_adj##SFX(const orc_grid* g, const REAL* v2dt2, REAL* sig,               \
                      const int* srcPos, int nSrc, const int* recPos, int nRec,       \
                      const REAL* dreg);                                              \

@@ -225,9 +225,13 @@ static void FN(swap)(REAL** a, REAL** b) { REAL* t = *a; *a = *b; *b = t; }
 static REAL* FN(zalloc)(long n) { return (REAL*)calloc((size_t)n, sizeof(REAL)); }
 
 /* propShotsFwdGpu, F/src/nonlinearShotsGpuFunctions.cu:227-309.
- * sig: nSrc x ntw (already scaled and time-interpolated); dreg: nRec x nts. */
-void FN(orc_nl_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* sig,
-                    const int* srcPos, int nSrc, const int* recPos, int nRec, REAL* dreg)
+ * sig: nSrc x ntw (already scaled and time-interpolated); dreg: nRec x nts.
+ * With wfld (nts x nz x nx, or NULL): propShotsFwdGpuWavefield (:310-390),
+ * which also captures the coarse wavefield (interpWavefield) after the
+ * damping of every fine step. */
+void FN(orc_nl_fwd_wfld)(const orc_grid* g, const REAL* v2dt2, const REAL* sig,
+                         const int* srcPos, int nSrc, const int* recPos, int nRec, REAL* dreg,
+                         REAL* wfld)
 {
     REAL zc[6], xc[6], f[2 * 101], cd[512];
     FN(orc_tables)(g, zc, xc, f, cd);
@@ -242,6 +246,7 @@ void FN(orc_nl_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* sig,
             FN(orc_step_fwd)(g, zc, xc, p0, p1, p0, v2dt2);
             FN(inject_source)(ntw, sig, p0, itw - 1, srcPos, nSrc);
             FN(orc_damp)(g, cd, p0, p1);
+            if (wfld) FN(capture2)(g, f, h, p0, wfld + (long)its * nzx, wfld + (long)(its + 1) * nzx, it2);
             FN(record_interp)(nts, h, f, p0, dreg, its, it2, recPos, nRec);
             FN(swap)(&p0, &p1);
         }
@@ -249,11 +254,20 @@ void FN(orc_nl_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* sig,
     free(p0); free(p1);
 }
 
+void FN(orc_nl_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* sig,
+                    const int* srcPos, int nSrc, const int* recPos, int nRec, REAL* dreg)
+{
+    FN(orc_nl_fwd_wfld)(g, v2dt2, sig, srcPos, nSrc, recPos, nRec, dreg, NULL);
+}
+
 /* propShotsAdjGpu, F/src/nonlinearShotsGpuFunctions.cu:579-652.
- * sig (output): nSrc x ntw, overwritten. */
-void FN(orc_nl_adj)(const orc_grid* g, const REAL* v2dt2, REAL* sig,
-                    const int* srcPos, int nSrc, const int* recPos, int nRec,
-                    const REAL* dreg)
+ * sig (output): nSrc x ntw, overwritten.  With wfld (nts x nz x nx, zeroed
+ * here, or NULL): propShotsAdjGpuWavefield (:653-740), which also captures
+ * the adjoint field at the coarse rate (interpWavefield with the backward
+ * fine index it2) after the damping. */
+void FN(orc_nl_adj_wfld)(const orc_grid* g, const REAL* v2dt2, REAL* sig,
+                         const int* srcPos, int nSrc, const int* recPos, int nRec,
+                         const REAL* dreg, REAL* wfld)
 {
     REAL zc[6], xc[6], f[2 * 101], cd[512];
     FN(orc_tables)(g, zc, xc, f, cd);
@@ -263,17 +277,26 @@ void FN(orc_nl_adj)(const orc_grid* g, const REAL* v2dt2, REAL* sig,
     REAL* p1 = FN(zalloc)(nzx);
     REAL* tmp = FN(zalloc)(nzx);
     memset(sig, 0, sizeof(REAL) * (size_t)nSrc * ntw);
+    if (wfld) memset(wfld, 0, sizeof(REAL) * (size_t)nzx * nts);
     for (int its = nts - 2; its > -1; --its) {
         for (int it2 = sub - 1; it2 > -1; --it2) {
             int itw = its * sub + it2;
             FN(orc_step_adj)(g, zc, xc, p0, p1, p0, v2dt2, tmp);
             FN(interp_inject)(nts, h, f, dreg, p0, its, it2, recPos, nRec);
             FN(orc_damp)(g, cd, p0, p1);
+            if (wfld) FN(capture2)(g, f, h, p0, wfld + (long)its * nzx, wfld + (long)(its + 1) * nzx, it2);
             FN(record_source)(ntw, p0, sig, itw, srcPos, nSrc);
             FN(swap)(&p0, &p1);
         }
     }
     free(p0); free(p1); free(tmp);
+}
+
+void FN(orc_nl_adj)(const orc_grid* g, const REAL* v2dt2, REAL* sig,
+                    const int* srcPos, int nSrc, const int* recPos, int nRec,
+                    const REAL* dreg)
+{
+    FN(orc_nl_adj_wfld)(g, v2dt2, sig, srcPos, nSrc, recPos, nRec, dreg, NULL);
 }
 
 /* 10th-order Laplacian of c at interior point i, in the reference's order. */
@@ -380,7 +403,7 @@ typedef void (*FN(ss_fn))(const orc_grid*, const void*, int, REAL*);
 static void FN(scat_pass)(const orc_grid* g, const REAL* zc, const REAL* xc,
                           const REAL* f, const REAL* cd, const REAL* v2dt2,
                           const int* recPos, int nRec, REAL* dreg,
-                          FN(ss_fn) make_ss, const void* ctx, int clearData)
+                          FN(ss_fn) make_ss, const void* ctx, int clearData, REAL* wsec)
 {
     const int nts = g->nts, sub = g->sub, h = sub + 1;
     const long nzx = (long)g->nz * g->nx;
@@ -396,6 +419,9 @@ static void FN(scat_pass)(const orc_grid* g, const REAL* zc, const REAL* xc,
             FN(orc_step_fwd)(g, zc, xc, p0, p1, p0, v2dt2);
             FN(inject_secondary)(g, f, h, ssL, ssR, p0, it2 - 1);
             FN(orc_damp)(g, cd, p0, p1);
+            /* BornShotsFwdGpuWavefield (F/src/BornShotsGpuFunctions.cu:485-486):
+             * the scattered field at the coarse rate (interpWavefield) */
+            if (wsec) FN(capture2)(g, f, h, p0, wsec + (long)its * nzx, wsec + (long)(its + 1) * nzx, it2);
             FN(record_interp)(nts, h, f, p0, dreg, its, it2, recPos, nRec);
             FN(swap)(&p0, &p1);
         }
@@ -410,10 +436,20 @@ static void FN(scat_pass)(const orc_grid* g, const REAL* zc, const REAL* xc,
  * condition for coarse sample k. */
 typedef void (*FN(img_fn))(const orc_grid*, void*, int, REAL*);
 
+/* recordScaleWavefield (F/src/kernelsGpu.cu:145-153): the receiver
+ * wavefield sample, scaled like the image, added into a zeroed slice. */
+static void FN(record_scale)(const orc_grid* g, REAL* out, const REAL* ss, const REAL* refl,
+                             const REAL* v2dt2)
+{
+    INTERIOR_BEGIN(g)
+        out[i] += ss[i] * refl[i] * v2dt2[i];
+    INTERIOR_END
+}
+
 static void FN(rec_pass)(const orc_grid* g, const REAL* zc, const REAL* xc,
                          const REAL* f, const REAL* cd, const REAL* v2dt2,
                          const int* recPos, int nRec, const REAL* dreg,
-                         FN(img_fn) image, void* ctx)
+                         FN(img_fn) image, void* ctx, REAL* wsec, const REAL* refl)
 {
     const int nts = g->nts, sub = g->sub, h = sub + 1;
     const long nzx = (long)g->nz * g->nx;
@@ -431,9 +467,12 @@ static void FN(rec_pass)(const orc_grid* g, const REAL* zc, const REAL* xc,
             FN(swap)(&p0, &p1);
         }
         image(g, ctx, its + 1, ssR);
+        /* BornShotsAdjGpuWavefield (F/src/BornShotsGpuFunctions.cu:1055-1056) */
+        if (wsec) FN(record_scale)(g, wsec + (long)(its + 1) * nzx, ssR, refl, v2dt2);
         FN(swap)(&ssR, &ssL);
         memset(ssL, 0, sizeof(REAL) * (size_t)nzx);
     }
+    if (wsec) FN(record_scale)(g, wsec, ssR, refl, v2dt2);
     image(g, ctx, 0, ssR);
     free(p0); free(p1); free(ssL); free(ssR); free(tmp);
 }
@@ -473,9 +512,10 @@ static void FN(scale_refl)(const orc_grid* g, REAL* m, const REAL* refl, const R
 
 /* BornShotsFwdGpu, F/src/BornShotsGpuFunctions.cu:260-388.  model: nz*nx
  * (read only, copied before scaling); W: nts*nz*nx scratch. */
-void FN(orc_born_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
-                      const REAL* sig, const int* srcPos, int nSrc,
-                      const int* recPos, int nRec, const REAL* model, REAL* dreg, REAL* W)
+void FN(orc_born_fwd_wfld)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
+                           const REAL* sig, const int* srcPos, int nSrc,
+                           const int* recPos, int nRec, const REAL* model, REAL* dreg, REAL* W,
+                           REAL* Wsec)
 {
     REAL zc[6], xc[6], f[2 * 101], cd[512];
     FN(orc_tables)(g, zc, xc, f, cd);
@@ -485,15 +525,26 @@ void FN(orc_born_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
     memcpy(m, model, sizeof(REAL) * (size_t)nzx);
     FN(scale_refl)(g, m, refl, v2dt2);
     FN(born_ctx) ctx = {m, W, v2dt2, refl};
-    FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(born_ss), &ctx, 1);
+    if (Wsec) memset(Wsec, 0, sizeof(REAL) * (size_t)nzx * g->nts);
+    FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(born_ss), &ctx, 1, Wsec);
     free(m);
+}
+
+/* W (output) is the coarse source wavefield; with Wsec (or NULL) the
+ * scattered wavefield of BornShotsFwdGpuWavefield (:390-520). */
+void FN(orc_born_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
+                      const REAL* sig, const int* srcPos, int nSrc,
+                      const int* recPos, int nRec, const REAL* model, REAL* dreg, REAL* W)
+{
+    FN(orc_born_fwd_wfld)(g, v2dt2, refl, sig, srcPos, nSrc, recPos, nRec, model, dreg, W, NULL);
 }
 
 /* BornShotsAdjGpu, F/src/BornShotsGpuFunctions.cu:829-957.  model (output)
  * is overwritten with this shot's image. */
-void FN(orc_born_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
-                      const REAL* sig, const int* srcPos, int nSrc,
-                      const int* recPos, int nRec, REAL* model, const REAL* dreg, REAL* W)
+void FN(orc_born_adj_wfld)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
+                           const REAL* sig, const int* srcPos, int nSrc,
+                           const int* recPos, int nRec, REAL* model, const REAL* dreg, REAL* W,
+                           REAL* Wsec)
 {
     REAL zc[6], xc[6], f[2 * 101], cd[512];
     FN(orc_tables)(g, zc, xc, f, cd);
@@ -501,8 +552,18 @@ void FN(orc_born_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
     FN(src_pass)(g, zc, xc, f, cd, v2dt2, sig, srcPos, nSrc, W);
     memset(model, 0, sizeof(REAL) * (size_t)nzx);
     FN(born_ctx) ctx = {model, W, v2dt2, refl};
-    FN(rec_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(born_img), &ctx);
+    if (Wsec) memset(Wsec, 0, sizeof(REAL) * (size_t)nzx * g->nts);
+    FN(rec_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(born_img), &ctx, Wsec, refl);
     FN(scale_refl)(g, model, refl, v2dt2);
+}
+
+/* W (output) is the coarse source wavefield; with Wsec (or NULL) the scaled
+ * receiver wavefield of BornShotsAdjGpuWavefield (:959-1080). */
+void FN(orc_born_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
+                      const REAL* sig, const int* srcPos, int nSrc,
+                      const int* recPos, int nRec, REAL* model, const REAL* dreg, REAL* W)
+{
+    FN(orc_born_adj_wfld)(g, v2dt2, refl, sig, srcPos, nSrc, recPos, nRec, model, dreg, W, NULL);
 }
 
 /* imagingOffsetFwdGpu (F/src/kernelsGpu.cu:381-396) followed by
@@ -577,7 +638,7 @@ void FN(orc_bornext_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
     memcpy(m, model, sizeof(REAL) * (size_t)nm);
     FN(scale_refl_ext)(g, m, refl);
     FN(born_ctx) ctx = {m, W, v2dt2, refl};
-    FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(ext_ss), &ctx, 1);
+    FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(ext_ss), &ctx, 1, NULL);
     free(m);
 }
 
@@ -593,7 +654,7 @@ void FN(orc_bornext_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
     FN(src_pass)(g, zc, xc, f, cd, v2dt2, sig, srcPos, nSrc, W);
     memset(model, 0, sizeof(REAL) * (size_t)nzx * g->nExt);
     FN(born_ctx) ctx = {model, W, v2dt2, refl};
-    FN(rec_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(ext_img), &ctx);
+    FN(rec_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(ext_img), &ctx, NULL, NULL);
     FN(scale_refl_ext)(g, model, refl);
 }
 
@@ -729,7 +790,7 @@ void FN(orc_borntime_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl
     memcpy(m, model, sizeof(REAL) * (size_t)nm);
     FN(scale_refl_time)(g, m, refl, v2dt2, sloth);
     FN(born_ctx) ctx = {m, W, v2dt2, refl};
-    FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(time_ss), &ctx, 1);
+    FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(time_ss), &ctx, 1, NULL);
     free(m);
 }
 
@@ -746,7 +807,7 @@ void FN(orc_borntime_adj)(const orc_grid* g, const REAL* v2dt2, const REAL* refl
     FN(src_pass)(g, zc, xc, f, cd, v2dt2, sig, srcPos, nSrc, W);
     memset(model, 0, sizeof(REAL) * (size_t)nzx * g->nExt);
     FN(born_ctx) ctx = {model, W, v2dt2, refl};
-    FN(rec_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(time_img), &ctx);
+    FN(rec_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(time_img), &ctx, NULL, NULL);
     FN(scale_refl_time)(g, model, refl, v2dt2, sloth);
 }
 
@@ -910,11 +971,11 @@ void FN(orc_tomo_fwd)(const orc_grid* g, const REAL* v2dt2, const REAL* refl,
     FN(born_ctx) rP = {r, P, v2dt2, refl}, mP = {m, P, v2dt2, refl};
     if (leg1) {
         FN(cap_pass)(g, zc, xc, f, cd, v2dt2, extSS, &rW, P);
-        FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(born_ss), &mP, 0);
+        FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, FN(born_ss), &mP, 0, NULL);
     }
     if (leg2) {
         FN(cap_pass)(g, zc, xc, f, cd, v2dt2, FN(born_ss), &mW, P);
-        FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, extSS, &rP, 0);
+        FN(scat_pass)(g, zc, xc, f, cd, v2dt2, recPos, nRec, dreg, extSS, &rP, 0, NULL);
     }
     free(m); free(r);
 }

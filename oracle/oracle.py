@@ -560,6 +560,70 @@ def born_adjoint(g, vel, sources, wavelets, receivers, data, prec="f", ext=False
     return image
 
 
+def nl_forward_wavefield(g, vel, sources, receivers, model, shot, prec="f"):
+    """The coarse wavefield [nts][nx][nz] of shot `shot` of nl_forward, as the
+    reference's forwardWavefield saves it (propShotsFwdGpuWavefield,
+    F/src/nonlinearShotsGpuFunctions.cu:310-390)."""
+    dt = _dt(prec)
+    vel = np.ascontiguousarray(vel, dt)
+    v2, _ = vel_scales(g, vel, prec)
+    src, rec = sources[shot], _shot_geom(receivers, shot)
+    m = model[0] if model.shape[0] == 1 else model[shot]
+    reg = src.adjoint(m, prec=prec)
+    scale_source(g, vel, src.regPos, reg, prec=prec)
+    fine = interp_fwd(g, reg, prec=prec)
+    dreg = np.zeros((len(rec.regPos), g.nts), dt)
+    wf = np.zeros((g.nts, g.nx, g.nz), dt)
+    gc = g.c()
+    _fn("orc_nl_fwd_wfld", prec)(C.byref(gc), _p(v2), _p(fine), _p(src.regPos), len(src.regPos),
+                                 _p(rec.regPos), len(rec.regPos), _p(dreg), _p(wf))
+    return wf
+
+
+def nl_adjoint_wavefield(g, vel, sources, receivers, data, shot, prec="f"):
+    """The coarse adjoint wavefield [nts][nx][nz] of shot `shot` of
+    nl_adjoint (propShotsAdjGpuWavefield, F/src/nonlinearShotsGpuFunctions.cu:653-740)."""
+    dt = _dt(prec)
+    vel = np.ascontiguousarray(vel, dt)
+    v2, _ = vel_scales(g, vel, prec)
+    src, rec = sources[shot], _shot_geom(receivers, shot)
+    dreg = np.ascontiguousarray(rec.adjoint(data[shot], prec=prec), dt)
+    sig = np.zeros((len(src.regPos), g.ntw), dt)
+    wf = np.zeros((g.nts, g.nx, g.nz), dt)
+    gc = g.c()
+    _fn("orc_nl_adj_wfld", prec)(C.byref(gc), _p(v2), _p(sig), _p(src.regPos), len(src.regPos),
+                                 _p(rec.regPos), len(rec.regPos), _p(dreg), _p(wf))
+    return wf
+
+
+def born_wavefields(g, vel, sources, wavelets, receivers, vec, shot, adjoint, prec="f"):
+    """The source and secondary wavefields [nts][nx][nz] of shot `shot` that
+    the reference's forwardWavefield / adjointWavefield save
+    (BornShotsFwdGpuWavefield / BornShotsAdjGpuWavefield,
+    F/src/BornShotsGpuFunctions.cu:390-520, 959-1080): the coarse source
+    wavefield, and the scattered wavefield (forward; vec = model) or the
+    receiver wavefield scaled by 2/v^3 v^2 dtw^2 (adjoint; vec = data)."""
+    dt = _dt(prec)
+    vel = np.ascontiguousarray(vel, dt)
+    v2, rf = vel_scales(g, vel, prec)
+    src, rec = sources[shot], _shot_geom(receivers, shot)
+    fine = born_source_fine(g, vel, src, _shot_geom(wavelets, shot), prec)
+    W = np.zeros((g.nts, g.nx, g.nz), dt)
+    Wsec = np.zeros((g.nts, g.nx, g.nz), dt)
+    gc = g.c()
+    if adjoint:
+        dreg = np.ascontiguousarray(rec.adjoint(vec[shot], prec=prec), dt)
+        model = np.zeros((g.nx, g.nz), dt)
+        _fn("orc_born_adj_wfld", prec)(C.byref(gc), _p(v2), _p(rf), _p(fine), _p(src.regPos), len(src.regPos),
+                                       _p(rec.regPos), len(rec.regPos), _p(model), _p(dreg), _p(W), _p(Wsec))
+    else:
+        dreg = np.zeros((len(rec.regPos), g.nts), dt)
+        model = np.ascontiguousarray(vec, dt)
+        _fn("orc_born_fwd_wfld", prec)(C.byref(gc), _p(v2), _p(rf), _p(fine), _p(src.regPos), len(src.regPos),
+                                       _p(rec.regPos), len(rec.regPos), _p(model), _p(dreg), _p(W), _p(Wsec))
+    return W, Wsec
+
+
 def bornext_forward(g, vel, sources, wavelets, receivers, model, prec="f"):
     return born_forward(g, vel, sources, wavelets, receivers, model, prec, ext=True)
 
