@@ -58,6 +58,7 @@ _SIGS = {
     "aiso_nl_forward": (_I, [_P, _I, _P, _I, _P, _PP, _I, _PP, _I, _I]),
     "aiso_nl_adjoint": (_I, [_P, _I, _P, _I, _P, _PP, _I, _PP, _I, _I]),
     "aiso_nl_velocity_gradient": (_I, [_P, _P, _I, _P, _P, _PP, _I, _PP, _I, _I, _I]),
+    "aiso_nl_forward_noalloc": (_I, [_P, _I, _P, _I, _P, _PP, _I, _PP, _I]),
     "aiso_born_forward": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_born_adjoint": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_bornext_forward": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
@@ -301,6 +302,18 @@ class Context:
         r, _kr = _devarray(receivers)
         _check(lib().aiso_nl_forward(self.h, int(add), _ptr(model), model.shape[0], _ptr(data), s, len(sources),
                                      r, len(receivers), mem))
+
+    def nl_forward_noalloc(self, model, data, sources, receivers, add=False):
+        """aiso_nl_forward_noalloc: model and data are GPU tensors."""
+        nSig = model.shape[0]
+        if nSig not in (1, len(sources)):
+            raise ValueError(f"model: shot axis {nSig}, expected 1 or {len(sources)}")
+        _check_arg(model, nSig * sources[0].nDev * self._nts(), MEM_DEVICE, "model")
+        _check_arg(data, self._data_n(sources, receivers), MEM_DEVICE, "data", writable=True)
+        s, _ks = _devarray(sources)
+        r, _kr = _devarray(receivers)
+        _check(lib().aiso_nl_forward_noalloc(self.h, int(add), _ptr(model), nSig, _ptr(data), s, len(sources), r,
+                                             len(receivers)))
 
     def nl_adjoint(self, model, data, sources, receivers, add=False, mem=MEM_HOST):
         nSig = model.shape[0]

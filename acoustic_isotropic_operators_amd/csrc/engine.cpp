@@ -29,6 +29,8 @@ namespace aiso {
 
 long next_geom_id() { return g_geom_id++; }
 
+void set_last_error(const char* msg) { g_err = msg; }
+
 void hip_check(hipError_t e, const char* what)
 {
     if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
@@ -40,7 +42,34 @@ void hip_check(hipError_t e, const char* what)
 int tiles_z(const aiso_ctx* c) { return (c->p.nz - 2 * kFat + c->tz - 1) / c->tz; }
 int tiles_x(const aiso_ctx* c) { return (c->p.nx - 2 * kFat + c->tx - 1) / c->tx; }
 
-// Bucket the unique points of a geometry by step-kernel tile.
+// Bucket the unique points of a geometry by tile of a tz x tx tiling of
+// the nz x nx grid's interior: per-tile CSR offsets and (local index, trace)
+// entries; points in the FAT ring are listed apart with their grid index.
+TileLists tile_lists(const Geometry& g, int nz, int nx, int tz, int tx)
+{
+    TileLists L;
+    const int tZ = (nz - 2 * kFat + tz - 1) / tz, nt = tZ * ((nx - 2 * kFat + tx - 1) / tx);
+    std::vector<std::vector<int2>> buckets(nt);
+    for (int r = 0; r < (int)g.regPos.size(); ++r) {
+        int pos = g.regPos[r], iz = pos % nz, ix = pos / nz;
+        if (iz >= kFat && iz < nz - kFat && ix >= kFat && ix < nx - kFat) {
+            int t = (iz - kFat) / tz + tZ * ((ix - kFat) / tx);
+            int local = ((ix - kFat) % tx) * tz + (iz - kFat) % tz;
+            buckets[t].push_back(make_int2(local, r));
+        } else {
+            L.ring.push_back(make_int2(pos, r));
+        }
+    }
+    L.ptr.assign(nt + 1, 0);
+    for (int t = 0; t < nt; ++t) {
+        L.ptr[t + 1] = L.ptr[t] + (int)buckets[t].size();
+        L.ent.insert(L.ent.end(), buckets[t].begin(), buckets[t].end());
+    }
+    return L;
+}
+
+// The tile lists of a geometry for the context's current tile, resident on
+// the device (cached by geometry and tile).
 DevGeom* dev_geom(aiso_ctx* c, const aiso_device* d)
 {
     auto key = std::make_tuple(d->id, c->tz, c->tx);
@@ -48,33 +77,15 @@ DevGeom* dev_geom(aiso_ctx* c, const aiso_device* d)
     if (it != c->geoms.end()) return it->second.get();
     auto dg = std::make_unique<DevGeom>();
     const Geometry& g = d->g;
-    const int nz = c->p.nz, nx = c->p.nx, tZ = tiles_z(c), nt = tZ * tiles_x(c);
-    std::vector<std::vector<int2>> buckets(nt);
-    std::vector<int2> ring;
-    for (int r = 0; r < (int)g.regPos.size(); ++r) {
-        int pos = g.regPos[r], iz = pos % nz, ix = pos / nz;
-        if (iz >= kFat && iz < nz - kFat && ix >= kFat && ix < nx - kFat) {
-            int t = (iz - kFat) / c->tz + tZ * ((ix - kFat) / c->tx);
-            int local = ((ix - kFat) % c->tx) * c->tz + (iz - kFat) % c->tz;
-            buckets[t].push_back(make_int2(local, r));
-        } else {
-            ring.push_back(make_int2(pos, r));
-        }
-    }
-    std::vector<int> ptr(nt + 1, 0);
-    std::vector<int2> ent;
-    for (int t = 0; t < nt; ++t) {
-        ptr[t + 1] = ptr[t] + (int)buckets[t].size();
-        ent.insert(ent.end(), buckets[t].begin(), buckets[t].end());
-    }
-    upload(dg->tilePtr, ptr, c->stream);
-    upload(dg->ent, ent, c->stream);
-    upload(dg->ring, ring, c->stream);
+    TileLists L = tile_lists(g, c->p.nz, c->p.nx, c->tz, c->tx);
+    upload(dg->tilePtr, L.ptr, c->stream);
+    upload(dg->ent, L.ent, c->stream);
+    upload(dg->ring, L.ring, c->stream);
     upload(dg->cornerReg, g.cornerReg, c->stream);
     upload(dg->weight, g.weight, c->stream);
     upload(dg->adjPtr, g.adjPtr, c->stream);
     upload(dg->adjEnt, g.adjEnt, c->stream);
-    dg->nRing = (int)ring.size();
+    dg->nRing = (int)L.ring.size();
     dg->nReg = (int)g.regPos.size();
     HIPC(hipStreamSynchronize(c->stream));
     DevGeom* out = dg.get();

@@ -215,6 +215,12 @@ struct Request {
 
 int tiles_z(const aiso_ctx* c);
 int tiles_x(const aiso_ctx* c);
+struct TileLists {
+    std::vector<int> ptr;        // tiles + 1 CSR offsets
+    std::vector<int2> ent;       // (local index in tile, trace)
+    std::vector<int2> ring;      // (grid index, trace) of points in the FAT ring
+};
+TileLists tile_lists(const Geometry& g, int nz, int nx, int tz, int tx);
 DevGeom* dev_geom(aiso_ctx* c, const aiso_device* d);
 // Tiling of the two-step kernel: output tiles of OZ x OX points.
 constexpr int kOZ2 = kPair2EZ - 2 * kFat, kOX2 = kPair2EX - 2 * kFat;
@@ -248,5 +254,23 @@ std::vector<float> nl_source(const aiso_ctx* c, const Geometry& src, const float
 
 // One operator call (passes.cpp).
 void run_call(aiso_ctx* c, const Request& q);
+
+// The C-ABI's error convention: run f, turn an exception into a status code
+// and the message aiso_last_error() returns.
+void set_last_error(const char* msg);
+template <class F>
+aiso_status guard_call(F&& f)
+{
+    try {
+        f();
+        return AISO_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return AISO_ERR_PARAM;
+    }
+}
 
 }  // namespace aiso

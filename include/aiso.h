@@ -216,6 +216,19 @@ aiso_status aiso_nl_adjoint(aiso_ctx* ctx, int add, float* model, int nShotSig,
                             const float* data, aiso_device* const* sources, int nShot,
                             aiso_device* const* receivers, int nRecGeom, int mem);
 
+/* aiso_nl_forward without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+ * buffer beyond the inputs and outputs lives in one 64 MiB device array per
+ * GPU, held until the process ends; a call whose per-tile lists and one
+ * shot's wavefield pair, regular traces and source signal exceed it fails
+ * with AISO_ERR_MEMORY (shots beyond what fits run in further batches).
+ * model and data are device memory (AISO_MEM_DEVICE).  No free surface.
+ * Calls that overlap on one GPU share the array and race: serialize them. */
+aiso_status aiso_nl_forward_noalloc(aiso_ctx* ctx, int add, const float* model, int nShotSig, float* data,
+                                    aiso_device* const* sources, int nShot, aiso_device* const* receivers,
+                                    int nRecGeom);
+
 /* Velocity gradient of a data misfit through the nonlinear operator, by the
  * adjoint-state method with checkpointed forward recomputation.  No
  * counterpart in the reference (its FWI takes the Born adjoint as the
