@@ -69,6 +69,16 @@ _SIGS = {
     "aiso_tomo_adjoint": (_I, [_P, _I, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_wemva_forward": (_I, [_P, _I, _P, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
     "aiso_wemva_adjoint": (_I, [_P, _I, _P, _P, _P, _PP, _I, _P, _I, _PP, _I, _I]),
+    "aiso_batch_create": (_I, [C.POINTER(FdParams), _I, _PP]),
+    "aiso_batch_destroy": (_I, [_P]),
+    "aiso_batch_set_windows": (_I, [_P, _I, _P, _P, _PP, _PP, _PP, _I]),
+    "aiso_batch_set_config": (_I, [_P, _I, _I, _I, _I, _I]),
+    "aiso_batch_get_config": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    "aiso_batch_tune": (_I, [_P, _I, C.POINTER(C.c_double)]),
+    "aiso_batch_nl_forward": (_I, [_P, _I, _PP, _I, _PP, _I]),
+    "aiso_batch_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(C.c_longlong), C.POINTER(_F),
+                             C.POINTER(C.c_double)]),
+    "aiso_batch_stream": (_I, [_P, _PP]),
     "aiso_pad_plus": (_I, [_I, _I, _I]),
     "aiso_pad_plus_free_surface": (_I, [_I, _I, _I]),
 }
@@ -419,3 +429,135 @@ class Context:
 
     def wemva_adjoint(self, model, image, data, sources, wavelets, receivers, add=False, mem=MEM_HOST):
         self._wemva(lib().aiso_wemva_adjoint, model, image, data, sources, wavelets, receivers, add, mem, True)
+
+
+def _fd_params(params):
+    fp = FdParams()
+    defaults = dict(nz=0, nx=0, oz=0.0, ox=0.0, ots=0.0, fat=5, blockSize=16, alphaCos=0.99, fMax=1000.0,
+                    freeSurface=0, nExt=1, extension=EXT_NONE)
+    merged = {**defaults, **params}
+    for k, _ in FdParams._fields_:
+        setattr(fp, k, merged[k])
+    return fp, merged
+
+
+def _ptrarray(xs):
+    arr = (C.c_void_p * len(xs))(*[_ptr(x).value for x in xs])
+    return C.cast(arr, _PP), arr
+
+
+class Batch:
+    """aiso_batch: many shots, each on a model window of its own size,
+    advanced together (one launch per fine step over every window of a
+    launch group, the time loop of each group captured in a HIP graph).
+
+    ``params`` holds the keys every window shares (spacing, time sampling,
+    pads, sponge, fMax); ``set_windows`` gives each window's padded velocity
+    [nx][nz] and its source and receiver devices built on that window's
+    grid."""
+
+    def __init__(self, params: dict, device=0):
+        fp, self.params = _fd_params(params)
+        self.h = C.c_void_p()
+        self.windows = []
+        _check(lib().aiso_batch_create(C.byref(fp), device, C.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().aiso_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        if _LIB is not None:
+            self.close()
+
+    def set_windows(self, vels, sources, receivers, mem=MEM_HOST):
+        n = len(vels)
+        if len(sources) != n or len(receivers) != n:
+            raise ValueError("one source and one receiver device per window")
+        if mem == MEM_HOST:
+            vels = [np.ascontiguousarray(v, np.float32) for v in vels]
+        nz = np.array([v.shape[1] for v in vels], np.int32)
+        nx = np.array([v.shape[0] for v in vels], np.int32)
+        for k, v in enumerate(vels):
+            _check_arg(v, int(nz[k]) * int(nx[k]), mem, f"velocity of window {k}")
+        vp, _kv = _ptrarray(vels)
+        s, _ks = _devarray(sources)
+        r, _kr = _devarray(receivers)
+        _check(lib().aiso_batch_set_windows(self.h, n, _ptr(nz), _ptr(nx), vp, s, r, mem))
+        self.windows = [(int(nz[k]), int(nx[k]), sources[k], receivers[k]) for k in range(n)]
+        self._keep = (sources, receivers)
+
+    def set_config(self, tz=32, tx=16, windows_per_launch=0, blocks_per_cu=0, graph=True):
+        _check(lib().aiso_batch_set_config(self.h, tz, tx, windows_per_launch, blocks_per_cu, int(graph)))
+
+    def config(self):
+        v = [_I() for _ in range(5)]
+        _check(lib().aiso_batch_get_config(self.h, *[C.byref(x) for x in v]))
+        return dict(tz=v[0].value, tx=v[1].value, windows_per_launch=v[2].value, blocks_per_cu=v[3].value,
+                    graph=bool(v[4].value))
+
+    def tune(self, steps=16, cache=None):
+        """Pick tile, windows per launch and workgroups per CU by timing
+        (aiso_batch_tune).  With ``cache`` (a JSON file path) the choice is
+        looked up by the windows' signature first and stored after tuning."""
+        key = self.signature()
+        if cache is not None:
+            import json
+            import os
+            table = {}
+            if os.path.exists(cache):
+                with open(cache) as f:
+                    table = json.load(f)
+            if key in table:
+                c = table[key]
+                self.set_config(c["tz"], c["tx"], c["windows_per_launch"], c["blocks_per_cu"],
+                                self.config()["graph"])
+                return dict(c, cached=True)
+        t = C.c_double()
+        _check(lib().aiso_batch_tune(self.h, int(steps), C.byref(t)))
+        c = self.config()
+        out = dict(tz=c["tz"], tx=c["tx"], windows_per_launch=c["windows_per_launch"],
+                   blocks_per_cu=c["blocks_per_cu"], ns_per_point_update=t.value * 1e6)
+        if cache is not None:
+            table[key] = out
+            tmp = cache + ".tmp"
+            with open(tmp, "w") as f:
+                json.dump(table, f, indent=1, sort_keys=True)
+            os.replace(tmp, cache)
+        return dict(out, cached=False)
+
+    def signature(self):
+        """Operator and window sizes: the autotuner's cache key."""
+        import hashlib
+        dims = ",".join(f"{nz}x{nx}" for nz, nx, _, _ in self.windows)
+        return "nl_fwd:" + hashlib.sha1(dims.encode()).hexdigest()[:16] + f":{len(self.windows)}"
+
+    def info(self):
+        n, g, pts, ms, cap = _I(), _I(), C.c_longlong(), _F(), C.c_double()
+        _check(lib().aiso_batch_info(self.h, C.byref(n), C.byref(g), C.byref(pts), C.byref(ms), C.byref(cap)))
+        return dict(windows=n.value, launch_groups=g.value, point_updates=pts.value, last_ms=ms.value,
+                    capture_ms=cap.value)
+
+    def stream(self):
+        s = C.c_void_p()
+        _check(lib().aiso_batch_stream(self.h, C.byref(s)))
+        return s.value
+
+    def nl_forward(self, models, datas, add=False, mem=MEM_HOST):
+        """models: one source signal [nSrcDev][nts] for all windows, or one per
+        window; datas[w]: [nRecDev_w][nts] outputs."""
+        n = len(self.windows)
+        if len(datas) != n:
+            raise ValueError(f"data: {len(datas)} arrays, expected {n}")
+        if len(models) not in (1, n):
+            raise ValueError(f"models: {len(models)} arrays, expected 1 or {n}")
+        nts = int(self.params["nts"])
+        nsrc = self.windows[0][2].nDev
+        for k, m in enumerate(models):
+            _check_arg(m, nsrc * nts, mem, f"model {k}")
+        for k, d in enumerate(datas):
+            _check_arg(d, self.windows[k][3].nDev * nts, mem, f"data {k}", writable=True)
+        mp, _km = _ptrarray(models)
+        dp, _kd = _ptrarray(datas)
+        _check(lib().aiso_batch_nl_forward(self.h, int(add), mp, len(models), dp, mem))

@@ -43,7 +43,7 @@ def build_lib():
     objs = []
     bdir = os.path.join(PKG, "_build")
     os.makedirs(bdir, exist_ok=True)
-    for src in ("kernels.hip", "noalloc.hip", "engine.cpp", "passes.cpp", "host.cpp"):
+    for src in ("kernels.hip", "noalloc.hip", "engine.cpp", "passes.cpp", "windows.cpp", "host.cpp"):
         obj = os.path.join(bdir, src.replace(".", "_") + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
         _run([HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])

@@ -218,14 +218,23 @@ __device__ __forceinline__ void step_block(const StepArgs& a, int group, int til
         cgf* pc = G(S.p[a.cur]);
         gf* po = G(S.p[a.cur ^ 1]);
 
-        // Per-point operands first, so their latency overlaps the staging.
+        // Per-point operands first, so their latency overlaps the staging;
+        // with the imaging window this includes the W sample and the image.
         float os[PX], pv[ADJ ? PX : 1];
         float sA[HAS_SS ? PX : 1], sB[HAS_SS ? PX : 1], mm[SSB ? PX : 1];
+        float wImg[WIN ? PX : 1], iImg[WIN ? PX : 1];
+        const bool winImg = WIN && a.win && a.winOut == WIN_IMG;
 #pragma unroll
         for (int j = 0; j < PX; ++j) {
             const unsigned o = off0 + j * nz;
             os[j] = ok[j] ? po[o] : 0.f;
             if constexpr (ADJ) pv[j] = ok[j] ? pc[o] : 0.f;
+            if constexpr (WIN) {
+                if (winImg && ok[j]) {
+                    wImg[j] = ldg_s(G(S.W + a.wImg) + o, a.ntW);
+                    iImg[j] = G(S.img)[o];
+                }
+            }
             if (ok[j]) {
                 if constexpr (SSB) {
                     mm[j] = G(a.mtil)[o];
@@ -341,7 +350,7 @@ __device__ __forceinline__ void step_block(const StepArgs& a, int group, int til
                     switch (a.winOut) {
                     case WIN_W: stg_s(G(S.W + a.wA) + gi, v, a.ntW); break;
                     case WIN_SS: stg_s(G(S.W + a.wA) + gi, G(a.mtil)[gi] * v, a.ntW); break;
-                    case WIN_IMG: G(S.img)[gi] += ldg_s(G(S.W + a.wImg) + gi, a.ntW) * v; break;
+                    case WIN_IMG: G(S.img)[gi] = iImg[j] + wImg[j] * v; break;
                     default: G(S.ss[a.ssA] + a.ssOffA)[gi] = v; break;
                     }
                 }
@@ -410,6 +419,46 @@ __global__ __launch_bounds__(NT) void k_step(const StepArgs a)
     if (t.full && !t.sponge) step_block<TZ, TX, MODE, SPB, true, false>(a, t.group, t.tile, t.z0, t.x0, sC, sV, sN);
     else if (t.full) step_block<TZ, TX, MODE, SPB, true, true>(a, t.group, t.tile, t.z0, t.x0, sC, sV, sN);
     else step_block<TZ, TX, MODE, SPB, false, true>(a, t.group, t.tile, t.z0, t.x0, sC, sV, sN);
+}
+
+// One fine step of a launch of windows of uneven size (k_step's work on each
+// window's own grid).  The tiles of all windows form one list; XCD k
+// (workgroup id mod 8) owns the k-th eighth of it, so neighbouring tiles of a
+// window share that XCD's L2, and its workgroups stride through that range.
+// With fewer workgroups than tiles the launch is persistent: each workgroup
+// loops over several tiles.  Every point receives the operations k_step gives
+// it, in the same order.
+template <int TZ, int TX, unsigned MODE>
+__global__ __launch_bounds__(NT) void k_step_win(const WinStepArgs w)
+{
+    using T = TileShape<TZ, TX>;
+    constexpr bool ADJ = MODE & M_ADJ;
+    constexpr bool SPARSE = MODE & (M_INJ_SIG | M_INJ_DATA | M_REC_DATA | M_REC_SIG);
+    __shared__ float sC[T::LX * T::LZ];
+    __shared__ float sV[ADJ ? T::LX * T::LZ : 1];
+    __shared__ float sN[SPARSE ? TX * TZ : 1];
+    const int xcd = blockIdx.x & 7, per = gridDim.x >> 3;
+    const int chunk = (w.ntotal + 7) >> 3;
+    const int end = imin((xcd + 1) * chunk, w.ntotal);
+    for (int l = xcd * chunk + (int)(blockIdx.x >> 3); l < end; l += per) {   // block-uniform
+        const int2 tm = w.tileMap[l];
+        const WinDesc d = w.win[tm.x];
+        StepArgs a = w.a;
+        a.slots = w.a.slots + tm.x;
+        a.v2dt2 = d.v2dt2;
+        a.nz = d.nz;
+        a.nx = d.nx;
+        a.tilesZ = d.tilesZ;
+        a.ntiles = d.ntiles;
+        a.nshots = 1;
+        const int z0 = FAT + (tm.y % d.tilesZ) * TZ, x0 = FAT + (tm.y / d.tilesZ) * TX;
+        const bool full = z0 + TZ <= d.nz - FAT && x0 + TX <= d.nx - FAT;
+        const int edge = imin(imin(z0 - FAT, d.nz - FAT - (z0 + TZ)), imin(x0 - FAT, d.nx - FAT - (x0 + TX)));
+        if (full && edge >= a.minPad) step_block<TZ, TX, MODE, 1, true, false>(a, 0, tm.y, z0, x0, sC, sV, sN);
+        else if (full) step_block<TZ, TX, MODE, 1, true, true>(a, 0, tm.y, z0, x0, sC, sV, sN);
+        else step_block<TZ, TX, MODE, 1, false, true>(a, 0, tm.y, z0, x0, sC, sV, sN);
+        __syncthreads();                                  // sC, sV, sN are reused by the next tile
+    }
 }
 
 // Lockstep Born forward, sub == 2.  One launch advances the source field
@@ -2118,6 +2167,28 @@ hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a
     StepArgs b = a;
     b.gridLogical = a.ntiles * ((a.nshots + spb - 1) / spb);
     return launch_k(k, (b.gridLogical + 7) / 8 * 8, b, s, e0, e1);
+}
+
+using WinKernel = void (*)(const WinStepArgs);
+
+template <unsigned MODE>
+static WinKernel pick_win_tile(int tz, int tx)
+{
+    if (tz == 64 && tx == 16) return k_step_win<64, 16, MODE>;
+    if (tz == 64 && tx == 32) return k_step_win<64, 32, MODE>;
+    if (tz == 32 && tx == 16) return k_step_win<32, 16, MODE>;
+    if (tz == 32 && tx == 32) return k_step_win<32, 32, MODE>;
+    return nullptr;
+}
+
+hipError_t launch_step_windows(unsigned mode, int tz, int tx, int grid, const WinStepArgs& w, hipStream_t s,
+                               hipEvent_t e0, hipEvent_t e1)
+{
+    WinKernel k = mode == MODE_NL_FWD ? pick_win_tile<MODE_NL_FWD>(tz, tx) : nullptr;
+    if (!k || grid < 8 || grid % 8 != 0 || w.ntotal < 1) return hipErrorInvalidValue;
+    if (e0 && e1) hipExtLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, e0, e1, 0, w);
+    else hipLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, w);
+    return hipGetLastError();
 }
 
 template <unsigned MODE>

@@ -149,6 +149,21 @@ struct Rec2Args {
     int ntW;
 };
 
+// Shots on model windows of their own (uneven apertures): window w has its
+// own padded grid nz x nx and v^2 dtw^2; every other step parameter is
+// common.  One launch advances every tile of every window of the launch.
+struct WinDesc {
+    const float* v2dt2;     // [nx][nz] of the window
+    int nz, nx, tilesZ, ntiles;
+};
+struct WinStepArgs {
+    StepArgs a;             // common fields; a.slots: one ShotSlot per window of the launch
+                            // (a.nz, a.nx, a.v2dt2, a.tilesZ, a.ntiles are taken from WinDesc)
+    const WinDesc* win;     // per window of the launch
+    const int2* tileMap;    // (window, tile in window) of every tile of the launch, window-major
+    int ntotal;             // tiles of the launch
+};
+
 // Velocity-gradient imaging of the nonlinear operator: img += L(p_{t-1}) * nu_t,
 // with p_{t-1} a forward slice stored in W and nu_t the adjoint slice p[nu].
 struct GradArgs {
@@ -195,6 +210,11 @@ hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a
 hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a, hipStream_t s,
                                hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 hipError_t launch_ring(unsigned mode, const StepArgs& a, int maxRing, hipStream_t s);
+// One fine step of every window of a launch (MODE_NL_FWD).  grid: workgroups,
+// a multiple of 8; workgroups beyond the tile count loop over further tiles
+// (a persistent launch when grid < w.ntotal).
+hipError_t launch_step_windows(unsigned mode, int tz, int tx, int grid, const WinStepArgs& w, hipStream_t s,
+                               hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 hipError_t launch_fill(float* p, long n, hipStream_t s);
 hipError_t launch_scale_model(const float* m, float* out, const float* v2dt2, const float* refl,
                               int nz, int nx, hipStream_t s);

@@ -212,3 +212,78 @@ def nonlinear_small(seed=7, nts=1000, dts=0.004, sub=4):
     rec = (1, izs, 1, nx, FAT + pad, 1)
     wav = ricker(nts, dts, 15.0, 0.1)[None, :]
     return Workload("nonlinear_1x200x400", params, vp, wav, src, rec, np.zeros_like(vp))
+
+
+@dataclass
+class WindowedWorkload:
+    """Shots on model windows of uneven size (the batched form): window k is
+    columns [x0[k], x0[k] + nx[k]) of a wide model, padded on its own."""
+    name: str
+    params: dict          # keys every window shares (nz, nx: those of window 0)
+    vels: list            # padded [NX_k][NZ] per window
+    src: list             # per window (iz, ix) on its padded grid
+    rec: list             # per window (nzDev, ozDev, dzDev, nxDev, oxDev, dxDev)
+    x0: list              # first model column of each window
+    nx: list              # physical width of each window
+    wavelet: np.ndarray   # [1][nts]
+
+    @property
+    def nShot(self):
+        return len(self.vels)
+
+    @property
+    def fine_steps(self):
+        return (self.params["nts"] - 1) * self.params["sub"]
+
+    def interior(self, k):
+        NX, NZ = self.vels[k].shape
+        return (NZ - 2 * FAT) * (NX - 2 * FAT)
+
+    def window_params(self, k):
+        """The single-window parameters of window k (a plain context)."""
+        NX, NZ = self.vels[k].shape
+        return dict(self.params, nz=NZ, nx=NX, xPadPlus=NX - 2 * FAT - self.params["xPadMinus"] - self.nx[k])
+
+
+def batched_windows(nShot=128, seed=303, nts=3000, dts=0.002, sub=2, nx_range=(300, 1500), nz=400):
+    """Shots on windows of uneven aperture: nz = 400 at 10 m, window widths
+    drawn uniformly from nx_range, each window centred on its source, sources
+    every 40 columns along a wide model.  Velocity: 1500 m/s at the top,
+    growing with depth to a bottom value that varies smoothly and randomly
+    along x (2800-3800 m/s), plus 24 seeded Gaussian lenses (+/-15 %,
+    radius 100-300 m), clipped to 1400-4400 m/s.  50-point sponge (alpha
+    0.95), source and receivers (every column of the window) at 20 m depth,
+    15 Hz Ricker, nts at dts with sub 2 (Courant <= 0.44)."""
+    rng = np.random.default_rng(seed)
+    d, pad, step, margin = 10.0, 50, 40, 800
+    widths = rng.integers(nx_range[0], nx_range[1] + 1, nShot)
+    xs = margin + step * np.arange(nShot)
+    nxFull = int(xs[-1] + margin)
+    bottom = _smooth(rng.standard_normal(nxFull), 200.0)
+    bottom = 3300.0 + 500.0 * bottom / np.abs(bottom).max()
+    z = np.arange(nz) / (nz - 1)
+    v = 1500.0 + (bottom[:, None] - 1500.0) * z[None, :]
+    zz, xx = np.meshgrid(np.arange(nz, dtype=np.float64), np.arange(nxFull, dtype=np.float64))
+    for _ in range(24):
+        cz, cx = rng.uniform(0.1, 0.9) * nz, rng.uniform(0, nxFull)
+        r = rng.uniform(10.0, 30.0)
+        amp = rng.uniform(-0.15, 0.15)
+        v *= 1.0 + amp * np.exp(-((zz - cz) ** 2 + (xx - cx) ** 2) / (2 * r * r))
+    v = np.clip(v, 1400.0, 4400.0).astype(np.float32)
+    vels, src, rec, x0s = [], [], [], []
+    zPlus = pad_plus(nz, pad)
+    izs = 3 - 1 + pad + FAT
+    for k in range(nShot):
+        w = int(widths[k])
+        x0 = int(xs[k] - w // 2)
+        vp, _, _ = pad_model(v[x0:x0 + w], pad, pad)
+        vels.append(vp)
+        x0s.append(x0)
+        src.append((izs, FAT + pad + (int(xs[k]) - x0)))
+        rec.append((1, izs, 1, w, FAT + pad, 1))
+    NX0, NZ = vels[0].shape
+    params = dict(nz=NZ, nx=NX0, dz=d, dx=d, nts=nts, dts=dts, sub=sub, zPadMinus=pad, zPadPlus=zPlus,
+                  xPadMinus=pad, xPadPlus=NX0 - 2 * FAT - pad - int(widths[0]), alphaCos=0.95, fMax=37.5)
+    wav = ricker(nts, dts, 15.0, 0.1)[None, :]
+    return WindowedWorkload(f"windows_{nShot}x{nz}x{nx_range[0]}-{nx_range[1]}", params, vels, src, rec, x0s,
+                            [int(w) for w in widths], wav)

@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--opt", action="append", default=[], help="engine option key=value (aiso_ctx_set_option)")
     ap.add_argument("--workload", default="born",
                     choices=["born", "bornfs", "bornext21", "borntime21", "tomo21", "wemva21", "tomotime21",
-                             "wemvatime21", "fwi32", "serve"],
+                             "wemvatime21", "fwi32", "serve", "windows128"],
                     help="born: the headline Born fwd+adj; bornfs: the same with a free surface; bornext21: subsurface-offset extended Born "
                          "fwd+adj with 21 offsets on the same grid and shots; borntime21: the same with 21 "
                          "time lags; tomo21: the offset-extended tomographic operator fwd+adj about a "
@@ -62,9 +62,12 @@ def parse():
                          "(21 offsets); tomotime21, wemvatime21: the same with 21 time lags; fwi32: L2 misfit and "
                          "its adjoint-state velocity gradient through torch.autograd, 32 shots on 400x1000 "
                          "(use --shots 32); serve: a Poisson stream of one-shot requests (nonlinear, Born "
-                         "forward/adjoint, 21-offset extended Born adjoint) on 4 resident models")
+                         "forward/adjoint, 21-offset extended Born adjoint) on 4 resident models; windows128: "
+                         "the batched form, 128 nonlinear-forward shots on windows of width 300-1500 at nz 400 "
+                         "(nts 3000), tuned and graph-captured, beside the same shots one per launch")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests in the stream")
     ap.add_argument("--rate", type=float, default=50.0, help="serve: mean arrivals per second")
+    ap.add_argument("--slots", type=int, default=4, help="serve: engine calls running at once")
     ap.add_argument("--profile-every", type=int, default=16,
                     help="profiling in the timed region: the pair kernel timed over windows of N consecutive "
                          "launches between two events, other step kernels on every N-th launch (1: events on "
@@ -189,6 +192,8 @@ def main():
         return bench_fwi(a, torch, A)
     if a.workload == "serve":
         return bench_serve(a)
+    if a.workload == "windows128":
+        return bench_windows(a, torch, A)
     if a.workload in ("tomo21", "wemva21", "tomotime21", "wemvatime21"):
         return bench_tomo(a, torch, A, born_lsrtm, wemva=a.workload.startswith("wemva"),
                           time_lags="time" in a.workload)
@@ -445,62 +450,179 @@ def bench_fwi(a, torch, A):
     print(json.dumps(out), flush=True)
 
 
+def bench_windows(a, torch, A):
+    """Batched form over windows of uneven aperture: 128 nonlinear-forward
+    shots, each on its own window (width 300-1500, nz 400, nts 3000 at 2 ms,
+    sub 2), advanced together by k_step_win, one launch per fine step over
+    every window of a launch group, the time loop captured in a HIP graph;
+    tile, windows per launch and workgroups per CU picked by the autotuner.
+    One step = one call over all 128 shots with wavelet and data resident in
+    HBM.  Beside it, the plain form: the same 128 shots through
+    aiso_nl_forward one shot per call and per launch, on per-window contexts
+    made before the clock starts.  Work unit: interior point-updates of one
+    fine step; the step kernel's algorithmic bytes are 16 per point-update."""
+    from acoustic_isotropic_operators_amd.workloads import batched_windows
+    w = batched_windows()
+    p = w.params
+    dev = torch.device("cuda:0")
+    model = torch.from_numpy(w.wavelet.astype(np.float32)).to(dev)
+    b = A.Batch(p)
+    devs = []
+    for k in range(w.nShot):
+        NX, NZ = w.vels[k].shape
+        iz, ix = w.src[k]
+        devs.append((A.Device.grid(1, iz, 1, 1, ix, 1, NZ, NX, p["nts"]), A.Device.grid(*w.rec[k], NZ, NX, p["nts"])))
+    b.set_windows(w.vels, [s for s, _ in devs], [r for _, r in devs])
+    t0 = time.perf_counter()
+    tuned = b.tune(steps=12)
+    tune_s = time.perf_counter() - t0
+    if a.tile:
+        c = b.config()
+        tz, tx = (int(v) for v in a.tile.split(","))
+        b.set_config(tz, tx, c["windows_per_launch"], c["blocks_per_cu"], True)
+    out_d = [torch.zeros((r.nDev, p["nts"]), dtype=torch.float32, device=dev) for _, r in devs]
+    torch.cuda.synchronize()
+    for _ in range(max(1, a.warmup)):     # the first call captures the graphs
+        b.nl_forward([model], out_d, mem=A.MEM_DEVICE)
+    capture_ms = b.info()["capture_ms"]
+    torch.cuda.synchronize()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        b.nl_forward([model], out_d, mem=A.MEM_DEVICE)
+        dev_ms.append(b.info()["last_ms"])
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / a.steps
+    info = b.info()
+    pts = float(info["point_updates"])
+    steps_per_call = w.fine_steps * info["launch_groups"]
+    avg_launch_ms = float(np.mean(dev_ms)) / steps_per_call
+    achieved = 16.0 * pts / (float(np.mean(dev_ms)) / 1e3) / 1e9
+    # the plain form: one shot per call and per launch
+    ctxs = []
+    for k in range(w.nShot):
+        c = A.Context(w.window_params(k))
+        c.set_batch(1)
+        c.set_velocity(w.vels[k])
+        ctxs.append(c)
+    plain_d = [torch.zeros((1, r.nDev, p["nts"]), dtype=torch.float32, device=dev) for _, r in devs]
+    m3 = model[None]
+    ctxs[0].nl_forward(m3, plain_d[0], [devs[0][0]], [devs[0][1]], mem=A.MEM_DEVICE)   # warm
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for k in range(w.nShot):
+        ctxs[k].nl_forward(m3, plain_d[k], [devs[k][0]], [devs[k][1]], mem=A.MEM_DEVICE)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t1
+    same = all(torch.equal(out_d[k], plain_d[k][0]) for k in range(w.nShot))
+    out = {"metric": "shots/s, batched nonlinear forward, 128 shots on windows nx 300-1500 at nz 400, nts 3000",
+           "value": w.nShot / t, "unit": "shots/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": t * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": w.name, "shots": w.nShot, "nts": p["nts"], "sub": p["sub"],
+                      "window_nx": [min(w.nx), max(w.nx)], "tuned": tuned, "tune_s": tune_s,
+                      "graph_capture_ms": capture_ms, "launch_groups": info["launch_groups"],
+                      "inputs": "wavelet and data resident in HBM"},
+           "gpts_per_s": pts / t / 1e9,
+           "roofline": {"bound": "hbm", "kernel": "k_step_win (MODE_NL_FWD)", "achieved": achieved,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "avg_launch_ms": avg_launch_ms, "launches_per_call": steps_per_call,
+                        "bytes_per_point": 16.0, "timing": "HIP events around each call on the batch stream"},
+           "plain_one_shot_per_launch": {"shots_per_s": w.nShot / t_plain, "s": t_plain,
+                                         "gpts_per_s": pts / t_plain / 1e9},
+           "speedup_vs_plain": t_plain / t,
+           "bitwise_equal_to_plain": bool(same),
+           "finite": all(bool(torch.isfinite(d).all().item()) for d in out_d)}
+    print(json.dumps(out), flush=True)
+
+
 def bench_serve(a):
     """Serving: a seeded Poisson stream (mean --rate per second) of --requests
     one-shot requests, mixed nonlinear modeling (30 %), Born forward (30 %),
     Born adjoint (25 %) and 21-offset extended-Born adjoint (15 %), on 4
     resident models from 200 x 400 (nts 4000) to 1000 x 3000 (nts 1000), the
     smaller grids drawn more often.  Host buffers in and out (each request
-    brings its data and takes its result).  Reports requests (shots) per
-    second over the stream and the latency from arrival to completion."""
+    brings its data and takes its result).  The stream runs twice: through
+    the continuously batching server (--slots concurrent calls, up to 32
+    shots per call, admission by a 200 GB HBM budget), and through plain
+    one-request-at-a-time execution (one call per request, each after the
+    previous).  Reports requests (shots) per second over each stream and
+    the latency from arrival to completion."""
     from acoustic_isotropic_operators_amd.serve import Request, Server, run_stream
     from acoustic_isotropic_operators_amd.workloads import serving_models
-    rng = np.random.default_rng(2026)
     models = serving_models()
-    srv = Server(max_models=4)
-    info = {}
+    kinds = ["nl_forward", "born_forward", "born_adjoint", "bornext_adjoint"]
+    refl_of = {}
     for mid, params, vel, rec, wav, (iz, x0, x1) in models:
-        srv.add_model(mid, params, vel, rec, wav)
         refl = np.zeros((params["nx"], params["nz"]), np.float32)
         refl[x0:x1, iz + 40:iz + 43] = 0.1
-        info[mid] = (params, rec[3], iz, x0, x1, refl)
-    kinds = ["nl_forward", "born_forward", "born_adjoint", "bornext_adjoint"]
+        refl_of[mid] = refl
 
-    def make(k, arrival=0.0):
-        mid = models[rng.choice(4, p=[0.4, 0.3, 0.2, 0.1])][0]
-        params, nrec, iz, x0, x1, refl = info[mid]
-        kind = kinds[rng.choice(4, p=[0.3, 0.3, 0.25, 0.15])]
-        r = Request(kind, mid, (iz, int(rng.integers(x0, x1))), arrival=arrival)
+    def one(srv, mid, kind, ix, rng, arrival=0.0):
+        params = srv.models[mid].params
+        iz = [m for m in models if m[0] == mid][0][5][0]
+        r = Request(kind, mid, (iz, ix), arrival=arrival)
         if kind == "born_forward":
-            r.payload, r.reflectivity_id = refl, mid
+            r.payload, r.reflectivity_id = refl_of[mid], mid
         elif kind != "nl_forward":
-            r.payload = rng.standard_normal((nrec, params["nts"])).astype(np.float32)
+            r.payload = rng.standard_normal((srv.models[mid].rec[0].nDev, params["nts"])).astype(np.float32)
         return r
 
-    # untimed: one request of every (model, kind), so buffers and geometries exist
-    for mid in info:
-        for kind in kinds:
-            params, nrec, iz, x0, x1, refl = info[mid]
-            r = Request(kind, mid, (iz, x0))
-            if kind == "born_forward":
-                r.payload, r.reflectivity_id = refl, mid
-            elif kind != "nl_forward":
-                r.payload = np.zeros((nrec, params["nts"]), np.float32)
+    def stream(srv):
+        rng = np.random.default_rng(2026)
+        arrivals = np.cumsum(rng.exponential(1.0 / a.rate, a.requests))
+        reqs = []
+        for t in arrivals:
+            m = models[rng.choice(4, p=[0.4, 0.3, 0.2, 0.1])]
+            kind = kinds[rng.choice(4, p=[0.3, 0.3, 0.25, 0.15])]
+            _, _, _, _, _, (iz, x0, x1) = m
+            reqs.append(one(srv, m[0], kind, int(rng.integers(x0, x1)), rng, float(t)))
+        return reqs, float(arrivals[-1])
+
+    def run(slots, max_shots, threads):
+        srv = Server(max_models=4, max_shots=max_shots, slots=slots)
+        for mid, params, vel, rec, wav, _ in models:
+            srv.add_model(mid, params, vel, rec, wav)
+        rng = np.random.default_rng(7)
+        # untimed: one call of every (model, kind) on each slot's context, so
+        # buffers and geometries exist
+        warm = [one(srv, m[0], kind, m[5][1], rng) for m in models for kind in kinds for _ in range(slots)]
+        srv.max_shots = 1                  # one call per warm-up request, spread over the slots
+        if threads:
+            srv.start()
+        for r in warm:
             srv.submit(r)
-            srv.step()
-    arrivals = np.cumsum(rng.exponential(1.0 / a.rate, a.requests))
-    reqs = [make(k, float(t)) for k, t in enumerate(arrivals)]
-    calls0 = srv.calls
-    t = run_stream(srv, reqs)
-    lat = np.array([r.done - r.arrival for r in reqs]) * 1e3
+        if threads:
+            srv.drain()
+        else:
+            while srv.pending():
+                srv.step()
+        srv.max_shots = max_shots
+        reqs, span = stream(srv)
+        calls0 = srv.calls
+        t = run_stream(srv, reqs)
+        srv.close()
+        lat = np.array([r.done - r.arrival for r in reqs]) * 1e3
+        return {"shots_per_s": len(reqs) / t, "s": t, "stream_s": span, "calls": srv.calls - calls0,
+                "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+                "max_calls_running": srv.max_running,
+                "errors": sum(r.error is not None for r in reqs),
+                "finite": all(r.result is not None and np.isfinite(r.result).all() for r in reqs)}, reqs
+
+    batched, reqs = run(a.slots, 32, True)
+    plain, _ = run(1, 1, False)
     by_kind = {k: sum(r.kind == k for r in reqs) for k in kinds}
     out = {"metric": "shots/s, serving mix (NL, Born fwd/adj, 21-offset extended Born adj), 4 models 200x400-1000x3000",
-           "value": len(reqs) / t, "unit": "shots/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": t * 1e3,
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "value": batched["shots_per_s"], "unit": "shots/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": batched["s"] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
            "config": {"workload": "serve_mix_4models", "requests": len(reqs), "arrival_rate_per_s": a.rate,
-                      "requests_by_kind": by_kind, "stream_s": float(arrivals[-1])},
-           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-           "engine_calls": srv.calls - calls0, "finite": all(np.isfinite(r.result).all() for r in reqs)}
+                      "requests_by_kind": by_kind, "slots": a.slots, "max_shots_per_call": 32,
+                      "hbm_budget_gb": 200},
+           "latency_ms": batched["latency_ms"], "continuous_batching": batched,
+           "plain_one_request_at_a_time": plain,
+           "speedup_vs_plain": batched["shots_per_s"] / plain["shots_per_s"],
+           "finite": batched["finite"] and plain["finite"]}
     print(json.dumps(out), flush=True)
 
 

@@ -308,6 +308,49 @@ aiso_status aiso_wemva_adjoint(aiso_ctx* ctx, int add, float* model, const float
                                const float* wavelets, int nWavelet,
                                aiso_device* const* receivers, int nRecGeom, int mem);
 
+/* ------------------------------------- batched windows (uneven apertures)
+ * No counterpart in the reference, which runs one shot at a time on the full
+ * model per GPU (F/src/nonlinearPropShotsGpu.cpp:62-152,
+ * F/src/nonlinearShotsGpuFunctions.cu:227-309); this batches many shots,
+ * each on a window of its own size, in one launch per fine step.  Window w
+ * has its own padded grid nz[w] x nx[w], velocity [nx][nz], source and
+ * receiver geometry (built on that window's grid); every other parameter
+ * (spacing, time sampling, pads, sponge, fMax) comes from `par` (its nz/nx
+ * are ignored).  Each window's data equal those of aiso_nl_forward for that
+ * shot alone on a context of that window.  No free surface, no device in a
+ * window's FAT ring, the same source count in every window. */
+typedef struct aiso_batch aiso_batch;
+aiso_status aiso_batch_create(const aiso_fd_params* par, int device, aiso_batch** out);
+aiso_status aiso_batch_destroy(aiso_batch* b);
+/* Velocity checks (CFL, dispersion: F/src/fdParam.cpp:228-256) per window;
+ * vel[w] host or device memory by mem.  Replaces any previous windows. */
+aiso_status aiso_batch_set_windows(aiso_batch* b, int nWin, const int* nz, const int* nx, const float* const* vel,
+                                   aiso_device* const* sources, aiso_device* const* receivers, int mem);
+/* Launch configuration: tile tz in {32,64} x tx in {16,32}; windows per
+ * launch (0 = all in one launch; more groups run one after the other);
+ * workgroups per CU (0 = one workgroup per tile, else a persistent launch
+ * of that many per CU, each looping over tiles); graph != 0: capture each
+ * group's time loop in a HIP graph on first use and replay it later. */
+aiso_status aiso_batch_set_config(aiso_batch* b, int tz, int tx, int windowsPerLaunch, int blocksPerCU, int graph);
+aiso_status aiso_batch_get_config(const aiso_batch* b, int* tz, int* tx, int* windowsPerLaunch, int* blocksPerCU,
+                                  int* graph);
+/* Autotuner: times `steps` fine steps of the first launch group under each
+ * candidate (tile shapes x windows per launch {8,16,32,64,all} x workgroups
+ * per CU {0,4,8}) and keeps the fastest per point-update (the graph flag is
+ * left as set).  msPerPointUpdate (optional) gets the winner's time. */
+aiso_status aiso_batch_tune(aiso_batch* b, int steps, double* msPerPointUpdate);
+/* Nonlinear forward of every window: model[k] [nSrcDev][nts] (nModel 1: one
+ * signal for all windows, or one per window); data[w] [nRecDev_w][nts]. */
+aiso_status aiso_batch_nl_forward(aiso_batch* b, int add, const float* const* model, int nModel,
+                                  float* const* data, int mem);
+/* Windows, launch groups of the current configuration, interior
+ * point-updates of one call, device time of the last call (ms, HIP events
+ * around its launches) and host time spent capturing and instantiating
+ * graphs so far (ms). */
+aiso_status aiso_batch_info(const aiso_batch* b, int* nWin, int* launchGroups, long long* pointUpdates,
+                            float* lastMs, double* captureMs);
+aiso_status aiso_batch_stream(const aiso_batch* b, void** stream);
+
 /* ---------------------------------------------------------------- misc */
 /* padFileGpuMain padding rule (F/src/padFileGpuMain.cpp:49-66, no free
  * surface): returns padPlus for a side of n samples with pad on the minus side. */

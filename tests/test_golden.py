@@ -88,3 +88,28 @@ def test_benchmark_workload_sources():
         np.testing.assert_array_equal(o.regPos, g["src_regPos"][s])
     o = O.Device.grid(*w.rec, w.params["nz"], w.params["nx"], 10)
     np.testing.assert_array_equal(o.regPos, g["rec_regPos"])
+
+
+def test_windowed_workload_is_consistent():
+    """The batched windows workload: 128 windows of width 300-1500 on
+    nz = 400, each padded by the reference's rule on its own, its source in
+    the window centre and a receiver on every column, within the CFL and
+    dispersion limits the engine checks."""
+    from acoustic_isotropic_operators_amd.workloads import batched_windows, pad_plus
+    w = batched_windows()
+    assert w.nShot == 128 and min(w.nx) >= 300 and max(w.nx) <= 1500
+    p = w.params
+    dtw = p["dts"] / p["sub"]
+    for k in range(w.nShot):
+        NX, NZ = w.vels[k].shape
+        assert NZ == 400 + 50 + pad_plus(400, 50) + 10
+        assert NX == w.nx[k] + 50 + pad_plus(w.nx[k], 50) + 10
+        assert (NX - 10) % 16 == 0 and (NZ - 10) % 16 == 0
+        q = w.window_params(k)
+        assert (q["nz"], q["nx"]) == (NZ, NX) and q["xPadPlus"] == pad_plus(w.nx[k], 50)
+        iz, ix = w.src[k]
+        assert (iz, ix) == (57, 55 + w.nx[k] // 2)
+        assert w.rec[k][3] == w.nx[k]
+        v = w.vels[k][5:-5, 5:-5]
+        assert v.max() * dtw / 10.0 <= 0.45
+        assert v.min() / (p["fMax"] * 10.0) >= 3.0

@@ -1,12 +1,15 @@
 """The request server on the GPU: requests of several callers batched into
 shared engine calls give, each, what the same shot gives alone (bitwise),
-for every request kind, with two resident models and interleaved arrivals;
+for every request kind, with two resident models and interleaved arrivals,
+both with batches run one at a time and with concurrent slots (continuous
+batching: calls overlapping on the device while later requests arrive);
 the engine's per-shot image option returns the images whose sum the plain
 adjoint returns."""
 import numpy as np
 import pytest
 
 from cases import make_case
+from acoustic_isotropic_operators_amd.serve import Request, Server
 
 pytestmark = pytest.mark.gpu
 
@@ -21,20 +24,12 @@ def _models():
     return out
 
 
-def test_batched_requests_equal_single_shots():
-    from acoustic_isotropic_operators_amd import _capi as A
-    from acoustic_isotropic_operators_amd.serve import Request, Server
-    rng = np.random.default_rng(203)
-    srv = Server(max_models=2, max_shots=4, nExt=5)
-    refl = {}
-    for mid, c, params in _models():
-        rec = c.rec_args[0][1]
-        srv.add_model(mid, params, c.vel, rec, c.wavelet)
-        refl[mid] = rng.standard_normal((c.nx, c.nz)).astype(np.float32)
+def _requests(srv, rng, n=24):
     cases = {mid: (c, params) for mid, c, params in _models()}
+    refl = {mid: rng.standard_normal((c.nx, c.nz)).astype(np.float32) for mid, (c, _) in cases.items()}
     kinds = ["nl_forward", "born_forward", "born_adjoint", "bornext_adjoint"]
     reqs = []
-    for k in range(24):
+    for k in range(n):
         mid = f"m{k % 2}"
         c, params = cases[mid]
         kind = kinds[(k // 2) % 4]
@@ -44,18 +39,26 @@ def test_batched_requests_equal_single_shots():
         elif kind != "nl_forward":
             r.payload = rng.standard_normal((srv.models[mid].rec[0].nDev, c.nts)).astype(np.float32)
         reqs.append(r)
-    for r in reqs:
-        srv.submit(r)
-    while srv.pending():
-        srv.step()
-    assert srv.calls < len(reqs)                     # requests did share calls
-    # each request against the same shot alone, on a fresh context
+    return reqs
+
+
+def _server(**kw):
+    srv = Server(max_models=2, nExt=5, **kw)
+    for mid, c, params in _models():
+        srv.add_model(mid, params, c.vel, c.rec_args[0][1], c.wavelet)
+    return srv
+
+
+def _check_against_alone(reqs):
+    """Each request against the same shot alone, on a fresh context."""
+    from acoustic_isotropic_operators_amd import _capi as A
     for mid, c, params in _models():
         ctx = A.Context(dict(params, nExt=5, extension=A.EXT_OFFSET))
         ctx.set_velocity(c.vel)
         rec = [A.Device.grid(*c.rec_args[0][1], c.nz, c.nx, c.nts)]
         wav = c.wavelet[None, :].astype(np.float32)
         for r in (r for r in reqs if r.model == mid):
+            assert r.error is None, r.error
             src = [A.Device.grid(1, r.source[0], 1, 1, r.source[1], 1, c.nz, c.nx, c.nts)]
             if r.kind == "nl_forward":
                 d = np.zeros((1, rec[0].nDev, c.nts), np.float32)
@@ -73,6 +76,40 @@ def test_batched_requests_equal_single_shots():
                 ctx.bornext_adjoint(ref, r.payload[None], src, wav, rec)
             assert np.abs(ref).max() > 0
             np.testing.assert_array_equal(r.result, ref, err_msg=f"{mid} {r.kind} {r.source}")
+
+
+def test_batched_requests_equal_single_shots():
+    print("synchronous batches, 24 requests on 2 models", flush=True)
+    srv = _server(max_shots=4)
+    reqs = _requests(srv, np.random.default_rng(203))
+    for r in reqs:
+        srv.submit(r)
+    while srv.pending():
+        srv.step()
+    assert srv.calls < len(reqs)                     # requests did share calls
+    _check_against_alone(reqs)
+
+
+def test_continuous_batching_results_equal_single_shots():
+    """Three slots: calls run concurrently on their own contexts and streams
+    while later requests arrive; every result equals its shot alone."""
+    import time
+    print("continuous batching, 3 slots, 40 requests in 4 waves", flush=True)
+    srv = _server(max_shots=3, slots=3)
+    reqs = _requests(srv, np.random.default_rng(206), n=40)
+    srv.start()
+    try:
+        for w in range(4):
+            for r in reqs[w * 10:(w + 1) * 10]:
+                srv.submit(r)
+            time.sleep(0.01)                         # the next wave arrives while calls run
+        srv.drain()
+    finally:
+        srv.stop()
+    print(f"calls {srv.calls}, most calls running at once {srv.max_running}", flush=True)
+    assert srv.max_running >= 2
+    assert srv.calls < len(reqs)
+    _check_against_alone(reqs)
 
 
 def test_image_per_shot_option():
