@@ -124,7 +124,7 @@ def _nelem(x):
 
 def _check_arg(x, n, mem, name, writable=False):
     """Refuse an operand the engine would misread or overrun: it must be a
-    float32, C-contiguous array (numpy for MEM_HOST, a GPU tensor for
+    float32, C-contiguous array (numpy or a CPU tensor for MEM_HOST, a GPU tensor for
     MEM_DEVICE) of exactly ``n`` elements, writable when it is an output."""
     if x is None:
         raise ValueError(f"{name}: missing operand")
@@ -138,9 +138,17 @@ def _check_arg(x, n, mem, name, writable=False):
             raise ValueError(f"{name}: dtype {x.dtype}, expected float32")
         if not x.is_contiguous():
             raise ValueError(f"{name}: tensor is not contiguous")
+    elif mem == MEM_HOST and hasattr(x, "data_ptr") and hasattr(x, "is_cuda"):
+        import torch
+        if x.is_cuda:
+            raise ValueError(f"{name}: MEM_HOST tensor is on the GPU")
+        if x.dtype != torch.float32:
+            raise ValueError(f"{name}: dtype {x.dtype}, expected float32")
+        if not x.is_contiguous():
+            raise ValueError(f"{name}: tensor is not contiguous")
     elif mem == MEM_HOST:
         if not isinstance(x, np.ndarray):
-            raise ValueError(f"{name}: MEM_HOST operands must be numpy arrays")
+            raise ValueError(f"{name}: MEM_HOST operands must be numpy arrays or CPU tensors")
         if x.dtype != np.float32:
             raise ValueError(f"{name}: dtype {x.dtype}, expected float32")
         if not x.flags.c_contiguous:

@@ -65,3 +65,22 @@ def test_ctx_create_rejects_bad_parameters_on_the_host():
     with pytest.raises(RuntimeError):
         _capi.Context(dict(nz=58, nx=74, dz=10.0, dx=10.0, nts=60, dts=0.004, sub=4, zPadMinus=10, zPadPlus=10,
                            xPadMinus=10, xPadPlus=10, fat=4))
+
+
+def test_operand_checks_take_numpy_and_cpu_tensors_for_host_memory():
+    """Host operands may be float32 numpy arrays or CPU tensors (the autograd
+    layer passes CPU tensors); wrong dtype, layout or element count is refused
+    before any call into the library."""
+    import torch
+    _capi._check_arg(np.zeros(6, np.float32), 6, _capi.MEM_HOST, "x", writable=True)
+    _capi._check_arg(torch.zeros(2, 3), 6, _capi.MEM_HOST, "x", writable=True)
+    with pytest.raises(ValueError, match="float32"):
+        _capi._check_arg(torch.zeros(6, dtype=torch.float64), 6, _capi.MEM_HOST, "x")
+    with pytest.raises(ValueError, match="contiguous"):
+        _capi._check_arg(torch.zeros(3, 2).t(), 6, _capi.MEM_HOST, "x")
+    with pytest.raises(ValueError, match="elements"):
+        _capi._check_arg(torch.zeros(5), 6, _capi.MEM_HOST, "x")
+    with pytest.raises(ValueError, match="GPU tensors"):
+        _capi._check_arg(np.zeros(6, np.float32), 6, _capi.MEM_DEVICE, "x")
+    with pytest.raises(ValueError, match="not on the GPU"):
+        _capi._check_arg(torch.zeros(6), 6, _capi.MEM_DEVICE, "x")

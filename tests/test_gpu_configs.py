@@ -56,8 +56,19 @@ def test_nonlinear_forward_200x400_matches_the_oracle():
     np.testing.assert_array_equal(ma, O.nl_adjoint(g, w.vel, osrc, orec, dr, 1, "f"))
     a = float(np.vdot(d.astype(np.float64).ravel(), dr.astype(np.float64).ravel()))
     b = float(np.vdot(m.astype(np.float64).ravel(), ma.astype(np.float64).ravel()))
-    print(f"<Fm,d> {a:.9e} <m,F'd> {b:.9e} rel {abs(a - b) / abs(b):.2e}", flush=True)
-    assert abs(a - b) <= 1e-5 * abs(b)
+    # The same pair through the FP64 oracle: the operators are exact transposes,
+    # so its dot test closes to rounding of doubles.
+    ad = O.nl_adjoint(g, w.vel, osrc, orec, dr.astype(np.float64), 1, "d")
+    a64 = float(np.vdot(od.ravel(), dr.astype(np.float64).ravel()))
+    b64 = float(np.vdot(m.astype(np.float64).ravel(), ad.ravel()))
+    print(f"<Fm,d> {a:.9e} <m,F'd> {b:.9e} rel {abs(a - b) / abs(b):.2e}; FP64 oracle rel "
+          f"{abs(a64 - b64) / abs(b64):.2e}", flush=True)
+    assert abs(a64 - b64) <= 1e-12 * abs(b64)
+    # Float rounding over 4000 fine steps leaves a dot-test gap of a few 1e-6 to
+    # ~1.4e-5 depending on the vectors (the float oracle, whose data and
+    # adjoint are bitwise those above, gives 3.5e-6 / 1.2e-5 / 4.5e-6 on three
+    # seeded uniform pairs): bound it at 5e-5, with the exact figure printed.
+    assert abs(a - b) <= 5e-5 * abs(b)
 
 
 @pytest.fixture(scope="module")
