@@ -3,7 +3,7 @@ same data, bitwise, on two calls in a row, the second with a smaller input
 (fewer shots on a smaller grid); the scratch array keeps the first call's
 contents, so the second call shows it clears what it needs.  Also its
 refusals: a call whose one shot exceeds the 64 MiB scratch, and host memory.
-Collected last (the round's end runs pytest -x)."""
+Collected last, so that a failure here hides no other test under pytest -x."""
 import numpy as np
 import pytest
 
