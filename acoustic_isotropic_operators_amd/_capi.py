@@ -41,6 +41,7 @@ _SIGS = {
     "aiso_ctx_set_batch": (_I, [_P, _I]),
     "aiso_ctx_set_tile": (_I, [_P, _I, _I]),
     "aiso_ctx_set_option": (_I, [_P, C.c_char_p, _I]),
+    "aiso_ctx_get_option": (_I, [_P, C.c_char_p, C.POINTER(_I)]),
     "aiso_ctx_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(C.c_size_t)]),
     "aiso_ctx_stream": (_I, [_P, _PP]),
     "aiso_ctx_profile": (_I, [_P, _I]),
@@ -268,6 +269,11 @@ class Context:
     def set_option(self, key, value):
         _check(lib().aiso_ctx_set_option(self.h, key.encode(), int(value)))
         self.options[key] = int(value)
+
+    def get_option(self, key):
+        v = _I()
+        _check(lib().aiso_ctx_get_option(self.h, key.encode(), C.byref(v)))
+        return v.value
 
     # operand sizes ---------------------------------------------------------
     def _nzx(self):

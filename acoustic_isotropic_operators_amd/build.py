@@ -40,14 +40,18 @@ def _run(cmd):
 
 
 def build_lib():
-    objs = []
+    from concurrent.futures import ThreadPoolExecutor
     bdir = os.path.join(PKG, "_build")
     os.makedirs(bdir, exist_ok=True)
+    cmds, objs = [], []
     for src in ("kernels.hip", "noalloc.hip", "engine.cpp", "passes.cpp", "windows.cpp", "host.cpp"):
         obj = os.path.join(bdir, src.replace(".", "_") + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
-        _run([HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # one compiler per object, side by side (each is single-threaded)
+    with ThreadPoolExecutor(max_workers=min(len(cmds), max(1, (os.cpu_count() or 1) // 2))) as ex:
+        list(ex.map(_run, cmds))
     out = os.path.join(PKG, "libaiso.so")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out, *objs])
     return out

@@ -223,7 +223,25 @@ void join_lanes(aiso_ctx* c)
 // upper bounds.
 bool sample_event(aiso_ctx* c, int kind)
 {
-    return c->profile && c->profSeq[kind]++ % c->profEvery == 0;
+    // With lanes the dispatches of different lanes overlap, so per-dispatch
+    // durations are not the kernel's; such passes are timed whole instead.
+    return c->profile && c->curLanes == 1 && c->profSeq[kind]++ % c->profEvery == 0;
+}
+
+hipEvent_t pass_timer_begin(aiso_ctx* c)
+{
+    if (!c->profile || c->curLanes <= 1) return nullptr;
+    hipEvent_t e0 = pooled_event(c);
+    HIPC(hipEventRecord(e0, c->stream));
+    return e0;
+}
+
+void pass_timer_end(aiso_ctx* c, hipEvent_t e0, int kind, double pts, int steps)
+{
+    if (!e0) return;
+    hipEvent_t e1 = pooled_event(c);
+    HIPC(hipEventRecord(e1, c->stream));
+    c->ev.push_back({e0, e1, kind, pts, steps});
 }
 
 void launch_timed(aiso_ctx* c, unsigned mode, int kind, const StepArgs& a, bool generic)
@@ -740,6 +758,34 @@ aiso_status aiso_ctx_set_option(aiso_ctx* c, const char* key, int value)
             if (value < 1) throw Error(AISO_ERR_PARAM, "profileEvery must be >= 1");
             c->profEvery = value;
         }
+        else throw Error(AISO_ERR_PARAM, "unknown option " + k);
+    });
+}
+
+aiso_status aiso_ctx_get_option(const aiso_ctx* c, const char* key, int* value)
+{
+    return guard([&] {
+        if (!c || !key || !value) throw Error(AISO_ERR_PARAM, "null argument");
+        const std::string k(key);
+        if (k == "streamNT") *value = c->streamNT;
+        else if (k == "lockstep") *value = c->lockstep;
+        else if (k == "pair2") *value = c->pair2;
+        else if (k == "leg1") *value = c->leg1;
+        else if (k == "leg2") *value = c->leg2;
+        else if (k == "sloth") *value = c->sloth;
+        else if (k == "imagePerShot") *value = c->imagePerShot;
+        else if (k == "saveWavefield") *value = c->saveWavefield;
+        else if (k == "wavefieldShot") *value = c->wavefieldShot;
+        else if (k == "extBatch") *value = c->extBatch;
+        else if (k == "lanes") *value = c->lanes;
+        else if (k == "tileFwd") *value = c->tzFwd * 100 + c->txFwd;
+        else if (k == "tileAdj") *value = c->tzAdj * 100 + c->txAdj;
+        else if (k == "batchFwd") *value = c->batchFwd;
+        else if (k == "batchAdj") *value = c->batchAdj;
+        else if (k == "batchRec") *value = c->batchRec;
+        else if (k == "shotsPerBlock") *value = c->spb;
+        else if (k == "cacheBudgetMB") *value = (int)(c->cacheBudget >> 20);
+        else if (k == "profileEvery") *value = c->profEvery;
         else throw Error(AISO_ERR_PARAM, "unknown option " + k);
     });
 }

@@ -143,10 +143,11 @@ struct aiso_ctx {
     hipEvent_t stageEv = nullptr;
     bool stagePending = false;
     // Lanes: the shots of a batch split over streams whose kernels run
-    // concurrently, so one lane's launch gaps and tails are filled by the
-    // other's.  On MI355X two lanes gain about 1 % on the Born benchmark, so
-    // the option is off by default.
-    int lanes = 1;
+    // concurrently, so one lane's launch tail (CUs idle while its last
+    // workgroups finish) and launch gap are filled by the other's.  On
+    // MI355X two lanes raise the Born benchmark from 279 to 296 Gpts/s;
+    // three and four lanes measured slower than one.
+    int lanes = 2;
     int extBatch = 16;                  // coarse samples per offset-imaging / secondary-source launch
     int curLanes = 1;                   // lanes of the call in progress
     std::vector<hipStream_t> lane;
@@ -237,6 +238,13 @@ void ensure_lanes(aiso_ctx* c);
 void fork_lanes(aiso_ctx* c);
 void join_lanes(aiso_ctx* c);
 bool sample_event(aiso_ctx* c, int kind);
+// Pass-level timing of a pass whose shots run on several lanes: an event on
+// the context's stream before the lanes fork and one after they join, the
+// interval credited to `kind` with the point-updates of every lane and
+// `steps` launches (one per fine step, counting the lanes' concurrent
+// dispatches as one).  No-op unless profiling with more than one lane.
+hipEvent_t pass_timer_begin(aiso_ctx* c);
+void pass_timer_end(aiso_ctx* c, hipEvent_t e0, int kind, double pts, int steps);
 void launch_timed(aiso_ctx* c, unsigned mode, int kind, const StepArgs& a, bool generic = false);
 void drain_profile(aiso_ctx* c);
 

@@ -667,6 +667,7 @@ void Call::pass_lockstep()
     a.wa = f[h + 1];
     a.wb = f[h + 2];
     a.wc = f[1];
+    hipEvent_t passT = pass_timer_begin(c);
     fork_lanes(c);
     // With sampled profiling on one lane, the pair kernel is timed over
     // windows of profEvery consecutive launches between two plain events
@@ -766,6 +767,7 @@ void Call::pass_lockstep()
     }
     if (windows) close_window();
     join_lanes(c);
+    pass_timer_end(c, passT, AISO_K_BORN_PAIR, 2.0 * double(p.interior()) * nb * (T - 2), T - 2);
     record_data();
 }
 
@@ -855,6 +857,7 @@ void Call::pass_source()
     a.wa = f[h + 1];
     a.wb = f[h + 2 < 2 * h ? h + 2 : h + 1];
     a.wc = f[1];
+    hipEvent_t passT = pass_timer_begin(c);
     fork_lanes(c);
     for (int its = 0; its < p.nts - 1; ++its)
         for (int it2 = 1; it2 <= p.sub; ++it2) {
@@ -874,6 +877,7 @@ void Call::pass_source()
             a.cur ^= 1;
         }
     join_lanes(c);
+    pass_timer_end(c, passT, AISO_K_SRC, double(p.interior()) * nb * (p.ntw - 1), p.ntw - 1);
     if (win)   // W[nts-1] = p(last-1)*f[h+1] + p(last)*f[h+2]
         HIPC(launch_win_tail(dSlots, nb, a.cur, winOut, f[h + 1], f[h + 2], (long)(p.nts - 1) * nzx, 0, 0, mtil,
                              p.nz, p.nx, s));
@@ -1136,6 +1140,7 @@ void Call::pass_forward()
             nextSS += e.nk;
         }
     };
+    hipEvent_t passT = pass_timer_begin(c);
     fork_lanes(c);
     for (int its = 0; its < p.nts - 1; ++its) {
         if (q.op == Op::ExtFwd) {
@@ -1162,6 +1167,7 @@ void Call::pass_forward()
         std::swap(L, R);
     }
     join_lanes(c);
+    pass_timer_end(c, passT, kind, double(p.interior()) * nb * (p.ntw - 1), p.ntw - 1);
     record_data();
 }
 
@@ -1218,6 +1224,7 @@ void Call::pass_adjoint()
             }
             pend.clear();
         };
+        hipEvent_t passT = pass_timer_begin(c);
         fork_lanes(c);
         for (int its = p.nts - 2; its >= 0; --its) {
             for (int it2 = p.sub - 1; it2 >= 0; --it2) {
@@ -1252,6 +1259,7 @@ void Call::pass_adjoint()
             std::swap(L, R);
         }
         join_lanes(c);
+        pass_timer_end(c, passT, kind, double(p.interior()) * nbr * (p.ntw - 1), p.ntw - 1);
         if (q.op == Op::BornAdj) {
             if (win) {   // img += W[0] * (q(1)*f[1] + q(0)*f[0]), then the scaling
                 HIPC(launch_win_tail(sl, nbr, r.cur, WIN_IMG, f[1], f[0], 0, 0, 0, nullptr, p.nz, p.nx, s));

@@ -244,6 +244,7 @@ def main():
     avg_ms = st["ms"] / max(st["launches"], 1)
     per_launch_bytes = bpp * st["point_updates"] / max(st["launches"], 1)
     achieved = per_launch_bytes / (avg_ms / 1e3) / 1e9
+    lanes = ctx.get_option("lanes")
     bms, bsh = ctx.batch_read()
     info = ctx.info()
     lat = shot_latencies(bms, bsh, w.nShot, 2)           # the forward call, then the adjoint call
@@ -264,7 +265,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": w.name, "shots": w.nShot, "grid": "500x1200 (padded %dx%d)" % (p["nz"], p["nx"]),
-                   "nts": p["nts"], "sub": p["sub"], "shots_per_launch_adj": info["batch"], "tile": a.tile or "32x16 forward, 32x32 adjoint",
+                   "nts": p["nts"], "sub": p["sub"], "shots_per_launch_adj": info["batch"], "tile": a.tile or "32x16 forward, 32x32 adjoint", "lanes": lanes,
                    "inputs": "model/data/image resident in HBM"},
         "job_roofline_frac": job_bytes / t / 1e9 / HBM_PEAK_GBS,
         "shot_latency_ms": {"p50": float(np.percentile(lat, 50)) if lat else None,
@@ -277,7 +278,12 @@ def main():
                      "traffic_counts": "L2-to-fabric bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated), "
                                        "HBM plus Infinity Cache hits", "avg_launch_ms": avg_ms,
                      "launches": st["launches"], "bytes_per_point": bpp,
-                     "point_updates_per_launch": st["point_updates"] / max(st["launches"], 1)},
+                     "point_updates_per_launch": st["point_updates"] / max(st["launches"], 1),
+                     "lanes": lanes,
+                     "timing": ("HIP events on the engine's stream around each forward pass, before its "
+                                f"{lanes} lanes fork and after they join; a launch is one fine step of every "
+                                "lane's shots (the lanes' kernels run concurrently)") if lanes > 1 else
+                               ("HIP events on the engine's stream around windows of consecutive launches")},
         "kernels": kern,
         "finite": ok,
     }

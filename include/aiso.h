@@ -114,7 +114,9 @@ aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
  *       (for callers that batch independent requests; default 0);
  *   "extBatch" 1..16: coarse samples per offset-imaging / secondary-source
  *       launch of the extended Born operators (default 16);
- *   "lanes" 1..4: streams the shots of a launch are split over (default 1);
+ *   "lanes" 1..4: streams the shots of a launch are split over, whose
+ *       kernels run concurrently (default 2; the extended, tomographic and
+ *       WEMVA operators and the two-step kernels run on one);
  *   "leg1", "leg2" 0/1: legs of the tomographic operator (parfile keys
  *       leg1/leg2 of F/src/tomoExtGpu.cpp:10-11, default 1);
  *   "sloth" 0/1: time-lag extended Born model is a slowness-squared
@@ -126,6 +128,8 @@ aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
  *       aiso_ctx_wavefield; parfile keys saveWavefield / wavefieldShotNumber
  *       of F/src/BornShotsGpu.cpp:18-19). */
 aiso_status aiso_ctx_set_option(aiso_ctx* ctx, const char* key, int value);
+/* The current value of an aiso_ctx_set_option knob (tiles as tz * 100 + tx). */
+aiso_status aiso_ctx_get_option(const aiso_ctx* ctx, const char* key, int* value);
 /* Derived sizes: ntw, sub, nShotsPerLaunch actually used by the last call. */
 aiso_status aiso_ctx_info(const aiso_ctx* ctx, int* ntw, int* batchUsed, size_t* bytesPerShot);
 /* The HIP stream (hipStream_t) every kernel of this context runs on. */
@@ -145,7 +149,9 @@ enum {
     AISO_K_COUNT = 9
 };
 /* Per-launch timing of the time-step kernels with HIP events on that stream
- * (enable != 0), reset of the counters. */
+ * (enable != 0), reset of the counters.  With more than one lane, a pass is
+ * timed whole instead (events before its lanes fork and after they join),
+ * and each of its fine steps counts as one launch of all lanes' shots. */
 aiso_status aiso_ctx_profile(aiso_ctx* ctx, int enable);
 /* Launch count, summed duration (ms) and summed interior point-updates of
  * the time-step kernels of variant `which` (AISO_K_*) timed since profiling
