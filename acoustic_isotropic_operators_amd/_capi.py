@@ -73,8 +73,9 @@ _SIGS = {
     "aiso_batch_create": (_I, [C.POINTER(FdParams), _I, _PP]),
     "aiso_batch_destroy": (_I, [_P]),
     "aiso_batch_set_windows": (_I, [_P, _I, _P, _P, _PP, _PP, _PP, _I]),
-    "aiso_batch_set_config": (_I, [_P, _I, _I, _I, _I, _I]),
-    "aiso_batch_get_config": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    "aiso_batch_set_config": (_I, [_P, _I, _I, _I, _I, _I, _I]),
+    "aiso_batch_get_config": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I), C.POINTER(_I),
+                                   C.POINTER(_I)]),
     "aiso_batch_tune": (_I, [_P, _I, C.POINTER(C.c_double)]),
     "aiso_batch_nl_forward": (_I, [_P, _I, _PP, _I, _PP, _I]),
     "aiso_batch_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(C.c_longlong), C.POINTER(_F),
@@ -502,17 +503,17 @@ class Batch:
         self.windows = [(int(nz[k]), int(nx[k]), sources[k], receivers[k]) for k in range(n)]
         self._keep = (sources, receivers)
 
-    def set_config(self, tz=32, tx=16, windows_per_launch=0, blocks_per_cu=0, graph=True):
-        _check(lib().aiso_batch_set_config(self.h, tz, tx, windows_per_launch, blocks_per_cu, int(graph)))
+    def set_config(self, tz=32, tx=16, windows_per_launch=0, blocks_per_cu=0, graph=True, lanes=2):
+        _check(lib().aiso_batch_set_config(self.h, tz, tx, windows_per_launch, blocks_per_cu, lanes, int(graph)))
 
     def config(self):
-        v = [_I() for _ in range(5)]
+        v = [_I() for _ in range(6)]
         _check(lib().aiso_batch_get_config(self.h, *[C.byref(x) for x in v]))
         return dict(tz=v[0].value, tx=v[1].value, windows_per_launch=v[2].value, blocks_per_cu=v[3].value,
-                    graph=bool(v[4].value))
+                    lanes=v[4].value, graph=bool(v[5].value))
 
     def tune(self, steps=16, cache=None):
-        """Pick tile, windows per launch and workgroups per CU by timing
+        """Pick tile, windows per launch, workgroups per CU and lanes by timing
         (aiso_batch_tune).  With ``cache`` (a JSON file path) the choice is
         looked up by the windows' signature first and stored after tuning."""
         key = self.signature()
@@ -526,13 +527,13 @@ class Batch:
             if key in table:
                 c = table[key]
                 self.set_config(c["tz"], c["tx"], c["windows_per_launch"], c["blocks_per_cu"],
-                                self.config()["graph"])
+                                self.config()["graph"], c.get("lanes", 2))
                 return dict(c, cached=True)
         t = C.c_double()
         _check(lib().aiso_batch_tune(self.h, int(steps), C.byref(t)))
         c = self.config()
         out = dict(tz=c["tz"], tx=c["tx"], windows_per_launch=c["windows_per_launch"],
-                   blocks_per_cu=c["blocks_per_cu"], ns_per_point_update=t.value * 1e6)
+                   blocks_per_cu=c["blocks_per_cu"], lanes=c["lanes"], ns_per_point_update=t.value * 1e6)
         if cache is not None:
             table[key] = out
             tmp = cache + ".tmp"

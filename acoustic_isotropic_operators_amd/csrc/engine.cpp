@@ -258,7 +258,7 @@ void launch_timed(aiso_ctx* c, unsigned mode, int kind, const StepArgs& a, bool 
             c->ev.push_back({e0, e1, kind, double(c->p.interior()) * cnt});
         }
         if (generic) HIPC(launch_step_generic(mode, c->tz, c->tx, y, lane_stream(c, l), e0, e1));
-        else HIPC(launch_step(mode, c->tz, c->tx, c->spb, y, lane_stream(c, l), e0, e1));
+        else HIPC(launch_step(mode, c->tz, c->tx, c->spb, y, lane_stream(c, l), e0, e1, c->spb == 1 ? c->occ : 0));
     }
 }
 
@@ -753,6 +753,10 @@ aiso_status aiso_ctx_set_option(aiso_ctx* c, const char* key, int value)
             if (value != 1 && value != 2) throw Error(AISO_ERR_PARAM, "shotsPerBlock must be 1 or 2");
             c->spb = value;
         }
+        else if (k == "occupancy") {
+            if (value != 0 && value != 8) throw Error(AISO_ERR_PARAM, "occupancy must be 0 or 8");
+            c->occ = value;
+        }
         else if (k == "cacheBudgetMB") c->cacheBudget = (long)std::max(1, value) << 20;
         else if (k == "profileEvery") {
             if (value < 1) throw Error(AISO_ERR_PARAM, "profileEvery must be >= 1");
@@ -784,6 +788,7 @@ aiso_status aiso_ctx_get_option(const aiso_ctx* c, const char* key, int* value)
         else if (k == "batchAdj") *value = c->batchAdj;
         else if (k == "batchRec") *value = c->batchRec;
         else if (k == "shotsPerBlock") *value = c->spb;
+        else if (k == "occupancy") *value = c->occ;
         else if (k == "cacheBudgetMB") *value = (int)(c->cacheBudget >> 20);
         else if (k == "profileEvery") *value = c->profEvery;
         else throw Error(AISO_ERR_PARAM, "unknown option " + k);

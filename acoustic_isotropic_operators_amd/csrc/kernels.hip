@@ -399,7 +399,7 @@ __device__ __forceinline__ bool tile_info(const StepArgs& a, TileInfo& t)
 }
 
 template <int TZ, int TX, unsigned MODE, int SPB>
-__global__ __launch_bounds__(NT) void k_step(const StepArgs a)
+__device__ __forceinline__ void step_tile(const StepArgs& a)
 {
     using T = TileShape<TZ, TX>;
     constexpr bool ADJ = MODE & M_ADJ;
@@ -414,15 +414,72 @@ __global__ __launch_bounds__(NT) void k_step(const StepArgs a)
     else step_block<TZ, TX, MODE, SPB, false, true>(a, t.group, t.tile, t.z0, t.x0, sC, sV, sN);
 }
 
-// One fine step of a launch of windows of uneven size (k_step's work on each
-// window's own grid).  The tiles of all windows form one list; XCD k
-// (workgroup id mod 8) owns the k-th eighth of it, so neighbouring tiles of a
-// window share that XCD's L2, and its workgroups stride through that range.
-// With fewer workgroups than tiles the launch is persistent: each workgroup
-// loops over several tiles.  Every point receives the operations k_step gives
-// it, in the same order.
+template <int TZ, int TX, unsigned MODE, int SPB>
+__global__ __launch_bounds__(NT) void k_step(const StepArgs a)
+{
+    step_tile<TZ, TX, MODE, SPB>(a);
+}
+
+// The same step with the register budget of 8 waves per SIMD (64 VGPRs):
+// at most 8 workgroups of 256 threads per CU either way, but more of them
+// resident to cover the latency of the tile staging (the adjoint kernels
+// otherwise compile to 76 VGPRs, 6 waves).
 template <int TZ, int TX, unsigned MODE>
-__global__ __launch_bounds__(NT) void k_step_win(const WinStepArgs w)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_step_o8(const StepArgs a)
+{
+    step_tile<TZ, TX, MODE, 1>(a);
+}
+
+// One tile of a window (k_step's work on that window's own grid): the
+// window, the tile and its classification from the tile list entry l.
+template <int TZ, int TX, unsigned MODE>
+__device__ __forceinline__ void win_tile(const WinStepArgs& w, int l, float* sC, float* sV, float* sN)
+{
+    const int2 tm = w.tileMap[l];
+    const WinDesc d = w.win[tm.x];
+    StepArgs a = w.a;
+    a.slots = w.a.slots + tm.x;
+    a.v2dt2 = d.v2dt2;
+    a.nz = d.nz;
+    a.nx = d.nx;
+    a.tilesZ = d.tilesZ;
+    a.ntiles = d.ntiles;
+    a.nshots = 1;
+    const int z0 = FAT + (tm.y % d.tilesZ) * TZ, x0 = FAT + (tm.y / d.tilesZ) * TX;
+    const bool full = z0 + TZ <= d.nz - FAT && x0 + TX <= d.nx - FAT;
+    const int edge = imin(imin(z0 - FAT, d.nz - FAT - (z0 + TZ)), imin(x0 - FAT, d.nx - FAT - (x0 + TX)));
+    if (full && edge >= a.minPad) step_block<TZ, TX, MODE, 1, true, false>(a, 0, tm.y, z0, x0, sC, sV, sN);
+    else if (full) step_block<TZ, TX, MODE, 1, true, true>(a, 0, tm.y, z0, x0, sC, sV, sN);
+    else step_block<TZ, TX, MODE, 1, false, true>(a, 0, tm.y, z0, x0, sC, sV, sN);
+}
+
+// One fine step of a launch of windows of uneven size.  The tiles of all
+// windows form one list; XCD k (workgroup id mod 8) owns the k-th eighth of
+// it, so neighbouring tiles of a window share that XCD's L2.  One workgroup
+// per tile: the launch has 8 * ceil(ntotal / 8) workgroups.  Every point
+// receives the operations k_step gives it, in the same order.  The 32-row
+// tiles fit the register budget of 8 waves per SIMD without spilling.
+template <int TZ, int TX, unsigned MODE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(TZ == 32 ? 8 : 1, 8)))
+void k_step_win(const WinStepArgs w)
+{
+    using T = TileShape<TZ, TX>;
+    constexpr bool ADJ = MODE & M_ADJ;
+    constexpr bool SPARSE = MODE & (M_INJ_SIG | M_INJ_DATA | M_REC_DATA | M_REC_SIG);
+    __shared__ float sC[T::LX * T::LZ];
+    __shared__ float sV[ADJ ? T::LX * T::LZ : 1];
+    __shared__ float sN[SPARSE ? TX * TZ : 1];
+    const int xcd = blockIdx.x & 7;
+    const int chunk = (w.ntotal + 7) >> 3;
+    const int l = xcd * chunk + (int)(blockIdx.x >> 3);
+    if (l >= imin((xcd + 1) * chunk, w.ntotal)) return;   // block-uniform
+    win_tile<TZ, TX, MODE>(w, l, sC, sV, sN);
+}
+
+// The persistent form: fewer workgroups than tiles, each looping over the
+// tiles of its XCD's eighth of the list.
+template <int TZ, int TX, unsigned MODE>
+__global__ __launch_bounds__(NT) void k_step_win_loop(const WinStepArgs w)
 {
     using T = TileShape<TZ, TX>;
     constexpr bool ADJ = MODE & M_ADJ;
@@ -434,22 +491,7 @@ __global__ __launch_bounds__(NT) void k_step_win(const WinStepArgs w)
     const int chunk = (w.ntotal + 7) >> 3;
     const int end = imin((xcd + 1) * chunk, w.ntotal);
     for (int l = xcd * chunk + (int)(blockIdx.x >> 3); l < end; l += per) {   // block-uniform
-        const int2 tm = w.tileMap[l];
-        const WinDesc d = w.win[tm.x];
-        StepArgs a = w.a;
-        a.slots = w.a.slots + tm.x;
-        a.v2dt2 = d.v2dt2;
-        a.nz = d.nz;
-        a.nx = d.nx;
-        a.tilesZ = d.tilesZ;
-        a.ntiles = d.ntiles;
-        a.nshots = 1;
-        const int z0 = FAT + (tm.y % d.tilesZ) * TZ, x0 = FAT + (tm.y / d.tilesZ) * TX;
-        const bool full = z0 + TZ <= d.nz - FAT && x0 + TX <= d.nx - FAT;
-        const int edge = imin(imin(z0 - FAT, d.nz - FAT - (z0 + TZ)), imin(x0 - FAT, d.nx - FAT - (x0 + TX)));
-        if (full && edge >= a.minPad) step_block<TZ, TX, MODE, 1, true, false>(a, 0, tm.y, z0, x0, sC, sV, sN);
-        else if (full) step_block<TZ, TX, MODE, 1, true, true>(a, 0, tm.y, z0, x0, sC, sV, sN);
-        else step_block<TZ, TX, MODE, 1, false, true>(a, 0, tm.y, z0, x0, sC, sV, sN);
+        win_tile<TZ, TX, MODE>(w, l, sC, sV, sN);
         __syncthreads();                                  // sC, sV, sN are reused by the next tile
     }
 }
@@ -2118,25 +2160,36 @@ static StepKernel pick_tile_spb(int tz, int tx)
 }
 
 template <unsigned MODE>
-static StepKernel pick_tile(int tz, int tx, int spb)
+static StepKernel pick_tile_o8(int tz, int tx)
+{
+    if (tz == 64 && tx == 16) return k_step_o8<64, 16, MODE>;
+    if (tz == 64 && tx == 32) return k_step_o8<64, 32, MODE>;
+    if (tz == 32 && tx == 16) return k_step_o8<32, 16, MODE>;
+    if (tz == 32 && tx == 32) return k_step_o8<32, 32, MODE>;
+    return nullptr;
+}
+
+template <unsigned MODE>
+static StepKernel pick_tile(int tz, int tx, int spb, int occ)
 {
     if (spb == 2) return pick_tile_spb<MODE, 2>(tz, tx);
+    if (occ == 8) return pick_tile_o8<MODE>(tz, tx);
     return pick_tile_spb<MODE, 1>(tz, tx);
 }
 
 // The extended-Born modes (MODE_EXT_SCAT, MODE_EXT_REC) are not instantiated
 // here: they run on k_step_generic only, and launch_step refuses them.
-static StepKernel pick_step(unsigned mode, int tz, int tx, int spb)
+static StepKernel pick_step(unsigned mode, int tz, int tx, int spb, int occ)
 {
     switch (mode) {
-    case MODE_NL_FWD: return pick_tile<MODE_NL_FWD>(tz, tx, spb);
-    case MODE_NL_ADJ: return pick_tile<MODE_NL_ADJ>(tz, tx, spb);
-    case MODE_SRC: return pick_tile<MODE_SRC>(tz, tx, spb);
-    case MODE_BORN_SCAT: return pick_tile<MODE_BORN_SCAT>(tz, tx, spb);
-    case MODE_BORN_REC: return pick_tile<MODE_BORN_REC>(tz, tx, spb);
-    case MODE_SRC_WIN: return pick_tile<MODE_SRC_WIN>(tz, tx, spb);
-    case MODE_BORN_SCAT_PRE: return pick_tile<MODE_BORN_SCAT_PRE>(tz, tx, spb);
-    case MODE_REC_WIN: return pick_tile<MODE_REC_WIN>(tz, tx, spb);
+    case MODE_NL_FWD: return pick_tile<MODE_NL_FWD>(tz, tx, spb, occ);
+    case MODE_NL_ADJ: return pick_tile<MODE_NL_ADJ>(tz, tx, spb, occ);
+    case MODE_SRC: return pick_tile<MODE_SRC>(tz, tx, spb, occ);
+    case MODE_BORN_SCAT: return pick_tile<MODE_BORN_SCAT>(tz, tx, spb, occ);
+    case MODE_BORN_REC: return pick_tile<MODE_BORN_REC>(tz, tx, spb, occ);
+    case MODE_SRC_WIN: return pick_tile<MODE_SRC_WIN>(tz, tx, spb, occ);
+    case MODE_BORN_SCAT_PRE: return pick_tile<MODE_BORN_SCAT_PRE>(tz, tx, spb, occ);
+    case MODE_REC_WIN: return pick_tile<MODE_REC_WIN>(tz, tx, spb, occ);
     default: return nullptr;
     }
 }
@@ -2151,9 +2204,10 @@ static hipError_t launch_k(StepKernel k, int grid, const StepArgs& b, hipStream_
 }
 
 hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a, hipStream_t s, hipEvent_t e0,
-                       hipEvent_t e1)
+                       hipEvent_t e1, int occ)
 {
-    StepKernel k = pick_step(mode, tz, tx, spb);
+    if (occ != 0 && occ != 8) return hipErrorInvalidValue;
+    StepKernel k = pick_step(mode, tz, tx, spb, occ);
     if (!k) return hipErrorInvalidValue;
     if ((mode & M_WIN) && a.winOut == WIN_SLICE) return hipErrorInvalidValue;   // extended-Born capture: generic only
     // One workgroup per (tile, group of spb shots).
@@ -2165,20 +2219,23 @@ hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a
 using WinKernel = void (*)(const WinStepArgs);
 
 template <unsigned MODE>
-static WinKernel pick_win_tile(int tz, int tx)
+static WinKernel pick_win_tile(int tz, int tx, bool loop)
 {
-    if (tz == 64 && tx == 16) return k_step_win<64, 16, MODE>;
-    if (tz == 64 && tx == 32) return k_step_win<64, 32, MODE>;
-    if (tz == 32 && tx == 16) return k_step_win<32, 16, MODE>;
-    if (tz == 32 && tx == 32) return k_step_win<32, 32, MODE>;
+    if (tz == 64 && tx == 16) return loop ? k_step_win_loop<64, 16, MODE> : k_step_win<64, 16, MODE>;
+    if (tz == 64 && tx == 32) return loop ? k_step_win_loop<64, 32, MODE> : k_step_win<64, 32, MODE>;
+    if (tz == 32 && tx == 16) return loop ? k_step_win_loop<32, 16, MODE> : k_step_win<32, 16, MODE>;
+    if (tz == 32 && tx == 32) return loop ? k_step_win_loop<32, 32, MODE> : k_step_win<32, 32, MODE>;
     return nullptr;
 }
 
 hipError_t launch_step_windows(unsigned mode, int tz, int tx, int grid, const WinStepArgs& w, hipStream_t s,
                                hipEvent_t e0, hipEvent_t e1)
 {
-    WinKernel k = mode == MODE_NL_FWD ? pick_win_tile<MODE_NL_FWD>(tz, tx) : nullptr;
-    if (!k || grid < 8 || grid % 8 != 0 || w.ntotal < 1) return hipErrorInvalidValue;
+    if (grid < 8 || grid % 8 != 0 || w.ntotal < 1) return hipErrorInvalidValue;
+    // one workgroup per tile when the grid covers the list, else the persistent loop
+    const bool loop = (long)grid < (long)(w.ntotal + 7) / 8 * 8;
+    WinKernel k = mode == MODE_NL_FWD ? pick_win_tile<MODE_NL_FWD>(tz, tx, loop) : nullptr;
+    if (!k) return hipErrorInvalidValue;
     if (e0 && e1) hipExtLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, e0, e1, 0, w);
     else hipLaunchKernelGGL(k, dim3(grid), dim3(NT), 0, s, w);
     return hipGetLastError();

@@ -37,6 +37,8 @@ struct BatchConfig {
     int tz = 32, tx = 16;
     int spl = 0;        // windows per launch (0: all)
     int persist = 0;    // workgroups per CU (0: one workgroup per tile)
+    int lanes = 2;      // streams the launch groups alternate over
+    // the device tables depend on the tile, the grouping and the grid only
     bool operator==(const BatchConfig& o) const
     {
         return tz == o.tz && tx == o.tx && spl == o.spl && persist == o.persist;
@@ -74,6 +76,11 @@ struct aiso_batch {
     std::vector<hipGraphExec_t> graphs;
     double lastCaptureMs = 0;
     hipEvent_t evIn = nullptr, e0 = nullptr, e1 = nullptr;
+    // Lanes: launch group g runs on lane g % lanes; the lanes' kernels run
+    // concurrently, so one group's launch tail is filled by another's work.
+    std::vector<hipStream_t> lane;
+    hipEvent_t forkEv = nullptr;
+    std::vector<hipEvent_t> joinEv;
     aiso::PinnedBuf stage;
     float lastMs = 0;
 };
@@ -208,8 +215,45 @@ std::pair<int, int> group_range(const aiso_batch* b, int g)
     return {g * spl, std::min(n, (g + 1) * spl)};
 }
 
-// Fine steps [t0, t1) of group g, t counted from 0 (its = t / sub).
-void launch_steps(aiso_batch* b, int g, int t0, int t1, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
+void ensure_lanes(aiso_batch* b, int n)
+{
+    while ((int)b->lane.size() < n) {
+        hipStream_t st = nullptr;
+        HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));   // default priority
+        b->lane.push_back(st);
+        hipEvent_t e = nullptr;
+        HIPC(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->joinEv.push_back(e);
+    }
+    if (!b->forkEv) HIPC(hipEventCreateWithFlags(&b->forkEv, hipEventDisableTiming));
+}
+
+// Lanes used for G launch groups, and the stream of group g.
+int lanes_for(const aiso_batch* b, const BatchConfig& c, int G) { return std::max(1, std::min(c.lanes, G)); }
+hipStream_t group_stream(const aiso_batch* b, int g, int L) { return L > 1 ? b->lane[g % L] : b->stream; }
+
+// The lanes start after the work queued on the batch stream...
+void fork_lanes(aiso_batch* b, int L)
+{
+    if (L <= 1) return;
+    ensure_lanes(b, L);
+    HIPC(hipEventRecord(b->forkEv, b->stream));
+    for (int l = 0; l < L; ++l) HIPC(hipStreamWaitEvent(b->lane[l], b->forkEv, 0));
+}
+
+// ...and the batch stream continues after all of them.
+void join_lanes(aiso_batch* b, int L)
+{
+    if (L <= 1) return;
+    for (int l = 0; l < L; ++l) {
+        HIPC(hipEventRecord(b->joinEv[l], b->lane[l]));
+        HIPC(hipStreamWaitEvent(b->stream, b->joinEv[l], 0));
+    }
+}
+
+// Fine steps [t0, t1) of group g on stream st, t counted from 0 (its = t / sub).
+void launch_steps(aiso_batch* b, int g, int t0, int t1, hipStream_t st, hipEvent_t e0 = nullptr,
+                  hipEvent_t e1 = nullptr)
 {
     const FdParams& p = b->win[0]->p;
     const int h = p.h();
@@ -224,48 +268,50 @@ void launch_steps(aiso_batch* b, int g, int t0, int t1, hipEvent_t e0 = nullptr,
         w.a.fe = f[it2];
         w.a.ff = f[h + it2];
         hipEvent_t a0 = t == t0 ? e0 : nullptr, a1 = t == t1 - 1 ? e1 : nullptr;
-        if (a0 && !a1) HIPC(hipEventRecord(a0, b->stream));
-        HIPC(launch_step_windows(MODE_NL_FWD, b->prepared.tz, b->prepared.tx, grid, w, b->stream,
+        if (a0 && !a1) HIPC(hipEventRecord(a0, st));
+        HIPC(launch_step_windows(MODE_NL_FWD, b->prepared.tz, b->prepared.tx, grid, w, st,
                                  a0 && a1 ? a0 : nullptr, a0 && a1 ? a1 : nullptr));
-        if (a1 && !a0) HIPC(hipEventRecord(a1, b->stream));
+        if (a1 && !a0) HIPC(hipEventRecord(a1, st));
     }
 }
 
-// Zero the wavefield pairs and receiver traces of group g.
-void clear_group(aiso_batch* b, int g)
+// Zero the wavefield pairs and receiver traces of group g (on stream st).
+void clear_group(aiso_batch* b, int g, hipStream_t st)
 {
     auto r = group_range(b, g);
     const int last = r.second - 1;
     const size_t p0 = b->pairOff[r.first], p1 = b->pairOff[last] + 2 * b->win[last]->p.nzx();
     const size_t d0 = b->dregOff[r.first], d1 = b->dregOff[last] + (size_t)b->win[last]->rec.regPos.size() *
                                                                       b->win[last]->p.nts;
-    HIPC(launch_fill(b->pairs.as<float>() + p0, (long)(p1 - p0), b->stream));
-    HIPC(launch_fill(b->dreg.as<float>() + d0, (long)(d1 - d0), b->stream));
+    HIPC(launch_fill(b->pairs.as<float>() + p0, (long)(p1 - p0), st));
+    HIPC(launch_fill(b->dreg.as<float>() + d0, (long)(d1 - d0), st));
 }
 
-void run_group(aiso_batch* b, int g)
+// The whole time loop of group g on stream st: replayed from its graph,
+// captured on the first call (a graph runs on whichever stream launches it).
+void run_group(aiso_batch* b, int g, hipStream_t st)
 {
     const FdParams& p = b->win[0]->p;
     const int T = p.ntw - 1;
     if (!b->useGraph) {
-        clear_group(b, g);
-        launch_steps(b, g, 0, T);
+        clear_group(b, g, st);
+        launch_steps(b, g, 0, T, st);
         return;
     }
     if (!b->graphs[g]) {
         const auto h0 = std::chrono::steady_clock::now();
         hipGraph_t graph = nullptr;
-        HIPC(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         try {
-            clear_group(b, g);
-            launch_steps(b, g, 0, T);
+            clear_group(b, g, st);
+            launch_steps(b, g, 0, T, st);
         } catch (...) {
             hipGraph_t dummy = nullptr;
-            (void)hipStreamEndCapture(b->stream, &dummy);
+            (void)hipStreamEndCapture(st, &dummy);
             if (dummy) (void)hipGraphDestroy(dummy);
             throw;
         }
-        HIPC(hipStreamEndCapture(b->stream, &graph));
+        HIPC(hipStreamEndCapture(st, &graph));
         hipGraphExec_t exec = nullptr;
         const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
@@ -274,7 +320,7 @@ void run_group(aiso_batch* b, int g)
         b->lastCaptureMs +=
             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
     }
-    HIPC(hipGraphLaunch(b->graphs[g], b->stream));
+    HIPC(hipGraphLaunch(b->graphs[g], st));
 }
 
 // Source chain of a nonlinear shot on window W (F/src/nonlinearPropGpu.cpp:132-139).
@@ -327,17 +373,21 @@ void batch_nl_forward(aiso_batch* b, int add, const float* const* model, int nMo
     }
     HIPC(hipEventRecord(b->e0, s));
     const int G = (int)b->groupTiles.size();
+    const int L = lanes_for(b, b->cfg, G);
+    fork_lanes(b, L);
     for (int g = 0; g < G; ++g) {
-        run_group(b, g);
+        hipStream_t gs = group_stream(b, g, L);
+        run_group(b, g, gs);
         auto r = group_range(b, g);
         for (int w = r.first; w < r.second; ++w) {
             aiso_batch::Win& W = *b->win[w];
             float* dst = mem == AISO_MEM_DEVICE ? data[w] : b->outStage.as<float>() + b->outOff[w];
             const int addHere = mem == AISO_MEM_DEVICE ? add : 0;
             HIPC(launch_dev_forward(b->dreg.as<float>() + b->dregOff[w], dst, W.cornerReg.as<int>(),
-                                    W.weight.as<float>(), W.rec.nDev, W.rec.nb, W.p.nts, addHere, s));
+                                    W.weight.as<float>(), W.rec.nDev, W.rec.nb, W.p.nts, addHere, gs));
         }
     }
+    join_lanes(b, L);
     HIPC(hipEventRecord(b->e1, s));
     if (mem != AISO_MEM_DEVICE) {
         HIPC(hipStreamSynchronize(s));
@@ -356,18 +406,28 @@ void batch_nl_forward(aiso_batch* b, int add, const float* const* model, int nMo
     HIPC(hipEventElapsedTime(&b->lastMs, b->e0, b->e1));
 }
 
-// Time `steps` fine steps (after `warm` untimed ones) of launch group 0 under
-// configuration c; returns ms per interior point-update of that group.
-double time_config(aiso_batch* b, const BatchConfig& c, int warm, int steps)
+// Time `steps` fine steps of the first launch group of each lane under
+// configuration c, the lanes running concurrently; returns ms per interior
+// point-update of those groups.
+double time_config(aiso_batch* b, const BatchConfig& c, int steps)
 {
     prepare(b, c);
-    clear_group(b, 0);
-    launch_steps(b, 0, 0, warm);
-    launch_steps(b, 0, warm, warm + steps, b->e0, b->e1);
-    HIPC(hipStreamSynchronize(b->stream));
+    const int L = lanes_for(b, c, (int)b->groupTiles.size());
+    hipStream_t s = b->stream;
+    for (int g = 0; g < L; ++g) clear_group(b, g, s);
+    HIPC(hipEventRecord(b->e0, s));      // after the clears, before the lanes fork
+    fork_lanes(b, L);
+    double pts = 0;
+    for (int g = 0; g < L; ++g) {
+        launch_steps(b, g, 0, steps, group_stream(b, g, L));
+        pts += (double)b->groupPts[g] * steps;
+    }
+    join_lanes(b, L);
+    HIPC(hipEventRecord(b->e1, s));
+    HIPC(hipStreamSynchronize(s));
     float ms = 0;
     HIPC(hipEventElapsedTime(&ms, b->e0, b->e1));
-    return ms / ((double)b->groupPts[0] * steps);
+    return ms / pts;
 }
 
 }  // namespace
@@ -399,8 +459,10 @@ aiso_status aiso_batch_destroy(aiso_batch* b)
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
     drop_graphs(b);
-    for (hipEvent_t e : {b->evIn, b->e0, b->e1})
+    for (hipEvent_t e : {b->evIn, b->e0, b->e1, b->forkEv})
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : b->joinEv) (void)hipEventDestroy(e);
+    for (hipStream_t st : b->lane) (void)hipStreamDestroy(st);
     (void)hipStreamDestroy(b->stream);
     delete b;
     return AISO_OK;
@@ -484,14 +546,16 @@ aiso_status aiso_batch_set_windows(aiso_batch* b, int nWin, const int* nz, const
     });
 }
 
-aiso_status aiso_batch_set_config(aiso_batch* b, int tz, int tx, int windowsPerLaunch, int blocksPerCU, int graph)
+aiso_status aiso_batch_set_config(aiso_batch* b, int tz, int tx, int windowsPerLaunch, int blocksPerCU, int lanes,
+                                  int graph)
 {
     return guard_call([&] {
         if (!b) throw Error(AISO_ERR_PARAM, "null argument");
         if (tz != 32 && tz != 64) throw Error(AISO_ERR_PARAM, "tile tz must be 32 or 64");
         if (tx != 16 && tx != 32) throw Error(AISO_ERR_PARAM, "tile tx must be 16 or 32");
         if (windowsPerLaunch < 0 || blocksPerCU < 0) throw Error(AISO_ERR_PARAM, "negative configuration value");
-        b->cfg = BatchConfig{tz, tx, windowsPerLaunch, blocksPerCU};
+        if (lanes < 1 || lanes > 4) throw Error(AISO_ERR_PARAM, "lanes must be 1..4");
+        b->cfg = BatchConfig{tz, tx, windowsPerLaunch, blocksPerCU, lanes};
         if (graph != b->useGraph) {
             HIPC(hipSetDevice(b->device));
             HIPC(hipStreamSynchronize(b->stream));
@@ -503,13 +567,14 @@ aiso_status aiso_batch_set_config(aiso_batch* b, int tz, int tx, int windowsPerL
 }
 
 aiso_status aiso_batch_get_config(const aiso_batch* b, int* tz, int* tx, int* windowsPerLaunch, int* blocksPerCU,
-                                  int* graph)
+                                  int* lanes, int* graph)
 {
     if (!b) return AISO_ERR_PARAM;
     if (tz) *tz = b->cfg.tz;
     if (tx) *tx = b->cfg.tx;
     if (windowsPerLaunch) *windowsPerLaunch = b->cfg.spl;
     if (blocksPerCU) *blocksPerCU = b->cfg.persist;
+    if (lanes) *lanes = b->cfg.lanes;
     if (graph) *graph = b->useGraph;
     return AISO_OK;
 }
@@ -527,7 +592,6 @@ aiso_status aiso_batch_tune(aiso_batch* b, int steps, double* msPerPointUpdate)
         const int n = (int)b->win.size();
         const int T = b->win[0]->p.ntw - 1;
         steps = std::max(1, std::min(steps > 0 ? steps : 16, T));
-        const int warm = std::min(4, T - steps);
         std::vector<int> spls;
         for (int s : {8, 16, 32, 64})
             if (s < n) spls.push_back(s);
@@ -537,14 +601,15 @@ aiso_status aiso_batch_tune(aiso_batch* b, int steps, double* msPerPointUpdate)
         for (int tz : {32, 64})
             for (int tx : {16, 32})
                 for (int spl : spls)
-                    for (int persist : {0, 4, 8}) {
-                        const BatchConfig c{tz, tx, spl, persist};
-                        const double t = time_config(b, c, warm, steps);
-                        if (t < bestT) {
-                            bestT = t;
-                            best = c;
+                    for (int persist : {0, 4, 8})
+                        for (int lanes : {1, 2}) {
+                            const BatchConfig c{tz, tx, spl, persist, lanes};
+                            const double t = time_config(b, c, steps);
+                            if (t < bestT) {
+                                bestT = t;
+                                best = c;
+                            }
                         }
-                    }
         b->cfg = best;
         if (msPerPointUpdate) *msPerPointUpdate = bestT;
     });

@@ -106,6 +106,8 @@ aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
  *       (0 = aiso_ctx_set_batch's value, or automatic);
  *   "batchRec": group size of the adjoint receiver pass within a batch;
  *   "shotsPerBlock" 1/2: shots advanced by one workgroup of the step kernel;
+ *   "occupancy" 0/8: 8 runs the specialized step kernels compiled to the
+ *       register budget of 8 waves per SIMD (shotsPerBlock 1; default 8);
  *   "cacheBudgetMB": on-die cache budget the automatic batch size fills;
  *   "profileEvery" N >= 1: with profiling on, events on every N-th dispatch
  *       of each kernel kind (default 1: every dispatch);
@@ -333,17 +335,21 @@ aiso_status aiso_batch_destroy(aiso_batch* b);
 aiso_status aiso_batch_set_windows(aiso_batch* b, int nWin, const int* nz, const int* nx, const float* const* vel,
                                    aiso_device* const* sources, aiso_device* const* receivers, int mem);
 /* Launch configuration: tile tz in {32,64} x tx in {16,32}; windows per
- * launch (0 = all in one launch; more groups run one after the other);
+ * launch (0 = all in one launch; more groups make more launch groups);
  * workgroups per CU (0 = one workgroup per tile, else a persistent launch
- * of that many per CU, each looping over tiles); graph != 0: capture each
- * group's time loop in a HIP graph on first use and replay it later. */
-aiso_status aiso_batch_set_config(aiso_batch* b, int tz, int tx, int windowsPerLaunch, int blocksPerCU, int graph);
+ * of that many per CU, each looping over tiles); lanes 1..4: streams the
+ * launch groups alternate over, whose kernels run concurrently (default 2);
+ * graph != 0: capture each group's time loop in a HIP graph on first use
+ * and replay it later. */
+aiso_status aiso_batch_set_config(aiso_batch* b, int tz, int tx, int windowsPerLaunch, int blocksPerCU, int lanes,
+                                  int graph);
 aiso_status aiso_batch_get_config(const aiso_batch* b, int* tz, int* tx, int* windowsPerLaunch, int* blocksPerCU,
-                                  int* graph);
-/* Autotuner: times `steps` fine steps of the first launch group under each
- * candidate (tile shapes x windows per launch {8,16,32,64,all} x workgroups
- * per CU {0,4,8}) and keeps the fastest per point-update (the graph flag is
- * left as set).  msPerPointUpdate (optional) gets the winner's time. */
+                                  int* lanes, int* graph);
+/* Autotuner: times `steps` fine steps of the first launch group of each
+ * lane, the lanes concurrent, under each candidate (tile shapes x windows
+ * per launch {8,16,32,64,all} x workgroups per CU {0,4,8} x lanes {1,2})
+ * and keeps the fastest per point-update (the graph flag is left as set).
+ * msPerPointUpdate (optional) gets the winner's time. */
 aiso_status aiso_batch_tune(aiso_batch* b, int steps, double* msPerPointUpdate);
 /* Nonlinear forward of every window: model[k] [nSrcDev][nts] (nModel 1: one
  * signal for all windows, or one per window); data[w] [nRecDev_w][nts]. */

@@ -65,15 +65,16 @@ def small():
     return A, w, model, ref
 
 
-@pytest.mark.parametrize("tz,tx,spl,persist,graph", [
-    (32, 16, 0, 0, True), (32, 32, 3, 1, True), (64, 16, 2, 0, False), (64, 32, 0, 2, True),
-    (32, 16, 4, 1, False),
+@pytest.mark.parametrize("tz,tx,spl,persist,graph,lanes", [
+    (32, 16, 0, 0, True, 1), (32, 32, 3, 1, True, 2), (64, 16, 2, 0, False, 2), (64, 32, 0, 2, True, 1),
+    (32, 16, 4, 1, False, 1), (32, 32, 1, 0, True, 3), (32, 16, 2, 0, True, 2),
 ])
-def test_windows_match_plain_launches(small, tz, tx, spl, persist, graph):
+def test_windows_match_plain_launches(small, tz, tx, spl, persist, graph, lanes):
     A, w, model, ref = small
-    print(f"tile {tz}x{tx}, windows per launch {spl}, workgroups per CU {persist}, graph {graph}", flush=True)
+    print(f"tile {tz}x{tx}, windows per launch {spl}, workgroups per CU {persist}, graph {graph}, lanes {lanes}",
+          flush=True)
     b, _devs = _batch(A, w)
-    b.set_config(tz, tx, spl, persist, graph)
+    b.set_config(tz, tx, spl, persist, graph, lanes)
     for call in range(2):                                  # a graph is captured, then replayed
         out = [np.full((r.shape[0], r.shape[1]), np.nan, np.float32) for r in ref]
         b.nl_forward([model], out)
