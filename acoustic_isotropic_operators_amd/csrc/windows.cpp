@@ -406,19 +406,20 @@ void batch_nl_forward(aiso_batch* b, int add, const float* const* model, int nMo
     HIPC(hipEventElapsedTime(&b->lastMs, b->e0, b->e1));
 }
 
-// Time `steps` fine steps of the first launch group of each lane under
-// configuration c, the lanes running concurrently; returns ms per interior
-// point-update of those groups.
+// Time `steps` fine steps of every launch group under configuration c, the
+// groups on their lanes as a call runs them; returns ms per interior
+// point-update.
 double time_config(aiso_batch* b, const BatchConfig& c, int steps)
 {
     prepare(b, c);
-    const int L = lanes_for(b, c, (int)b->groupTiles.size());
+    const int G = (int)b->groupTiles.size();
+    const int L = lanes_for(b, c, G);
     hipStream_t s = b->stream;
-    for (int g = 0; g < L; ++g) clear_group(b, g, s);
+    for (int g = 0; g < G; ++g) clear_group(b, g, s);
     HIPC(hipEventRecord(b->e0, s));      // after the clears, before the lanes fork
     fork_lanes(b, L);
     double pts = 0;
-    for (int g = 0; g < L; ++g) {
+    for (int g = 0; g < G; ++g) {
         launch_steps(b, g, 0, steps, group_stream(b, g, L));
         pts += (double)b->groupPts[g] * steps;
     }
@@ -596,22 +597,30 @@ aiso_status aiso_batch_tune(aiso_batch* b, int steps, double* msPerPointUpdate)
         for (int s : {8, 16, 32, 64})
             if (s < n) spls.push_back(s);
         spls.push_back(0);
-        BatchConfig best = b->cfg;
-        double bestT = 1e300;
+        // Screen every candidate, then re-time the four fastest three times
+        // over 3 x steps and keep the best of those times.
+        std::vector<std::pair<double, BatchConfig>> cand;
         for (int tz : {32, 64})
             for (int tx : {16, 32})
                 for (int spl : spls)
                     for (int persist : {0, 4, 8})
                         for (int lanes : {1, 2}) {
                             const BatchConfig c{tz, tx, spl, persist, lanes};
-                            const double t = time_config(b, c, steps);
-                            if (t < bestT) {
-                                bestT = t;
-                                best = c;
-                            }
+                            cand.emplace_back(time_config(b, c, steps), c);
                         }
-        b->cfg = best;
-        if (msPerPointUpdate) *msPerPointUpdate = bestT;
+        std::stable_sort(cand.begin(), cand.end(),
+                         [](const auto& x, const auto& y) { return x.first < y.first; });
+        cand.resize(std::min<size_t>(cand.size(), 4));
+        const int longSteps = std::min(3 * steps, T);
+        for (auto& cd : cand) {
+            double t = 1e300;
+            for (int r = 0; r < 3; ++r) t = std::min(t, time_config(b, cd.second, longSteps));
+            cd.first = t;
+        }
+        const auto best = *std::min_element(cand.begin(), cand.end(),
+                                            [](const auto& x, const auto& y) { return x.first < y.first; });
+        b->cfg = best.second;
+        if (msPerPointUpdate) *msPerPointUpdate = best.first;
     });
 }
 

@@ -345,11 +345,11 @@ aiso_status aiso_batch_set_config(aiso_batch* b, int tz, int tx, int windowsPerL
                                   int graph);
 aiso_status aiso_batch_get_config(const aiso_batch* b, int* tz, int* tx, int* windowsPerLaunch, int* blocksPerCU,
                                   int* lanes, int* graph);
-/* Autotuner: times `steps` fine steps of the first launch group of each
- * lane, the lanes concurrent, under each candidate (tile shapes x windows
- * per launch {8,16,32,64,all} x workgroups per CU {0,4,8} x lanes {1,2})
- * and keeps the fastest per point-update (the graph flag is left as set).
- * msPerPointUpdate (optional) gets the winner's time. */
+/* Autotuner: times `steps` fine steps of every launch group, on its lanes,
+ * under each candidate (tile shapes x windows per launch {8,16,32,64,all} x
+ * workgroups per CU {0,4,8} x lanes {1,2}), re-times the four fastest three
+ * times over 3 x steps, and keeps the fastest per point-update (the graph
+ * flag is left as set).  msPerPointUpdate (optional) gets its time. */
 aiso_status aiso_batch_tune(aiso_batch* b, int steps, double* msPerPointUpdate);
 /* Nonlinear forward of every window: model[k] [nSrcDev][nts] (nModel 1: one
  * signal for all windows, or one per window); data[w] [nRecDev_w][nts]. */
