@@ -754,7 +754,7 @@ aiso_status aiso_ctx_set_option(aiso_ctx* c, const char* key, int value)
             c->spb = value;
         }
         else if (k == "occupancy") {
-            if (value != 0 && value != 8) throw Error(AISO_ERR_PARAM, "occupancy must be 0 or 8");
+            if (value != 0 && value != 6 && value != 8) throw Error(AISO_ERR_PARAM, "occupancy must be 0, 6 or 8");
             c->occ = value;
         }
         else if (k == "cacheBudgetMB") c->cacheBudget = (long)std::max(1, value) << 20;

@@ -162,11 +162,9 @@ struct aiso_ctx {
     int batchFwd = 0, batchAdj = 0;     // per-direction overrides of batchReq
     int batchRec = 0;                   // receiver-pass group size within an adjoint batch
     int spb = 1;                        // shots per workgroup of the step kernel
-    // 8: the specialized step kernels compiled to the register budget of 8
-    // waves per SIMD (k_step_o8); 0: the compiler's own budget.  The adjoint
-    // source pass runs 3 % faster at 8 (76 -> 64 VGPRs); the receiver pass
-    // with imaging keeps the compiler's budget either way.
-    int occ = 8;
+    // 6 / 8: the specialized step kernels compiled to the register budget
+    // of that many waves per SIMD (k_step_w); 0: the compiler's own budget.
+    int occ = 6;
     long cacheBudget = 192l << 20;      // bytes of hot slices per launch (Infinity Cache is 256 MB)
     std::map<std::tuple<long, int, int>, std::unique_ptr<aiso::DevGeom>> geoms;
     // profiling of the step kernels

@@ -439,12 +439,12 @@ __global__ __launch_bounds__(NT) void k_step(const StepArgs a)
     step_tile<TZ, TX, MODE, SPB>(a);
 }
 
-// The same step with the register budget of 8 waves per SIMD (64 VGPRs):
-// at most 8 workgroups of 256 threads per CU either way, but more of them
-// resident to cover the latency of the tile staging (the adjoint kernels
-// otherwise compile to 76 VGPRs, 6 waves).
-template <int TZ, int TX, unsigned MODE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_step_o8(const StepArgs a)
+// The same step with the register budget of W waves per SIMD (W = 6: 80
+// VGPRs, W = 8: 64): more workgroups resident to cover the latency of the
+// tile staging (the adjoint source pass otherwise compiles to 76 VGPRs, 6
+// waves, and the receiver pass with its early imaging operands to 84, 5).
+template <int TZ, int TX, unsigned MODE, int W>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_step_w(const StepArgs a)
 {
     step_tile<TZ, TX, MODE, 1>(a);
 }
@@ -2178,24 +2178,29 @@ static StepKernel pick_tile_spb(int tz, int tx)
     return nullptr;
 }
 
-template <unsigned MODE>
-static StepKernel pick_tile_o8(int tz, int tx)
+template <unsigned MODE, int W>
+static StepKernel pick_tile_w(int tz, int tx)
 {
-    if (tz == 64 && tx == 16) return k_step_o8<64, 16, MODE>;
-    if (tz == 64 && tx == 32) return k_step_o8<64, 32, MODE>;
-    if (tz == 32 && tx == 16) return k_step_o8<32, 16, MODE>;
-    if (tz == 32 && tx == 32) return k_step_o8<32, 32, MODE>;
+    if (tz == 64 && tx == 16) return k_step_w<64, 16, MODE, W>;
+    if (tz == 64 && tx == 32) return k_step_w<64, 32, MODE, W>;
+    if (tz == 32 && tx == 16) return k_step_w<32, 16, MODE, W>;
+    if (tz == 32 && tx == 32) return k_step_w<32, 32, MODE, W>;
     return nullptr;
 }
 
-// The adjoint receiver pass with imaging keeps the compiler's register
-// budget: its early imaging operands spill at 64 VGPRs.
+// occ: 0 (the compiler's register budget), 6 or 8 waves per SIMD.  The
+// adjoint receiver pass with imaging runs at 6 when 8 is asked for: its
+// early imaging operands spill at 64 VGPRs.
 template <unsigned MODE>
 static StepKernel pick_tile(int tz, int tx, int spb, int occ)
 {
     if (spb == 2) return pick_tile_spb<MODE, 2>(tz, tx);
-    if constexpr (MODE != MODE_REC_WIN)
-        if (occ == 8) return pick_tile_o8<MODE>(tz, tx);
+    if (occ == 6) return pick_tile_w<MODE, 6>(tz, tx);
+    if constexpr (MODE == MODE_REC_WIN) {
+        if (occ == 8) return pick_tile_w<MODE, 6>(tz, tx);
+    } else {
+        if (occ == 8) return pick_tile_w<MODE, 8>(tz, tx);
+    }
     return pick_tile_spb<MODE, 1>(tz, tx);
 }
 
@@ -2228,7 +2233,7 @@ static hipError_t launch_k(StepKernel k, int grid, const StepArgs& b, hipStream_
 hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a, hipStream_t s, hipEvent_t e0,
                        hipEvent_t e1, int occ)
 {
-    if (occ != 0 && occ != 8) return hipErrorInvalidValue;
+    if (occ != 0 && occ != 6 && occ != 8) return hipErrorInvalidValue;
     StepKernel k = pick_step(mode, tz, tx, spb, occ);
     if (!k) return hipErrorInvalidValue;
     if ((mode & M_WIN) && a.winOut == WIN_SLICE) return hipErrorInvalidValue;   // extended-Born capture: generic only

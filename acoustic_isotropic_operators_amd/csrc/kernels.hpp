@@ -201,8 +201,8 @@ constexpr unsigned MODE_BORN_SCAT_CAP = M_SS_BORN | M_CAPTURE | M_CAP_ALT;
 
 // spb: shots advanced per workgroup (1 or 2), sharing the tile's v2dt2 and sponge.
 // e0/e1 (optional): events timestamped by the dispatch itself.
-// occ: 0, or 8 for the kernels compiled to the register budget of 8 waves
-// per SIMD (spb 1 only).
+// occ: 0, or 6 / 8 for the kernels compiled to the register budget of that
+// many waves per SIMD (spb 1 only).
 hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a, hipStream_t s,
                        hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, int occ = 0);
 // The unspecialized step kernel: the extended-Born, tomographic and WEMVA

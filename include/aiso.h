@@ -106,8 +106,9 @@ aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
  *       (0 = aiso_ctx_set_batch's value, or automatic);
  *   "batchRec": group size of the adjoint receiver pass within a batch;
  *   "shotsPerBlock" 1/2: shots advanced by one workgroup of the step kernel;
- *   "occupancy" 0/8: 8 runs the specialized step kernels compiled to the
- *       register budget of 8 waves per SIMD (shotsPerBlock 1; default 8);
+ *   "occupancy" 0/6/8: the specialized step kernels compiled to the
+ *       register budget of that many waves per SIMD, 0 the compiler's own
+ *       (shotsPerBlock 1; default 6);
  *   "cacheBudgetMB": on-die cache budget the automatic batch size fills;
  *   "profileEvery" N >= 1: with profiling on, events on every N-th dispatch
  *       of each kernel kind (default 1: every dispatch);
