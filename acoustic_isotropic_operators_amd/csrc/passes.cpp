@@ -507,7 +507,7 @@ void Call::begin_batch(int first, int count)
     a = base;
     a.nshots = nb;
     a.gridLogical = a.ntiles * nb;
-    c->curLanes = is_ext(q.op) || tomo || pair2 || adj2 ? 1 : std::max(1, std::min(c->lanes, nb));
+    c->curLanes = tomo || pair2 || adj2 ? 1 : std::max(1, std::min(c->lanes, nb));
     if (c->curLanes > 1) ensure_lanes(c);
 }
 
@@ -1131,11 +1131,15 @@ void Call::pass_forward()
                 e.ssOff[j] = ss_slot(nextSS + j);
                 e.wOff[j] = (long)(nextSS + j) * nzx;
             }
-            if (tlag) {
-                time_lags(e, 1);
-                HIPC(launch_time_ss_batch(e, nb, s));
-            } else {
-                HIPC(launch_ext_ss_batch(e, nb, s));
+            if (tlag) time_lags(e, 1);
+            for (int l = 0; l < c->curLanes; ++l) {   // each lane's shots, on its stream
+                int k0, cnt;
+                lane_range(nb, c->curLanes, l, k0, cnt);
+                if (cnt == 0) continue;
+                ExtBatchArgs el = e;
+                el.slots = dSlots + k0;
+                if (tlag) HIPC(launch_time_ss_batch(el, cnt, lane_stream(c, l)));
+                else HIPC(launch_ext_ss_batch(el, cnt, lane_stream(c, l)));
             }
             nextSS += e.nk;
         }
@@ -1205,6 +1209,7 @@ void Call::pass_adjoint()
         // Completed receiver-wavefield samples awaiting the offset imaging,
         // newest last; imaged extK at a time.
         std::vector<int> pend;
+        int lanesNow = c->curLanes;
         auto ext_img_flush = [&] {
             if (pend.empty()) return;
             ExtBatchArgs e{};
@@ -1216,11 +1221,18 @@ void Call::pass_adjoint()
                 e.ssOff[j] = ss_slot(pend[j]);
                 e.wOff[j] = (long)pend[j] * nzx;
             }
-            if (tlag) {
-                time_lags(e, 1);
-                HIPC(launch_time_img_batch(e, nbr, s));
-            } else {
-                HIPC(launch_ext_img_batch(e, nbr, s));
+            if (tlag) time_lags(e, 1);
+            // Inside the time loop each lane images its own shots on its
+            // stream; after the lanes join (lanesNow 1) the whole group on s.
+            for (int l = 0; l < lanesNow; ++l) {
+                int k0, cnt;
+                lane_range(nbr, lanesNow, l, k0, cnt);
+                if (cnt == 0) continue;
+                ExtBatchArgs el = e;
+                el.slots = sl + k0;
+                hipStream_t ls = lanesNow > 1 ? lane_stream(c, l) : s;
+                if (tlag) HIPC(launch_time_img_batch(el, cnt, ls));
+                else HIPC(launch_ext_img_batch(el, cnt, ls));
             }
             pend.clear();
         };
@@ -1259,6 +1271,7 @@ void Call::pass_adjoint()
             std::swap(L, R);
         }
         join_lanes(c);
+        lanesNow = 1;
         pass_timer_end(c, passT, kind, double(p.interior()) * nbr * (p.ntw - 1), p.ntw - 1);
         if (q.op == Op::BornAdj) {
             if (win) {   // img += W[0] * (q(1)*f[1] + q(0)*f[0]), then the scaling

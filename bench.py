@@ -347,7 +347,12 @@ def bench_ext(a, torch, A, born_lsrtm, nExt=21, time_lags=False):
            "config": {"workload": f"born{'time' if time_lags else 'ext'}{nExt}_{w.nShot}x500x1200", "nExt": nExt,
                       "shots": w.nShot,
                       "shots_per_launch_adj": ctx.info()["batch"]},
-           "step_kernels_avg_us_sampled": {n: 1e3 * v["ms"] / v["launches"] for n, v in stats.items() if v["launches"]},
+           "lanes": ctx.get_option("lanes"),
+           "step_kernels_us_per_fine_step": {n: 1e3 * v["ms"] / v["launches"] for n, v in stats.items()
+                                             if v["launches"]},
+           "step_kernel_timing": ("pass-timed: events around each pass, before its lanes fork and after they join, "
+                                  "per fine step of all the batch's shots" if ctx.get_option("lanes") > 1 else
+                                  "events on sampled dispatches (upper bounds)"),
            "finite": bool(torch.isfinite(image).all().item())}
     print(json.dumps(out), flush=True)
 
@@ -403,7 +408,8 @@ def bench_tomo(a, torch, A, born_lsrtm, nExt=21, wemva=False, time_lags=False):
                       "nExt": nExt,
                       "shots": w.nShot,
                       "shots_per_launch": ctx.info()["batch"]},
-           "step_kernels_avg_us_sampled": {n: 1e3 * v["ms"] / v["launches"] for n, v in stats.items() if v["launches"]},
+           "step_kernels_us_per_launch_sampled": {n: 1e3 * v["ms"] / v["launches"] for n, v in stats.items()
+                                                  if v["launches"]},
            "finite": bool(torch.isfinite(grad).all().item()) and bool(torch.isfinite(data).all().item())}
     print(json.dumps(out), flush=True)
 
