@@ -507,7 +507,7 @@ void Call::begin_batch(int first, int count)
     a = base;
     a.nshots = nb;
     a.gridLogical = a.ntiles * nb;
-    c->curLanes = tomo || pair2 || adj2 ? 1 : std::max(1, std::min(c->lanes, nb));
+    c->curLanes = pair2 || adj2 ? 1 : std::max(1, std::min(c->lanes, nb));
     if (c->curLanes > 1) ensure_lanes(c);
 }
 
@@ -897,6 +897,28 @@ void Call::passes_tomo()
     // W <- its coarse second time derivative, in place
     const float d2 = p.dts * p.dts;
     HIPC(launch_dt2_coarse(dSlots, nb, p.nts, (float)(1.0 / (double)d2), (float)(-2.0 / (double)d2), p.nz, p.nx, s));
+    // Lanes: inside a pass the batch's shots run on the context's lanes;
+    // every launch in the time loop goes to each lane with its own shots.
+    bool forked = false;
+    auto lanes = [&](const std::function<void(int, int, hipStream_t)>& fn) {
+        const int L = forked ? c->curLanes : 1;
+        for (int l = 0; l < L; ++l) {
+            int k0, cnt;
+            lane_range(nb, L, l, k0, cnt);
+            if (cnt > 0) fn(k0, cnt, L > 1 ? lane_stream(c, l) : s);
+        }
+    };
+    hipEvent_t passT = nullptr;
+    auto fork = [&] {
+        passT = pass_timer_begin(c);
+        fork_lanes(c);
+        forked = c->curLanes > 1;
+    };
+    auto join = [&] {
+        join_lanes(c);
+        forked = false;
+        pass_timer_end(c, passT, AISO_K_TOMO, double(p.interior()) * nb * (p.ntw - 1), p.ntw - 1);
+    };
     enum { SS_EXT, SS_MUL, SS_ADJ };   // ring producers
     auto produce = [&](int kind, const ShotSlot* from, const float* scale, int k0, int n) {
         ExtBatchArgs e{};
@@ -909,16 +931,15 @@ void Call::passes_tomo()
             e.ssOff[j] = ss_slot(k0 + j);
             e.wOff[j] = (long)(k0 + j) * nzx;
         }
-        if (kind == SS_MUL) {
-            HIPC(launch_mul_batch(e, nb, s));
-        } else if (tlag) {   // imagingTimeFwdGpu / imagingTimeTomoAdjGpu
-            time_lags(e, kind == SS_EXT ? 1 : -1);
-            HIPC(launch_time_ss_batch(e, nb, s));
-        } else if (kind == SS_EXT) {
-            HIPC(launch_ext_ss_batch(e, nb, s));
-        } else {
-            HIPC(launch_tomo_adj_ss(e, nb, s));
-        }
+        if (kind != SS_MUL && tlag) time_lags(e, kind == SS_EXT ? 1 : -1);
+        lanes([&](int s0, int cnt, hipStream_t st) {
+            ExtBatchArgs el = e;
+            el.slots = from + s0;
+            if (kind == SS_MUL) HIPC(launch_mul_batch(el, cnt, st));
+            else if (tlag) HIPC(launch_time_ss_batch(el, cnt, st));   // imagingTimeFwdGpu / imagingTimeTomoAdjGpu
+            else if (kind == SS_EXT) HIPC(launch_ext_ss_batch(el, cnt, st));
+            else HIPC(launch_tomo_adj_ss(el, cnt, st));
+        });
     };
     // Forward in time.  ssKind >= 0: inject ring samples made from `from`'s
     // W (scaled by `scale`); capture: 1 = into the launch slots' W at the
@@ -933,6 +954,7 @@ void Call::passes_tomo()
         x.win = 0;
         int next = 0;
         auto off = [&](int k) { return capture == 1 ? (long)k * nzx : cap_off(k); };
+        fork();
         for (int its = 0; its < p.nts - 1; ++its) {
             while (ssKind >= 0 && next <= its + 1) {
                 const int n = std::min(extK, p.nts - next);
@@ -969,6 +991,7 @@ void Call::passes_tomo()
             if (done) done(its);
         }
         if (done) done(p.nts - 1);
+        join();
     };
     // Backward in time on the adjoint stepper from ring samples made from
     // `from`'s W, injected with the weights of fine index it2 + 1, captured
@@ -982,6 +1005,7 @@ void Call::passes_tomo()
         y.cur = 1;
         y.win = 0;
         int next = p.nts - 1;   // highest sample not yet in the ring
+        fork();
         for (int its = p.nts - 2; its >= 0; --its) {
             while (next >= its) {
                 const int n = std::min(extK, next + 1);
@@ -1009,10 +1033,15 @@ void Call::passes_tomo()
             done(its + 1);
         }
         done(0);
+        join();
     };
     // img += X[k] * captured sample k, one coarse sample at a time
     auto image_with = [&](const ShotSlot* X) {
-        return [&, X](int k) { HIPC(launch_img_acc(X, (long)k * nzx, dSlotsC, cap_off(k), nb, p.nz, p.nx, s)); };
+        return [&, X](int k) {
+            lanes([&](int s0, int cnt, hipStream_t st) {
+                HIPC(launch_img_acc(X + s0, (long)k * nzx, dSlotsC + s0, cap_off(k), cnt, p.nz, p.nx, st));
+            });
+        };
     };
     if (dataInput) {   // receiver wavefield R into P (computeTomoRecWfld, :1481-1515)
         HIPC(launch_fill(pairs, (long)nb * 2 * nzx, s));
@@ -1020,6 +1049,7 @@ void Call::passes_tomo()
         x.slots = dSlotsQ;
         x.cur = 1;
         x.win = 0;
+        fork();
         for (int its = p.nts - 2; its >= 0; --its)
             for (int it2 = p.sub - 1; it2 >= 0; --it2) {
                 x.its = its;
@@ -1038,6 +1068,7 @@ void Call::passes_tomo()
                 ring(MODE_TOMO_REC, x);
                 x.cur ^= 1;
             }
+        join();
     }
     if (img) HIPC(launch_fill(img, (long)nb * nImg * nzx, s));
     if (q.op == Op::TomoFwd) {
@@ -1080,12 +1111,14 @@ void Call::passes_tomo()
                 e.ssOff[j] = rSide ? (long)k * nzx : cap_off(k);   // receiver side: R[k] / sample k
                 e.wOff[j] = rSide ? cap_off(k) : (long)k * nzx;    // source side: sample k / W''[k] (R[k])
             }
-            if (tlag) {   // imagingWemvaTimeAdjGpu / imagingTimeAdjGpu
-                time_lags(e, wemvaFwd ? -1 : 1);
-                HIPC(launch_time_img_batch(e, nb, s));
-            } else {
-                HIPC(launch_ext_img_batch(e, nb, s, wemvaFwd));
-            }
+            if (tlag) time_lags(e, wemvaFwd ? -1 : 1);   // imagingWemvaTimeAdjGpu / imagingTimeAdjGpu
+            const ShotSlot* base0 = e.slots;
+            lanes([&](int s0, int cnt, hipStream_t st) {
+                ExtBatchArgs el = e;
+                el.slots = base0 + s0;
+                if (tlag) HIPC(launch_time_img_batch(el, cnt, st));
+                else HIPC(launch_ext_img_batch(el, cnt, st, wemvaFwd));
+            });
             pend.clear();
         };
         if (c->leg1) {   // forward from m * W'', imaged against R (batches of extK samples)
