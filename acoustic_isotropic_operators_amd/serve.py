@@ -79,7 +79,7 @@ class _Slot:
 
 
 class _Model:
-    def __init__(self, params, vel, receivers, wavelet, nExt, device):
+    def __init__(self, params, vel, receivers, wavelet, nExt, device, options=None):
         p = dict(params, nExt=nExt, extension=A.EXT_OFFSET)
         self.params = p
         self.device = device
@@ -87,6 +87,7 @@ class _Model:
         self.rec = [A.Device.grid(*receivers, p["nz"], p["nx"], p["nts"])]
         self.wavelet = np.ascontiguousarray(wavelet, np.float32).reshape(1, -1)
         self.nExt = nExt
+        self.options = dict(options or {})
         self.sources = {}
         first = _Slot()
         first.ctx = self.new_context()     # the velocity checks run when the model is added
@@ -105,6 +106,8 @@ class _Model:
         ctx = A.Context(self.params, self.device)
         ctx.set_velocity(self.vel)
         ctx.set_option("imagePerShot", 1)
+        for k, v in self.options.items():
+            ctx.set_option(k, v)
         return ctx
 
     def source(self, iz, ix):
@@ -122,13 +125,14 @@ class _Model:
 
 
 class Server:
-    def __init__(self, max_models=4, hbm_budget_gb=200.0, max_shots=32, nExt=21, device=0, slots=4):
+    def __init__(self, max_models=4, hbm_budget_gb=200.0, max_shots=32, nExt=21, device=0, slots=4, options=None):
         self.max_models = max_models
         self.budget = hbm_budget_gb * 1e9
         self.max_shots = max_shots
         self.nExt = nExt
         self.device = device
         self.slots = slots
+        self.options = dict(options or {})   # engine options (aiso_ctx_set_option) of every context
         self.models = collections.OrderedDict()
         self.queue = collections.deque()
         self.cv = threading.Condition()
@@ -158,7 +162,7 @@ class Server:
                 old = self.models.pop(victim)
                 self.resident_bytes -= sum(sl.reserved for sl in old.slots)
                 old.close()
-        m = _Model(params, vel, receivers, wavelet, self.nExt, self.device)
+        m = _Model(params, vel, receivers, wavelet, self.nExt, self.device, self.options)
         with self.cv:
             self.models[model_id] = m
 

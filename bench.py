@@ -597,8 +597,10 @@ def bench_serve(a):
             reqs.append(one(srv, m[0], kind, int(rng.integers(x0, x1)), rng, 0.0 if burst else float(t)))
         return reqs, 0.0 if burst else float(arrivals[-1])
 
+    opts = {k: int(v) for k, v in (kv.split("=") for kv in a.opt)}
+
     def run(slots, max_shots, threads, burst):
-        srv = Server(max_models=4, max_shots=max_shots, slots=slots)
+        srv = Server(max_models=4, max_shots=max_shots, slots=slots, options=opts)
         for mid, params, vel, rec, wav, _ in models:
             srv.add_model(mid, params, vel, rec, wav)
         rng = np.random.default_rng(7)
@@ -640,7 +642,7 @@ def bench_serve(a):
            "dtype": "f32", "data": "synthetic",
            "config": {"workload": "serve_mix_4models", "requests": len(reqs), "arrival_rate_per_s": a.rate,
                       "requests_by_kind": by_kind, "slots": a.slots, "max_shots_per_call": 32,
-                      "hbm_budget_gb": 200},
+                      "hbm_budget_gb": 200, "engine_options": opts},
            "latency_ms": lat_b["latency_ms"],
            "capacity_continuous_batching": cap_b,
            "capacity_plain_one_request_at_a_time": cap_p,
