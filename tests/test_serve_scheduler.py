@@ -17,8 +17,9 @@ from acoustic_isotropic_operators_amd import serve
 class _FakeModel:
     closed = []
 
-    def __init__(self, params, vel, receivers, wavelet, nExt, device):
+    def __init__(self, params, vel, receivers, wavelet, nExt, device, options=None):
         self.params = params
+        self.options = options
         self.bytes_per_shot = {k: params.get("bytes", 1) for k in serve.KINDS}
         self.runs = []
         self.active = 0
@@ -193,6 +194,13 @@ def test_errors_stay_with_their_requests(monkeypatch):
     finally:
         srv.stop()
     assert "engine refused" in bad.error and good.error is None and good.result is not None
+
+
+def test_engine_options_reach_every_model(monkeypatch):
+    monkeypatch.setattr(serve, "_Model", _FakeModel)
+    srv = serve.Server(max_models=2, options={"lanes": 1, "occupancy": 8})
+    srv.add_model("a", {"name": "a"}, None, None, None)
+    assert srv.models["a"].options == {"lanes": 1, "occupancy": 8}
 
 
 def test_stream_latency_accounting(server):
