@@ -523,10 +523,28 @@ void run_nl_grad(aiso_ctx* c, const float* model, int nShotSig, const float* dat
         a.gridLogical = a.ntiles * nb;
         ga.gridLogical = a.gridLogical;
         auto ckSlice = [&](int k, int j, int which) { return ck + (((size_t)k * nCk + j) * 2 + which) * nzx; };
+        // Lanes: the batch's shots split over the context's streams; every
+        // per-shot copy and every launch goes to the lane of its shots.
+        auto lane_of = [&](int k) {
+            for (int l = 0; l < c->curLanes; ++l) {
+                int k0, cnt;
+                lane_range(nb, c->curLanes, l, k0, cnt);
+                if (k >= k0 && k < k0 + cnt) return lane_stream(c, l);
+            }
+            return s;
+        };
         auto save = [&](int j, int cur) {
             for (int k = 0; k < nb; ++k) {
-                HIPC(hipMemcpyAsync(ckSlice(k, j, 0), fs[k].p[cur], nzx * 4, hipMemcpyDeviceToDevice, s));
-                HIPC(hipMemcpyAsync(ckSlice(k, j, 1), fs[k].p[cur ^ 1], nzx * 4, hipMemcpyDeviceToDevice, s));
+                hipStream_t ls = lane_of(k);
+                HIPC(hipMemcpyAsync(ckSlice(k, j, 0), fs[k].p[cur], nzx * 4, hipMemcpyDeviceToDevice, ls));
+                HIPC(hipMemcpyAsync(ckSlice(k, j, 1), fs[k].p[cur ^ 1], nzx * 4, hipMemcpyDeviceToDevice, ls));
+            }
+        };
+        auto ring_lanes = [&](unsigned mode, const StepArgs& x, int mr) {
+            for (int l = 0; l < c->curLanes; ++l) {
+                int k0, cnt;
+                lane_range(nb, c->curLanes, l, k0, cnt);
+                if (cnt > 0) HIPC(launch_ring(mode, c->curLanes > 1 ? lane_args(x, k0, cnt) : x, mr, lane_stream(c, l)));
             }
         };
         // forward: one fine step = MODE_SRC without capture (step, injection, sponge)
@@ -543,8 +561,11 @@ void run_nl_grad(aiso_ctx* c, const float* model, int nShotSig, const float* dat
             x.assignB = 0;
             x.skipB = 1;
             launch_timed(c, MODE_SRC, AISO_K_SRC, x);
-            if (maxSrcRing > 0) HIPC(launch_ring(M_INJ_SIG, x, maxSrcRing, s));
+            if (maxSrcRing > 0) ring_lanes(M_INJ_SIG, x, maxSrcRing);
         };
+        c->curLanes = std::max(1, std::min(c->lanes, nb));
+        if (c->curLanes > 1) ensure_lanes(c);
+        fork_lanes(c);
         int cur = 1;
         save(0, cur);
         for (int t = 1; t < T; ++t) {
@@ -560,9 +581,10 @@ void run_nl_grad(aiso_ctx* c, const float* model, int nShotSig, const float* dat
         for (int j = nCk - 1; j >= 0; --j) {
             const int t0 = j * K, t1 = std::min(t0 + K, T);
             for (int k = 0; k < nb; ++k) {
-                HIPC(hipMemcpyAsync(fs[k].p[1], ckSlice(k, j, 0), nzx * 4, hipMemcpyDeviceToDevice, s));
-                HIPC(hipMemcpyAsync(fs[k].p[0], ckSlice(k, j, 1), nzx * 4, hipMemcpyDeviceToDevice, s));
-                HIPC(hipMemcpyAsync(fs[k].W, ckSlice(k, j, 0), nzx * 4, hipMemcpyDeviceToDevice, s));
+                hipStream_t ls = lane_of(k);
+                HIPC(hipMemcpyAsync(fs[k].p[1], ckSlice(k, j, 0), nzx * 4, hipMemcpyDeviceToDevice, ls));
+                HIPC(hipMemcpyAsync(fs[k].p[0], ckSlice(k, j, 1), nzx * 4, hipMemcpyDeviceToDevice, ls));
+                HIPC(hipMemcpyAsync(fs[k].W, ckSlice(k, j, 0), nzx * 4, hipMemcpyDeviceToDevice, ls));
             }
             int fc = 1;
             for (int t = t0 + 1; t < t1; ++t) {          // W[t - t0] = p_t
@@ -577,13 +599,23 @@ void run_nl_grad(aiso_ctx* c, const float* model, int nShotSig, const float* dat
                 r.fd = f[h + it2 + 1];
                 launch_timed(c, MODE_NL_ADJ, AISO_K_NL_ADJ, r);
                 if (maxSrcRing > 0 || maxRecRing > 0)
-                    HIPC(launch_ring(M_INJ_DATA | M_REC_SIG, r, std::max(maxSrcRing, maxRecRing), s));
+                    ring_lanes(M_INJ_DATA | M_REC_SIG, r, std::max(maxSrcRing, maxRecRing));
                 r.cur ^= 1;
                 ga.wF = (long)(t - 1 - t0) * nzx;
                 ga.nu = r.cur;
-                HIPC(launch_grad(c->tz, c->tx, ga, s));
+                for (int l = 0; l < c->curLanes; ++l) {
+                    int k0, cnt;
+                    lane_range(nb, c->curLanes, l, k0, cnt);
+                    if (cnt == 0) continue;
+                    GradArgs gl = ga;
+                    gl.slots = dA + k0;
+                    gl.gridLogical = ga.ntiles * cnt;
+                    HIPC(launch_grad(c->tz, c->tx, gl, lane_stream(c, l)));
+                }
             }
         }
+        join_lanes(c);
+        c->curLanes = 1;
         for (int k = 0; k < nb; ++k) HIPC(launch_accumulate(acc, img + (size_t)k * nzx, nzx, s));
         std::vector<float> sa((size_t)nb * nSrcRegMax * p.ntw);
         HIPC(hipMemcpyAsync(sa.data(), sigA, sa.size() * 4, hipMemcpyDeviceToHost, s));
