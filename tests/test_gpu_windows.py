@@ -75,12 +75,15 @@ def test_windows_match_plain_launches(small, tz, tx, spl, persist, graph, lanes)
           flush=True)
     b, _devs = _batch(A, w)
     b.set_config(tz, tx, spl, persist, graph, lanes)
-    for call in range(2):                                  # a graph is captured, then replayed
+    # a graph is captured on the first call and replayed by the next three,
+    # with eager work of the program (a plain call and a device sync) between
+    for call in range(4):
         out = [np.full((r.shape[0], r.shape[1]), np.nan, np.float32) for r in ref]
         b.nl_forward([model], out)
         for k in range(w.nShot):
             assert np.abs(ref[k]).max() > 0
             np.testing.assert_array_equal(out[k], ref[k], err_msg=f"window {k}, call {call}")
+        np.testing.assert_array_equal(_plain(A, w, call % w.nShot, model), ref[call % w.nShot])
     info = b.info()
     print(f"info {info}", flush=True)
     assert info["windows"] == w.nShot
