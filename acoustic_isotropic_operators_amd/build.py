@@ -39,6 +39,9 @@ def _run(cmd):
     return r
 
 
+RESOURCE_USAGE = os.path.join(PKG, "_build", "kernels_resource_usage.txt")
+
+
 def build_lib():
     from concurrent.futures import ThreadPoolExecutor
     bdir = os.path.join(PKG, "_build")
@@ -47,11 +50,15 @@ def build_lib():
     for src in ("kernels.hip", "noalloc.hip", "engine.cpp", "passes.cpp", "windows.cpp", "host.cpp"):
         obj = os.path.join(bdir, src.replace(".", "_") + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
-        cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
+        # the kernels' register, scratch and occupancy figures, kept for the tests
+        remarks = ["-Rpass-analysis=kernel-resource-usage"] if src == "kernels.hip" else []
+        cmds.append([HIPCC, *HIP_FLAGS, *remarks, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     # one compiler per object, side by side (each is single-threaded)
     with ThreadPoolExecutor(max_workers=min(len(cmds), max(1, (os.cpu_count() or 1) // 2))) as ex:
-        list(ex.map(_run, cmds))
+        results = list(ex.map(_run, cmds))
+    with open(RESOURCE_USAGE, "w") as f:
+        f.write(results[0].stderr)
     out = os.path.join(PKG, "libaiso.so")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out, *objs])
     return out

@@ -443,8 +443,10 @@ __global__ __launch_bounds__(NT) void k_step(const StepArgs a)
 // VGPRs, W = 8: 64): more workgroups resident to cover the latency of the
 // tile staging (the adjoint source pass otherwise compiles to 76 VGPRs, 6
 // waves, and the receiver pass with its early imaging operands to 84, 5).
+// The budget applies to the 32-row tiles only: the 64-row ones spill in it.
 template <int TZ, int TX, unsigned MODE, int W>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_step_w(const StepArgs a)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(TZ == 32 ? W : 1, 8)))
+void k_step_w(const StepArgs a)
 {
     step_tile<TZ, TX, MODE, 1>(a);
 }
@@ -647,11 +649,13 @@ __device__ __forceinline__ void pair_block(const StepArgs& a, int shot, int tile
     }
 }
 
-// Compiled to the register budget of 6 waves per SIMD (80 VGPRs, one
-// spilled): the sixth wave covers more staging latency than the spill costs
-// (40.85 -> 40.23 us per fine step of the Born benchmark on MI355X).
+// The 32 x 16 instance (the default tile) is compiled to the register
+// budget of 6 waves per SIMD (80 VGPRs, one spilled): the sixth wave covers
+// more staging latency than the spill costs (40.85 -> 40.23 us per fine step
+// of the Born benchmark on MI355X).  The other tiles spill heavily in it.
 template <int TZ, int TX>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_fwd_pair(const StepArgs a)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(TZ == 32 && TX == 16 ? 6 : 1, 8)))
+void k_fwd_pair(const StepArgs a)
 {
     using T = TileShape<TZ, TX>;
     __shared__ float sA[T::LX * T::LZ];
@@ -2192,14 +2196,14 @@ static StepKernel pick_tile_w(int tz, int tx)
 }
 
 // occ: 0 (the compiler's register budget), 6 or 8 waves per SIMD.  The
-// adjoint receiver pass with imaging runs at 6 when 8 is asked for: its
-// early imaging operands spill at 64 VGPRs.
+// adjoint receiver passes with imaging run at 6 when 8 is asked for: they
+// spill at 64 VGPRs.
 template <unsigned MODE>
 static StepKernel pick_tile(int tz, int tx, int spb, int occ)
 {
     if (spb == 2) return pick_tile_spb<MODE, 2>(tz, tx);
     if (occ == 6) return pick_tile_w<MODE, 6>(tz, tx);
-    if constexpr (MODE == MODE_REC_WIN) {
+    if constexpr (MODE == MODE_REC_WIN || MODE == MODE_BORN_REC) {
         if (occ == 8) return pick_tile_w<MODE, 6>(tz, tx);
     } else {
         if (occ == 8) return pick_tile_w<MODE, 8>(tz, tx);
