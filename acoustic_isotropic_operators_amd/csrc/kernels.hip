@@ -135,6 +135,37 @@ __device__ __forceinline__ void stage_tile(float* sT, cgf* src, const float* sMu
         }
 }
 
+// The adjoint Laplacian below a free surface (stepAdjFsGpu,
+// F/src/kernelsGpu.cu:613-700): the z terms, minus those the forward folds
+// onto the four rows under the surface, then the x terms, in the reference's
+// order.  gz is the point's row; away from the surface the sum equals
+// lap_at's.
+template <int TZ, int TX, class A>
+__device__ __forceinline__ float lap_adj_fs(const float* sT, const float* w, int j, int lx0, int lane, int gz,
+                                            const A& a)
+{
+    using T = TileShape<TZ, TX>;
+    const float* s = &sT[(lx0 + j + FAT) * T::LZ + lane + FAT];
+    const float cc = w[j + FAT];
+    float lap = a.zc[0] * cc
+              + a.zc[1] * (s[-1] + s[1])
+              + a.zc[2] * (s[-2] + s[2])
+              + a.zc[3] * (s[-3] + s[3])
+              + a.zc[4] * (s[-4] + s[4])
+              + a.zc[5] * (s[-5] + s[5]);
+    if (gz == FAT + 1) lap = lap - a.zc[2] * s[0] - a.zc[3] * s[1] - a.zc[4] * s[2] - a.zc[5] * s[3];
+    else if (gz == FAT + 2) lap = lap - a.zc[3] * s[-1] - a.zc[4] * s[0] - a.zc[5] * s[1];
+    else if (gz == FAT + 3) lap = lap - a.zc[4] * s[-2] - a.zc[5] * s[-1];
+    else if (gz == FAT + 4) lap = lap - a.zc[5] * s[-3];
+    return lap
+         + a.xc[0] * cc
+         + a.xc[1] * (w[j + 6] + w[j + 4])
+         + a.xc[2] * (w[j + 7] + w[j + 3])
+         + a.xc[3] * (w[j + 8] + w[j + 2])
+         + a.xc[4] * (w[j + 9] + w[j + 1])
+         + a.xc[5] * (w[j + 10] + w[j + 0]);
+}
+
 // 10th-order Laplacian at the PX points of a thread, in the reference's
 // summation order (z terms, then x terms).  w holds the x-window.
 template <int TZ, int TX, class A>
@@ -157,13 +188,36 @@ __device__ __forceinline__ float lap_at(const float* sT, const float* w, int j, 
          + a.xc[5] * (w[j + 10] + w[j + 0]);
 }
 
+// Free surface on a staged top tile (setFsConditionFwdGpu,
+// F/src/kernelsGpu.cu:555-567), for the interior columns: the surface row is
+// zero and the FAT rows above it mirror the rows below with the opposite
+// sign.  The stored slice keeps its surface value; the staged copy and the
+// old-slice read take it as zero, as the reference's slice holds it after
+// the condition.  The caller synchronizes before and after.
+template <int TZ, int TX>
+__device__ __forceinline__ void fs_mirror(float* sT, int x0, int nx, int lane, int wy)
+{
+    using T = TileShape<TZ, TX>;
+    if (lane != 0) return;
+    for (int lx = wy; lx < T::LX; lx += T::NXW) {
+        const int gx = x0 - FAT + lx;
+        if (gx >= FAT && gx < nx - FAT) {
+            float* col = &sT[lx * T::LZ];
+            col[FAT] = 0.f;
+#pragma unroll
+            for (int k = 0; k < FAT; ++k) col[k] = -col[2 * FAT - k];
+        }
+    }
+}
+
 // Sponge factors of a thread's points; distZ is constant along the thread's
-// x-run.  FULL: every point is interior.
-template <int PX, bool FULL, bool SPONGE>
+// x-run.  FULL: every point is interior.  FS: a free surface on the top row,
+// with no sponge above (dampCosineEdgeFs, F/src/kernelsGpu.cu:221-239).
+template <int PX, bool FULL, bool SPONGE, bool FS = false>
 __device__ __forceinline__ void tile_masks(bool (&ok)[PX], float (&dmp)[PX], const StepArgs& a, int gz, int gx0)
 {
     const int nz = a.nz, nx = a.nx;
-    const int distZ = imin(gz - FAT, nz - 1 - FAT - gz);
+    const int distZ = FS ? nz - 1 - FAT - gz : imin(gz - FAT, nz - 1 - FAT - gz);
 #pragma unroll
     for (int j = 0; j < PX; ++j) {
         const int gx = gx0 + j;
@@ -177,7 +231,7 @@ __device__ __forceinline__ void tile_masks(bool (&ok)[PX], float (&dmp)[PX], con
     }
 }
 
-template <int TZ, int TX, unsigned MODE, int SPB, bool FULL, bool SPONGE>
+template <int TZ, int TX, unsigned MODE, int SPB, bool FULL, bool SPONGE, bool FS = false>
 __device__ __forceinline__ void step_block(const StepArgs& a, int group, int tile, int z0, int x0, float* sC,
                                            float* sV, float* sN)
 {
@@ -201,7 +255,9 @@ __device__ __forceinline__ void step_block(const StepArgs& a, int group, int til
 
     bool ok[PX];
     float dmp[PX], vc[ADJ ? 1 : PX];
-    tile_masks<PX, FULL, SPONGE>(ok, dmp, a, gz, x0 + lx0);
+    tile_masks<PX, FULL, SPONGE, FS>(ok, dmp, a, gz, x0 + lx0);
+    const bool surfTile = FS && z0 == FAT;          // block-uniform
+    const bool surf = surfTile && gz == FAT;        // the thread's points lie on the surface row
     if constexpr (!ADJ) {
 #pragma unroll
         for (int j = 0; j < PX; ++j) vc[j] = ok[j] ? V[off0 + j * nz] : 0.f;
@@ -246,6 +302,12 @@ __device__ __forceinline__ void step_block(const StepArgs& a, int group, int til
         }
         stage_tile<TZ, TX, FULL, ADJ>(sC, pc, sV, soff, nz, nx, x0, z0, lane, wy);
         __syncthreads();
+        if constexpr (FS && !ADJ) {
+            if (surfTile) {
+                fs_mirror<TZ, TX>(sC, x0, nx, lane, wy);
+                __syncthreads();
+            }
+        }
         // The adjoint receiver pass's imaging operands (the W sample and the
         // image), issued once the tile is staged so their latency overlaps
         // the stencil (17 % less time per imaging launch on MI355X than
@@ -271,18 +333,26 @@ __device__ __forceinline__ void step_block(const StepArgs& a, int group, int til
         float cur[WIN ? PX : 1];
 #pragma unroll
         for (int j = 0; j < PX; ++j) {
-            const float lap = lap_at<TZ, TX>(sC, w, j, lx0, lane, a);
+            float lap;
+            if constexpr (FS && ADJ) lap = surfTile ? lap_adj_fs<TZ, TX>(sC, w, j, lx0, lane, gz, a)
+                                                    : lap_at<TZ, TX>(sC, w, j, lx0, lane, a);
+            else lap = lap_at<TZ, TX>(sC, w, j, lx0, lane, a);
             const float cc = w[j + FAT];
             float n = 0.f;
             if (ok[j]) {
-                const float o = SPONGE ? os[j] * dmp[j] : os[j];   // second damping of the old slice
+                // forward at a free surface: the old slice's surface value
+                // reads as zero, as the reference zeroed it when it was current
+                const float osj = (FS && !ADJ && surf) ? 0.f : os[j];
+                const float o = SPONGE ? osj * dmp[j] : osj;       // second damping of the old slice
                 if constexpr (ADJ) {
                     const float c = pv[j];
-                    n = lap + c + c - o;
+                    n = (FS && surf) ? o : lap + c + c - o;         // the adjoint leaves the surface row
                     if constexpr (WIN) cur[j] = c;
                 } else {
                     n = vc[j] * lap + cc + cc - o;
-                    if constexpr (WIN) cur[j] = cc;
+                    // the window takes the slices as stored; at the surface
+                    // the staged value is the mirrored one
+                    if constexpr (WIN) cur[j] = (FS && surf) ? pc[off0 + j * nz] : cc;
                 }
                 if constexpr (SSB) {
                     const float ssa = mm[j] * sA[j];
@@ -401,7 +471,7 @@ struct TileInfo {
     bool full, sponge;
 };
 
-template <int TZ, int TX>
+template <int TZ, int TX, bool FS = false>
 __device__ __forceinline__ bool tile_info(const StepArgs& a, TileInfo& t)
 {
     const int logical = xcd_logical(blockIdx.x, gridDim.x);
@@ -412,12 +482,13 @@ __device__ __forceinline__ bool tile_info(const StepArgs& a, TileInfo& t)
     t.z0 = FAT + tz * TZ;
     t.x0 = FAT + tx * TX;
     t.full = t.z0 + TZ <= a.nz - FAT && t.x0 + TX <= a.nx - FAT;
-    const int edge = imin(imin(t.z0 - FAT, a.nz - FAT - (t.z0 + TZ)), imin(t.x0 - FAT, a.nx - FAT - (t.x0 + TX)));
+    const int top = FS ? a.minPad : t.z0 - FAT;      // no sponge under a free surface
+    const int edge = imin(imin(top, a.nz - FAT - (t.z0 + TZ)), imin(t.x0 - FAT, a.nx - FAT - (t.x0 + TX)));
     t.sponge = !t.full || edge < a.minPad;
     return true;
 }
 
-template <int TZ, int TX, unsigned MODE, int SPB>
+template <int TZ, int TX, unsigned MODE, int SPB, bool FS = false>
 __device__ __forceinline__ void step_tile(const StepArgs& a)
 {
     using T = TileShape<TZ, TX>;
@@ -427,16 +498,27 @@ __device__ __forceinline__ void step_tile(const StepArgs& a)
     __shared__ float sV[ADJ ? T::LX * T::LZ : 1];   // v2dt2 window (adjoint), shared by the shots
     __shared__ float sN[SPARSE ? TX * TZ : 1];
     TileInfo t;
-    if (!tile_info<TZ, TX>(a, t)) return;
-    if (t.full && !t.sponge) step_block<TZ, TX, MODE, SPB, true, false>(a, t.group, t.tile, t.z0, t.x0, sC, sV, sN);
-    else if (t.full) step_block<TZ, TX, MODE, SPB, true, true>(a, t.group, t.tile, t.z0, t.x0, sC, sV, sN);
-    else step_block<TZ, TX, MODE, SPB, false, true>(a, t.group, t.tile, t.z0, t.x0, sC, sV, sN);
+    if (!tile_info<TZ, TX, FS>(a, t)) return;
+    if (t.full && !t.sponge)
+        step_block<TZ, TX, MODE, SPB, true, false, FS>(a, t.group, t.tile, t.z0, t.x0, sC, sV, sN);
+    else if (t.full) step_block<TZ, TX, MODE, SPB, true, true, FS>(a, t.group, t.tile, t.z0, t.x0, sC, sV, sN);
+    else step_block<TZ, TX, MODE, SPB, false, true, FS>(a, t.group, t.tile, t.z0, t.x0, sC, sV, sN);
 }
 
 template <int TZ, int TX, unsigned MODE, int SPB>
 __global__ __launch_bounds__(NT) void k_step(const StepArgs a)
 {
     step_tile<TZ, TX, MODE, SPB>(a);
+}
+
+// The step with a free surface on the top row (the nonlinear and Born
+// operators' surface condition, F/src/kernelsGpu.cu:555-700), one shot per
+// workgroup, at the compiler's register budget (the adjoint source pass
+// spills at 6 waves per SIMD).
+template <int TZ, int TX, unsigned MODE>
+__global__ __launch_bounds__(NT) void k_step_fs(const StepArgs a)
+{
+    step_tile<TZ, TX, MODE, 1, true>(a);
 }
 
 // The same step with the register budget of W waves per SIMD (W = 6: 80
@@ -525,7 +607,7 @@ __global__ __launch_bounds__(NT) void k_step_win_loop(const WinStepArgs w)
 // completed.  Velocity, sponge factor and tile bookkeeping are shared by the
 // two fields; the arithmetic per field is that of k_step (MODE_SRC_WIN with
 // WIN_SS, and MODE_BORN_SCAT_PRE).
-template <int TZ, int TX, bool FULL, bool SPONGE>
+template <int TZ, int TX, bool FULL, bool SPONGE, bool FS = false>
 __device__ __forceinline__ void pair_block(const StepArgs& a, int shot, int tile, int z0, int x0, float* sA, float* sB,
                                            float* sN)
 {
@@ -549,7 +631,9 @@ __device__ __forceinline__ void pair_block(const StepArgs& a, int shot, int tile
 
     bool ok[PX];
     float dmp[PX];
-    tile_masks<PX, FULL, SPONGE>(ok, dmp, a, gz, x0 + lx0);
+    tile_masks<PX, FULL, SPONGE, FS>(ok, dmp, a, gz, x0 + lx0);
+    const bool surfTile = FS && z0 == FAT;          // block-uniform
+    const bool surf = surfTile && gz == FAT;
     float oa[PX], ob[PX], v2[PX], qa[PX], qb[PX], mm[PX];
 #pragma unroll
     for (int j = 0; j < PX; ++j) {
@@ -573,6 +657,13 @@ __device__ __forceinline__ void pair_block(const StepArgs& a, int shot, int tile
     stage_tile<TZ, TX, FULL, false>(sA, pcA, nullptr, soff, nz, nx, x0, z0, lane, wy);
     stage_tile<TZ, TX, FULL, false>(sB, pcB, nullptr, soff, nz, nx, x0, z0, lane, wy);
     __syncthreads();
+    if constexpr (FS) {
+        if (surfTile) {
+            fs_mirror<TZ, TX>(sA, x0, nx, lane, wy);
+            fs_mirror<TZ, TX>(sB, x0, nx, lane, wy);
+            __syncthreads();
+        }
+    }
 
     float wA[PX + 2 * FAT], wB[PX + 2 * FAT];
 #pragma unroll
@@ -588,8 +679,10 @@ __device__ __forceinline__ void pair_block(const StepArgs& a, int shot, int tile
         const float ca = wA[j + FAT], cb = wB[j + FAT];
         float na = 0.f, nb = 0.f;
         if (ok[j]) {
-            na = v2[j] * lapA + ca + ca - (SPONGE ? oa[j] * dmp[j] : oa[j]);
-            nb = v2[j] * lapB + cb + cb - (SPONGE ? ob[j] * dmp[j] : ob[j]);
+            // at a free surface the old slices' surface values read as zero
+            const float oaj = (FS && surf) ? 0.f : oa[j], obj = (FS && surf) ? 0.f : ob[j];
+            na = v2[j] * lapA + ca + ca - (SPONGE ? oaj * dmp[j] : oaj);
+            nb = v2[j] * lapB + cb + cb - (SPONGE ? obj * dmp[j] : obj);
             if (useB) nb += qa[j] * a.fa + qb[j] * a.fb;
             else nb += qa[j] * a.fa;
         }
@@ -624,7 +717,10 @@ __device__ __forceinline__ void pair_block(const StepArgs& a, int shot, int tile
         const float nb = SPONGE ? nB[j] * dmp[j] : nB[j];
         nB[j] = nb;
         if (a.win) {
-            const float v = (oa[j] * a.wa + wA[j + FAT] * a.wb) + na * a.wc;
+            // the window takes the slices as stored (at a free surface the
+            // staged current value is the mirrored one)
+            const float cA = (FS && surf) ? pcA[gi] : wA[j + FAT];
+            const float v = (oa[j] * a.wa + cA * a.wb) + na * a.wc;
             stg_s(Wd + gi, mm[j] * v, a.ntW);
         }
         poA[gi] = na;
@@ -666,6 +762,22 @@ void k_fwd_pair(const StepArgs a)
     if (t.full && !t.sponge) pair_block<TZ, TX, true, false>(a, t.group, t.tile, t.z0, t.x0, sA, sB, sN);
     else if (t.full) pair_block<TZ, TX, true, true>(a, t.group, t.tile, t.z0, t.x0, sA, sB, sN);
     else pair_block<TZ, TX, false, true>(a, t.group, t.tile, t.z0, t.x0, sA, sB, sN);
+}
+
+// The lockstep pair with a free surface on the top row.
+template <int TZ, int TX>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(TZ == 32 && TX == 16 ? 6 : 1, 8)))
+void k_fwd_pair_fs(const StepArgs a)
+{
+    using T = TileShape<TZ, TX>;
+    __shared__ float sA[T::LX * T::LZ];
+    __shared__ float sB[T::LX * T::LZ];
+    __shared__ float sN[TX * TZ];
+    TileInfo t;
+    if (!tile_info<TZ, TX, true>(a, t)) return;
+    if (t.full && !t.sponge) pair_block<TZ, TX, true, false, true>(a, t.group, t.tile, t.z0, t.x0, sA, sB, sN);
+    else if (t.full) pair_block<TZ, TX, true, true, true>(a, t.group, t.tile, t.z0, t.x0, sA, sB, sN);
+    else pair_block<TZ, TX, false, true, true>(a, t.group, t.tile, t.z0, t.x0, sA, sB, sN);
 }
 
 // Stage the (EZ+10) x (EX+10) window of src whose first element is grid
@@ -2211,6 +2323,32 @@ static StepKernel pick_tile(int tz, int tx, int spb, int occ)
     return pick_tile_spb<MODE, 1>(tz, tx);
 }
 
+template <unsigned MODE>
+static StepKernel pick_tile_fs(int tz, int tx)
+{
+    if (tz == 64 && tx == 16) return k_step_fs<64, 16, MODE>;
+    if (tz == 64 && tx == 32) return k_step_fs<64, 32, MODE>;
+    if (tz == 32 && tx == 16) return k_step_fs<32, 16, MODE>;
+    if (tz == 32 && tx == 32) return k_step_fs<32, 32, MODE>;
+    return nullptr;
+}
+
+// The nonlinear and Born modes with a free surface.
+static StepKernel pick_step_fs(unsigned mode, int tz, int tx)
+{
+    switch (mode) {
+    case MODE_NL_FWD: return pick_tile_fs<MODE_NL_FWD>(tz, tx);
+    case MODE_NL_ADJ: return pick_tile_fs<MODE_NL_ADJ>(tz, tx);
+    case MODE_SRC: return pick_tile_fs<MODE_SRC>(tz, tx);
+    case MODE_BORN_SCAT: return pick_tile_fs<MODE_BORN_SCAT>(tz, tx);
+    case MODE_BORN_REC: return pick_tile_fs<MODE_BORN_REC>(tz, tx);
+    case MODE_SRC_WIN: return pick_tile_fs<MODE_SRC_WIN>(tz, tx);
+    case MODE_BORN_SCAT_PRE: return pick_tile_fs<MODE_BORN_SCAT_PRE>(tz, tx);
+    case MODE_REC_WIN: return pick_tile_fs<MODE_REC_WIN>(tz, tx);
+    default: return nullptr;
+    }
+}
+
 // The extended-Born modes (MODE_EXT_SCAT, MODE_EXT_REC) are not instantiated
 // here: they run on k_step_generic only, and launch_step refuses them.
 static StepKernel pick_step(unsigned mode, int tz, int tx, int spb, int occ)
@@ -2241,7 +2379,8 @@ hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a
                        hipEvent_t e1, int occ)
 {
     if (occ != 0 && occ != 6 && occ != 8) return hipErrorInvalidValue;
-    StepKernel k = pick_step(mode, tz, tx, spb, occ);
+    if (a.fs) spb = 1;                              // the surface instances advance one shot per workgroup
+    StepKernel k = a.fs ? pick_step_fs(mode, tz, tx) : pick_step(mode, tz, tx, spb, occ);
     if (!k) return hipErrorInvalidValue;
     if ((mode & M_WIN) && a.winOut == WIN_SLICE) return hipErrorInvalidValue;   // extended-Born capture: generic only
     // One workgroup per (tile, group of spb shots).
@@ -2315,10 +2454,10 @@ hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a,
 hipError_t launch_fwd_pair(int tz, int tx, const StepArgs& a, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
 {
     StepKernel k = nullptr;
-    if (tz == 64 && tx == 16) k = k_fwd_pair<64, 16>;
-    else if (tz == 64 && tx == 32) k = k_fwd_pair<64, 32>;
-    else if (tz == 32 && tx == 16) k = k_fwd_pair<32, 16>;
-    else if (tz == 32 && tx == 32) k = k_fwd_pair<32, 32>;
+    if (tz == 64 && tx == 16) k = a.fs ? k_fwd_pair_fs<64, 16> : k_fwd_pair<64, 16>;
+    else if (tz == 64 && tx == 32) k = a.fs ? k_fwd_pair_fs<64, 32> : k_fwd_pair<64, 32>;
+    else if (tz == 32 && tx == 16) k = a.fs ? k_fwd_pair_fs<32, 16> : k_fwd_pair<32, 16>;
+    else if (tz == 32 && tx == 32) k = a.fs ? k_fwd_pair_fs<32, 32> : k_fwd_pair<32, 32>;
     if (!k) return hipErrorInvalidValue;
     return launch_k(k, (a.gridLogical + 7) / 8 * 8, a, s, e0, e1);
 }

@@ -75,7 +75,7 @@ struct StepArgs {
     int win, winOut;        // M_WIN: capture at this step, and its destination
     float wa, wb, wc;       // M_WIN weights of the old, current and new slices
     int ntW;                // non-temporal accesses to the W / SS stream
-    int fs;                 // free surface at iz = FAT (k_step_generic only)
+    int fs;                 // free surface at iz = FAT (k_step_generic, k_step_fs, k_fwd_pair_fs)
     // lockstep Born forward (k_fwd_pair): scattered-field parity and the
     // secondary-source slices it reads
     int curQ;

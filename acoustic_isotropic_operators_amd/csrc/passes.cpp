@@ -118,11 +118,10 @@ void Call::select_tile()
 {
     c->tz = is_adj(q.op) ? c->tzAdj : c->tzFwd;
     c->tx = is_adj(q.op) ? c->txAdj : c->txFwd;
-    // The forward passes of the extended, tomographic and WEMVA operators,
-    // and every pass with a free surface, run on the generic step kernel,
-    // which is faster on 32 x 32 tiles than on the specialized forward
-    // kernels' 32 x 16 (2-6 % on MI355X).
-    const bool onGeneric = is_ext(q.op) || is_tomo(q.op) || is_wemva(q.op) || p.freeSurface != 0;
+    // The forward passes of the extended, tomographic and WEMVA operators
+    // run on the generic step kernel, which is faster on 32 x 32 tiles than
+    // on the specialized forward kernels' 32 x 16 (2-6 % on MI355X).
+    const bool onGeneric = is_ext(q.op) || is_tomo(q.op) || is_wemva(q.op);
     if (!is_adj(q.op) && !c->tileFwdSet && onGeneric) {
         c->tz = 32;
         c->tx = 32;
@@ -259,17 +258,18 @@ void Call::build_geometry()
 
 // Shots per launch and the passes' forms: the Born forward with sub == 2
 // runs its two propagations in lockstep (the secondary source then lives in
-// a 3-slice ring instead of nts slices); a free surface runs on the generic
-// step kernel without the lockstep pair or two steps per launch.
+// a 3-slice ring instead of nts slices); with a free surface the nonlinear
+// and Born passes run the surface instances of the specialized kernels
+// (k_step_fs, k_fwd_pair_fs), without two steps per launch.
 void Call::plan_batches()
 {
-    lockstep = q.op == Op::BornFwd && p.sub == 2 && c->lockstep && !fsurf;
+    lockstep = q.op == Op::BornFwd && p.sub == 2 && c->lockstep;
     // Two fine steps per launch (k_fwd_pair2, k_rec2) when no source or
     // receiver point lies in the FAT ring, whose values those kernels take as 0.
     bool noRing = true;
     for (int i = 0; noRing && i < q.nShot; ++i) noRing = srcG[i]->nRing == 0;
     for (int i = 0; noRing && i < q.nRec; ++i) noRing = recG[i]->nRing == 0;
-    pair2 = lockstep && c->pair2 && noRing;
+    pair2 = lockstep && c->pair2 && noRing && !fsurf;
     adj2 = q.op == Op::BornAdj && p.sub == 2 && c->pair2 && noRing && !fsurf;
     pairSlices = pair2 || adj2 ? 4 : 2;
     wSlices = lockstep ? 3 : p.nts;
@@ -291,7 +291,7 @@ void Call::plan_batches()
     // specialized kernel on the extended-Born path ended in a memory fault
     // whose cause was never shown; the specialized kernel is not built for
     // the extended modes.
-    generic = is_ext(q.op) || tomo || fsurf;
+    generic = is_ext(q.op) || tomo;
 
     size_t perShot = pairSlices * nzx * 4 + (size_t)nRecRegMax * p.nts * 4 + (size_t)nSrcRegMax * p.ntw * 4;
     if (needW) perShot += (size_t)wSlices * nzx * 4;
