@@ -67,21 +67,6 @@ __device__ __forceinline__ void st_stream(float* p, float v, int nt)
     else *p = v;
 }
 
-__device__ __forceinline__ float sponge(const float* __restrict__ cd, int gz, int gx,
-                                        int nz, int nx, int minPad)
-{
-    int dist = imin(imin(gz - FAT, gx - FAT), imin(nz - gz - 1 - FAT, nx - gx - 1 - FAT));
-    return dist < minPad ? cd[dist] : 1.0f;
-}
-
-// dampCosineEdgeFs (F/src/kernelsGpu.cu:221-239): no sponge at a free surface.
-__device__ __forceinline__ float sponge_fs(const float* __restrict__ cd, int gz, int gx,
-                                           int nz, int nx, int minPad)
-{
-    int dist = imin(gx - FAT, imin(nz - gz - 1 - FAT, nx - gx - 1 - FAT));
-    return dist < minPad ? cd[dist] : 1.0f;
-}
-
 // Global-memory views: pointers read from the ShotSlot table are generic to
 // the compiler; casting them lets it emit global (not flat) loads and stores
 // with a uniform 64-bit base and a 32-bit per-lane offset.
@@ -90,6 +75,19 @@ using gf = GBL float;
 using cgf = const GBL float;
 __device__ __forceinline__ gf* G(float* p) { return (gf*)p; }
 __device__ __forceinline__ cgf* G(const float* p) { return (cgf*)p; }
+
+__device__ __forceinline__ float sponge(cgf* cd, int gz, int gx, int nz, int nx, int minPad)
+{
+    int dist = imin(imin(gz - FAT, gx - FAT), imin(nz - gz - 1 - FAT, nx - gx - 1 - FAT));
+    return dist < minPad ? cd[dist] : 1.0f;
+}
+
+// dampCosineEdgeFs (F/src/kernelsGpu.cu:221-239): no sponge at a free surface.
+__device__ __forceinline__ float sponge_fs(cgf* cd, int gz, int gx, int nz, int nx, int minPad)
+{
+    int dist = imin(gx - FAT, imin(nz - gz - 1 - FAT, nx - gx - 1 - FAT));
+    return dist < minPad ? cd[dist] : 1.0f;
+}
 
 __device__ __forceinline__ float ldg_s(cgf* p, int nt) { return nt ? __builtin_nontemporal_load(p) : *p; }
 __device__ __forceinline__ void stg_s(gf* p, float v, int nt)
@@ -1204,10 +1202,13 @@ __global__ __launch_bounds__(NT) void k_grad(const GradArgs a)
     }
 }
 
-// Unspecialized form of the fused step: one shot per workgroup, every
-// operand addressed through generic pointers, bounds and sponge tested at
-// every point.  The extended-Born operators run on it.
-template <int TZ, int TX, unsigned MODE>
+// Unspecialized form of the fused step: one shot per workgroup, 64-bit
+// element offsets, bounds and sponge tested at every point.  The operands
+// are read through global address-space views, so their loads are not
+// counted with the LDS reads of the stencil.  FS: the free-surface
+// conditions, for launches with a surface; without it they are compiled
+// out.  The extended-Born, tomographic and WEMVA operators run on it.
+template <int TZ, int TX, unsigned MODE, bool FS>
 __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
 {
     constexpr bool ADJ = MODE & M_ADJ;
@@ -1227,9 +1228,9 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
     const int nz = a.nz, nx = a.nx;
     const int z0 = FAT + tz * TZ, x0 = FAT + tx * TX;
     const int lane = threadIdx.x % TZ, wy = threadIdx.x / TZ;
-    const float* __restrict__ pc = S.p[a.cur];
-    float* __restrict__ po = S.p[a.cur ^ 1];
-    const float* __restrict__ v2dt2 = a.v2dt2;
+    cgf* pc = G(S.p[a.cur]);
+    gf* po = G(S.p[a.cur ^ 1]);
+    cgf* v2dt2 = G(a.v2dt2);
 
     // Per-point operands.  The forward modes load them before staging the
     // tile, so that their latency overlaps it; the adjoint modes, whose
@@ -1258,36 +1259,50 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
                 os[j] = po[gi];
                 pv[j] = ADJ ? pc[gi] : v2dt2[gi];
                 if constexpr (SSB) {
-                    mm[j] = a.mtil[gi];
-                    sA[j] = S.W[a.wA + gi];
-                    if (useB) sB[j] = S.W[a.wB + gi];
+                    mm[j] = G(a.mtil)[gi];
+                    sA[j] = G(S.W)[a.wA + gi];
+                    if (useB) sB[j] = G(S.W)[a.wB + gi];
                 }
                 if constexpr (SSP) {
-                    sA[j] = S.W[a.wA + gi];
-                    if (useB) sB[j] = S.W[a.wB + gi];
+                    sA[j] = G(S.W)[a.wA + gi];
+                    if (useB) sB[j] = G(S.W)[a.wB + gi];
                 }
                 if constexpr (SSS) {
-                    sA[j] = S.ss[a.ssA][a.ssOffA + gi];
-                    if (useB) sB[j] = S.ss[a.ssB][a.ssOffB + gi];
+                    sA[j] = G(S.ss[a.ssA])[a.ssOffA + gi];
+                    if (useB) sB[j] = G(S.ss[a.ssB])[a.ssOffB + gi];
                 }
             }
         }
     };
     if constexpr (!ADJ) load_operands();
 
-    // Stage the current slice (times v2dt2 for the adjoint) with its halo.
-    for (int lx = wy; lx < LX; lx += NXW) {
-        const int gx = x0 - FAT + lx;
-        for (int lz = lane; lz < LZ; lz += TZ) {
-            const int gz = z0 - FAT + lz;
-            float v = 0.f;
-            if (gx < nx && gz < nz) {
+    // Stage the current slice (times v2dt2 for the adjoint) with its halo:
+    // every load of the thread's share first, then the LDS stores, so the
+    // loads are in flight together.  Points outside the grid stage as zero.
+    {
+        constexpr int KX = (LX + NXW - 1) / NXW, KZ = (LZ + TZ - 1) / TZ;
+        float tv[KX][KZ], tm[ADJ ? KX : 1][ADJ ? KZ : 1];
+#pragma unroll
+        for (int kx = 0; kx < KX; ++kx)
+#pragma unroll
+            for (int kz = 0; kz < KZ; ++kz) {
+                const int lx = wy + kx * NXW, lz = lane + kz * TZ;
+                const int gx = x0 - FAT + lx, gz = z0 - FAT + lz;
+                const bool in = lx < LX && lz < LZ && gx < nx && gz < nz;
                 const long gi = (long)gx * nz + gz;
-                v = pc[gi];
-                if constexpr (ADJ) v = v * v2dt2[gi];
+                tv[kx][kz] = in ? pc[gi] : 0.f;
+                if constexpr (ADJ) tm[kx][kz] = in ? v2dt2[gi] : 0.f;
             }
-            sC[lx * LZ + lz] = v;
-        }
+#pragma unroll
+        for (int kx = 0; kx < KX; ++kx)
+#pragma unroll
+            for (int kz = 0; kz < KZ; ++kz) {
+                const int lx = wy + kx * NXW, lz = lane + kz * TZ;
+                if (lx < LX && lz < LZ) {
+                    if constexpr (ADJ) sC[lx * LZ + lz] = tv[kx][kz] * tm[kx][kz];
+                    else sC[lx * LZ + lz] = tv[kx][kz];
+                }
+            }
     }
     __syncthreads();
     if constexpr (!ADJ) {
@@ -1297,7 +1312,7 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
         // opposite sign.  The stored slice keeps its surface value; the
         // old-slice read below takes it as zero, as the reference's slice
         // holds it after the condition.
-        if (a.fs && z0 == FAT) {
+        if (FS && z0 == FAT) {
             for (int lx = wy; lx < LX; lx += NXW) {
                 const int gx = x0 - FAT + lx;
                 if (gx >= FAT && gx < nx - FAT && lane == 0) {
@@ -1332,7 +1347,7 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
         if constexpr (ADJ) {
             // stepAdjFsGpu (F/src/kernelsGpu.cu:613-700): below a free
             // surface the four rows drop the z terms the forward folds onto them.
-            if (a.fs && gz <= FAT + 4) {
+            if (FS && gz <= FAT + 4) {
                 if (gz == FAT + 1) lap = lap - a.zc[2] * s[0] - a.zc[3] * s[1] - a.zc[4] * s[2] - a.zc[5] * s[3];
                 else if (gz == FAT + 2) lap = lap - a.zc[3] * s[-1] - a.zc[4] * s[0] - a.zc[5] * s[1];
                 else if (gz == FAT + 3) lap = lap - a.zc[4] * s[-2] - a.zc[5] * s[-1];
@@ -1348,8 +1363,8 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
             + a.xc[5] * (w[j + 10] + w[j + 0]);
         float n = 0.f, d = 1.f;
         if (ok[j]) {
-            const bool surf = a.fs && gz == FAT;
-            d = a.fs ? sponge_fs(a.cd, gz, gx, nz, nx, a.minPad) : sponge(a.cd, gz, gx, nz, nx, a.minPad);
+            const bool surf = FS && gz == FAT;
+            d = FS ? sponge_fs(G(a.cd), gz, gx, nz, nx, a.minPad) : sponge(G(a.cd), gz, gx, nz, nx, a.minPad);
             // forward: the reference zeroed the old slice's surface value when
             // it was the current slice
             const float osj = (!ADJ && surf) ? 0.f : os[j];
@@ -1397,9 +1412,9 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
                 const int2 en = te[e];
                 float v;
                 if constexpr (MODE & M_INJ_SIG) {
-                    v = S.sig[(long)en.y * a.ntw + a.itwInj];
+                    v = G(S.sig)[(long)en.y * a.ntw + a.itwInj];
                 } else {
-                    const float* dr = S.dreg + (long)en.y * a.nts + a.its;
+                    cgf* dr = G(S.dreg) + ((long)en.y * a.nts + a.its);
                     v = dr[0] * a.fc + dr[1] * a.fd;
                 }
                 sN[en.x] += v;
@@ -1423,8 +1438,8 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
             // adjoint receiver pass accumulates into the two ss slices.
             constexpr bool CAP_SS = ADJ && !(MODE & M_CAP_W);
             constexpr bool CAP_ALT = !ADJ && (MODE & M_CAP_ALT);
-            float* A = CAP_SS ? S.ss[a.ssA] + a.ssOffA : CAP_ALT ? S.img + a.wA : S.W + a.wA;
-            float* B = CAP_SS ? S.ss[a.ssB] + a.ssOffB : CAP_ALT ? S.img + a.wB : S.W + a.wB;
+            gf* A = G(CAP_SS ? S.ss[a.ssA] + a.ssOffA : CAP_ALT ? S.img + a.wA : S.W + a.wA);
+            gf* B = G(CAP_SS ? S.ss[a.ssB] + a.ssOffB : CAP_ALT ? S.img + a.wB : S.W + a.wB);
             // A zero weight leaves the slice as it is (x + n*0 == x); a
             // first touch writes instead of accumulating into a zeroed slice.
             if (!a.skipA) A[gi] = a.assignA ? n * a.fe : A[gi] + n * a.fe;
@@ -1439,7 +1454,7 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
                 bnew = B[gi];
             }
             if constexpr (MODE & M_IMAGE) {
-                if (a.image) S.img[gi] += S.W[a.wImg + gi] * bnew;
+                if (a.image) G(S.img)[gi] += G(S.W)[a.wImg + gi] * bnew;
             }
         }
         if constexpr (MODE & M_WIN) {
@@ -1448,10 +1463,10 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
                 // in the reference's order (old, current, new).
                 const float v = (ost[j] * a.wa + cur[j] * a.wb) + n * a.wc;
                 switch (a.winOut) {
-                case WIN_W: S.W[a.wA + gi] = v; break;
-                case WIN_SS: S.W[a.wA + gi] = a.mtil[gi] * v; break;
-                case WIN_IMG: S.img[gi] += S.W[a.wImg + gi] * v; break;
-                default: S.ss[a.ssA][a.ssOffA + gi] = v; break;
+                case WIN_W: G(S.W)[a.wA + gi] = v; break;
+                case WIN_SS: G(S.W)[a.wA + gi] = G(a.mtil)[gi] * v; break;
+                case WIN_IMG: G(S.img)[gi] += G(S.W)[a.wImg + gi] * v; break;
+                default: G(S.ss[a.ssA])[a.ssOffA + gi] = v; break;
                 }
             }
         }
@@ -1470,9 +1485,9 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
                 const int2 en = te[e];
                 const float n = sN[en.x];
                 if constexpr (MODE & M_REC_SIG) {
-                    S.sig[(long)en.y * a.ntw + a.itwRec] += n;
+                    G(S.sig)[(long)en.y * a.ntw + a.itwRec] += n;
                 } else {
-                    float* dr = S.dreg + (long)en.y * a.nts + a.its;
+                    gf* dr = G(S.dreg) + ((long)en.y * a.nts + a.its);
                     dr[0] += n * a.fe;
                     dr[1] += n * a.ff;
                 }
@@ -2414,35 +2429,42 @@ hipError_t launch_step_windows(unsigned mode, int tz, int tx, int grid, const Wi
     return hipGetLastError();
 }
 
-template <unsigned MODE>
-static StepKernel pick_generic_tile(int tz, int tx)
+template <unsigned MODE, bool FS>
+static StepKernel pick_generic_fs(int tz, int tx)
 {
-    if (tz == 64 && tx == 16) return k_step_generic<64, 16, MODE>;
-    if (tz == 64 && tx == 32) return k_step_generic<64, 32, MODE>;
-    if (tz == 32 && tx == 16) return k_step_generic<32, 16, MODE>;
-    if (tz == 32 && tx == 32) return k_step_generic<32, 32, MODE>;
+    if (tz == 64 && tx == 16) return k_step_generic<64, 16, MODE, FS>;
+    if (tz == 64 && tx == 32) return k_step_generic<64, 32, MODE, FS>;
+    if (tz == 32 && tx == 16) return k_step_generic<32, 16, MODE, FS>;
+    if (tz == 32 && tx == 32) return k_step_generic<32, 32, MODE, FS>;
     return nullptr;
+}
+
+template <unsigned MODE>
+static StepKernel pick_generic_tile(int tz, int tx, bool fs)
+{
+    return fs ? pick_generic_fs<MODE, true>(tz, tx) : pick_generic_fs<MODE, false>(tz, tx);
 }
 
 hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a, hipStream_t s, hipEvent_t e0,
                                hipEvent_t e1)
 {
     StepKernel k = nullptr;
+    const bool fs = a.fs != 0;
     switch (mode) {
-    case MODE_NL_FWD: k = pick_generic_tile<MODE_NL_FWD>(tz, tx); break;
-    case MODE_NL_ADJ: k = pick_generic_tile<MODE_NL_ADJ>(tz, tx); break;
-    case MODE_BORN_REC: k = pick_generic_tile<MODE_BORN_REC>(tz, tx); break;
-    case MODE_SRC: k = pick_generic_tile<MODE_SRC>(tz, tx); break;
-    case MODE_SRC_WIN: k = pick_generic_tile<MODE_SRC_WIN>(tz, tx); break;
-    case MODE_BORN_SCAT: k = pick_generic_tile<MODE_BORN_SCAT>(tz, tx); break;
-    case MODE_BORN_SCAT_PRE: k = pick_generic_tile<MODE_BORN_SCAT_PRE>(tz, tx); break;
-    case MODE_EXT_SCAT: k = pick_generic_tile<MODE_EXT_SCAT>(tz, tx); break;
-    case MODE_TOMO_CAP: k = pick_generic_tile<MODE_TOMO_CAP>(tz, tx); break;
-    case MODE_TOMO_ADJ_CAP: k = pick_generic_tile<MODE_TOMO_ADJ_CAP>(tz, tx); break;
-    case MODE_TOMO_REC: k = pick_generic_tile<MODE_TOMO_REC>(tz, tx); break;
-    case MODE_EXT_REC: k = pick_generic_tile<MODE_EXT_REC>(tz, tx); break;
-    case MODE_REC_WIN: k = pick_generic_tile<MODE_REC_WIN>(tz, tx); break;
-    case MODE_BORN_SCAT_CAP: k = pick_generic_tile<MODE_BORN_SCAT_CAP>(tz, tx); break;
+    case MODE_NL_FWD: k = pick_generic_tile<MODE_NL_FWD>(tz, tx, fs); break;
+    case MODE_NL_ADJ: k = pick_generic_tile<MODE_NL_ADJ>(tz, tx, fs); break;
+    case MODE_BORN_REC: k = pick_generic_tile<MODE_BORN_REC>(tz, tx, fs); break;
+    case MODE_SRC: k = pick_generic_tile<MODE_SRC>(tz, tx, fs); break;
+    case MODE_SRC_WIN: k = pick_generic_tile<MODE_SRC_WIN>(tz, tx, fs); break;
+    case MODE_BORN_SCAT: k = pick_generic_tile<MODE_BORN_SCAT>(tz, tx, fs); break;
+    case MODE_BORN_SCAT_PRE: k = pick_generic_tile<MODE_BORN_SCAT_PRE>(tz, tx, fs); break;
+    case MODE_EXT_SCAT: k = pick_generic_tile<MODE_EXT_SCAT>(tz, tx, fs); break;
+    case MODE_TOMO_CAP: k = pick_generic_tile<MODE_TOMO_CAP>(tz, tx, fs); break;
+    case MODE_TOMO_ADJ_CAP: k = pick_generic_tile<MODE_TOMO_ADJ_CAP>(tz, tx, fs); break;
+    case MODE_TOMO_REC: k = pick_generic_tile<MODE_TOMO_REC>(tz, tx, fs); break;
+    case MODE_EXT_REC: k = pick_generic_tile<MODE_EXT_REC>(tz, tx, fs); break;
+    case MODE_REC_WIN: k = pick_generic_tile<MODE_REC_WIN>(tz, tx, fs); break;
+    case MODE_BORN_SCAT_CAP: k = pick_generic_tile<MODE_BORN_SCAT_CAP>(tz, tx, fs); break;
     default: break;
     }
     if (!k) return hipErrorInvalidValue;
