@@ -1243,7 +1243,7 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
     constexpr bool HAS_SS = SSB || SSP || SSS;
     const bool useB = a.fb != 0.f;
     bool ok[PX];
-    float os[PX], pv[PX];
+    float os[PX], pv[PX], dmp[PX];
     float sA[HAS_SS ? PX : 1], sB[HAS_SS ? PX : 1], mm[SSB ? PX : 1];
     auto load_operands = [&] {
 #pragma unroll
@@ -1252,12 +1252,15 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
             ok[j] = zok && gx < nx - FAT;
             os[j] = 0.f;
             pv[j] = 0.f;
+            dmp[j] = 1.f;
             if constexpr (HAS_SS) { sA[j] = 0.f; sB[j] = 0.f; }
             if constexpr (SSB) mm[j] = 0.f;
             if (ok[j]) {
                 const long gi = (long)gx * nz + gz;
                 os[j] = po[gi];
                 pv[j] = ADJ ? pc[gi] : v2dt2[gi];
+                // sponge factor, its table read issued with the operands
+                dmp[j] = FS ? sponge_fs(G(a.cd), gz, gx, nz, nx, a.minPad) : sponge(G(a.cd), gz, gx, nz, nx, a.minPad);
                 if constexpr (SSB) {
                     mm[j] = G(a.mtil)[gi];
                     sA[j] = G(S.W)[a.wA + gi];
@@ -1331,7 +1334,7 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
 #pragma unroll
     for (int j = 0; j < PX + 2 * FAT; ++j) w[j] = sC[(lx0 + j) * LZ + lane + FAT];
 
-    float nv[PX], dmp[PX];
+    float nv[PX];
     float ost[(MODE & M_WIN) ? PX : 1], cur[(MODE & M_WIN) ? PX : 1];
 #pragma unroll
     for (int j = 0; j < PX; ++j) {
@@ -1361,10 +1364,10 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
             + a.xc[3] * (w[j + 8] + w[j + 2])
             + a.xc[4] * (w[j + 9] + w[j + 1])
             + a.xc[5] * (w[j + 10] + w[j + 0]);
-        float n = 0.f, d = 1.f;
+        float n = 0.f;
         if (ok[j]) {
             const bool surf = FS && gz == FAT;
-            d = FS ? sponge_fs(G(a.cd), gz, gx, nz, nx, a.minPad) : sponge(G(a.cd), gz, gx, nz, nx, a.minPad);
+            const float d = dmp[j];
             // forward: the reference zeroed the old slice's surface value when
             // it was the current slice
             const float osj = (!ADJ && surf) ? 0.f : os[j];
@@ -1396,10 +1399,23 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
             }
         }
         nv[j] = n;
-        dmp[j] = d;
     }
 
     const int t = tile;
+    // Receiver / source entries of the tile for the records after the
+    // stores, read before them (see the buffer pointers below).
+    int postE0 = 0, postE1 = 0;
+    const int2* postTe = nullptr;
+    gf* postSig = nullptr;
+    gf* postDreg = nullptr;
+    if constexpr (POST_SPARSE) {
+        const int* tp = (MODE & M_REC_SIG) ? S.srcTile : S.recTile;
+        postTe = (MODE & M_REC_SIG) ? S.srcEnt : S.recEnt;
+        postE0 = tp[t];
+        postE1 = tp[t + 1];
+        postSig = G(S.sig);
+        postDreg = G(S.dreg);
+    }
     if constexpr (PRE_SPARSE) {
         const int* tp = (MODE & M_INJ_SIG) ? S.srcTile : S.recTile;
         const int2* te = (MODE & M_INJ_SIG) ? S.srcEnt : S.recEnt;
@@ -1426,6 +1442,28 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
         }
     }
 
+    // The shot's buffers written below, read from its slot before the first
+    // store: the compiler cannot tell that the stores leave the slot table
+    // alone, and would read them again before every store.
+    constexpr bool CAP_SS = ADJ && !(MODE & M_CAP_W);
+    constexpr bool CAP_ALT = !ADJ && (MODE & M_CAP_ALT);
+    gf* capA = nullptr;
+    gf* capB = nullptr;
+    long capOffA = 0, capOffB = 0;
+    if constexpr (MODE & M_CAPTURE) {
+        // Forward passes capture the coarse source wavefield W; the
+        // adjoint receiver pass accumulates into the two ss slices.
+        capA = G(CAP_SS ? S.ss[a.ssA] : CAP_ALT ? S.img : S.W);
+        capB = G(CAP_SS ? S.ss[a.ssB] : CAP_ALT ? S.img : S.W);
+        capOffA = CAP_SS ? a.ssOffA : a.wA;
+        capOffB = CAP_SS ? a.ssOffB : a.wB;
+    }
+    gf* bImg = G(S.img);
+    gf* bW = G(S.W);
+    gf* winSlice = nullptr;
+    if constexpr (MODE & M_WIN) {
+        if (a.win && a.winOut == WIN_SLICE) winSlice = G(S.ss[a.ssA]);
+    }
 #pragma unroll
     for (int j = 0; j < PX; ++j) {
         if (!ok[j]) continue;
@@ -1434,12 +1472,8 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
         const float n = nv[j] * dmp[j];
         nv[j] = n;
         if constexpr (MODE & M_CAPTURE) {
-            // Forward passes capture the coarse source wavefield W; the
-            // adjoint receiver pass accumulates into the two ss slices.
-            constexpr bool CAP_SS = ADJ && !(MODE & M_CAP_W);
-            constexpr bool CAP_ALT = !ADJ && (MODE & M_CAP_ALT);
-            gf* A = G(CAP_SS ? S.ss[a.ssA] + a.ssOffA : CAP_ALT ? S.img + a.wA : S.W + a.wA);
-            gf* B = G(CAP_SS ? S.ss[a.ssB] + a.ssOffB : CAP_ALT ? S.img + a.wB : S.W + a.wB);
+            gf* A = capA + capOffA;
+            gf* B = capB + capOffB;
             // A zero weight leaves the slice as it is (x + n*0 == x); a
             // first touch writes instead of accumulating into a zeroed slice.
             if (!a.skipA) A[gi] = a.assignA ? n * a.fe : A[gi] + n * a.fe;
@@ -1454,7 +1488,7 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
                 bnew = B[gi];
             }
             if constexpr (MODE & M_IMAGE) {
-                if (a.image) G(S.img)[gi] += G(S.W)[a.wImg + gi] * bnew;
+                if (a.image) bImg[gi] += bW[a.wImg + gi] * bnew;
             }
         }
         if constexpr (MODE & M_WIN) {
@@ -1463,10 +1497,10 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
                 // in the reference's order (old, current, new).
                 const float v = (ost[j] * a.wa + cur[j] * a.wb) + n * a.wc;
                 switch (a.winOut) {
-                case WIN_W: G(S.W)[a.wA + gi] = v; break;
-                case WIN_SS: G(S.W)[a.wA + gi] = G(a.mtil)[gi] * v; break;
-                case WIN_IMG: G(S.img)[gi] += G(S.W)[a.wImg + gi] * v; break;
-                default: G(S.ss[a.ssA])[a.ssOffA + gi] = v; break;
+                case WIN_W: bW[a.wA + gi] = v; break;
+                case WIN_SS: bW[a.wA + gi] = G(a.mtil)[gi] * v; break;
+                case WIN_IMG: bImg[gi] += bW[a.wImg + gi] * v; break;
+                default: winSlice[a.ssOffA + gi] = v; break;
                 }
             }
         }
@@ -1474,9 +1508,8 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
     }
 
     if constexpr (POST_SPARSE) {
-        const int* tp = (MODE & M_REC_SIG) ? S.srcTile : S.recTile;
-        const int2* te = (MODE & M_REC_SIG) ? S.srcEnt : S.recEnt;
-        const int e0 = tp[t], e1 = tp[t + 1];
+        const int2* te = postTe;
+        const int e0 = postE0, e1 = postE1;
         if (e1 > e0) {                               // block-uniform
 #pragma unroll
             for (int j = 0; j < PX; ++j) sN[(lx0 + j) * TZ + lane] = nv[j];
@@ -1485,9 +1518,9 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
                 const int2 en = te[e];
                 const float n = sN[en.x];
                 if constexpr (MODE & M_REC_SIG) {
-                    G(S.sig)[(long)en.y * a.ntw + a.itwRec] += n;
+                    postSig[(long)en.y * a.ntw + a.itwRec] += n;
                 } else {
-                    gf* dr = G(S.dreg) + ((long)en.y * a.nts + a.its);
+                    gf* dr = postDreg + ((long)en.y * a.nts + a.its);
                     dr[0] += n * a.fe;
                     dr[1] += n * a.ff;
                 }
