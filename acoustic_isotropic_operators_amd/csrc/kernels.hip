@@ -27,6 +27,9 @@ namespace aiso {
 
 constexpr int FAT = 5;
 constexpr int NT = 256;
+// The step kernels address a slice as a 64-bit base plus a 32-bit element
+// offset; their launchers refuse larger slices.
+constexpr long kMaxSlicePoints = 0x7fffffffL;
 
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
 
@@ -1202,8 +1205,10 @@ __global__ __launch_bounds__(NT) void k_grad(const GradArgs a)
     }
 }
 
-// Unspecialized form of the fused step: one shot per workgroup, 64-bit
-// element offsets, bounds and sponge tested at every point.  The operands
+// Unspecialized form of the fused step: one shot per workgroup, bounds and
+// sponge tested at every point.  Every slice is addressed as a 64-bit base
+// (read once per workgroup) plus a 32-bit element offset within the slice
+// (launch_step_generic refuses grids of 2^31 points or more).  The operands
 // are read through global address-space views, so their loads are not
 // counted with the LDS reads of the stencil.  FS: the free-surface
 // conditions, for launches with a surface; without it they are compiled
@@ -1245,6 +1250,17 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
     bool ok[PX];
     float os[PX], pv[PX], dmp[PX];
     float sA[HAS_SS ? PX : 1], sB[HAS_SS ? PX : 1], mm[SSB ? PX : 1];
+    // the secondary-source slices
+    cgf* ssa = nullptr;
+    cgf* ssb = nullptr;
+    if constexpr (SSB || SSP) {
+        ssa = G(S.W) + a.wA;
+        if (useB) ssb = G(S.W) + a.wB;
+    }
+    if constexpr (SSS) {
+        ssa = G(S.ss[a.ssA]) + a.ssOffA;
+        if (useB) ssb = G(S.ss[a.ssB]) + a.ssOffB;
+    }
     auto load_operands = [&] {
 #pragma unroll
         for (int j = 0; j < PX; ++j) {
@@ -1256,23 +1272,15 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
             if constexpr (HAS_SS) { sA[j] = 0.f; sB[j] = 0.f; }
             if constexpr (SSB) mm[j] = 0.f;
             if (ok[j]) {
-                const long gi = (long)gx * nz + gz;
+                const unsigned gi = (unsigned)(gx * nz + gz);
                 os[j] = po[gi];
                 pv[j] = ADJ ? pc[gi] : v2dt2[gi];
                 // sponge factor, its table read issued with the operands
                 dmp[j] = FS ? sponge_fs(G(a.cd), gz, gx, nz, nx, a.minPad) : sponge(G(a.cd), gz, gx, nz, nx, a.minPad);
-                if constexpr (SSB) {
-                    mm[j] = G(a.mtil)[gi];
-                    sA[j] = G(S.W)[a.wA + gi];
-                    if (useB) sB[j] = G(S.W)[a.wB + gi];
-                }
-                if constexpr (SSP) {
-                    sA[j] = G(S.W)[a.wA + gi];
-                    if (useB) sB[j] = G(S.W)[a.wB + gi];
-                }
-                if constexpr (SSS) {
-                    sA[j] = G(S.ss[a.ssA])[a.ssOffA + gi];
-                    if (useB) sB[j] = G(S.ss[a.ssB])[a.ssOffB + gi];
+                if constexpr (SSB) mm[j] = G(a.mtil)[gi];
+                if constexpr (HAS_SS) {
+                    sA[j] = ssa[gi];
+                    if (useB) sB[j] = ssb[gi];
                 }
             }
         }
@@ -1285,14 +1293,14 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
     {
         constexpr int KX = (LX + NXW - 1) / NXW, KZ = (LZ + TZ - 1) / TZ;
         float tv[KX][KZ], tm[ADJ ? KX : 1][ADJ ? KZ : 1];
+        const unsigned soff = (unsigned)((x0 - FAT + wy) * nz + (z0 - FAT + lane));   // first staged element
 #pragma unroll
         for (int kx = 0; kx < KX; ++kx)
 #pragma unroll
             for (int kz = 0; kz < KZ; ++kz) {
                 const int lx = wy + kx * NXW, lz = lane + kz * TZ;
-                const int gx = x0 - FAT + lx, gz = z0 - FAT + lz;
-                const bool in = lx < LX && lz < LZ && gx < nx && gz < nz;
-                const long gi = (long)gx * nz + gz;
+                const bool in = lx < LX && lz < LZ && x0 - FAT + lx < nx && z0 - FAT + lz < nz;
+                const unsigned gi = soff + (unsigned)(kx * NXW * nz + kz * TZ);
                 tv[kx][kz] = in ? pc[gi] : 0.f;
                 if constexpr (ADJ) tm[kx][kz] = in ? v2dt2[gi] : 0.f;
             }
@@ -1381,7 +1389,7 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
                 // The window takes the slices as stored (their values when
                 // they were new); at a free surface the staged current value
                 // is the mirrored one, so the stored one is read there.
-                if constexpr (MODE & M_WIN) cur[j] = surf ? pc[(long)gx * nz + gz] : cc;
+                if constexpr (MODE & M_WIN) cur[j] = surf ? pc[(unsigned)(gx * nz + gz)] : cc;
             }
             if constexpr (MODE & M_WIN) ost[j] = os[j];
             if constexpr (SSB) {
@@ -1458,22 +1466,38 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
         capOffA = CAP_SS ? a.ssOffA : a.wA;
         capOffB = CAP_SS ? a.ssOffB : a.wB;
     }
-    gf* bImg = G(S.img);
-    gf* bW = G(S.W);
-    gf* winSlice = nullptr;
-    if constexpr (MODE & M_WIN) {
-        if (a.win && a.winOut == WIN_SLICE) winSlice = G(S.ss[a.ssA]);
+    gf* imgI = nullptr;       // image of the imaging condition and its W slice
+    cgf* imgW = nullptr;
+    if constexpr (MODE & M_IMAGE) {
+        if (a.image) {
+            imgI = G(S.img);
+            imgW = G(S.W) + a.wImg;
+        }
     }
+    gf* winDst = nullptr;     // window capture: destination, and the W slice of WIN_IMG
+    cgf* winW = nullptr;
+    if constexpr (MODE & M_WIN) {
+        if (a.win) {
+            if (a.winOut == WIN_W || a.winOut == WIN_SS) {
+                winDst = G(S.W) + a.wA;
+            } else if (a.winOut == WIN_IMG) {
+                winDst = G(S.img);
+                winW = G(S.W) + a.wImg;
+            } else {
+                winDst = G(S.ss[a.ssA]) + a.ssOffA;
+            }
+        }
+    }
+    gf* const A = capA ? capA + capOffA : nullptr;
+    gf* const B = capB ? capB + capOffB : nullptr;
 #pragma unroll
     for (int j = 0; j < PX; ++j) {
         if (!ok[j]) continue;
         const int gx = x0 + lx0 + j;
-        const long gi = (long)gx * nz + gz;
+        const unsigned gi = (unsigned)(gx * nz + gz);
         const float n = nv[j] * dmp[j];
         nv[j] = n;
         if constexpr (MODE & M_CAPTURE) {
-            gf* A = capA + capOffA;
-            gf* B = capB + capOffB;
             // A zero weight leaves the slice as it is (x + n*0 == x); a
             // first touch writes instead of accumulating into a zeroed slice.
             if (!a.skipA) A[gi] = a.assignA ? n * a.fe : A[gi] + n * a.fe;
@@ -1488,7 +1512,7 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
                 bnew = B[gi];
             }
             if constexpr (MODE & M_IMAGE) {
-                if (a.image) bImg[gi] += bW[a.wImg + gi] * bnew;
+                if (a.image) imgI[gi] += imgW[gi] * bnew;
             }
         }
         if constexpr (MODE & M_WIN) {
@@ -1497,10 +1521,10 @@ __global__ __launch_bounds__(NT) void k_step_generic(const StepArgs a)
                 // in the reference's order (old, current, new).
                 const float v = (ost[j] * a.wa + cur[j] * a.wb) + n * a.wc;
                 switch (a.winOut) {
-                case WIN_W: bW[a.wA + gi] = v; break;
-                case WIN_SS: bW[a.wA + gi] = G(a.mtil)[gi] * v; break;
-                case WIN_IMG: bImg[gi] += bW[a.wImg + gi] * v; break;
-                default: winSlice[a.ssOffA + gi] = v; break;
+                case WIN_W: winDst[gi] = v; break;
+                case WIN_SS: winDst[gi] = G(a.mtil)[gi] * v; break;
+                case WIN_IMG: winDst[gi] += winW[gi] * v; break;
+                default: winDst[gi] = v; break;
                 }
             }
         }
@@ -1747,23 +1771,31 @@ __global__ __launch_bounds__(256) void k_ext_ss_tile(ExtBatchArgs a, int nshots,
 // v2dt2[x+h] instead.  The thread's image values of a shot stay in
 // registers across the batch, so the extended image is read and written
 // once per batch; the two operands pass through LDS, the next pair's tiles
-// fetched into registers while the current one is computed.
-template <int HX, bool WF>
-__global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nbx, int nby, int nbz)
+// fetched into registers while the current one is computed.  A workgroup
+// images NG of the offsets (offset group = fastest block index, so the
+// groups of one tile run side by side and share its tiles in L2): fewer
+// image registers per thread, more waves in flight.
+// 21 offsets (HX 10) in groups of 7, imaging (not WF): compiled to 4 waves
+// per SIMD (128 VGPRs, two registers spilled), faster on MI355X than the
+// compiler's own budget (129 VGPRs, 3 waves).  The WF form would spill six
+// at that budget and keeps the compiler's.
+template <int HX, int NG, bool WF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(HX == 10 && !WF ? 4 : 1, 8))) void k_ext_img_tile(ExtBatchArgs a, int nbx, int ngrp, int nby, int nbz)
 {
-    constexpr int NE = 2 * HX + 1;
     constexpr int LW = kExtTX + 2 * HX;
-    constexpr int RW = kExtCX + 2 * HX;        // register windows of R and W columns
+    constexpr int RW = kExtCX + NG - 1;        // register windows of R and W columns
     constexpr int PF = (LW + 3) / 4;
     __shared__ float Rl[LW * kExtTZ];
     __shared__ float Wl[LW * kExtTZ];
-    Blk3 blk;                                   // blk.y: the shot
-    if (!xcd_block3(nbx, nby, nbz, blk)) return;     // padding blocks (block-uniform)
+    Blk3 blk;                                   // blk.x: x tile * ngrp + offset group, blk.y: the shot
+    if (!xcd_block3(nbx * ngrp, nby, nbz, blk)) return;     // padding blocks (block-uniform)
+    const int grp = blk.x % ngrp, bx = blk.x / ngrp;
+    const int e0 = grp * NG;                    // the group's first offset index
     const int hExt = a.hExt, ne = 2 * hExt + 1, lw = kExtTX + 2 * hExt;
     const long nz = a.nz, nzx = nz * a.nx;
     const int tz = threadIdx.x, g = threadIdx.y;
     const int z = FAT + blk.z * kExtTZ + tz;
-    const int x0 = FAT + hExt + blk.x * kExtTX;
+    const int x0 = FAT + hExt + bx * kExtTX;
     const int xEnd = a.nx - FAT - hExt;
     const bool zok = z < a.nz - FAT;
     const int xb = x0 + kExtCX * g;
@@ -1793,22 +1825,25 @@ __global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nbx, i
             pw[u] = in ? W[(long)gx * nz + z] : 0.f;
         }
     };
-    const int jr = xb - x0;                      // R window entry i: column xb + i - hExt
-    const int jw = xb - x0 - 2 * HX + 2 * hExt;  // W window entry i: column xb + i - 2*HX + hExt
+    // Offset e = e0 + q (q < NG), column xb + c: R at xb + c + h is R window
+    // entry c + q (column xb + e0 + i - hExt for entry i); W at xb + c - h is
+    // W window entry c - q + NG - 1 (column xb - e0 - NG + 1 + i + hExt).
+    const int jr = xb - x0 + e0;
+    const int jw = xb - x0 + 2 * hExt - e0 - NG + 1;
     float vw[WF ? RW : 1];                       // WF: v2dt2 on the R window
     if constexpr (WF) {
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
-            const int gx = xb + i - hExt;
+            const int gx = xb + e0 + i - hExt;
             vw[i] = (zok && gx < a.nx) ? G(a.v2dt2)[(long)gx * nz + z] : 0.f;
         }
     }
-    float m[NE][kExtCX];
+    float m[NG][kExtCX];
 #pragma unroll
-    for (int e = 0; e < NE; ++e)
+    for (int q = 0; q < NG; ++q)
 #pragma unroll
         for (int c = 0; c < kExtCX; ++c)
-            m[e][c] = (e < ne && zok && xb + c < xEnd) ? img[nzx * e + (long)(xb + c) * nz + z] : 0.f;
+            m[q][c] = (e0 + q < ne && zok && xb + c < xEnd) ? img[nzx * (e0 + q) + (long)(xb + c) * nz + z] : 0.f;
     fetch(0);
     for (int k = 0; k < a.nk; ++k) {
 #pragma unroll
@@ -1829,13 +1864,12 @@ __global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nbx, i
         }
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < NE; ++e)
+        for (int q = 0; q < NG; ++q)
 #pragma unroll
             for (int c = 0; c < kExtCX; ++c)
-                // R at xb+c+h is entry c + e; W at xb+c-h is entry c - e + 2*HX
-                if (e < ne) {
-                    if constexpr (WF) m[e][c] += (ww[c - e + 2 * HX] * rw[c + e]) * vw[c + e];
-                    else m[e][c] += rw[c + e] * ww[c - e + 2 * HX];
+                if (e0 + q < ne) {
+                    if constexpr (WF) m[q][c] += (ww[c - q + NG - 1] * rw[c + q]) * vw[c + q];
+                    else m[q][c] += rw[c + q] * ww[c - q + NG - 1];
                 }
     }
     // The store addresses are formed again here rather than kept live
@@ -1843,10 +1877,10 @@ __global__ __launch_bounds__(256) void k_ext_img_tile(ExtBatchArgs a, int nbx, i
     float* imgOut = S.img;
     asm volatile("" : "+v"(imgOut));
 #pragma unroll
-    for (int e = 0; e < NE; ++e)
+    for (int q = 0; q < NG; ++q)
 #pragma unroll
         for (int c = 0; c < kExtCX; ++c)
-            if (e < ne && zok && xb + c < xEnd) G(imgOut)[nzx * e + (long)(xb + c) * nz + z] = m[e][c];
+            if (e0 + q < ne && zok && xb + c < xEnd) G(imgOut)[nzx * (e0 + q) + (long)(xb + c) * nz + z] = m[q][c];
 }
 
 // The same two operations for offsets beyond the register-tiled kernels:
@@ -2431,6 +2465,7 @@ hipError_t launch_step(unsigned mode, int tz, int tx, int spb, const StepArgs& a
     StepKernel k = a.fs ? pick_step_fs(mode, tz, tx) : pick_step(mode, tz, tx, spb, occ);
     if (!k) return hipErrorInvalidValue;
     if ((mode & M_WIN) && a.winOut == WIN_SLICE) return hipErrorInvalidValue;   // extended-Born capture: generic only
+    if ((long)a.nz * a.nx > kMaxSlicePoints) return hipErrorInvalidValue;
     // One workgroup per (tile, group of spb shots).
     StepArgs b = a;
     b.gridLogical = a.ntiles * ((a.nshots + spb - 1) / spb);
@@ -2501,6 +2536,7 @@ hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a,
     default: break;
     }
     if (!k) return hipErrorInvalidValue;
+    if ((long)a.nz * a.nx > kMaxSlicePoints) return hipErrorInvalidValue;
     StepArgs b = a;
     b.gridLogical = a.ntiles * a.nshots;               // one workgroup per (tile, shot)
     return launch_k(k, (b.gridLogical + 7) / 8 * 8, b, s, e0, e1);
@@ -2514,6 +2550,7 @@ hipError_t launch_fwd_pair(int tz, int tx, const StepArgs& a, hipStream_t s, hip
     else if (tz == 32 && tx == 16) k = a.fs ? k_fwd_pair_fs<32, 16> : k_fwd_pair<32, 16>;
     else if (tz == 32 && tx == 32) k = a.fs ? k_fwd_pair_fs<32, 32> : k_fwd_pair<32, 32>;
     if (!k) return hipErrorInvalidValue;
+    if ((long)a.nz * a.nx > kMaxSlicePoints) return hipErrorInvalidValue;
     return launch_k(k, (a.gridLogical + 7) / 8 * 8, a, s, e0, e1);
 }
 
@@ -2631,12 +2668,15 @@ template <bool WF>
 static void ext_img_launch(const ExtBatchArgs& a, int nshots, hipStream_t s, int nzi, int nxr)
 {
     if (a.hExt <= 20) {
+        // offsets per workgroup: all of them up to 11, else groups of 7 (21 offsets) or 14
+        const int ng = a.hExt <= 5 ? 2 * a.hExt + 1 : a.hExt <= 10 ? 7 : 14;
+        const int ngrp = (2 * a.hExt + 1 + ng - 1) / ng;
         const int bx = (nxr + kExtTX - 1) / kExtTX, by = nshots, bz = (nzi + kExtTZ - 1) / kExtTZ;
-        const dim3 g(xcd_padded(bx * by * bz)), b(kExtTZ, 4);
-        if (a.hExt <= 2) hipLaunchKernelGGL((k_ext_img_tile<2, WF>), g, b, 0, s, a, bx, by, bz);
-        else if (a.hExt <= 5) hipLaunchKernelGGL((k_ext_img_tile<5, WF>), g, b, 0, s, a, bx, by, bz);
-        else if (a.hExt <= 10) hipLaunchKernelGGL((k_ext_img_tile<10, WF>), g, b, 0, s, a, bx, by, bz);
-        else hipLaunchKernelGGL((k_ext_img_tile<20, WF>), g, b, 0, s, a, bx, by, bz);
+        const dim3 g(xcd_padded(bx * ngrp * by * bz)), b(kExtTZ, 4);
+        if (a.hExt <= 2) hipLaunchKernelGGL((k_ext_img_tile<2, 5, WF>), g, b, 0, s, a, bx, 1, by, bz);
+        else if (a.hExt <= 5) hipLaunchKernelGGL((k_ext_img_tile<5, 11, WF>), g, b, 0, s, a, bx, 1, by, bz);
+        else if (a.hExt <= 10) hipLaunchKernelGGL((k_ext_img_tile<10, 7, WF>), g, b, 0, s, a, bx, ngrp, by, bz);
+        else hipLaunchKernelGGL((k_ext_img_tile<20, 14, WF>), g, b, 0, s, a, bx, ngrp, by, bz);
     } else {
         dim3 g((nzi + 63) / 64, nshots, nxr);
         hipLaunchKernelGGL(k_ext_img_any<WF>, g, dim3(64), 0, s, a);

@@ -2,8 +2,9 @@
 remarks that the build keeps (acoustic_isotropic_operators_amd/_build/
 kernels_resource_usage.txt): no kernel spills registers except the 32 x 16
 lockstep pair kernel's one register (measured faster at 6 waves per SIMD
-than without the spill at 5), and the default kernels of the Born benchmark
-keep the occupancy they were tuned at.  CPU only."""
+than without the spill at 5) and the 21-offset imaging kernel's two
+(faster at 4 waves than without them at 3), and the default kernels of the
+Born benchmark keep the occupancy they were tuned at.  CPU only."""
 import os
 import re
 
@@ -11,13 +12,17 @@ import pytest
 
 from acoustic_isotropic_operators_amd import build
 
-ALLOWED_SPILLS = {"_ZN4aiso10k_fwd_pairILi32ELi16EEEvNS_8StepArgsE": 1}
+ALLOWED_SPILLS = {
+    "_ZN4aiso10k_fwd_pairILi32ELi16EEEvNS_8StepArgsE": 1,
+    "_ZN4aiso14k_ext_img_tileILi10ELi7ELb0EEEvNS_12ExtBatchArgsEiiii": 2,
+}
 # MODE_SRC_WIN = 520, MODE_REC_WIN = 529, MODE_NL_FWD = 136
 TUNED_OCCUPANCY = {
     "_ZN4aiso10k_fwd_pairILi32ELi16EEEvNS_8StepArgsE": 6,
     "_ZN4aiso8k_step_wILi32ELi32ELj520ELi6EEEvNS_8StepArgsE": 6,
     "_ZN4aiso8k_step_wILi32ELi32ELj529ELi6EEEvNS_8StepArgsE": 6,
     "_ZN4aiso10k_step_winILi32ELi32ELj136EEEvNS_11WinStepArgsE": 8,
+    "_ZN4aiso14k_ext_img_tileILi10ELi7ELb0EEEvNS_12ExtBatchArgsEiiii": 4,
 }
 
 
