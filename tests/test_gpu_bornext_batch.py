@@ -24,7 +24,9 @@ def _engine(c):
     return ctx, srcs, recs
 
 
-@pytest.mark.parametrize("sub,nExt", [(2, 21), (3, 11), (2, 43), (1, 3)])
+# nExt 21 and 31 image in offset groups of 7 and 14 (31: three groups, the
+# last one partial); 43 takes the untiled kernels.
+@pytest.mark.parametrize("sub,nExt", [(2, 21), (3, 11), (2, 43), (1, 3), (2, 31)])
 def test_batched_offset_imaging_matches_oracle(sub, nExt):
     # 23 coarse samples: batches of 5 and 8 leave a partial last batch
     c = make_case(seed=61, nShot=3, nts=23, sub=sub, dts=0.001 * sub, nExt=nExt)

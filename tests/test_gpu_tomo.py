@@ -75,11 +75,11 @@ def test_tomo_add_and_dot_product():
     assert abs(a - b) <= 1e-5 * abs(b)
 
 
-@pytest.mark.parametrize("sub,nExt,extBatch", [(2, 5, 16), (3, 3, 4), (2, 21, 5), (2, 43, 3)])
+@pytest.mark.parametrize("sub,nExt,extBatch", [(2, 5, 16), (3, 3, 4), (2, 21, 5), (2, 43, 3), (2, 31, 7)])
 def test_wemva_matches_oracle(sub, nExt, extBatch):
     """WEMVA (aiso_wemva_forward / aiso_wemva_adjoint), both legs and each
-    alone, bitwise equal to the oracle; nExt = 43 takes the untiled
-    imaging kernel."""
+    alone, bitwise equal to the oracle; nExt = 21 and 31 image in offset
+    groups of 7 and 14, nExt = 43 takes the untiled imaging kernel."""
     c = make_case(seed=85, nShot=3, nts=23, sub=sub, dts=0.001 * sub, nExt=nExt)
     ctx, srcs, recs = _engine(c)
     ctx.set_option("extBatch", extBatch)
