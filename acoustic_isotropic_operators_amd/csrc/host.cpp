@@ -54,6 +54,8 @@ FdParams make_params(const aiso_fd_params& a)
     if ((p.nx - 2 * kFat) <= 0 || (p.nx - 2 * kFat) % a.blockSize != 0) fail(AISO_ERR_PARAM, "nx not a multiple of block size");
     if (p.nExt > 1 && p.extension == AISO_EXT_OFFSET && 2 * p.hExt >= p.nx - 2 * kFat)
         fail(AISO_ERR_PARAM, "extension longer than the model");
+    // The kernels address a slice as a 64-bit base plus a 32-bit element offset.
+    if ((long)p.nz * p.nx > 0x7fffffffL) fail(AISO_ERR_PARAM, "grid of 2^31 points or more");
 
     // Laplacian coefficients: double literal / float(d*d), stored as float.
     static const double C[6] = {-2.927222222, 1.666666667, -0.238095238,

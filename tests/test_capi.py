@@ -67,6 +67,15 @@ def test_ctx_create_rejects_bad_parameters_on_the_host():
                            xPadMinus=10, xPadPlus=10, fat=4))
 
 
+def test_ctx_create_rejects_slices_beyond_32bit_offsets():
+    """The kernels address a slice with 32-bit element offsets: a padded grid
+    of 2^31 points or more is refused when the context is made."""
+    n = 46346                                   # (n - 10) % 16 == 0, n * n > 2^31 - 1
+    with pytest.raises(RuntimeError, match="2\\^31"):
+        _capi.Context(dict(nz=n, nx=n, dz=10.0, dx=10.0, nts=60, dts=0.004, sub=4, zPadMinus=10, zPadPlus=10,
+                           xPadMinus=10, xPadPlus=10))
+
+
 def test_operand_checks_take_numpy_and_cpu_tensors_for_host_memory():
     """Host operands may be float32 numpy arrays or CPU tensors (the autograd
     layer passes CPU tensors); wrong dtype, layout or element count is refused
