@@ -2281,6 +2281,36 @@ __global__ void k_img_acc(const ShotSlot* X, long xOff, const ShotSlot* Y, long 
     SX.img[i] += SX.W[xOff + i] * SY.W[yOff + i];
 }
 
+// The same over nk coarse samples, in the order given: the image element
+// stays in a register across the samples (the same float operations in the
+// same order as nk launches of k_img_acc), so the image is read and written
+// once per batch.  All the batch's operand loads are issued first.
+__global__ __launch_bounds__(64) void k_img_acc_batch(const ImgAccArgs a)
+{
+    const ShotSlot& SX = a.X[blockIdx.y];
+    const ShotSlot& SY = a.Y[blockIdx.y];
+    const int iz = FAT + blockIdx.x * blockDim.x + threadIdx.x;
+    const int ix = FAT + blockIdx.z;
+    if (iz >= a.nz - FAT) return;
+    const unsigned i = (unsigned)(ix * a.nz + iz);
+    cgf* xw = G(SX.W);
+    cgf* yw = G(SY.W);
+    gf* img = G(SX.img);
+    float xv[kExtBatchMax], yv[kExtBatchMax];
+#pragma unroll
+    for (int j = 0; j < kExtBatchMax; ++j) {
+        if (j < a.nk) {
+            xv[j] = xw[a.xOff[j] + i];
+            yv[j] = yw[a.yOff[j] + i];
+        }
+    }
+    float v = img[i];
+#pragma unroll
+    for (int j = 0; j < kExtBatchMax; ++j)
+        if (j < a.nk) v += xv[j] * yv[j];
+    img[i] = v;
+}
+
 // acc += img (one shot, in shot order).
 __global__ void k_record_scale(const ShotSlot* slot, int ssIdx, float* __restrict__ out,
                                const float* __restrict__ refl, const float* __restrict__ v2dt2, int nz, int nx)
@@ -2731,6 +2761,14 @@ hipError_t launch_img_acc(const ShotSlot* X, long xOff, const ShotSlot* Y, long 
                           hipStream_t s)
 {
     hipLaunchKernelGGL(k_img_acc, column_grid(nz, nx, nshots), dim3(64), 0, s, X, xOff, Y, yOff, nz, nx);
+    return hipGetLastError();
+}
+
+hipError_t launch_img_acc_batch(const ImgAccArgs& a, int nshots, hipStream_t s)
+{
+    if (a.nk < 1 || a.nk > kExtBatchMax) return hipErrorInvalidValue;
+    if ((long)a.nz * a.nx > kMaxSlicePoints) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_img_acc_batch, column_grid(a.nz, a.nx, nshots), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

@@ -102,6 +102,17 @@ struct ExtBatchArgs {
     int nts;                // coarse samples of W (bounds check of the time-lag batches)
 };
 
+// Image accumulation over a batch of coarse samples (k_img_acc_batch):
+// X.img += X.W[xOff[j]] * Y.W[yOff[j]] for j = 0 .. nk-1, in that order.
+struct ImgAccArgs {
+    const ShotSlot* X;      // image, and the first operand's slices
+    const ShotSlot* Y;      // the second operand's slices
+    int nz, nx;
+    int nk;                 // samples in the batch, 1..kExtBatchMax
+    long xOff[kExtBatchMax];
+    long yOff[kExtBatchMax];
+};
+
 // Two fine steps of both fields of the lockstep Born forward (sub == 2) per
 // launch (k_fwd_pair2).  Launch m advances the source field by steps 2m+1,
 // 2m+2 and the scattered field by steps 2m-1, 2m.  Each field reads its
@@ -243,6 +254,7 @@ hipError_t launch_mul_batch(const ExtBatchArgs& a, int nshots, hipStream_t s);
 hipError_t launch_tomo_adj_ss(const ExtBatchArgs& a, int nshots, hipStream_t s);
 hipError_t launch_img_acc(const ShotSlot* X, long xOff, const ShotSlot* Y, long yOff, int nshots, int nz, int nx,
                           hipStream_t s);
+hipError_t launch_img_acc_batch(const ImgAccArgs& a, int nshots, hipStream_t s);
 hipError_t launch_accumulate(float* acc, const float* img, long n, hipStream_t s);
 // out = (ss[ssIdx] * refl) * v2dt2 on the interior of one shot slot
 // (recordScaleWavefield, F/src/kernelsGpu.cu:145-153, into a zeroed slice).

@@ -18,6 +18,7 @@
 // of a pass is one launch of the fused step kernel over the batch.
 #include <cmath>
 #include <functional>
+#include <memory>
 
 #include "engine_internal.hpp"
 
@@ -281,7 +282,9 @@ void Call::plan_batches()
     extK = std::min(c->extBatch, p.nts);
     ssSlices = is_ext(q.op) || tomo ? extK + 1 : 2;
     needImg = q.op == Op::BornAdj || q.op == Op::ExtAdj || q.op == Op::TomoAdj || wemva;
-    capSlices = wemva ? extK + 1 : 2;     // captured coarse samples kept per shot
+    // captured coarse samples kept per shot: the WEMVA forward's imaging and
+    // the tomographic / WEMVA adjoints' image accumulation take extK at a time
+    capSlices = wemva || q.op == Op::TomoAdj ? extK + 1 : 2;
     // Three consecutive fine slices hold everything a coarse sample needs
     // when sub == 2, so captures and imaging are written once per coarse
     // step instead of accumulated at every fine step.
@@ -1035,11 +1038,29 @@ void Call::passes_tomo()
         done(0);
         join();
     };
-    // img += X[k] * captured sample k, one coarse sample at a time
-    auto image_with = [&](const ShotSlot* X) {
-        return [&, X](int k) {
+    // img += X[k] * captured sample k, for the samples in the order they
+    // complete, extK of them per launch (the capture ring holds extK + 1:
+    // the pending samples and the one being captured); `last` is the pass's
+    // final sample.
+    auto image_with = [&](const ShotSlot* X, int last) {
+        auto pend = std::make_shared<std::vector<int>>();
+        return [&, X, last, pend](int k) {
+            pend->push_back(k);
+            if ((int)pend->size() < extK && k != last) return;
+            ImgAccArgs g{};
+            g.nz = p.nz;
+            g.nx = p.nx;
+            g.nk = (int)pend->size();
+            for (int j = 0; j < g.nk; ++j) {
+                g.xOff[j] = (long)(*pend)[j] * nzx;
+                g.yOff[j] = cap_off((*pend)[j]);
+            }
+            pend->clear();
             lanes([&](int s0, int cnt, hipStream_t st) {
-                HIPC(launch_img_acc(X + s0, (long)k * nzx, dSlotsC + s0, cap_off(k), cnt, p.nz, p.nx, st));
+                ImgAccArgs h = g;
+                h.X = X + s0;
+                h.Y = dSlotsC + s0;
+                HIPC(launch_img_acc_batch(h, cnt, st));
             });
         };
     };
@@ -1086,8 +1107,8 @@ void Call::passes_tomo()
         // imaged against R, and backward from the adjoint extended
         // scattering of R imaged against W''; the tomographic adjoint runs
         // them as legs 1, 2, WEMVA's as legs 2, 1.
-        auto fwdLeg = [&] { forward_pass(MODE_TOMO_CAP, dSlotsC, SS_EXT, dSlots, rExt, 2, image_with(dSlotsQ)); };
-        auto bwdLeg = [&] { backward_pass(SS_ADJ, dSlotsQ, rExt, image_with(dSlots)); };
+        auto fwdLeg = [&] { forward_pass(MODE_TOMO_CAP, dSlotsC, SS_EXT, dSlots, rExt, 2, image_with(dSlotsQ, p.nts - 1)); };
+        auto bwdLeg = [&] { backward_pass(SS_ADJ, dSlotsQ, rExt, image_with(dSlots, 0)); };
         if (q.op == Op::TomoAdj) {
             if (c->leg1) fwdLeg();
             if (c->leg2) bwdLeg();
