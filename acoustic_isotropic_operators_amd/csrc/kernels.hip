@@ -2268,23 +2268,12 @@ __global__ __launch_bounds__(256) void k_tomo_adj_ss_tile(ExtBatchArgs a, int ns
     }
 }
 
-// imagingAdjGpu (F/src/kernelsGpu.cu:318-325): img += X.W[xOff] * Y.W[yOff]
-// at interior points, img from X's slot.
-__global__ void k_img_acc(const ShotSlot* X, long xOff, const ShotSlot* Y, long yOff, int nz, int nx)
-{
-    const ShotSlot& SX = X[blockIdx.y];
-    const ShotSlot& SY = Y[blockIdx.y];
-    const int iz = FAT + blockIdx.x * blockDim.x + threadIdx.x;
-    const int ix = FAT + blockIdx.z;
-    if (iz >= nz - FAT) return;
-    const long i = (long)ix * nz + iz;
-    SX.img[i] += SX.W[xOff + i] * SY.W[yOff + i];
-}
-
-// The same over nk coarse samples, in the order given: the image element
-// stays in a register across the samples (the same float operations in the
-// same order as nk launches of k_img_acc), so the image is read and written
-// once per batch.  All the batch's operand loads are issued first.
+// imagingAdjGpu (F/src/kernelsGpu.cu:318-325), img += X.W[xOff] * Y.W[yOff]
+// at interior points with img from X's slot, over nk coarse samples in the
+// order given: the image element stays in a register across the samples
+// (the same float operations in the same order as one sample at a time), so
+// the image is read and written once per batch.  All the batch's operand
+// loads are issued first.
 __global__ __launch_bounds__(64) void k_img_acc_batch(const ImgAccArgs a)
 {
     const ShotSlot& SX = a.X[blockIdx.y];
@@ -2754,13 +2743,6 @@ hipError_t launch_tomo_adj_ss(const ExtBatchArgs& a, int nshots, hipStream_t s)
     } else {
         hipLaunchKernelGGL(k_tomo_adj_ss, column_grid(a.nz, a.nx, nshots), dim3(64), 0, s, a);
     }
-    return hipGetLastError();
-}
-
-hipError_t launch_img_acc(const ShotSlot* X, long xOff, const ShotSlot* Y, long yOff, int nshots, int nz, int nx,
-                          hipStream_t s)
-{
-    hipLaunchKernelGGL(k_img_acc, column_grid(nz, nx, nshots), dim3(64), 0, s, X, xOff, Y, yOff, nz, nx);
     return hipGetLastError();
 }
 

@@ -252,8 +252,6 @@ hipError_t launch_dt2_coarse(const ShotSlot* slots, int nshots, int nts, float c
                              hipStream_t s);
 hipError_t launch_mul_batch(const ExtBatchArgs& a, int nshots, hipStream_t s);
 hipError_t launch_tomo_adj_ss(const ExtBatchArgs& a, int nshots, hipStream_t s);
-hipError_t launch_img_acc(const ShotSlot* X, long xOff, const ShotSlot* Y, long yOff, int nshots, int nz, int nx,
-                          hipStream_t s);
 hipError_t launch_img_acc_batch(const ImgAccArgs& a, int nshots, hipStream_t s);
 hipError_t launch_accumulate(float* acc, const float* img, long n, hipStream_t s);
 // out = (ss[ssIdx] * refl) * v2dt2 on the interior of one shot slot
