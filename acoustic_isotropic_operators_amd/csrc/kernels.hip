@@ -2549,6 +2549,8 @@ hipError_t launch_step_generic(unsigned mode, int tz, int tx, const StepArgs& a,
     case MODE_TOMO_CAP: k = pick_generic_tile<MODE_TOMO_CAP>(tz, tx, fs); break;
     case MODE_TOMO_ADJ_CAP: k = pick_generic_tile<MODE_TOMO_ADJ_CAP>(tz, tx, fs); break;
     case MODE_TOMO_REC: k = pick_generic_tile<MODE_TOMO_REC>(tz, tx, fs); break;
+    case MODE_TOMO_CAP_WIN: k = pick_generic_tile<MODE_TOMO_CAP_WIN>(tz, tx, fs); break;
+    case MODE_TOMO_ADJ_CAP_WIN: k = pick_generic_tile<MODE_TOMO_ADJ_CAP_WIN>(tz, tx, fs); break;
     case MODE_EXT_REC: k = pick_generic_tile<MODE_EXT_REC>(tz, tx, fs); break;
     case MODE_REC_WIN: k = pick_generic_tile<MODE_REC_WIN>(tz, tx, fs); break;
     case MODE_BORN_SCAT_CAP: k = pick_generic_tile<MODE_BORN_SCAT_CAP>(tz, tx, fs); break;

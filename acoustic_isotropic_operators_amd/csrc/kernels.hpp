@@ -206,6 +206,10 @@ constexpr unsigned MODE_REC_WIN = M_ADJ | M_INJ_DATA | M_WIN;
 constexpr unsigned MODE_TOMO_CAP = M_SS_SLICE | M_CAPTURE;
 constexpr unsigned MODE_TOMO_ADJ_CAP = M_ADJ | M_SS_SLICE | M_CAPTURE | M_CAP_W;
 constexpr unsigned MODE_TOMO_REC = M_ADJ | M_INJ_DATA | M_CAPTURE | M_CAP_W;
+// Their sub == 2 forms (window capture into W, winOut WIN_W; the receiver
+// wavefield's is MODE_REC_WIN).
+constexpr unsigned MODE_TOMO_CAP_WIN = M_SS_SLICE | M_WIN;
+constexpr unsigned MODE_TOMO_ADJ_CAP_WIN = M_ADJ | M_SS_SLICE | M_WIN;
 // Wavefield saving: the Born scattered field captured at the coarse rate
 // into ShotSlot::img (the source wavefield W stays readable).
 constexpr unsigned MODE_BORN_SCAT_CAP = M_SS_BORN | M_CAPTURE | M_CAP_ALT;

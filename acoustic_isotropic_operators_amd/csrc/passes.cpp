@@ -957,6 +957,16 @@ void Call::passes_tomo()
         x.win = 0;
         int next = 0;
         auto off = [&](int k) { return capture == 1 ? (long)k * nzx : cap_off(k); };
+        // sub == 2: sample its written once at the first fine step of its
+        // interval from the old, current and new slices (as the source pass)
+        const bool wcap = capture && win;
+        if (wcap) {
+            mode = MODE_TOMO_CAP_WIN;
+            x.winOut = WIN_W;
+            x.wa = f[h + 1];
+            x.wb = f[h + 2 < 2 * h ? h + 2 : h + 1];
+            x.wc = f[1];
+        }
         fork();
         for (int its = 0; its < p.nts - 1; ++its) {
             while (ssKind >= 0 && next <= its + 1) {
@@ -975,7 +985,10 @@ void Call::passes_tomo()
                     x.ssOffA = ss_slot(its);
                     x.ssOffB = ss_slot(its + 1);
                 }
-                if (capture) {
+                if (wcap) {
+                    x.wA = off(its);
+                    x.win = it2 == 1;
+                } else if (capture) {
                     x.wA = off(its);
                     x.wB = off(its + 1);
                     x.skipA = x.fe == 0.f;
@@ -993,6 +1006,11 @@ void Call::passes_tomo()
             }
             if (done) done(its);
         }
+        if (wcap)   // sample nts-1 = p(last-1)*f[h+1] + p(last)*f[h+2]
+            lanes([&](int s0, int cnt, hipStream_t st) {
+                HIPC(launch_win_tail(on + s0, cnt, x.cur, WIN_W, f[h + 1], f[h + 2], off(p.nts - 1), 0, 0, nullptr,
+                                     p.nz, p.nx, st));
+            });
         if (done) done(p.nts - 1);
         join();
     };
@@ -1008,6 +1026,16 @@ void Call::passes_tomo()
         y.cur = 1;
         y.win = 0;
         int next = p.nts - 1;   // highest sample not yet in the ring
+        // sub == 2: sample its + 1 written once at the first fine step of
+        // interval its (in backward order), as the Born receiver pass
+        unsigned mode = MODE_TOMO_ADJ_CAP;
+        if (win) {
+            mode = MODE_TOMO_ADJ_CAP_WIN;
+            y.winOut = WIN_W;
+            y.wa = f[1];
+            y.wb = f[0];
+            y.wc = f[h + 1];
+        }
         fork();
         for (int its = p.nts - 2; its >= 0; --its) {
             while (next >= its) {
@@ -1022,19 +1050,25 @@ void Call::passes_tomo()
                 y.ssA = y.ssB = 0;
                 y.ssOffA = ss_slot(its);
                 y.ssOffB = ss_slot(its + 1);
-                y.wA = cap_off(its);
+                y.wA = win ? cap_off(its + 1) : cap_off(its);
                 y.wB = cap_off(its + 1);
+                y.win = it2 == 1;
                 y.fe = f[it2];
                 y.ff = f[h + it2];
                 y.skipA = y.fe == 0.f;
                 y.skipB = y.ff == 0.f;
                 y.assignA = it2 == p.sub - 1;
                 y.assignB = 0;
-                launch_timed(c, MODE_TOMO_ADJ_CAP, AISO_K_TOMO, y, true);
+                launch_timed(c, mode, AISO_K_TOMO, y, true);
                 y.cur ^= 1;
             }
             done(its + 1);
         }
+        if (win)   // sample 0 = q(1)*f[1] + q(0)*f[0]
+            lanes([&](int s0, int cnt, hipStream_t st) {
+                HIPC(launch_win_tail(dSlotsC + s0, cnt, y.cur, WIN_W, f[1], f[0], cap_off(0), 0, 0, nullptr, p.nz,
+                                     p.nx, st));
+            });
         done(0);
         join();
     };
@@ -1070,6 +1104,15 @@ void Call::passes_tomo()
         x.slots = dSlotsQ;
         x.cur = 1;
         x.win = 0;
+        // sub == 2: R[its + 1] written once at the first fine step of interval its
+        unsigned mode = MODE_TOMO_REC;
+        if (win) {
+            mode = MODE_REC_WIN;
+            x.winOut = WIN_W;
+            x.wa = f[1];
+            x.wb = f[0];
+            x.wc = f[h + 1];
+        }
         fork();
         for (int its = p.nts - 2; its >= 0; --its)
             for (int it2 = p.sub - 1; it2 >= 0; --it2) {
@@ -1077,18 +1120,23 @@ void Call::passes_tomo()
                 x.itwRec = its * p.sub + it2;
                 x.fc = f[it2 + 1];
                 x.fd = f[h + it2 + 1];
-                x.wA = (long)its * nzx;
+                x.wA = win ? (long)(its + 1) * nzx : (long)its * nzx;
                 x.wB = (long)(its + 1) * nzx;
+                x.win = it2 == 1;
                 x.fe = f[it2];
                 x.ff = f[h + it2];
                 x.skipA = x.fe == 0.f;
                 x.skipB = x.ff == 0.f;
                 x.assignA = it2 == p.sub - 1;
                 x.assignB = its == p.nts - 2 && it2 == p.sub - 1;
-                launch_timed(c, MODE_TOMO_REC, AISO_K_TOMO, x, true);
-                ring(MODE_TOMO_REC, x);
+                launch_timed(c, mode, AISO_K_TOMO, x, true);
+                ring(mode, x);
                 x.cur ^= 1;
             }
+        if (win)   // R[0] = q(1)*f[1] + q(0)*f[0]
+            lanes([&](int s0, int cnt, hipStream_t st) {
+                HIPC(launch_win_tail(dSlotsQ + s0, cnt, x.cur, WIN_W, f[1], f[0], 0, 0, 0, nullptr, p.nz, p.nx, st));
+            });
         join();
     }
     if (img) HIPC(launch_fill(img, (long)nb * nImg * nzx, s));
