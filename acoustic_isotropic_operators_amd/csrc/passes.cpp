@@ -959,7 +959,7 @@ void Call::passes_tomo()
         auto off = [&](int k) { return capture == 1 ? (long)k * nzx : cap_off(k); };
         // sub == 2: sample its written once at the first fine step of its
         // interval from the old, current and new slices (as the source pass)
-        const bool wcap = capture && win;
+        const bool wcap = capture && win && mode == MODE_TOMO_CAP;
         if (wcap) {
             mode = MODE_TOMO_CAP_WIN;
             x.winOut = WIN_W;
