@@ -425,7 +425,10 @@ void run_nl_grad(aiso_ctx* c, const float* model, int nShotSig, const float* dat
     // per shot: forward pair, adjoint pair, K segment slices, gradient, checkpoints
     const size_t perShot = (size_t)(2 + 2 + K + 1 + 2 * nCk) * nzx * 4 + (size_t)nRecRegMax * p.nts * 4 +
                            2 * (size_t)nSrcRegMax * p.ntw * 4;
-    int B = c->batchAdj > 0 ? c->batchAdj : (c->batchReq > 0 ? c->batchReq : 8);
+    // 16 shots per batch by default, 8 per lane: at 400 x 1000 with 8 per
+    // batch a lane's launch covers the GPU about once (MI355X: 230 against
+    // 254 Gpts/s for 16; 12 and 32 gave 249 and 251).
+    int B = c->batchAdj > 0 ? c->batchAdj : (c->batchReq > 0 ? c->batchReq : 16);
     {
         size_t freeB = 0, totalB = 0;
         HIPC(hipMemGetInfo(&freeB, &totalB));
