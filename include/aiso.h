@@ -103,7 +103,8 @@ aiso_status aiso_ctx_set_tile(aiso_ctx* ctx, int tz, int tx);
  *   "pair2" 0/1: two fine steps per launch in the Born forward (lockstep)
  *       and adjoint when sub == 2 and no device lies in the FAT ring;
  *   "batchFwd", "batchAdj": shots per launch of forward / adjoint calls
- *       (0 = aiso_ctx_set_batch's value, or automatic);
+ *       (0 = aiso_ctx_set_batch's value, or automatic; "batchAdj" also
+ *       sets the batch of aiso_nl_velocity_gradient, 16 by default);
  *   "batchRec": group size of the adjoint receiver pass within a batch;
  *   "shotsPerBlock" 1/2: shots advanced by one workgroup of the step kernel;
  *   "occupancy" 0/6/8: the specialized step kernels compiled to the
